@@ -1,0 +1,172 @@
+"""RAW->RGB throughput on one MI355X.
+
+Workload (default): 12 MP (4032x3024) RGGB uint16 12-bit mosaics, black 64,
+white 4095, six noise levels a in {0.0005, 0.001, 0.002, 0.004, 0.008, 0.016},
+b = a/200, seeded synthetic scenes generated on the device.  One step runs the
+full path (scale, raw noise fit, demosaic, highlights, YCbCr, despeckle,
+5-level bilateral pyramid with per-level noise fits, tone map to RGBA8) on
+all six frames; inputs are resident in HBM before timing starts.
+
+Prints one JSON line.  `roofline` is measured for the dominant stage with
+HIP events on the library's stream during the timed steps; `cpu_baseline`
+times the FP32 CPU restatement (the oracle) on one of the frames.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import climage_amd as ca  # noqa: E402
+
+NOISE_A = [0.0005, 0.001, 0.002, 0.004, 0.008, 0.016]
+HBM_PEAK_GBS = 8000.0
+
+
+def stage_bytes(stage, texels, levels_px=None):
+    """Algorithmic bytes of one stage: each input read once, each output written
+    once, FP32 texels (luma 4 B, RGBA 16 B), raw 2 B, RGBA8 4 B."""
+    per = {
+        "scale_raw": 2 + 4,             # u16 in, f32 out (per pixel)
+        "raw_stats": 16 + 32,           # 4 f32 texels in, mean+var RGBA out (per quad)
+        "raw_fit": 3 * 32,              # three passes over mean+var (per quad)
+        "green": 4 + 4,
+        "red_blue_ycc": 4 + 4 + 16,
+        "despeckle": 16 + 16,
+        "downsample": 4 * 16 + 16,      # per output texel: 4 input texels + 1 output
+        "level_stats": 16 + 16,
+        "level_fit": 3 * 16,
+        "denoise": 16 + 16,
+        "reassemble": 16 + 16 + 16 / 4 + 16 / 4,
+        "tone": 16 + 4,
+    }
+    return per[stage] * texels
+
+
+def cpu_baseline(w, h, bayer, a, seconds_cap=30.0):
+    from oracle import oracle as O
+    import subprocess
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    raw = O.generate_bayer(w, h, bayer, 1000, 2000, a, a / 200.0, 64.0, 4095.0)
+    o = O.Oracle(double=False)
+    p = O.default_params(bayer, 64.0, 4095.0, levels=5)
+    t0 = time.perf_counter()
+    o.raw_to_rgb(raw, p)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": w * h / dt / 1e6, "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "sample": f"one {w}x{h} RGGB frame (a={a}) through the FP32 CPU restatement, "
+                      f"{dt:.2f} s, OpenMP row bands"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=4032)
+    ap.add_argument("--height", type=int, default=3024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(frames are independent, see DESIGN.md for how the path shards)", file=sys.stderr)
+        sys.exit(2)
+
+    w, h, bayer = args.width, args.height, ca.RGGB
+    lib = ca.lib()
+    ctx = ca.Context(0)
+    raws, outs = [], []
+    for i, a in enumerate(NOISE_A):
+        r = ctx.image(w, h, ca.FMT_R16U)
+        ctx.generate_bayer(r, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
+        raws.append(r)
+        outs.append(ctx.image(w, h, ca.FMT_RGBA8))
+    params = ca.default_params(bayer, 64.0, 4095.0, levels=5, algorithm=ca.BILATERAL)
+    ctx.sync()
+
+    def step():
+        for r, o in zip(raws, outs):
+            ctx.raw_to_rgb(r, o, params, writeback=False)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+
+    import ctypes as C
+
+    class Prof(C.Structure):
+        _fields_ = [("ms", C.c_double * 12), ("launches", C.c_int64 * 12), ("texels", C.c_int64 * 12)]
+
+    prof_on = not args.no_profile
+    if prof_on:
+        lib.clim_profile_enable(ctx.h, 1)
+        lib.clim_profile_read(ctx.h, C.byref(Prof()))
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    dt = time.perf_counter() - t0
+
+    stages = {}
+    if prof_on:
+        pr = Prof()
+        lib.clim_profile_read(ctx.h, C.byref(pr))
+        lib.clim_profile_enable(ctx.h, 0)
+        lib.clim_profile_stage_name.restype = C.c_char_p
+        for i in range(12):
+            name = lib.clim_profile_stage_name(i).decode()
+            if pr.launches[i]:
+                stages[name] = {"ms_total": pr.ms[i], "launches": int(pr.launches[i]),
+                                "texels": int(pr.texels[i])}
+
+    frames = args.steps * len(raws)
+    mpix = frames * w * h / dt / 1e6
+    roofline = None
+    if stages:
+        dom = max(stages, key=lambda k: stages[k]["ms_total"])
+        s = stages[dom]
+        avg_s = s["ms_total"] / s["launches"] / 1e3
+        bytes_per_launch = stage_bytes(dom, s["texels"] / s["launches"])
+        ach = bytes_per_launch / avg_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                    "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+        for k, v in stages.items():
+            v["us_per_frame"] = round(v["ms_total"] * 1e3 / frames, 2)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(w, h, bayer, 0.002)
+
+    res = {
+        "metric": "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP",
+        "value": round(mpix, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scenes generated on the device)",
+        "config": {"workload": f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, 5-level bilateral pyramid",
+                   "frames_per_step": len(raws), "pixels_per_frame": w * h},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages": stages,
+    }
+    print(json.dumps(res))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,300 @@
+"""climage_amd - MI355X (gfx950) RAW->RGB path of CLImage behind a C ABI.
+
+This module is a thin ctypes front end of climage_amd/lib/libclimage_hip.so
+(include/climage.h).  It mirrors the reference's operator names
+(cpp/demosaic_cl.hpp, cpp/pyramidal_denoise.hpp, cpp/raw_converter.hpp) so
+that tests and tools read like the reference's call sites.  There is no CPU
+fallback: every call runs the HIP kernels, and importing raises if the
+library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libclimage_hip.so")
+
+# clim_status
+OK, EINVAL, ENOMEM, EHIP, ENUMERIC, ENODEV = range(6)
+# clim_format
+FMT_R16U, FMT_R32F, FMT_RGBA32F, FMT_RGBA8, FMT_RG32F = 1, 2, 3, 4, 5
+# BayerPattern (cpp/demosaic.hpp:23-28)
+GRBG, GBRG, RGGB, BGGR = 0, 1, 2, 3
+# PyramidalDenoise::DenoiseAlgorithm (cpp/pyramidal_denoise.hpp:38)
+BILATERAL, GUIDED_FAST, GUIDED_PRECISE = 0, 1, 2
+
+_NP = {FMT_R16U: (np.uint16, 1), FMT_R32F: (np.float32, 1), FMT_RGBA32F: (np.float32, 4),
+       FMT_RGBA8: (np.uint8, 4), FMT_RG32F: (np.float32, 2)}
+
+
+class ClimError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{msg} (status {status})")
+        self.status = status
+
+
+class Image(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32),
+                ("pitch_bytes", C.c_int32), ("format", C.c_int32)]
+
+
+class DenoiseParameters(C.Structure):
+    _fields_ = [("luma", C.c_float), ("chroma", C.c_float), ("chromaBoost", C.c_float),
+                ("gradientBoost", C.c_float), ("sharpening", C.c_float)]
+
+
+class RGBConversionParameters(C.Structure):
+    _fields_ = [("contrast", C.c_float), ("saturation", C.c_float), ("toneCurveSlope", C.c_float),
+                ("exposureBias", C.c_float), ("blacks", C.c_float), ("localToneMapping", C.c_int32)]
+
+
+class NoiseModel(C.Structure):
+    _fields_ = [("rawNlf", C.c_float * 4), ("pyramidNlf", (C.c_float * 6) * 5)]
+
+
+class LTMParameters(C.Structure):
+    _fields_ = [("guidedFilterEps", C.c_float), ("shadows", C.c_float), ("highlights", C.c_float),
+                ("detail", C.c_float)]
+
+
+class DemosaicParameters(C.Structure):
+    _fields_ = [("bayerPattern", C.c_int32), ("black_level", C.c_float), ("white_level", C.c_float),
+                ("exposure_multiplier", C.c_float), ("scale_mul", C.c_float * 4),
+                ("rgb_cam", (C.c_float * 3) * 3), ("noiseModel", NoiseModel),
+                ("denoiseParameters", DenoiseParameters * 5), ("noiseLevel", C.c_float),
+                ("rgbConversionParameters", RGBConversionParameters), ("ltmParameters", LTMParameters),
+                ("levels", C.c_int32), ("denoiseAlgorithm", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"climage_amd: {LIB_PATH} is missing; build it with "
+                              f"`python climage_amd/build.py` (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        L.clim_last_error.restype = C.c_char_p
+        L.clim_status_string.restype = C.c_char_p
+        L.clim_stream.restype = C.c_void_p
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return [n for n in ("clim_create", "clim_destroy", "clim_last_error", "clim_status_string", "clim_sync",
+                        "clim_stream", "clim_image_alloc", "clim_image_free", "clim_copy_from_host",
+                        "clim_copy_to_host", "clim_format_texel_bytes", "clim_scale_raw_data",
+                        "clim_raw_noise_statistics", "clim_compute_raw_noise_statistics",
+                        "clim_interpolate_green", "clim_interpolate_red_blue", "clim_blend_highlights",
+                        "clim_transform_image", "clim_despeckle_image", "clim_downsample_image",
+                        "clim_noise_statistics", "clim_compute_noise_statistics", "clim_denoise_image",
+                        "clim_denoise_image_guided", "clim_guided_filter", "clim_reassemble_image",
+                        "clim_convert_to_srgb", "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
+                        "clim_pyramidal_denoise", "clim_raw_to_rgb", "clim_generate_bayer")]
+
+
+def _f(vals):
+    vals = [float(v) for v in np.asarray(vals, np.float32).ravel()]
+    return (C.c_float * len(vals))(*vals)
+
+
+def default_params(bayer=RGGB, black=64.0, white=4095.0, levels=5, algorithm=BILATERAL):
+    """DemosaicParameters for a synthetic identity-colour camera (clim_default_params)."""
+    p = DemosaicParameters()
+    lib().clim_default_params(C.byref(p), C.c_int32(bayer), C.c_float(black), C.c_float(white))
+    p.levels = levels
+    p.denoiseAlgorithm = algorithm
+    return p
+
+
+def cam_ycbcr(rgb_cam):
+    out = (C.c_float * 9)()
+    lib().clim_cam_ycbcr(_f(rgb_cam), out)
+    return np.array(out[:], np.float32).reshape(3, 3)
+
+
+def inverse3x3(m):
+    out = (C.c_float * 9)()
+    st = lib().clim_inverse3x3(_f(m), out)
+    if st != OK:
+        raise ClimError(st, "inverse3x3: singular matrix")
+    return np.array(out[:], np.float32).reshape(3, 3)
+
+
+class DeviceImage:
+    """A device image owned by a Context (cl_image_2d counterpart)."""
+
+    def __init__(self, ctx, width, height, fmt):
+        self.ctx = ctx
+        self.im = Image()
+        ctx._check(lib().clim_image_alloc(ctx.h, width, height, fmt, C.byref(self.im)), "image_alloc")
+
+    @property
+    def width(self):
+        return self.im.width
+
+    @property
+    def height(self):
+        return self.im.height
+
+    @property
+    def format(self):
+        return self.im.format
+
+    def ref(self):
+        return C.byref(self.im)
+
+    def upload(self, arr):
+        dt, ch = _NP[self.format]
+        arr = np.ascontiguousarray(arr, dt)
+        shape = (self.height, self.width) if ch == 1 else (self.height, self.width, ch)
+        if arr.shape != shape:
+            raise ValueError(f"upload: shape {arr.shape} != {shape}")
+        self.ctx._check(lib().clim_copy_from_host(self.ctx.h, arr.ctypes.data_as(C.c_void_p),
+                                                  C.c_int64(arr.strides[0]), self.ref()), "copy_from_host")
+
+    def download(self):
+        dt, ch = _NP[self.format]
+        shape = (self.height, self.width) if ch == 1 else (self.height, self.width, ch)
+        out = np.empty(shape, dt)
+        self.ctx._check(lib().clim_copy_to_host(self.ctx.h, self.ref(), out.ctypes.data_as(C.c_void_p),
+                                                C.c_int64(out.strides[0])), "copy_to_host")
+        return out
+
+    def free(self):
+        if self.im.data:
+            lib().clim_image_free(self.ctx.h, C.byref(self.im))
+
+
+class Context:
+    """One HIP stream plus work buffers on one gfx950 device (OpenCLContext counterpart)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        st = lib().clim_create(C.byref(self.h), device)
+        if st != OK:
+            raise ClimError(st, f"clim_create(device={device}): {lib().clim_status_string(st).decode()}")
+
+    def close(self):
+        if self.h:
+            lib().clim_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st, what):
+        if st != OK:
+            raise ClimError(st, f"{what}: {lib().clim_last_error(self.h).decode()}")
+
+    def image(self, width, height, fmt):
+        return DeviceImage(self, width, height, fmt)
+
+    def upload(self, arr, fmt):
+        arr = np.asarray(arr)
+        im = self.image(arr.shape[1], arr.shape[0], fmt)
+        im.upload(arr)
+        return im
+
+    def sync(self):
+        self._check(lib().clim_sync(self.h), "sync")
+
+    # ---- operators (cpp/demosaic_cl.hpp names) ---------------------------
+    def scaleRawData(self, raw, scaled, bayer, scale_mul, black_level):
+        self._check(lib().clim_scale_raw_data(self.h, raw.ref(), scaled.ref(), bayer, _f(scale_mul),
+                                              C.c_float(black_level)), "scaleRawData")
+
+    def rawNoiseStatistics(self, scaled, bayer, mean, var):
+        self._check(lib().clim_raw_noise_statistics(self.h, scaled.ref(), bayer, mean.ref(), var.ref()),
+                    "rawNoiseStatistics")
+
+    def computeRawNoiseStatistics(self, scaled, bayer):
+        out = (C.c_float * 8)()
+        self._check(lib().clim_compute_raw_noise_statistics(self.h, scaled.ref(), bayer, out),
+                    "computeRawNoiseStatistics")
+        return np.array(out[:], np.float32)
+
+    def interpolateGreen(self, scaled, green, bayer, luma_variance):
+        self._check(lib().clim_interpolate_green(self.h, scaled.ref(), green.ref(), bayer,
+                                                 C.c_float(luma_variance)), "interpolateGreen")
+
+    def interpolateRedBlue(self, scaled, green, rgb, bayer, chroma_variance, rotate_180=False):
+        self._check(lib().clim_interpolate_red_blue(self.h, scaled.ref(), green.ref(), rgb.ref(), bayer,
+                                                    C.c_float(chroma_variance), int(bool(rotate_180))),
+                    "interpolateRedBlue")
+
+    def blendHighlightsImage(self, inp, clip, out):
+        self._check(lib().clim_blend_highlights(self.h, inp.ref(), C.c_float(clip), out.ref()), "blendHighlights")
+
+    def transformImage(self, inp, out, transform):
+        self._check(lib().clim_transform_image(self.h, inp.ref(), out.ref(), _f(transform)), "transformImage")
+
+    def despeckleImage(self, inp, var_a, var_b, out):
+        self._check(lib().clim_despeckle_image(self.h, inp.ref(), _f(var_a), _f(var_b), out.ref()),
+                    "despeckleImage")
+
+    def downsampleImage(self, inp, out):
+        self._check(lib().clim_downsample_image(self.h, inp.ref(), out.ref()), "downsampleImage")
+
+    def noiseStatistics(self, inp, out):
+        self._check(lib().clim_noise_statistics(self.h, inp.ref(), out.ref()), "noiseStatistics")
+
+    def computeNoiseStatistics(self, inp):
+        out = (C.c_float * 6)()
+        self._check(lib().clim_compute_noise_statistics(self.h, inp.ref(), out), "computeNoiseStatistics")
+        return np.array(out[:], np.float32)
+
+    def denoiseImage(self, inp, var_a, var_b, chroma_boost, gradient_boost, out):
+        self._check(lib().clim_denoise_image(self.h, inp.ref(), _f(var_a), _f(var_b), C.c_float(chroma_boost),
+                                             C.c_float(gradient_boost), out.ref()), "denoiseImage")
+
+    def denoiseImageGuided(self, inp, var_a, var_b, out):
+        self._check(lib().clim_denoise_image_guided(self.h, inp.ref(), _f(var_a), _f(var_b), out.ref()),
+                    "denoiseImageGuided")
+
+    def guidedFilter(self, inp, filter_size, eps, out):
+        self._check(lib().clim_guided_filter(self.h, inp.ref(), int(filter_size), _f(eps), out.ref()),
+                    "guidedFilter")
+
+    def reassembleImage(self, d0, n1, d1, sharpening, nlf, out):
+        self._check(lib().clim_reassemble_image(self.h, d0.ref(), n1.ref(), d1.ref(), C.c_float(sharpening),
+                                                _f(nlf), out.ref()), "reassembleImage")
+
+    def convertTosRGB(self, linear, out8, transform, params, out_f32=None):
+        self._check(lib().clim_convert_to_srgb(self.h, linear.ref(), None, out8.ref(),
+                                               out_f32.ref() if out_f32 is not None else None, _f(transform),
+                                               C.byref(params)), "convertTosRGB")
+
+    def pyramidalDenoise(self, image, levels, algorithm, params, out, nlf=None):
+        dp = (DenoiseParameters * 5)(*params)
+        buf = _f(np.zeros(30) if nlf is None else np.asarray(nlf, np.float32).ravel().tolist()
+                 + [0.0] * (30 - 6 * levels))
+        self._check(lib().clim_pyramidal_denoise(self.h, image.ref(), levels, algorithm, dp, out.ref(), buf),
+                    "pyramidalDenoise")
+        return np.array(buf[:6 * levels], np.float32).reshape(levels, 6)
+
+    def raw_to_rgb(self, raw, out8, params, out_f32=None, writeback=True):
+        """RawConverter::demosaicImage.  With writeback, blocks and returns the
+        measured noise model (rawNlf, pyramidNlf)."""
+        nm = NoiseModel() if writeback else None
+        self._check(lib().clim_raw_to_rgb(self.h, raw.ref(), out8.ref(),
+                                          out_f32.ref() if out_f32 is not None else None, C.byref(params),
+                                          C.byref(nm) if writeback else None), "raw_to_rgb")
+        if not writeback:
+            return None
+        return (np.array(nm.rawNlf[:], np.float32),
+                np.array([nm.pyramidNlf[i][:] for i in range(params.levels)], np.float32))
+
+    def generate_bayer(self, raw, bayer, scene_seed, noise_seed, a, b, black, white, clean=None):
+        self._check(lib().clim_generate_bayer(self.h, raw.ref() if raw is not None else None,
+                                              clean.ref() if clean is not None else None, bayer,
+                                              C.c_uint32(scene_seed), C.c_uint32(noise_seed), C.c_float(a),
+                                              C.c_float(b), C.c_float(black), C.c_float(white)), "generate_bayer")

@@ -1,0 +1,94 @@
+// ctx.h - the library context behind clim_ctx (internal).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "climage.h"
+#include "kernels.h"
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+// device image owned by the context
+struct OwnedImg {
+    clim::Img img{nullptr, 0, 0, 0};
+    int format = 0;
+};
+
+// work buffers of one frame geometry (RawConverter::allocateTextures,
+// cpp/raw_converter.cpp:24-40, plus the pyramid of cpp/pyramidal_denoise.cpp:69-93)
+struct Workspace {
+    int w = 0, h = 0, levels = 0, algorithm = -1;
+    OwnedImg scaled, green, rgbA, rgbB, raw_mean, raw_var;
+    OwnedImg pyr[CLIM_MAX_LEVELS];    // pyr[i], i >= 1: downsampled level i
+    OwnedImg den[CLIM_MAX_LEVELS];    // den[i], i >= 1: denoised level i (den[0] is rgbA)
+    OwnedImg stats[CLIM_MAX_LEVELS];  // noiseStatistics of level i
+    OwnedImg g_meanI, g_s1, g_s2, g_ipr, g_ipg, g_ipb, g_tmp4, g_tmp2;  // GuidedPrecise scratch
+};
+
+constexpr int kFitJobs = 1 + CLIM_MAX_LEVELS;  // raw + levels
+constexpr int kFitMaxBlocks = 1024;
+constexpr int kFitK = 17;                       // 4 sums x 4 channels + count
+
+enum Stage {
+    ST_SCALE = 0, ST_RAW_STATS, ST_RAW_FIT, ST_GREEN, ST_RED_BLUE, ST_DESPECKLE,
+    ST_DOWNSAMPLE, ST_LEVEL_STATS, ST_LEVEL_FIT, ST_DENOISE, ST_REASSEMBLE, ST_TONE
+};
+
+struct ProfRec {
+    int stage;
+    long long texels;
+    hipEvent_t a, b;
+};
+
+struct clim_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    Workspace ws;
+    clim::FitState* fit = nullptr;            // device [kFitJobs]
+    clim_fx128* partials = nullptr;           // device [kFitJobs][kFitMaxBlocks][kFitK]
+    clim::FitState* fit_host = nullptr;       // pinned mirror for read-back
+    clim::SceneParams* scene = nullptr;       // device
+    clim::SceneParams* scene_host = nullptr;  // pinned staging
+    // scratch of the stand-alone fit operators
+    OwnedImg op_mean, op_var;
+    clim::GuidedScratch op_guided{};
+    OwnedImg op_guided_imgs[8];
+    // stage profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<ProfRec> prof_recs;
+    size_t ev_next = 0;
+};
+
+// brackets the launches issued in its scope with two events when profiling
+struct ProfScope {
+    clim_ctx* ctx;
+    int idx = -1;
+    ProfScope(clim_ctx* c, int stage, long long texels);
+    ~ProfScope();
+};
+
+// helpers shared by the translation units of the library
+clim_status ctx_fail(clim_ctx* ctx, clim_status s, const std::string& msg);
+clim_status ctx_hip(clim_ctx* ctx, hipError_t e, const char* what);
+clim_status owned_alloc(clim_ctx* ctx, OwnedImg& o, int w, int h, int format);
+void owned_free(OwnedImg& o);
+clim::Bayer make_bayer(int bayer);
+int scene_colors(int bayer);
+
+#define CLIM_TRY_HIP(ctx, call)                                  \
+    do {                                                         \
+        hipError_t e_ = (call);                                  \
+        if (e_ != hipSuccess) return ctx_hip((ctx), e_, #call);  \
+    } while (0)
+
+#define CLIM_TRY(call)                      \
+    do {                                    \
+        clim_status s_ = (call);            \
+        if (s_ != CLIM_OK) return s_;       \
+    } while (0)

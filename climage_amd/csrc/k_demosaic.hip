@@ -1,0 +1,383 @@
+// k_demosaic.hip - gfx950 kernels for the demosaic half of the RAW->RGB path:
+// scaleRawData, rawNoiseStatistics, interpolateGreen, interpolateRedBlue
+// (optionally fused with blendHighlightsImage and the camera->YCbCr
+// transformImage), despeckleLumaMedianChromaImage, and the stand-alone
+// pointwise operators.
+//
+// Reference: cl/demosaic.cl (cl/ = CLImage/app/src/main/cpp/OpenCL/); line
+// numbers are cited per kernel.  All loads clamp to the image.
+#include "kernels.h"
+
+namespace clim {
+
+// ------------------------------------------------------------ scaleRawData
+// cl/demosaic.cl:69-78: max(scaleMul[c] * (raw/65535 - black), 0).
+// One thread per pixel (the reference works on quads; the per-site
+// arithmetic is the same).
+__global__ void __launch_bounds__(256) k_scale_raw(Img raw, Img out, Bayer b, float4 smul, float black) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    const uint16_t v16 = reinterpret_cast<const uint16_t*>(raw.p + (size_t)y * raw.pitch)[x];
+    const int c = b.site(x, y);
+    const float sm = c == 0 ? smul.x : c == 1 ? smul.y : c == 2 ? smul.z : smul.w;
+    const float v = (float)v16 / 65535.0f;
+    const float s = sm * (v - black);
+    wrow_f(out, y)[x] = mmax(s, 0.0f);
+}
+
+void launch_scale_raw(const Img& raw, const Img& out, const Bayer& b, const float smul[4], float black,
+                      hipStream_t st) {
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    hipLaunchKernelGGL(k_scale_raw, grd, blk, 0, st, raw, out, b, make_float4(smul[0], smul[1], smul[2], smul[3]),
+                       black);
+}
+
+// ------------------------------------------------------ rawNoiseStatistics
+// cl/demosaic.cl:1288-1324.  Output grid (W/2) x (H/2); per quad the 5x5-quad
+// sums of each CFA channel in (R, G, B, G2) order, then
+// mean = sum/25, var = (sumSq - sum^2/25)/25.
+// The quads of a 16x16-quad tile plus a 2-quad halo are staged in LDS as
+// float4 (one quad per texel).
+__global__ void __launch_bounds__(256) k_raw_stats(Img scaled, Bayer b, Img mean, Img var) {
+    constexpr int T = 16, R = 2, S = T + 2 * R;
+    __shared__ float4 q[S][S];
+    const int qx0 = blockIdx.x * T, qy0 = blockIdx.y * T;
+    const int tid = threadIdx.y * T + threadIdx.x;
+    for (int i = tid; i < S * S; i += 256) {
+        const int sy = i / S, sx = i % S;
+        const int qx = qx0 - R + sx, qy = qy0 - R + sy;
+        float4 v;
+        v.x = ld1(scaled, 2 * qx + b.ox[0], 2 * qy + b.oy[0]);
+        v.y = ld1(scaled, 2 * qx + b.ox[1], 2 * qy + b.oy[1]);
+        v.z = ld1(scaled, 2 * qx + b.ox[2], 2 * qy + b.oy[2]);
+        v.w = ld1(scaled, 2 * qx + b.ox[3], 2 * qy + b.oy[3]);
+        q[sy][sx] = v;
+    }
+    __syncthreads();
+    const int qx = qx0 + threadIdx.x, qy = qy0 + threadIdx.y;
+    if (qx >= mean.w || qy >= mean.h) return;
+    float s0 = 0, s1 = 0, s2 = 0, s3 = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const float4 v = q[threadIdx.y + j][threadIdx.x + i];
+            s0 += v.x; q0 += v.x * v.x;
+            s1 += v.y; q1 += v.y * v.y;
+            s2 += v.z; q2 += v.z * v.z;
+            s3 += v.w; q3 += v.w * v.w;
+        }
+    }
+    float4 m, vv;
+    m.x = s0 / 25.0f; m.y = s1 / 25.0f; m.z = s2 / 25.0f; m.w = s3 / 25.0f;
+    vv.x = (q0 - (s0 * s0) / 25.0f) / 25.0f;
+    vv.y = (q1 - (s1 * s1) / 25.0f) / 25.0f;
+    vv.z = (q2 - (s2 * s2) / 25.0f) / 25.0f;
+    vv.w = (q3 - (s3 * s3) / 25.0f) / 25.0f;
+    wrow_f4(mean, qy)[qx] = m;
+    wrow_f4(var, qy)[qx] = vv;
+}
+
+void launch_raw_stats(const Img& scaled, const Bayer& b, const Img& mean, const Img& var, hipStream_t st) {
+    dim3 blk(16, 16), grd((mean.w + 15) / 16, (mean.h + 15) / 16);
+    hipLaunchKernelGGL(k_raw_stats, grd, blk, 0, st, scaled, b, mean, var);
+}
+
+// --------------------------------------------------------- interpolateGreen
+// cl/demosaic.cl:96-129 and :160-227.  Tile of 32x16 output pixels; the raw
+// tile carries a 3-pixel halo, the |Sobel| responses a 2-pixel halo, so every
+// Sobel response is computed once per block instead of 26 times per pixel.
+namespace {
+constexpr int GT_X = 32, GT_Y = 16;
+constexpr int GR_X = GT_X + 6, GR_Y = GT_Y + 6;   // raw tile
+constexpr int GS_X = GT_X + 4, GS_Y = GT_Y + 4;   // sobel tile
+}
+
+__global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int gyo, ParamRef lv) {
+    __shared__ float r[GR_Y][GR_X + 1];
+    __shared__ float sgx[GS_Y][GS_X + 1];
+    __shared__ float sgy[GS_Y][GS_X + 1];
+    __shared__ float sgl[GS_Y][GS_X + 1];
+    const int x0 = blockIdx.x * GT_X, y0 = blockIdx.y * GT_Y;
+    const int tid = threadIdx.y * 32 + threadIdx.x;
+    for (int i = tid; i < GR_X * GR_Y; i += 256) {
+        const int sy = i / GR_X, sx = i % GR_X;
+        r[sy][sx] = ld1(raw, x0 - 3 + sx, y0 - 3 + sy);
+    }
+    __syncthreads();
+    // sobel(), cl/demosaic.cl:96-112, value / sqrt(4.5) then abs()
+    for (int i = tid; i < GS_X * GS_Y; i += 256) {
+        const int sy = i / GS_X, sx = i % GS_X;
+        float vx = 0.0f, vy = 0.0f;
+        const float kx[3][3] = {{1, 0, -1}, {2, 0, -2}, {1, 0, -1}};
+        const float ky[3][3] = {{1, 2, 1}, {0, 0, 0}, {-1, -2, -1}};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float s = r[sy + j][sx + k];
+                vx += kx[j][k] * s;
+                vy += ky[j][k] * s;
+            }
+        }
+        const float ax = fabsf(vx / 2.12132034f), ay = fabsf(vy / 2.12132034f);
+        sgx[sy][sx] = ax;
+        sgy[sy][sx] = ay;
+        sgl[sy][sx] = len2(ax, ay);
+    }
+    __syncthreads();
+    const float lumaVariance = mmax(lv.get(0), 1e-4f);
+#pragma unroll 1
+    for (int rr = 0; rr < 2; rr++) {
+        const int ty = threadIdx.y + rr * 8, tx = threadIdx.x;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= green.w || y >= green.h) continue;
+        const int X = tx + 3, Y = ty + 3;  // raw tile coordinates
+        const int xg = (y & 1) == (gyo & 1) ? gxo + 1 : gxo;
+        float out;
+        if ((xg & 1) != (x & 1)) {
+            out = r[Y][X];
+        } else {
+            const float g_left = r[Y][X - 1], g_right = r[Y][X + 1];
+            const float g_up = r[Y - 1][X], g_down = r[Y + 1][X];
+            const float c_xy = r[Y][X];
+            const float c_left = r[Y][X - 2], c_right = r[Y][X + 2];
+            const float c_up = r[Y - 2][X], c_down = r[Y + 2][X];
+            const float c2_ave = (r[Y - 1][X - 1] + r[Y - 1][X + 1] + r[Y + 1][X - 1] + r[Y + 1][X + 1]) / 4.0f;
+            const float g_ave = (g_left + g_right + g_up + g_down) / 4.0f;
+            const float lumaStdDev = sqrtf(lumaVariance * g_ave);
+
+            // bilateralSobel(), cl/demosaic.cl:114-129
+            const int SX = tx + 2, SY = ty + 2;  // sobel tile coordinates
+            const float cl = sgl[SY][SX];
+            const float e0 = 4.0f * lumaStdDev, e1 = 16.0f * lumaStdDev;
+            float sx = 0.0f, sy = 0.0f, wx = 0.0f, wy = 0.0f;
+#pragma unroll
+            for (int j = -2; j <= 2; j++) {
+#pragma unroll
+                for (int i = -2; i <= 2; i++) {
+                    const float gx = sgx[SY + j][SX + i], gy = sgy[SY + j][SX + i];
+                    const float d = 0.05f * fabsf(sgl[SY + j][SX + i] - cl);
+                    const float wt = 1.0f - smoothstep(e0, e1, d);
+                    sx += wt * gx;
+                    sy += wt * gy;
+                    wx += wt;
+                    wy += wt;
+                }
+            }
+            const float gradx = sx / wx, grady = sy / wy;
+
+            const float g_lf_x = (g_left + g_right) / 2.0f, g_lf_y = (g_up + g_down) / 2.0f;
+            const float g_hf_x = (2.0f * c_xy - (c_left + c_right)) / 4.0f;
+            const float g_hf_y = (2.0f * c_xy - (c_up + c_down)) / 4.0f;
+            const float whiteness =
+                clampf(mmin(c_xy, mmin(g_ave, c2_ave)) / mmax(c_xy, mmax(g_ave, c2_ave)), 0.0f, 1.0f);
+            const float glen = len2(gradx, grady);
+            const float lgt = smoothstep(lumaStdDev, 4.0f * lumaStdDev, glen);
+            const float hf_gain =
+                lgt * mmin(0.5f * whiteness + smoothstep(0.0f, 0.3f, glen / 1.41421356f), 1.0f);
+            const float g_est_x = g_lf_x + hf_gain * g_hf_x;
+            const float g_est_y = g_lf_y + hf_gain * g_hf_y;
+            float direction = 2.0f * clim_atan2pif(grady, gradx);
+            direction = direction < 0.5f
+                            ? mixf(direction, 0.0f, 1.0f - smoothstep(0.3f * lgt, 0.45f * lgt, direction))
+                            : mixf(direction, 1.0f,
+                                   smoothstep((1.0f - 0.45f) * lgt, (1.0f - 0.3f) * lgt, direction));
+            direction = mixf(0.5f, direction, lgt);
+            out = mixf(g_est_y, g_est_x, direction);
+        }
+        wrow_f(green, y)[x] = out;
+    }
+}
+
+void launch_green(const Img& raw, const Img& green, int bayer, const ParamRef& lv, hipStream_t st) {
+    // raw_green site of the OpenCL table
+    const int gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 1, 1};
+    dim3 blk(32, 8), grd((green.w + GT_X - 1) / GT_X, (green.h + GT_Y - 1) / GT_Y);
+    hipLaunchKernelGGL(k_green, grd, blk, 0, st, raw, green, gx[bayer], gy[bayer], lv);
+}
+
+// ------------------------------------------------------- interpolateRedBlue
+// cl/demosaic.cl:229-318, optionally followed by blendHighlightsImage
+// (cl/demosaic.cl:338-380, clip) and transformImage (cl/demosaic.cl:728-733)
+// on the same pixel.  chroma variance = (B_R + B_B) / 2 of the raw noise
+// model (cpp/raw_converter.cpp:127-128) when taken from the device.
+struct Mat3 { float m[9]; };
+
+__device__ __forceinline__ void blend_highlights_px(float& px, float& py, float& pz, float clip) {
+    // cl/demosaic.cl:338-370
+    if (px > clip || py > clip || pz > clip) {
+        const float s3 = 1.7320508f;
+        const float c0[3] = {px, py, pz};
+        const float c1[3] = {mmin(px, clip), mmin(py, clip), mmin(pz, clip)};
+        const float l00 = 1.0f * c0[0] + 1.0f * c0[1] + 1.0f * c0[2];
+        const float l01 = s3 * c0[0] + -s3 * c0[1] + 0.0f * c0[2];
+        const float l02 = -1.0f * c0[0] + -1.0f * c0[1] + 2.0f * c0[2];
+        const float l11 = s3 * c1[0] + -s3 * c1[1] + 0.0f * c1[2];
+        const float l12 = -1.0f * c1[0] + -1.0f * c1[1] + 2.0f * c1[2];
+        const float sum0 = l01 * l01 + l02 * l02;
+        const float sum1 = l11 * l11 + l12 * l12;
+        const float chratio = sum0 > 0 ? sqrtf(sum1 / sum0) : 1.0f;
+        const float a = l00, bb = l01 * chratio, cc = l02 * chratio;
+        const float h = s3 / 2.0f;
+        px = (1.0f * a + h * bb + -0.5f * cc) / 3.0f;
+        py = (1.0f * a + -h * bb + -0.5f * cc) / 3.0f;
+        pz = (1.0f * a + 0.0f * bb + 1.0f * cc) / 3.0f;
+    }
+}
+
+template <bool kFuse>
+__global__ void __launch_bounds__(256) k_red_blue(Img raw, Img green, Img out, Bayer b, int red_is_c1_site,
+                                                  ParamRef cv, int rotate_180, float clip, Mat3 m) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    float chromaVariance = cv.dev ? (cv.dev[4] + cv.dev[6]) / 2.0f : cv.host[0];
+    chromaVariance = mmax(chromaVariance, 1e-6f);
+    const int color = b.site(x, y);
+    const float g = ld1(green, x, y);
+    float red, blue;
+    if (color == 0 || color == 2) {
+        const float c1 = ld1(raw, x, y);
+        const float c2_tl = ld1(green, x - 1, y - 1) - ld1(raw, x - 1, y - 1);
+        const float c2_tr = ld1(green, x + 1, y - 1) - ld1(raw, x + 1, y - 1);
+        const float c2_bl = ld1(green, x - 1, y + 1) - ld1(raw, x - 1, y + 1);
+        const float c2_br = ld1(green, x + 1, y + 1) - ld1(raw, x + 1, y + 1);
+        const float chromaStdDev = sqrtf(chromaVariance * g);
+        const float dvx = fabsf(c2_tl - c2_br), dvy = fabsf(c2_tr - c2_bl);
+        const float flatness = 1.0f - smoothstep(0.25f * chromaStdDev, chromaStdDev, len2(dvx, dvy));
+        const float alpha = mixf(2.0f * clim_atan2pif(dvy, dvx), 0.5f, flatness);
+        const float c2 = g - mixf((c2_tr + c2_bl) / 2.0f, (c2_tl + c2_br) / 2.0f, alpha);
+        if (color == 0) { red = c1; blue = c2; } else { blue = c1; red = c2; }
+    } else {
+        const float c1_left = ld1(green, x - 1, y) - ld1(raw, x - 1, y);
+        const float c1_right = ld1(green, x + 1, y) - ld1(raw, x + 1, y);
+        const float c2_up = ld1(green, x, y - 1) - ld1(raw, x, y - 1);
+        const float c2_down = ld1(green, x, y + 1) - ld1(raw, x, y + 1);
+        const float c1 = g - (c1_left + c1_right) / 2.0f;
+        const float c2 = g - (c2_up + c2_down) / 2.0f;
+        if (color == red_is_c1_site) { red = c1; blue = c2; } else { blue = c1; red = c2; }
+    }
+    float px = red, py = g, pz = blue;
+    if (kFuse) {
+        blend_highlights_px(px, py, pz, clip);
+        const float tx = m.m[0] * px + m.m[1] * py + m.m[2] * pz;
+        const float ty = m.m[3] * px + m.m[4] * py + m.m[5] * pz;
+        const float tz = m.m[6] * px + m.m[7] * py + m.m[8] * pz;
+        px = tx; py = ty; pz = tz;
+    }
+    int ox = x, oy = y;
+    if (rotate_180) {
+        ox = out.w - x;
+        oy = out.h - y;
+        if (ox >= out.w || oy >= out.h) return;
+    }
+    wrow_f4(out, oy)[ox] = make_float4(px, py, pz, 0.0f);
+}
+
+void launch_red_blue(const Img& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
+                     const ParamRef& cv, int rotate_180, bool fuse, float clip, const float c2y[9],
+                     hipStream_t st) {
+    // cl/demosaic.cl:301: at green sites red is the horizontal difference on
+    // raw_green for bggr/grbg, on raw_green2 otherwise
+    const int red_is_c1 = (bayer == 3 || bayer == 0) ? 1 : 3;
+    Mat3 m;
+    for (int i = 0; i < 9; i++) m.m[i] = c2y ? c2y[i] : (i % 4 == 0 ? 1.0f : 0.0f);
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    if (fuse)
+        hipLaunchKernelGGL(k_red_blue<true>, grd, blk, 0, st, raw, green, out, b, red_is_c1, cv, 0, clip, m);
+    else
+        hipLaunchKernelGGL(k_red_blue<false>, grd, blk, 0, st, raw, green, out, b, red_is_c1, cv, rotate_180,
+                           clip, m);
+}
+
+// ------------------------------------------- stand-alone pointwise operators
+__global__ void __launch_bounds__(256) k_blend_highlights(Img in, Img out, float clip) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    float4 p = row_f4(in, y)[x];
+    blend_highlights_px(p.x, p.y, p.z, clip);
+    wrow_f4(out, y)[x] = make_float4(p.x, p.y, p.z, 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_transform(Img in, Img out, Mat3 m) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    const float4 p = row_f4(in, y)[x];
+    const float tx = m.m[0] * p.x + m.m[1] * p.y + m.m[2] * p.z;
+    const float ty = m.m[3] * p.x + m.m[4] * p.y + m.m[5] * p.z;
+    const float tz = m.m[6] * p.x + m.m[7] * p.y + m.m[8] * p.z;
+    wrow_f4(out, y)[x] = make_float4(tx, ty, tz, 0.0f);
+}
+
+void launch_blend_highlights(const Img& in, const Img& out, float clip, hipStream_t st) {
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    hipLaunchKernelGGL(k_blend_highlights, grd, blk, 0, st, in, out, clip);
+}
+
+void launch_transform(const Img& in, const Img& out, const float mat[9], hipStream_t st) {
+    Mat3 m;
+    for (int i = 0; i < 9; i++) m.m[i] = mat[i];
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    hipLaunchKernelGGL(k_transform, grd, blk, 0, st, in, out, m);
+}
+
+// ----------------------------------------- despeckleLumaMedianChromaImage
+// cl/demosaic.cl:384-413 (forgetful-selection median of Cb and Cr) and
+// :443-479 (luma clamped between its noise-adaptive second extremes), in FP32.
+#define CLIM_S(a, b) { const float t_ = a; a = mmin(a, b); b = mmax(t_, b); }
+
+__device__ __forceinline__ float median9(float v0, float v1, float v2, float v3, float v4, float v5, float v6,
+                                         float v7, float v8) {
+    float a0 = v0, a1 = v1, a2 = v2, a3 = v3, a4 = v4, a5 = v5;
+    CLIM_S(a0, a1) CLIM_S(a2, a3) CLIM_S(a4, a5) CLIM_S(a0, a2) CLIM_S(a1, a3) CLIM_S(a0, a4) CLIM_S(a3, a5)
+    a0 = v6;
+    CLIM_S(a0, a1) CLIM_S(a2, a3) CLIM_S(a0, a2) CLIM_S(a1, a3) CLIM_S(a0, a4) CLIM_S(a3, a4)
+    a0 = v7;
+    CLIM_S(a0, a1) CLIM_S(a2, a3) CLIM_S(a0, a2) CLIM_S(a1, a3)
+    a0 = v8;
+    CLIM_S(a0, a1) CLIM_S(a0, a2) CLIM_S(a1, a2)
+    return a1;
+}
+
+__global__ void __launch_bounds__(256) k_despeckle(Img in, Img out, float var_a, float var_b) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    // read order of fast_median3x3
+    const int ox[9] = {0, 1, 0, 1, 0, 1, -1, -1, -1};
+    const int oy[9] = {-1, -1, 0, 0, 1, 1, 1, 0, -1};
+    float sample = 0.0f, firstMax = 0.0f, secMax = 0.0f, firstMin = 65535.0f, secMin = 65535.0f;
+    float cb[9], cr[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const float4 p = ld4(in, x + ox[k], y + oy[k]);
+        const float v = p.x;
+        secMax = v <= firstMax && v > secMax ? v : secMax;
+        secMax = v > firstMax ? firstMax : secMax;
+        firstMax = v > firstMax ? v : firstMax;
+        secMin = v >= firstMin && v < secMin ? v : secMin;
+        secMin = v < firstMin ? firstMin : secMin;
+        firstMin = v < firstMin ? v : firstMin;
+        if (k == 2) sample = v;
+        cb[k] = p.y;
+        cr[k] = p.z;
+    }
+    const float mcb = median9(cb[0], cb[1], cb[2], cb[3], cb[4], cb[5], cb[6], cb[7], cb[8]);
+    const float mcr = median9(cr[0], cr[1], cr[2], cr[3], cr[4], cr[5], cr[6], cr[7], cr[8]);
+    const float sigma = sqrtf(var_a + var_b * sample);
+    const float minVal = mixf(secMin, firstMin, smoothstep(sigma, 4.0f * sigma, secMin - firstMin));
+    const float maxVal = mixf(secMax, firstMax, smoothstep(sigma, 4.0f * sigma, firstMax - secMax));
+    sample = clampf(sample, minVal, maxVal);
+    wrow_f4(out, y)[x] = make_float4(sample, mcb, mcr, 0.0f);
+}
+
+void launch_despeckle(const Img& in, const Img& out, float var_a, float var_b, hipStream_t st) {
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    hipLaunchKernelGGL(k_despeckle, grd, blk, 0, st, in, out, var_a, var_b);
+}
+
+}  // namespace clim

@@ -1,0 +1,352 @@
+// k_pyramid.hip - gfx950 kernels for the pyramidal denoise and the tone map:
+// downsampleImage, noiseStatistics, denoiseImage (hard sigma filter),
+// denoiseImageGuided, reassembleImage and convertTosRGB (optionally preceded
+// by the YCbCr->camera transformImage on the same pixel).
+//
+// Reference: cl/demosaic.cl (cl/ = CLImage/app/src/main/cpp/OpenCL/).
+#include "kernels.h"
+
+namespace clim {
+
+// CL_FILTER_LINEAR | CLK_ADDRESS_CLAMP_TO_EDGE with normalised coordinates:
+// u = x*W - 0.5, i0 = floor(u), weight frac(u) rounded to 8 fractional bits.
+__device__ __forceinline__ float4 bilinear4(const Img& im, float un, float vn) {
+    const float u = un * (float)im.w - 0.5f;
+    const float v = vn * (float)im.h - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    float a = u - fu, b = v - fv;
+    a = floorf(a * 256.0f + 0.5f) * 0.00390625f;
+    b = floorf(b * 256.0f + 0.5f) * 0.00390625f;
+    const int i0 = (int)fu, j0 = (int)fv;
+    const float4 t00 = ld4(im, i0, j0), t10 = ld4(im, i0 + 1, j0);
+    const float4 t01 = ld4(im, i0, j0 + 1), t11 = ld4(im, i0 + 1, j0 + 1);
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b);
+    const float w01 = (1.0f - a) * b, w11 = a * b;
+    float4 r;
+    r.x = w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x;
+    r.y = w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y;
+    r.z = w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z;
+    r.w = w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w;
+    return r;
+}
+
+// ------------------------------------------------------------ downsampleImage
+// cl/demosaic.cl:853-865: four linear taps at +-1/2 output texel, averaged.
+__global__ void __launch_bounds__(256) k_downsample(Img in, Img out) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
+    const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
+    const float sx = 0.5f * nx, sy = 0.5f * ny;
+    float4 acc = bilinear4(in, px - sx, py - sy);
+    float4 t = bilinear4(in, px + sx, py - sy);
+    acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
+    t = bilinear4(in, px - sx, py + sy);
+    acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
+    t = bilinear4(in, px + sx, py + sy);
+    acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
+    wrow_f4(out, y)[x] = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.25f * acc.w);
+}
+
+void launch_downsample(const Img& in, const Img& out, hipStream_t st) {
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    hipLaunchKernelGGL(k_downsample, grd, blk, 0, st, in, out);
+}
+
+// 5x5 stencils over an RGBA level: a 32x16 output tile plus a 2-texel halo in
+// LDS (xyz only; alpha never enters these filters).
+namespace {
+constexpr int TX = 32, TY = 16, HR = 2, SX = TX + 2 * HR, SY = TY + 2 * HR;
+}
+
+struct Tile3 {
+    float x[SY][SX + 1], y[SY][SX + 1], z[SY][SX + 1];
+};
+
+__device__ __forceinline__ void load_tile3(Tile3& t, const Img& in, int x0, int y0) {
+    const int tid = threadIdx.y * 32 + threadIdx.x;
+    for (int i = tid; i < SX * SY; i += 256) {
+        const int sy = i / SX, sx = i % SX;
+        const float4 p = ld4(in, x0 - HR + sx, y0 - HR + sy);
+        t.x[sy][sx] = p.x;
+        t.y[sy][sx] = p.y;
+        t.z[sy][sx] = p.z;
+    }
+}
+
+// ------------------------------------------------------------ noiseStatistics
+// cl/demosaic.cl:1267-1286: (mean Y, var Y, var Cb, var Cr) over 5x5.
+__global__ void __launch_bounds__(256) k_noise_stats(Img in, Img out) {
+    __shared__ Tile3 t;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    load_tile3(t, in, x0, y0);
+    __syncthreads();
+#pragma unroll 1
+    for (int rr = 0; rr < 2; rr++) {
+        const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= out.w || y >= out.h) continue;
+        float s0 = 0, s1 = 0, s2 = 0, q0 = 0, q1 = 0, q2 = 0;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
+                s0 += a; q0 += a * a;
+                s1 += b; q1 += b * b;
+                s2 += c; q2 += c * c;
+            }
+        }
+        wrow_f4(out, y)[x] = make_float4(s0 / 25.0f, (q0 - (s0 * s0) / 25.0f) / 25.0f,
+                                         (q1 - (s1 * s1) / 25.0f) / 25.0f, (q2 - (s2 * s2) / 25.0f) / 25.0f);
+    }
+}
+
+void launch_noise_stats(const Img& in, const Img& out, hipStream_t st) {
+    dim3 blk(32, 8), grd((out.w + TX - 1) / TX, (out.h + TY - 1) / TY);
+    hipLaunchKernelGGL(k_noise_stats, grd, blk, 0, st, in, out);
+}
+
+// --------------------------------------------------------------- denoiseImage
+// cl/demosaic.cl:735-765.  nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr.
+__global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
+                                                   float gradientBoost) {
+    __shared__ Tile3 t;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    load_tile3(t, in, x0, y0);
+    __syncthreads();
+    const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
+    const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
+#pragma unroll 1
+    for (int rr = 0; rr < 2; rr++) {
+        const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= out.w || y >= out.h) continue;
+        const int X = tx + HR, Y = ty + HR;
+        const float cY = t.x[Y][X], cCb = t.y[Y][X], cCr = t.z[Y][X];
+        const float sig0 = sqrtf(va0 + vb0 * cY), sig1 = sqrtf(va1 + vb1 * cY), sig2 = sqrtf(va2 + vb2 * cY);
+        float lumaBoost = 1.0f;
+        if (gradientBoost > 1.0f) {
+            // sobel() of the luma channel, cl/demosaic.cl:96-112
+            const float kx[3][3] = {{1, 0, -1}, {2, 0, -2}, {1, 0, -1}};
+            const float ky[3][3] = {{1, 2, 1}, {0, 0, 0}, {-1, -2, -1}};
+            float vx = 0.0f, vy = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const float s = t.x[Y - 1 + j][X - 1 + i];
+                    vx += kx[j][i] * s;
+                    vy += ky[j][i] * s;
+                }
+            vx = vx / 2.12132034f;
+            vy = vy / 2.12132034f;
+            lumaBoost = gradientBoost -
+                        (gradientBoost - 1.0f) * smoothstep(0.5f, 2.0f, len2(vx / sig0, vy / sig0));
+        }
+        const float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
+        float f0 = 0, f1 = 0, f2 = 0, n0 = 0, n1 = 0, n2 = 0;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
+                const float d0 = (a - cY) / sig0, d1 = (b - cCb) / sig1, d2 = (c - cCr) / sig2;
+                const float l = len3(d0, d1, d2);
+                const float w0 = 1.0f - (l < e0 ? 0.0f : 1.0f);
+                const float w1 = 1.0f - (l < e1 ? 0.0f : 1.0f);
+                f0 += w0 * a; n0 += w0;
+                f1 += w1 * b; n1 += w1;
+                f2 += w1 * c; n2 += w1;
+            }
+        }
+        wrow_f4(out, y)[x] = make_float4(f0 / n0, f1 / n1, f2 / n2, 0.0f);
+    }
+}
+
+void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
+                      hipStream_t st) {
+    dim3 blk(32, 8), grd((out.w + TX - 1) / TX, (out.h + TY - 1) / TY);
+    hipLaunchKernelGGL(k_bilateral, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost);
+}
+
+// --------------------------------------------------------- denoiseImageGuided
+// cl/demosaic.cl:767-851: one-pass 5x5 self-guided colour guided filter.
+__global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef nlf) {
+    __shared__ Tile3 t;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    load_tile3(t, in, x0, y0);
+    __syncthreads();
+    const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
+    const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
+    const float norm = 1.0f / 25.0f;
+#pragma unroll 1
+    for (int rr = 0; rr < 2; rr++) {
+        const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= out.w || y >= out.h) continue;
+        const int X = tx + HR, Y = ty + HR;
+        const float ix = t.x[Y][X], iy = t.y[Y][X], iz = t.z[Y][X];
+        const float eps0 = va0 + vb0 * ix, eps1 = va1 + vb1 * ix, eps2 = va2 + vb2 * ix;
+        float m0 = 0, m1 = 0, m2 = 0, rr_ = 0, rg = 0, rb = 0, gg = 0, gb = 0, bb = 0;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+                const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
+                m0 += a; m1 += b; m2 += c;
+                rr_ += a * a; rg += a * b; rb += a * c;
+                gg += b * b; gb += b * c; bb += c * c;
+            }
+        }
+        m0 *= norm; m1 *= norm; m2 *= norm;
+        rr_ *= norm; rg *= norm; rb *= norm; gg *= norm; gb *= norm; bb *= norm;
+        const float vrr = rr_ - m0 * m0, vrg = rg - m0 * m1, vrb = rb - m0 * m2;
+        const float vgg = gg - m1 * m1, vgb = gb - m1 * m2, vbb = bb - m2 * m2;
+        const float vrr_e = vrr + eps0, vgg_e = vgg + eps1, vbb_e = vbb + eps2;
+        float invrr = vgg_e * vbb_e - vgb * vgb;
+        float invrg = vgb * vrb - vrg * vbb_e;
+        float invrb = vrg * vgb - vgg_e * vrb;
+        float invgg = vrr_e * vbb_e - vrb * vrb;
+        float invgb = vrb * vrg - vrr_e * vgb;
+        float invbb = vrr_e * vgg_e - vrg * vrg;
+        const float invCovDet = 1.0f / (invrr * vrr_e + invrg * vrg + invrb * vrb);
+        invrr *= invCovDet; invrg *= invCovDet; invrb *= invCovDet;
+        invgg *= invCovDet; invgb *= invCovDet; invbb *= invCovDet;
+        const float cr[3] = {vrr, vrg, vrb}, cg[3] = {vrg, vgg, vgb}, cb[3] = {vrb, vgb, vbb};
+        const float mI[3] = {m0, m1, m2};
+        float ar[3], ag[3], ab[3], q[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            ar[c] = invrr * cr[c] + invrg * cg[c] + invrb * cb[c];
+            ag[c] = invrg * cr[c] + invgg * cg[c] + invgb * cb[c];
+            ab[c] = invrb * cr[c] + invgb * cg[c] + invbb * cb[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float b = mI[c] - ar[c] * m0 - ag[c] * m1 - ab[c] * m2;
+            q[c] = ar[c] * ix + ag[c] * iy + ab[c] * iz + b;
+        }
+        wrow_f4(out, y)[x] = make_float4(q[0], q[1], q[2], 0.0f);
+    }
+}
+
+void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st) {
+    dim3 blk(32, 8), grd((out.w + TX - 1) / TX, (out.h + TY - 1) / TY);
+    hipLaunchKernelGGL(k_guided_fast, grd, blk, 0, st, in, out, nlf);
+}
+
+// ------------------------------------------------------------ reassembleImage
+// cl/demosaic.cl:867-896.  nlf: (A_Y, B_Y) at nlf.get(0), nlf.get(3).
+// out may alias d0 (each texel is read, then written, by one thread).
+__global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img out, float sharpening0,
+                                                    ParamRef nlf) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
+    const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
+    const float4 p0 = row_f4(d0, y)[x];
+    const float4 a1 = bilinear4(n1, px, py);
+    const float4 b1 = bilinear4(d1, px, py);
+    float4 d;
+    d.x = p0.x - (a1.x - b1.x);
+    d.y = p0.y - (a1.y - b1.y);
+    d.z = p0.z - (a1.z - b1.z);
+    d.w = p0.w - (a1.w - b1.w);
+    float sharpening = sharpening0;
+    if (sharpening > 1.0f) {
+        const float dx = (bilinear4(d1, px + nx, py).x - bilinear4(d1, px - nx, py).x) / 2.0f;
+        const float dy = (bilinear4(d1, px, py + ny).x - bilinear4(d1, px, py - ny).x) / 2.0f;
+        const float threshold = 0.25f * sqrtf(nlf.get(0) + nlf.get(3) * b1.x);
+        const float detail = smoothstep(0.25f * threshold, threshold, len2(dx, dy)) *
+                             (1.0f - smoothstep(0.95f, 1.0f, d.x)) * (0.6f + 0.4f * smoothstep(0.0f, 0.1f, d.x));
+        sharpening = 1.0f + (sharpening - 1.0f) * detail;
+    }
+    d.x = mixf(b1.x, d.x, sharpening);
+    wrow_f4(out, y)[x] = d;
+}
+
+void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& out, float sharpening,
+                       const ParamRef& nlf, hipStream_t st) {
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    hipLaunchKernelGGL(k_reassemble, grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf);
+}
+
+// -------------------------------------------------------------- convertTosRGB
+// cl/demosaic.cl:1391-1474 (localToneMapping = 0), optionally preceded by
+// transformImage with `pre` (the YCbCr->camera matrix of
+// cpp/raw_converter.cpp:170).
+struct ToneArgs {
+    float pre[9];
+    float cam[9];
+    float contrast, saturation, slope, ebmul, blacks;
+    float sig0, sig1;
+    int has_pre;
+};
+
+__device__ __forceinline__ float sigmoidf(float x, float s) { return 0.5f * (clim_tanhf(s * x - 0.3f * s) + 1.0f); }
+
+__global__ void __launch_bounds__(256) k_tone(Img in, Img out8, Img outf, ToneArgs a) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out8.w || y >= out8.h) return;
+    const float4 p = row_f4(in, y)[x];
+    float v[3] = {p.x, p.y, p.z};
+    if (a.has_pre) {
+        const float t0 = a.pre[0] * v[0] + a.pre[1] * v[1] + a.pre[2] * v[2];
+        const float t1 = a.pre[3] * v[0] + a.pre[4] * v[1] + a.pre[5] * v[2];
+        const float t2 = a.pre[6] * v[0] + a.pre[7] * v[1] + a.pre[8] * v[2];
+        v[0] = t0; v[1] = t1; v[2] = t2;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] *= a.ebmul;
+    if (a.saturation != 1.0f) {
+        const float luma = 0.2126f * v[0] + 0.7152f * v[1] + 0.0722f * v[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            v[c] = mixf(luma, v[c], mixf(a.saturation, 1.0f, smoothstep(0.75f, 2.0f, v[c])));
+    }
+    if (a.contrast != 1.0f) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = mixf(0.2f, v[c], mixf(a.contrast, 1.0f, smoothstep(0.9f, 2.0f, v[c])));
+    }
+    float rgb[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) rgb[r] = a.cam[r * 3 + 0] * v[0] + a.cam[r * 3 + 1] * v[1] + a.cam[r * 3 + 2] * v[2];
+    uint32_t packed = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float t = sqrtf(0.95f * rgb[c]);
+        const float tc = (sigmoidf(t, a.slope) - a.sig0) / (a.sig1 - a.sig0);
+        rgb[c] = clampf(tc, 0.0f, 1.0f);
+        if (a.blacks > 0.0f) rgb[c] = (rgb[c] - a.blacks) / (1.0f - a.blacks);
+        const uint32_t q = (uint32_t)rintf(clampf(rgb[c], 0.0f, 1.0f) * 255.0f);
+        packed |= q << (8 * c);
+    }
+    reinterpret_cast<uint32_t*>(out8.p + (size_t)y * out8.pitch)[x] = packed;
+    if (outf.p) wrow_f4(outf, y)[x] = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
+}
+
+void launch_tone(const Img& in, const Img& out8, const Img& outf, const float* pre, const float cam[9],
+                 float contrast, float saturation, float slope, float exposureBias, float blacks, hipStream_t st) {
+    ToneArgs a;
+    a.has_pre = pre != nullptr;
+    for (int i = 0; i < 9; i++) {
+        a.pre[i] = pre ? pre[i] : 0.0f;
+        a.cam[i] = cam[i];
+    }
+    a.contrast = contrast;
+    a.saturation = saturation;
+    a.slope = slope;
+    // powr(2, exposureBias) as exp(exposureBias * ln 2)
+    a.ebmul = exposureBias != 0.0f ? clim_expf(exposureBias * 0.693147182f) : 1.0f;
+    a.blacks = blacks;
+    a.sig0 = 0.5f * (clim_tanhf(slope * 0.0f - 0.3f * slope) + 1.0f);
+    a.sig1 = 0.5f * (clim_tanhf(slope * 1.0f - 0.3f * slope) + 1.0f);
+    dim3 blk(64, 4), grd((out8.w + 63) / 64, (out8.h + 3) / 4);
+    hipLaunchKernelGGL(k_tone, grd, blk, 0, st, in, out8, outf, a);
+}
+
+}  // namespace clim

@@ -1,0 +1,78 @@
+// kernels.h - launchers of the gfx950 kernels (internal to the library).
+#pragma once
+
+#include "device_common.h"
+
+namespace clim {
+
+// ----------------------------------------------------- noise-model regression
+struct FitState {
+    double A[4], B[4], err2[4];
+    unsigned long long n0;
+    int status;            // 0 fitted, 1 no sample in range (fallback), 2 no inliers (first fit)
+    unsigned int counter;  // blocks finished in the current pass
+    float out[8];          // effective model: A[0..nch), B[0..nch)
+    float cal[8];          // out * mult (calibrated per-level model)
+};
+
+struct FitJob {
+    const char* m;        // mean samples (float4 texels)
+    const char* v;        // variance samples (float4 texels)
+    int w, h, pitch_m, pitch_v;
+    int m_elem, v_elem;   // float offset of the first mean / variance inside a texel
+    int m_nch;            // 1: one mean shared by all channels, else per channel
+    FitState* st;
+    clim_fx128* partials;
+    float nlf_in[8];      // fallback model (written to out when no sample is in range)
+    float mult[8];        // calibration multipliers
+};
+
+struct FitJobs {
+    FitJob j[5];
+};
+
+void launch_fit(const FitJobs& jobs, int njobs, int nch, int nblocks, int pass, hipStream_t st);
+
+// ------------------------------------------------------------ demosaic stages
+void launch_scale_raw(const Img& raw, const Img& out, const Bayer& b, const float smul[4], float black,
+                      hipStream_t st);
+void launch_raw_stats(const Img& scaled, const Bayer& b, const Img& mean, const Img& var, hipStream_t st);
+void launch_green(const Img& raw, const Img& green, int bayer, const ParamRef& lv, hipStream_t st);
+void launch_red_blue(const Img& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
+                     const ParamRef& cv, int rotate_180, bool fuse, float clip, const float c2y[9],
+                     hipStream_t st);
+void launch_blend_highlights(const Img& in, const Img& out, float clip, hipStream_t st);
+void launch_transform(const Img& in, const Img& out, const float mat[9], hipStream_t st);
+void launch_despeckle(const Img& in, const Img& out, float var_a, float var_b, hipStream_t st);
+
+// ------------------------------------------------------------- pyramid stages
+void launch_downsample(const Img& in, const Img& out, hipStream_t st);
+void launch_noise_stats(const Img& in, const Img& out, hipStream_t st);
+void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
+                      hipStream_t st);
+void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st);
+void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& out, float sharpening,
+                       const ParamRef& nlf, hipStream_t st);
+void launch_tone(const Img& in, const Img& out8, const Img& outf, const float* pre, const float cam[9],
+                 float contrast, float saturation, float slope, float exposureBias, float blacks, hipStream_t st);
+
+// ----------------------------------------------- multi-pass guided filter
+// scratch: 7 images of w*h texels (float4) + 1 of float2, see k_guided.hip
+struct GuidedScratch {
+    Img mean_I, s1, s2, ipr, ipg, ipb, tmp4, tmp2;
+};
+void launch_guided_filter(const Img& in, const Img& out, int fs, const ParamRef& eps, const GuidedScratch& g,
+                          hipStream_t st);
+
+// -------------------------------------------------------- synthetic scenes
+constexpr int kSceneBlobs = 200, kSceneRects = 50;
+struct SceneParams {
+    float base[3], gx[3], gy[3], gxy[3], amp[3], fx[3], fy[3], ph[3];
+    float bcx[kSceneBlobs], bcy[kSceneBlobs], bw[kSceneBlobs], binv[kSceneBlobs], ba[kSceneBlobs][3];
+    float rx0[kSceneRects], ry0[kSceneRects], rx1[kSceneRects], ry1[kSceneRects], rc[kSceneRects][3];
+};
+void scene_draw_params(int w, int h, uint32_t seed, SceneParams* p);
+void launch_scene(const Img& raw, const Img& clean, const SceneParams* dparams, int bayer_site_color_packed,
+                  uint32_t noise_seed, float a, float b, float black, float white, hipStream_t st);
+
+}  // namespace clim

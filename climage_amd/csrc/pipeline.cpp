@@ -1,0 +1,301 @@
+// pipeline.cpp - RawConverter::demosaicImage and PyramidalDenoise::denoise as
+// device-resident launch sequences (C ABI entry points clim_raw_to_rgb,
+// clim_pyramidal_denoise and clim_generate_bayer).
+//
+// Differences from the reference orchestration (cpp/raw_converter.cpp:73-186,
+// cpp/pyramidal_denoise.cpp:103-149), none of which changes a value:
+//  - the raw and per-level noise-model fits run on the device; kernels that
+//    need a fitted value read it from device memory, so the frame has no
+//    host round trip (the reference maps six images per frame);
+//  - blendHighlightsImage and the camera->YCbCr transformImage run inside
+//    the interpolateRedBlue kernel, and the YCbCr->camera transformImage
+//    inside the convertTosRGB kernel (same per-pixel operations);
+//  - work buffers follow the frame size (the reference allocates once and
+//    never checks, cpp/raw_converter.cpp:27).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "ctx.h"
+
+using clim::Img;
+using clim::ParamRef;
+
+clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, int njobs, int nch);
+clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& m, const Img& v, int m_elem, int v_elem, int m_nch,
+                          const float* nlf_in, const float* mult);
+clim_status guided_scratch(clim_ctx* ctx, OwnedImg* imgs, int w, int h, clim::GuidedScratch* g);
+int inverse3x3_scaled(const float mf[9], double scale, float out[9]);
+
+static ParamRef dev_params(const float* p) {
+    ParamRef r{};
+    r.dev = p;
+    return r;
+}
+
+static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int algorithm) {
+    Workspace& ws = ctx->ws;
+    CLIM_TRY(owned_alloc(ctx, ws.scaled, w, h, CLIM_FMT_R32F));
+    CLIM_TRY(owned_alloc(ctx, ws.green, w, h, CLIM_FMT_R32F));
+    CLIM_TRY(owned_alloc(ctx, ws.rgbA, w, h, CLIM_FMT_RGBA32F));
+    CLIM_TRY(owned_alloc(ctx, ws.rgbB, w, h, CLIM_FMT_RGBA32F));
+    CLIM_TRY(owned_alloc(ctx, ws.raw_mean, w / 2, h / 2, CLIM_FMT_RGBA32F));
+    CLIM_TRY(owned_alloc(ctx, ws.raw_var, w / 2, h / 2, CLIM_FMT_RGBA32F));
+    for (int i = 0, scale = 1; i < levels; i++, scale *= 2) {
+        const int lw = w / scale, lh = h / scale;
+        if (i > 0) {
+            CLIM_TRY(owned_alloc(ctx, ws.pyr[i], lw, lh, CLIM_FMT_RGBA32F));
+            CLIM_TRY(owned_alloc(ctx, ws.den[i], lw, lh, CLIM_FMT_RGBA32F));
+        }
+        CLIM_TRY(owned_alloc(ctx, ws.stats[i], lw, lh, CLIM_FMT_RGBA32F));
+    }
+    if (algorithm == CLIM_DENOISE_GUIDED_PRECISE) {
+        OwnedImg* g[8] = {&ws.g_meanI, &ws.g_s1, &ws.g_ipr, &ws.g_ipg, &ws.g_ipb, &ws.g_tmp4, &ws.g_tmp2, &ws.g_s2};
+        for (int i = 0; i < 7; i++) CLIM_TRY(owned_alloc(ctx, *g[i], w, h, CLIM_FMT_RGBA32F));
+        CLIM_TRY(owned_alloc(ctx, *g[7], w, h, CLIM_FMT_RG32F));
+    }
+    ws.w = w;
+    ws.h = h;
+    ws.levels = levels;
+    ws.algorithm = algorithm;
+    return CLIM_OK;
+}
+
+static Img sub_view(const Img& im, int w, int h) { return Img{im.p, w, h, im.pitch}; }
+
+// The pyramid of cpp/pyramidal_denoise.cpp:103-149 on device images.
+// level0: despeckled YCbCr; out: denoised level 0 (may not alias level0).
+// Fit slots 1..levels receive the per-level noise models.
+static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int levels, int algorithm,
+                           const clim_denoise_params* dp, const float (*nlf_in)[6]) {
+    Workspace& ws = ctx->ws;
+    hipStream_t st = ctx->stream;
+    Img lvl[CLIM_MAX_LEVELS], den[CLIM_MAX_LEVELS];
+    lvl[0] = level0;
+    den[0] = out;
+    for (int i = 1; i < levels; i++) {
+        lvl[i] = ws.pyr[i].img;
+        den[i] = ws.den[i].img;
+    }
+    // analysis: downsample and statistics of every level
+    clim::FitJobs jobs{};
+    for (int i = 0; i < levels; i++) {
+        if (i < levels - 1) {
+            ProfScope ps(ctx, ST_DOWNSAMPLE, (long long)lvl[i + 1].w * lvl[i + 1].h);
+            clim::launch_downsample(lvl[i], lvl[i + 1], st);
+        }
+        {
+            ProfScope ps(ctx, ST_LEVEL_STATS, (long long)lvl[i].w * lvl[i].h);
+            clim::launch_noise_stats(lvl[i], ws.stats[i].img, st);
+        }
+        const float lm = dp[i].luma * dp[i].luma, cm = dp[i].chroma * dp[i].chroma;
+        const float mult[8] = {lm, cm, cm, lm, cm, cm, 1.0f, 1.0f};
+        float in8[8] = {0};
+        std::memcpy(in8, nlf_in[i], 6 * sizeof(float));
+        jobs.j[i] = make_fit_job(ctx, 1 + i, ws.stats[i].img, ws.stats[i].img, 0, 1, 1, in8, mult);
+    }
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    {
+        long long n = 0;
+        for (int i = 0; i < levels; i++) n += (long long)lvl[i].w * lvl[i].h;
+        ProfScope ps(ctx, ST_LEVEL_FIT, n);
+        CLIM_TRY(run_fit(ctx, jobs, levels, 3));
+    }
+
+    // synthesis, coarse to fine
+    for (int i = levels - 1; i >= 0; i--) {
+        const ParamRef cal = dev_params(ctx->fit[1 + i].cal);
+        const bool bilateral = algorithm == CLIM_DENOISE_BILATERAL ||
+                               (algorithm == CLIM_DENOISE_GUIDED_FAST && i == levels - 1);
+        ProfScope ps(ctx, ST_DENOISE, (long long)lvl[i].w * lvl[i].h);
+        if (algorithm == CLIM_DENOISE_GUIDED_PRECISE) {
+            clim::GuidedScratch g{ws.g_meanI.img, ws.g_s1.img, ws.g_s2.img, ws.g_ipr.img,
+                                  ws.g_ipg.img,   ws.g_ipb.img, ws.g_tmp4.img, ws.g_tmp2.img};
+            clim::GuidedScratch gv{sub_view(g.mean_I, lvl[i].w, lvl[i].h), sub_view(g.s1, lvl[i].w, lvl[i].h),
+                                   sub_view(g.s2, lvl[i].w, lvl[i].h),     sub_view(g.ipr, lvl[i].w, lvl[i].h),
+                                   sub_view(g.ipg, lvl[i].w, lvl[i].h),    sub_view(g.ipb, lvl[i].w, lvl[i].h),
+                                   sub_view(g.tmp4, lvl[i].w, lvl[i].h),   sub_view(g.tmp2, lvl[i].w, lvl[i].h)};
+            // GuidedPreciseDenoiser passes var_b as eps (cpp/pyramidal_denoise.cpp:65)
+            clim::launch_guided_filter(lvl[i], den[i], 5, dev_params(ctx->fit[1 + i].cal + 3), gv, st);
+        } else if (bilateral) {
+            clim::launch_bilateral(lvl[i], den[i], cal, dp[i].chromaBoost, dp[i].gradientBoost, st);
+        } else {
+            clim::launch_guided_fast(lvl[i], den[i], cal, st);
+        }
+        if (i < levels - 1) {
+            ProfScope pr(ctx, ST_REASSEMBLE, (long long)lvl[i].w * lvl[i].h);
+            clim::launch_reassemble(den[i], lvl[i + 1], den[i + 1], den[i], dp[i].sharpening, cal, st);
+        }
+    }
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
+static clim_status check_params(clim_ctx* ctx, const clim_demosaic_params* p) {
+    if (p->bayerPattern < 0 || p->bayerPattern > 3) return ctx_fail(ctx, CLIM_EINVAL, "bad bayerPattern");
+    if (p->levels < 2 || p->levels > CLIM_MAX_LEVELS) return ctx_fail(ctx, CLIM_EINVAL, "levels must be 2..5");
+    if (p->denoiseAlgorithm < 0 || p->denoiseAlgorithm > 2) return ctx_fail(ctx, CLIM_EINVAL, "bad denoiseAlgorithm");
+    if (p->noiseLevel > 0.6f)
+        return ctx_fail(ctx, CLIM_EINVAL, "noiseLevel > 0.6 selects the high-noise raw branch, which is not built");
+    if (p->rgbConversionParameters.localToneMapping)
+        return ctx_fail(ctx, CLIM_EINVAL, "local tone mapping is not built");
+    return CLIM_OK;
+}
+
+static clim_status check_image(clim_ctx* ctx, const clim_image* im, int format, const char* name) {
+    if (!im || !im->data || im->format != format || im->width <= 0 || im->height <= 0 ||
+        im->pitch_bytes < im->width * clim_format_texel_bytes(format))
+        return ctx_fail(ctx, CLIM_EINVAL, std::string(name) + ": bad image");
+    if (format == CLIM_FMT_RGBA32F && im->pitch_bytes % 16)
+        return ctx_fail(ctx, CLIM_EINVAL, std::string(name) + ": pitch must be a multiple of 16 bytes");
+    return CLIM_OK;
+}
+
+static Img view(const clim_image* im) { return Img{(char*)im->data, im->width, im->height, im->pitch_bytes}; }
+
+extern "C" clim_status clim_pyramidal_denoise(clim_ctx* ctx, const clim_image* image, int32_t levels,
+                                              int32_t algorithm, const clim_denoise_params* params, clim_image* out,
+                                              float* nlf_out) {
+    if (!ctx || !params) return ctx_fail(ctx, CLIM_EINVAL, "pyramidal denoise: null argument");
+    if (levels < 2 || levels > CLIM_MAX_LEVELS || algorithm < 0 || algorithm > 2)
+        return ctx_fail(ctx, CLIM_EINVAL, "pyramidal denoise: bad levels or algorithm");
+    CLIM_TRY(check_image(ctx, image, CLIM_FMT_RGBA32F, "image"));
+    CLIM_TRY(check_image(ctx, out, CLIM_FMT_RGBA32F, "out"));
+    if (image->width != out->width || image->height != out->height || image->data == out->data)
+        return ctx_fail(ctx, CLIM_EINVAL, "pyramidal denoise: out must be a distinct image of the same size");
+    CLIM_TRY(ensure_workspace(ctx, image->width, image->height, levels, algorithm));
+    float nlf_in[CLIM_MAX_LEVELS][6];
+    if (nlf_out) std::memcpy(nlf_in, nlf_out, sizeof(float) * 6 * levels);
+    else std::memset(nlf_in, 0, sizeof(nlf_in));
+    CLIM_TRY(pyramid(ctx, view(image), view(out), levels, algorithm, params, nlf_in));
+    if (nlf_out) {
+        CLIM_TRY_HIP(ctx, hipMemcpyAsync(ctx->fit_host + 1, ctx->fit + 1, sizeof(clim::FitState) * levels,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+        CLIM_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < levels; i++) std::memcpy(nlf_out + 6 * i, ctx->fit_host[1 + i].out, 6 * sizeof(float));
+    }
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8,
+                                       clim_image* out_f32, const clim_demosaic_params* p, clim_noise_model* nlf_out) {
+    if (!ctx || !p) return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: null argument");
+    CLIM_TRY(check_params(ctx, p));
+    CLIM_TRY(check_image(ctx, raw, CLIM_FMT_R16U, "raw"));
+    CLIM_TRY(check_image(ctx, out_rgba8, CLIM_FMT_RGBA8, "out_rgba8"));
+    const int w = raw->width, h = raw->height;
+    if ((w & 1) || (h & 1)) return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: mosaic size must be even");
+    if ((w >> (p->levels - 1)) < 1 || (h >> (p->levels - 1)) < 1)
+        return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: image too small for the pyramid depth");
+    if (out_rgba8->width != w || out_rgba8->height != h)
+        return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: output size differs from the mosaic");
+    Img outf{nullptr, 0, 0, 0};
+    if (out_f32) {
+        CLIM_TRY(check_image(ctx, out_f32, CLIM_FMT_RGBA32F, "out_f32"));
+        if (out_f32->width != w || out_f32->height != h)
+            return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: float output size differs from the mosaic");
+        outf = view(out_f32);
+    }
+    float c2y[9], y2c[9], cam[9];
+    std::memcpy(cam, p->rgb_cam, sizeof(cam));
+    clim_cam_ycbcr(cam, c2y);
+    if (inverse3x3_scaled(c2y, (double)p->exposure_multiplier, y2c))
+        return ctx_fail(ctx, CLIM_ENUMERIC, "raw_to_rgb: camera->YCbCr matrix is singular");
+    CLIM_TRY(ensure_workspace(ctx, w, h, p->levels, p->denoiseAlgorithm));
+    Workspace& ws = ctx->ws;
+    hipStream_t st = ctx->stream;
+    const int bayer = p->bayerPattern;
+    const clim::Bayer b = make_bayer(bayer);
+
+    // scaleRawData; black level normalised as in cpp/raw_converter.cpp:105-106
+    const long long npx = (long long)w * h;
+    {
+        ProfScope ps(ctx, ST_SCALE, npx);
+        clim::launch_scale_raw(view(raw), ws.scaled.img, b, p->scale_mul, p->black_level / 65535.0f, st);
+    }
+    // computeRawNoiseStatistics (cpp/raw_converter.cpp:108-109): fit slot 0,
+    // rawNlf = B; the input rawNlf is the fallback when no quad is in range
+    {
+        ProfScope ps(ctx, ST_RAW_STATS, npx / 4);
+        clim::launch_raw_stats(ws.scaled.img, b, ws.raw_mean.img, ws.raw_var.img, st);
+    }
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    {
+        ProfScope ps(ctx, ST_RAW_FIT, npx / 4);
+        clim::FitJobs jobs{};
+        float in8[8] = {0, 0, 0, 0, p->noiseModel.rawNlf[0], p->noiseModel.rawNlf[1], p->noiseModel.rawNlf[2],
+                        p->noiseModel.rawNlf[3]};
+        jobs.j[0] = make_fit_job(ctx, 0, ws.raw_mean.img, ws.raw_var.img, 0, 0, 4, in8, nullptr);
+        CLIM_TRY(run_fit(ctx, jobs, 1, 4));
+    }
+    const float* rawfit = ctx->fit[0].out;  // device: A[4], B[4]
+    // interpolateGreen(rawNlf[1]) and interpolateRedBlue((rawNlf[0]+rawNlf[2])/2)
+    // + blendHighlightsImage(clip = 1) + transformImage(cam_ycbcr(rgb_cam))
+    {
+        ProfScope ps(ctx, ST_GREEN, npx);
+        clim::launch_green(ws.scaled.img, ws.green.img, bayer, dev_params(rawfit + 5), st);
+    }
+    {
+        ProfScope ps(ctx, ST_RED_BLUE, npx);
+        clim::launch_red_blue(ws.scaled.img, ws.green.img, ws.rgbA.img, bayer, b, dev_params(rawfit), 0, true, 1.0f,
+                              c2y, st);
+    }
+    // despeckleImage with the caller's pyramidNlf[0] (cpp/raw_converter.cpp:144-145)
+    const float* np0 = p->noiseModel.pyramidNlf[0];
+    {
+        ProfScope ps(ctx, ST_DESPECKLE, npx);
+        clim::launch_despeckle(ws.rgbA.img, ws.rgbB.img, np0[0], np0[3], st);
+    }
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+
+    CLIM_TRY(pyramid(ctx, ws.rgbB.img, ws.rgbA.img, p->levels, p->denoiseAlgorithm, p->denoiseParameters,
+                     p->noiseModel.pyramidNlf));
+
+    // transformImage(inverse(cam_to_ycbcr) * exposure) + convertTosRGB
+    const clim_rgb_conversion_params& rc = p->rgbConversionParameters;
+    {
+        ProfScope ps(ctx, ST_TONE, npx);
+        clim::launch_tone(ws.rgbA.img, view(out_rgba8), outf, y2c, cam, rc.contrast, rc.saturation, rc.toneCurveSlope,
+                          rc.exposureBias, rc.blacks, st);
+    }
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+
+    if (nlf_out) {
+        CLIM_TRY_HIP(ctx, hipMemcpyAsync(ctx->fit_host, ctx->fit, sizeof(clim::FitState) * (1 + p->levels),
+                                         hipMemcpyDeviceToHost, st));
+        CLIM_TRY_HIP(ctx, hipStreamSynchronize(st));
+        std::memcpy(nlf_out->rawNlf, ctx->fit_host[0].out + 4, 4 * sizeof(float));
+        for (int i = 0; i < p->levels; i++)
+            std::memcpy(nlf_out->pyramidNlf[i], ctx->fit_host[1 + i].out, 6 * sizeof(float));
+    }
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_generate_bayer(clim_ctx* ctx, clim_image* raw, clim_image* clean, int32_t bayer,
+                                           uint32_t scene_seed, uint32_t noise_seed, float noise_a, float noise_b,
+                                           float black_level, float white_level) {
+    if (!ctx || bayer < 0 || bayer > 3 || !(noise_a > 0.0f) || noise_b < 0.0f)
+        return ctx_fail(ctx, CLIM_EINVAL, "generate_bayer: bad argument");
+    if (!raw && !clean) return ctx_fail(ctx, CLIM_EINVAL, "generate_bayer: no output");
+    Img r{nullptr, 0, 0, 0}, c{nullptr, 0, 0, 0};
+    if (raw) {
+        CLIM_TRY(check_image(ctx, raw, CLIM_FMT_R16U, "raw"));
+        r = view(raw);
+    }
+    if (clean) {
+        CLIM_TRY(check_image(ctx, clean, CLIM_FMT_RGBA32F, "clean"));
+        c = view(clean);
+        if (raw && (clean->width != raw->width || clean->height != raw->height))
+            return ctx_fail(ctx, CLIM_EINVAL, "generate_bayer: raw and clean sizes differ");
+    }
+    const int w = raw ? raw->width : clean->width, h = raw ? raw->height : clean->height;
+    // the pinned staging buffer is reused: wait for the previous upload
+    CLIM_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    clim::scene_draw_params(w, h, scene_seed, ctx->scene_host);
+    CLIM_TRY_HIP(ctx, hipMemcpyAsync(ctx->scene, ctx->scene_host, sizeof(clim::SceneParams), hipMemcpyHostToDevice,
+                                     ctx->stream));
+    clim::launch_scene(r, c, ctx->scene, scene_colors(bayer), noise_seed, noise_a, noise_b, black_level, white_level,
+                       ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}

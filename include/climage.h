@@ -1,0 +1,292 @@
+/*
+ * climage.h - C ABI of the MI355X (gfx950) RAW->RGB path.
+ *
+ * This is the boundary the reference's OpenCL operator layer is replaced at.
+ * Every entry point below names the reference function it stands in for
+ * (paths relative to the reference tree, cpp/ = CLImage/app/src/main/cpp/,
+ * cl/ = cpp/OpenCL/).  The C++ typed wrappers in gls_hip_image.hpp,
+ * demosaic.hpp, pyramidal_denoise.hpp and raw_converter.hpp call these and
+ * keep the reference's names and argument order.
+ *
+ * Conventions
+ *  - Images are device-resident, pitch-linear (clim_image).  Texel formats
+ *    follow the reference textures (src/gls_cl_image.hpp:232-292): R16 raw
+ *    reads as value/65535 (UNORM16); luma and RGBA float images are FP32
+ *    (the reference's USE_FP16_FLOATS=0 build); RGBA8 is UNORM8.
+ *  - All calls are asynchronous on the context's HIP stream, except those
+ *    documented as blocking (copies, the *_nlf fits that return host values,
+ *    and clim_raw_to_rgb when it is asked to write the measured noise model
+ *    back).  clim_sync() waits for the stream.
+ *  - Errors are status codes, never exceptions; clim_last_error() gives the
+ *    message of the last failing call on a context.
+ *  - One context per host thread; a context is not thread-safe.
+ *  - Image borders: every stencil clamps coordinates to the image (the
+ *    reference leaves sampler-less out-of-bounds reads undefined).
+ */
+#ifndef CLIMAGE_H
+#define CLIMAGE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct clim_ctx clim_ctx;
+
+typedef enum {
+    CLIM_OK = 0,
+    CLIM_EINVAL = 1,     /* bad argument: null pointer, size or format mismatch */
+    CLIM_ENOMEM = 2,     /* device or pinned-host allocation failed */
+    CLIM_EHIP = 3,       /* a HIP runtime call failed */
+    CLIM_ENUMERIC = 4,   /* singular matrix, empty noise-model sample */
+    CLIM_ENODEV = 5      /* no usable gfx950 device */
+} clim_status;
+
+typedef enum {
+    CLIM_FMT_R16U = 1,      /* uint16 raw mosaic, read as UNORM16 */
+    CLIM_FMT_R32F = 2,      /* float luma */
+    CLIM_FMT_RGBA32F = 3,   /* float4 */
+    CLIM_FMT_RGBA8 = 4,     /* UNORM8 x4 */
+    CLIM_FMT_RG32F = 5      /* float2 (guided-filter scratch) */
+} clim_format;
+
+/* device image, pitch-linear; pitch_bytes >= width * texel size */
+typedef struct {
+    void* data;
+    int32_t width;
+    int32_t height;
+    int32_t pitch_bytes;
+    int32_t format;
+} clim_image;
+
+/* cpp/demosaic.hpp:23-28, identical numbering */
+typedef enum { CLIM_GRBG = 0, CLIM_GBRG = 1, CLIM_RGGB = 2, CLIM_BGGR = 3 } clim_bayer_pattern;
+
+/* cpp/pyramidal_denoise.hpp:38 */
+typedef enum {
+    CLIM_DENOISE_BILATERAL = 0,
+    CLIM_DENOISE_GUIDED_FAST = 1,
+    CLIM_DENOISE_GUIDED_PRECISE = 2
+} clim_denoise_algorithm;
+
+/* cpp/demosaic.hpp:32-38 (defaults 1,1,1,1,1) */
+typedef struct {
+    float luma;
+    float chroma;
+    float chromaBoost;
+    float gradientBoost;
+    float sharpening;
+} clim_denoise_params;
+
+/* cpp/demosaic.hpp:40-47, layout of cl/demosaic.cl:1423-1430
+ * (defaults contrast 1.05, saturation 1, toneCurveSlope 3.5, 0, 0, 0) */
+typedef struct {
+    float contrast;
+    float saturation;
+    float toneCurveSlope;
+    float exposureBias;
+    float blacks;
+    int32_t localToneMapping;
+} clim_rgb_conversion_params;
+
+/* cpp/demosaic.hpp:49-52 */
+typedef struct {
+    float rawNlf[4];
+    float pyramidNlf[5][6];   /* per level: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr */
+} clim_noise_model;
+
+/* cpp/demosaic.hpp:54-59 */
+typedef struct {
+    float guidedFilterEps;
+    float shadows;
+    float highlights;
+    float detail;
+} clim_ltm_params;
+
+#define CLIM_MAX_LEVELS 5
+
+/* cpp/demosaic.hpp:61-81, plus the two knobs the reference fixes at compile
+ * time: the pyramid depth (PyramidalDenoise<5>, cpp/raw_converter.hpp:33) and
+ * the per-level filter (constructor argument, cpp/pyramidal_denoise.hpp:45). */
+typedef struct {
+    int32_t bayerPattern;
+    float black_level;                 /* DN */
+    float white_level;                 /* DN */
+    float exposure_multiplier;
+    float scale_mul[4];
+    float rgb_cam[3][3];
+    clim_noise_model noiseModel;       /* pyramidNlf[0] is read by despeckle */
+    clim_denoise_params denoiseParameters[CLIM_MAX_LEVELS];
+    float noiseLevel;                  /* must be <= 0.6 (high-noise branch not built) */
+    clim_rgb_conversion_params rgbConversionParameters;
+    clim_ltm_params ltmParameters;
+    int32_t levels;                    /* 2..5 */
+    int32_t denoiseAlgorithm;          /* clim_denoise_algorithm */
+} clim_demosaic_params;
+
+/* ------------------------------------------------------------ context */
+clim_status clim_create(clim_ctx** ctx, int device);
+void clim_destroy(clim_ctx* ctx);
+const char* clim_last_error(const clim_ctx* ctx);
+const char* clim_status_string(clim_status s);
+clim_status clim_sync(clim_ctx* ctx);
+/* the context's stream, as a hipStream_t cast to void* */
+void* clim_stream(clim_ctx* ctx);
+
+/* -------------------------------------------------- images and copies
+ * cl_image_2d constructor / copyPixelsFrom / copyPixelsTo
+ * (src/gls_cl_image.hpp:53-90).  Copies are blocking like the reference's. */
+clim_status clim_image_alloc(clim_ctx* ctx, int32_t width, int32_t height, int32_t format, clim_image* out);
+clim_status clim_image_free(clim_ctx* ctx, clim_image* img);
+clim_status clim_copy_from_host(clim_ctx* ctx, const void* host, int64_t host_pitch_bytes, clim_image* dst);
+clim_status clim_copy_to_host(clim_ctx* ctx, const clim_image* src, void* host, int64_t host_pitch_bytes);
+int32_t clim_format_texel_bytes(int32_t format);
+
+/* -------------------------------------------------------- operators */
+
+/* scaleRawData, cpp/demosaic_cl.cpp:26-44, cl/demosaic.cl:69-78.
+ * black_level is normalised (DN / 65535). */
+clim_status clim_scale_raw_data(clim_ctx* ctx, const clim_image* raw, clim_image* scaled,
+                                int32_t bayer, const float scale_mul[4], float black_level);
+
+/* rawNoiseStatistics, cpp/demosaic_cl.cpp:107-128, cl/demosaic.cl:1288-1324.
+ * mean, var: (W/2)x(H/2) RGBA32F, channels in (R, G, B, G2) order of the
+ * OpenCL Bayer table (cl/demosaic.cl:29-34). */
+clim_status clim_raw_noise_statistics(clim_ctx* ctx, const clim_image* scaled, int32_t bayer,
+                                      clim_image* mean, clim_image* var);
+
+/* computeRawNoiseStatistics, cpp/pyramidal_denoise.cpp:259-362: statistics
+ * plus the three-pass FP64 regression, all on the device.  Blocking; writes
+ * A[4], B[4]. */
+clim_status clim_compute_raw_noise_statistics(clim_ctx* ctx, const clim_image* scaled, int32_t bayer,
+                                              float nlf_out[8]);
+
+/* interpolateGreen, cpp/demosaic_cl.cpp:46-63, cl/demosaic.cl:160-227 */
+clim_status clim_interpolate_green(clim_ctx* ctx, const clim_image* scaled, clim_image* green,
+                                   int32_t bayer, float luma_variance);
+
+/* interpolateRedBlue, cpp/demosaic_cl.cpp:65-85, cl/demosaic.cl:229-318 */
+clim_status clim_interpolate_red_blue(clim_ctx* ctx, const clim_image* scaled, const clim_image* green,
+                                      clim_image* rgb, int32_t bayer, float chroma_variance,
+                                      int32_t rotate_180);
+
+/* blendHighlightsImage, cpp/demosaic_cl.cpp:506-522, cl/demosaic.cl:338-380
+ * (in-place allowed) */
+clim_status clim_blend_highlights(clim_ctx* ctx, const clim_image* in, float clip, clim_image* out);
+
+/* transformImage, cpp/demosaic_cl.cpp:210-234, cl/demosaic.cl:728-733
+ * (row-major 3x3; in-place allowed) */
+clim_status clim_transform_image(clim_ctx* ctx, const clim_image* in, clim_image* out,
+                                 const float transform[9]);
+
+/* despeckleImage, cpp/demosaic_cl.cpp:267-287, cl/demosaic.cl:384-479 */
+clim_status clim_despeckle_image(clim_ctx* ctx, const clim_image* in, const float var_a[3],
+                                 const float var_b[3], clim_image* out);
+
+/* resampleImage("downsampleImage"), cpp/demosaic_cl.cpp:157-178,
+ * cl/demosaic.cl:853-865 */
+clim_status clim_downsample_image(clim_ctx* ctx, const clim_image* in, clim_image* out);
+
+/* applyKernel("noiseStatistics"), cpp/demosaic_cl.cpp:130-145,
+ * cl/demosaic.cl:1267-1286: out = (mean Y, var Y, var Cb, var Cr) */
+clim_status clim_noise_statistics(clim_ctx* ctx, const clim_image* in, clim_image* out);
+
+/* computeNoiseStatistics, cpp/pyramidal_denoise.cpp:161-256 (device-side
+ * fit).  Blocking; writes A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr. */
+clim_status clim_compute_noise_statistics(clim_ctx* ctx, const clim_image* in, float nlf_out[6]);
+
+/* denoiseImage, cpp/demosaic_cl.cpp:292-315, cl/demosaic.cl:735-765 */
+clim_status clim_denoise_image(clim_ctx* ctx, const clim_image* in, const float var_a[3],
+                               const float var_b[3], float chroma_boost, float gradient_boost,
+                               clim_image* out);
+
+/* denoiseImageGuided, cpp/demosaic_cl.cpp:317-337, cl/demosaic.cl:767-851 */
+clim_status clim_denoise_image_guided(clim_ctx* ctx, const clim_image* in, const float var_a[3],
+                                      const float var_b[3], clim_image* out);
+
+/* GuidedFilter::filter, cpp/guided_filter.cpp:195-225, cl/guided_filter.cl.
+ * Scratch comes from the context. */
+clim_status clim_guided_filter(clim_ctx* ctx, const clim_image* in, int32_t filter_size,
+                               const float eps[3], clim_image* out);
+
+/* reassembleImage, cpp/demosaic_cl.cpp:180-208, cl/demosaic.cl:867-896
+ * (out may alias denoised0) */
+clim_status clim_reassemble_image(clim_ctx* ctx, const clim_image* denoised0, const clim_image* image1,
+                                  const clim_image* denoised1, float sharpening, const float nlf[2],
+                                  clim_image* out);
+
+/* convertTosRGB, cpp/demosaic_cl.cpp:236-265, cl/demosaic.cl:1380-1474.
+ * out_rgba8 receives the UNORM8 result; out_f32 (nullable) the same values
+ * before quantisation.  ltm_mask must be null: local tone mapping is not
+ * part of this build (params->localToneMapping must be 0). */
+clim_status clim_convert_to_srgb(clim_ctx* ctx, const clim_image* linear, const clim_image* ltm_mask,
+                                 clim_image* out_rgba8, clim_image* out_f32, const float transform[9],
+                                 const clim_rgb_conversion_params* params);
+
+/* ------------------------------------------------------ host-side math */
+
+/* cam_ycbcr, cpp/demosaic_utils.cpp:809-822 (computed in FP64, rounded to float) */
+void clim_cam_ycbcr(const float rgb_cam[9], float out[9]);
+/* inverse, src/gls_linalg.hpp:544-557, in FP64; CLIM_ENUMERIC when singular */
+clim_status clim_inverse3x3(const float m[9], float out[9]);
+/* DemosaicParameters defaults of cpp/demosaic.hpp:32-81 for a synthetic
+ * identity-colour camera: rgb_cam = I, scale_mul = 65535/(white-black),
+ * exposure 1, noiseLevel 0, 5 Bilateral levels, pyramidNlf row 0 of
+ * cpp/raw_pipeline.cpp:105-111. */
+void clim_default_params(clim_demosaic_params* p, int32_t bayer, float black_level, float white_level);
+
+/* -------------------------------------------------------------- pipeline */
+
+/* PyramidalDenoise<levels>::denoise, cpp/pyramidal_denoise.cpp:103-149.
+ * `image` is the despeckled YCbCr level 0; the result is written to `out`.
+ * Per-level noise models are measured on the device; if nlf_out is not
+ * null the call blocks and writes levels x 6 floats. */
+clim_status clim_pyramidal_denoise(clim_ctx* ctx, const clim_image* image, int32_t levels,
+                                   int32_t algorithm, const clim_denoise_params* params,
+                                   clim_image* out, float* nlf_out);
+
+/* RawConverter::demosaicImage, cpp/raw_converter.cpp:73-186: raw mosaic to
+ * RGBA8 (and optionally the FP32 values before quantisation).  Work
+ * buffers are owned by the context and resized when the frame size changes.
+ * If nlf_out is not null the call blocks at the end and writes the measured
+ * noise model (rawNlf = B of the raw fit, pyramidNlf per level), as the
+ * reference writes it back into DemosaicParameters::noiseModel. */
+clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8,
+                            clim_image* out_f32, const clim_demosaic_params* params,
+                            clim_noise_model* nlf_out);
+
+/* ------------------------------------------------- synthetic scenes
+ * Seeded synthetic RAW frames (smooth background, 200 Gaussian blobs, 50
+ * hard-edged rectangles; Poisson-Gaussian noise y = a*Poisson(x/a) + N(0,b);
+ * DN = clamp(rint(black + y*(white-black)), 0, white)), generated on the
+ * device with Philox4x32-10 keyed by the seeds.  clean (nullable, RGBA32F)
+ * receives the noise-free linear scene.  Not a reference function: the
+ * reference reads DNG files (src/gls_image_tiff.cpp:255); this is the input
+ * generator for tests, benchmarks and learned-stage training data. */
+clim_status clim_generate_bayer(clim_ctx* ctx, clim_image* raw, clim_image* clean, int32_t bayer,
+                                uint32_t scene_seed, uint32_t noise_seed, float noise_a,
+                                float noise_b, float black_level, float white_level);
+
+/* ------------------------------------------------------------ profiling
+ * Per-stage device timing with HIP events on the context's stream (the
+ * counterpart of the reference's PRINT_EXECUTION_TIME wall-clock block,
+ * cpp/raw_converter.cpp:22,97-99,176-183).  While enabled, every kernel
+ * launch of clim_raw_to_rgb is bracketed by two events; clim_profile_read
+ * waits for the stream, adds the elapsed times per stage and clears the
+ * record.  Stage names: clim_profile_stage_name(i), i < CLIM_PROFILE_STAGES. */
+#define CLIM_PROFILE_STAGES 12
+typedef struct {
+    double ms[CLIM_PROFILE_STAGES];         /* summed kernel time per stage */
+    int64_t launches[CLIM_PROFILE_STAGES];  /* launches per stage */
+    int64_t texels[CLIM_PROFILE_STAGES];    /* output texels (or fitted samples) per stage */
+} clim_profile;
+clim_status clim_profile_enable(clim_ctx* ctx, int32_t enable);
+clim_status clim_profile_read(clim_ctx* ctx, clim_profile* out);
+const char* clim_profile_stage_name(int32_t stage);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLIMAGE_H */
