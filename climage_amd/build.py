@@ -67,7 +67,24 @@ def build(verbose=True, jobs=None):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_examples(verbose)
     return LIB
+
+
+def build_examples(verbose=True):
+    """C++ drivers of the typed API (include/*.hpp), linked against the library."""
+    ex_dir = os.path.join(HERE, "examples")
+    for f in sorted(os.listdir(ex_dir)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(LIB_DIR, f[:-4])
+        cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), os.path.join(ex_dir, f),
+               "-o", exe, "-L", LIB_DIR, "-lclimage_hip", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {f}:\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print(f"built {exe}")
 
 
 if __name__ == "__main__":

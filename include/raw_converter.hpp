@@ -1,0 +1,68 @@
+// raw_converter.hpp - RawConverter (cpp/raw_converter.hpp:22-62) on gfx950.
+//
+// demosaicImage uploads the mosaic (blocking, like copyPixelsFrom at
+// cpp/raw_converter.cpp:101), runs the whole path on the device and writes
+// the measured noise model back into demosaicParameters->noiseModel, as the
+// reference does.  The returned image is owned by the converter and valid
+// until the next call.  Work buffers follow the frame size.
+#pragma once
+
+#include <memory>
+
+#include "pyramidal_denoise.hpp"
+
+class RawConverter {
+   public:
+    explicit RawConverter(gls::HIPContext* ctx, int levels = 5,
+                          PyramidalDenoise<5>::DenoiseAlgorithm algorithm = PyramidalDenoise<5>::Bilateral)
+        : _ctx(ctx), _levels(levels), _algorithm(algorithm) {}
+
+    gls::hip_image_2d<gls::rgba_pixel>* demosaicImage(const gls::image<gls::luma_pixel_16>& rawImage,
+                                                      DemosaicParameters* demosaicParameters,
+                                                      const gls::rectangle* /*gmb_position*/, bool rotate_180) {
+        if (rotate_180) throw gls::clim_error(CLIM_EINVAL, "demosaicImage: rotate_180 is not supported");
+        allocateTextures(rawImage.width, rawImage.height);
+        _raw->copyPixelsFrom(rawImage);
+        return demosaicImage(*_raw, demosaicParameters);
+    }
+
+    // device-resident input: no upload
+    gls::hip_image_2d<gls::rgba_pixel>* demosaicImage(const gls::hip_image_2d<gls::luma_pixel_16>& raw,
+                                                      DemosaicParameters* demosaicParameters) {
+        allocateTextures(raw.width, raw.height);
+        clim_demosaic_params p = to_clim(*demosaicParameters, _levels, _algorithm);
+        clim_noise_model nm;
+        _ctx->check(clim_raw_to_rgb(_ctx->ctx(), raw.clim(), _srgb->clim(), nullptr, &p, &nm), "demosaicImage");
+        for (int c = 0; c < 4; c++) demosaicParameters->noiseModel.rawNlf[c] = nm.rawNlf[c];
+        for (int l = 0; l < _levels; l++)
+            for (int k = 0; k < 6; k++) demosaicParameters->noiseModel.pyramidNlf[l][k] = nm.pyramidNlf[l][k];
+        return _srgb.get();
+    }
+
+    // cpp/raw_converter.cpp:226-237
+    static gls::image<gls::rgb_pixel>::unique_ptr convertToRGBImage(
+        const gls::hip_image_2d<gls::rgba_pixel>& rgbaImage) {
+        auto rgba = rgbaImage.mapImage();
+        auto rgb = std::make_unique<gls::image<gls::rgb_pixel>>(rgbaImage.width, rgbaImage.height);
+        for (int y = 0; y < rgbaImage.height; y++)
+            for (int x = 0; x < rgbaImage.width; x++) {
+                const auto& p = rgba[y][x];
+                (*rgb)[y][x] = {p.red, p.green, p.blue};
+            }
+        return rgb;
+    }
+
+   private:
+    void allocateTextures(int width, int height) {
+        if (!_raw || _raw->width != width || _raw->height != height) {
+            _raw = std::make_unique<gls::hip_image_2d<gls::luma_pixel_16>>(_ctx, width, height);
+            _srgb = std::make_unique<gls::hip_image_2d<gls::rgba_pixel>>(_ctx, width, height);
+        }
+    }
+
+    gls::HIPContext* _ctx;
+    int _levels;
+    int _algorithm;
+    std::unique_ptr<gls::hip_image_2d<gls::luma_pixel_16>> _raw;
+    std::unique_ptr<gls::hip_image_2d<gls::rgba_pixel>> _srgb;
+};
