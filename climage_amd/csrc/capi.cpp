@@ -146,7 +146,7 @@ extern "C" clim_status clim_create(clim_ctx** out, int device) {
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&ctx->fit, sizeof(clim::FitState) * kFitJobs);
     if (e == hipSuccess) e = hipMemset(ctx->fit, 0, sizeof(clim::FitState) * kFitJobs);
-    if (e == hipSuccess) e = hipMalloc(&ctx->partials, sizeof(clim_fx128) * kFitJobs * kFitMaxBlocks * kFitK);
+    if (e == hipSuccess) e = hipMalloc(&ctx->partials, sizeof(clim_fx128) * kFitJobs * clim::kFitBlocksPerJob * kFitK);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->fit_host, sizeof(clim::FitState) * kFitJobs, 0);
     if (e == hipSuccess) e = hipMalloc(&ctx->scene, sizeof(clim::SceneParams));
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->scene_host, sizeof(clim::SceneParams), 0);
@@ -176,8 +176,6 @@ extern "C" void clim_destroy(clim_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_workspace(ctx->ws);
-    owned_free(ctx->op_mean);
-    owned_free(ctx->op_var);
     for (auto& o : ctx->op_guided_imgs) owned_free(o);
     if (ctx->fit) (void)hipFree(ctx->fit);
     if (ctx->partials) (void)hipFree(ctx->partials);
@@ -273,33 +271,18 @@ extern "C" clim_status clim_raw_noise_statistics(clim_ctx* ctx, const clim_image
     return CLIM_OK;
 }
 
-// runs the three regression passes for njobs jobs of nch channels
-clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, int njobs, int nch) {
-    long long maxcount = 1;
-    for (int j = 0; j < njobs; j++) maxcount = std::max(maxcount, (long long)jobs.j[j].w * jobs.j[j].h);
-    int nblocks = (int)std::min<long long>((maxcount + 4095) / 4096, kFitMaxBlocks);
-    nblocks = std::max(nblocks, 1);
-    for (int pass = 0; pass < 3; pass++) {
-        clim::launch_fit(jobs, njobs, nch, nblocks, pass, ctx->stream);
-        CLIM_TRY_HIP(ctx, hipGetLastError());
-    }
+// runs the three regression passes of the jobs (one kind per launch)
+clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, int kind) {
+    clim::launch_fit(jobs, kind, ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
 }
 
-clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& m, const Img& v, int m_elem, int v_elem, int m_nch,
-                          const float* nlf_in, const float* mult) {
+clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& src, const float* nlf_in, const float* mult) {
     clim::FitJob jb{};
-    jb.m = m.p;
-    jb.v = v.p;
-    jb.w = m.w;
-    jb.h = m.h;
-    jb.pitch_m = m.pitch;
-    jb.pitch_v = v.pitch;
-    jb.m_elem = m_elem;
-    jb.v_elem = v_elem;
-    jb.m_nch = m_nch;
+    jb.src = src;
     jb.st = ctx->fit + slot;
-    jb.partials = ctx->partials + (size_t)slot * kFitMaxBlocks * kFitK;
+    jb.partials = ctx->partials + (size_t)slot * clim::kFitBlocksPerJob * kFitK;
     for (int k = 0; k < 8; k++) {
         jb.nlf_in[k] = nlf_in ? nlf_in[k] : 0.0f;
         jb.mult[k] = mult ? mult[k] : 1.0f;
@@ -317,15 +300,13 @@ static clim_status read_fit(clim_ctx* ctx, int slot, int n) {
 extern "C" clim_status clim_compute_raw_noise_statistics(clim_ctx* ctx, const clim_image* scaled, int32_t bayer,
                                                          float nlf_out[8]) {
     if (!ctx || !nlf_out) return ctx_fail(ctx, CLIM_EINVAL, "computeRawNoiseStatistics: null argument");
+    if (bayer < 0 || bayer > 3) return ctx_fail(ctx, CLIM_EINVAL, "computeRawNoiseStatistics: bad bayer");
     CLIM_TRY(check_img(ctx, scaled, CLIM_FMT_R32F, "scaled"));
-    CLIM_TRY(owned_alloc(ctx, ctx->op_mean, scaled->width / 2, scaled->height / 2, CLIM_FMT_RGBA32F));
-    CLIM_TRY(owned_alloc(ctx, ctx->op_var, scaled->width / 2, scaled->height / 2, CLIM_FMT_RGBA32F));
-    clim_image m{ctx->op_mean.img.p, ctx->op_mean.img.w, ctx->op_mean.img.h, ctx->op_mean.img.pitch, CLIM_FMT_RGBA32F};
-    clim_image v{ctx->op_var.img.p, ctx->op_var.img.w, ctx->op_var.img.h, ctx->op_var.img.pitch, CLIM_FMT_RGBA32F};
-    CLIM_TRY(clim_raw_noise_statistics(ctx, scaled, bayer, &m, &v));
     clim::FitJobs jobs{};
-    jobs.j[0] = make_fit_job(ctx, 0, ctx->op_mean.img, ctx->op_var.img, 0, 0, 4, nullptr, nullptr);
-    CLIM_TRY(run_fit(ctx, jobs, 1, 4));
+    jobs.njobs = 1;
+    jobs.j[0] = make_fit_job(ctx, 0, view(scaled), nullptr, nullptr);
+    jobs.j[0].b = make_bayer(bayer);
+    CLIM_TRY(run_fit(ctx, jobs, clim::FIT_RAW_F32));
     CLIM_TRY(read_fit(ctx, 0, 1));
     if (ctx->fit_host[0].status == 1)
         return ctx_fail(ctx, CLIM_ENUMERIC, "computeRawNoiseStatistics: no sample in range");
@@ -414,12 +395,10 @@ extern "C" clim_status clim_noise_statistics(clim_ctx* ctx, const clim_image* in
 extern "C" clim_status clim_compute_noise_statistics(clim_ctx* ctx, const clim_image* in, float nlf_out[6]) {
     if (!ctx || !nlf_out) return ctx_fail(ctx, CLIM_EINVAL, "computeNoiseStatistics: null argument");
     CLIM_TRY(check_img(ctx, in, CLIM_FMT_RGBA32F, "in"));
-    CLIM_TRY(owned_alloc(ctx, ctx->op_mean, in->width, in->height, CLIM_FMT_RGBA32F));
-    clim::launch_noise_stats(view(in), ctx->op_mean.img, ctx->stream);
-    CLIM_TRY_HIP(ctx, hipGetLastError());
     clim::FitJobs jobs{};
-    jobs.j[0] = make_fit_job(ctx, 1, ctx->op_mean.img, ctx->op_mean.img, 0, 1, 1, nullptr, nullptr);
-    CLIM_TRY(run_fit(ctx, jobs, 1, 3));
+    jobs.njobs = 1;
+    jobs.j[0] = make_fit_job(ctx, 1, view(in), nullptr, nullptr);
+    CLIM_TRY(run_fit(ctx, jobs, clim::FIT_LEVEL));
     CLIM_TRY(read_fit(ctx, 1, 1));
     if (ctx->fit_host[1].status == 1)
         return ctx_fail(ctx, CLIM_ENUMERIC, "computeNoiseStatistics: no sample in range");

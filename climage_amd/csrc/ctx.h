@@ -30,8 +30,7 @@ struct Workspace {
 };
 
 constexpr int kFitJobs = 1 + CLIM_MAX_LEVELS;  // raw + levels
-constexpr int kFitMaxBlocks = 1024;
-constexpr int kFitK = 17;                       // 4 sums x 4 channels + count
+constexpr int kFitK = 17;                       // largest accumulator count (raw: 4 sums x 4 channels + count)
 
 enum Stage {
     ST_SCALE = 0, ST_RAW_STATS, ST_RAW_FIT, ST_GREEN, ST_RED_BLUE, ST_DESPECKLE,
@@ -50,12 +49,11 @@ struct clim_ctx {
     std::string last_error;
     Workspace ws;
     clim::FitState* fit = nullptr;            // device [kFitJobs]
-    clim_fx128* partials = nullptr;           // device [kFitJobs][kFitMaxBlocks][kFitK]
+    clim_fx128* partials = nullptr;           // device [kFitJobs][kFitBlocksPerJob][kFitK]
     clim::FitState* fit_host = nullptr;       // pinned mirror for read-back
     clim::SceneParams* scene = nullptr;       // device
     clim::SceneParams* scene_host = nullptr;  // pinned staging
-    // scratch of the stand-alone fit operators
-    OwnedImg op_mean, op_var;
+    // scratch of the stand-alone guided filter
     clim::GuidedScratch op_guided{};
     OwnedImg op_guided_imgs[8];
     // stage profiling

@@ -75,6 +75,75 @@ struct ParamRef {
     __device__ __forceinline__ float get(int i) const { return dev ? dev[i] : host[i]; }
 };
 
+// Stages N elements into LDS with NT threads: every thread issues all of its
+// global loads before it waits for any (a loop with a run-time trip count
+// would wait for each load before issuing the next).
+template <int N, int NT, typename LoadF, typename StoreF>
+__device__ __forceinline__ void stage(int tid, LoadF load, StoreF store) {
+    constexpr int IT = (N + NT - 1) / NT;
+    using T = decltype(load(0));
+    T v[IT];
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int i = tid + k * NT;
+        if (i < N) v[k] = load(i);
+    }
+#pragma unroll
+    for (int k = 0; k < IT; k++) {
+        const int i = tid + k * NT;
+        if (i < N) store(i, v[k]);
+    }
+}
+
+// One raw quad (texels (2qx + ox[c], 2qy + oy[c]), c = R, G, B, G2) of the
+// uint16 mosaic, scaled as scaleRawData does (cl/demosaic.cl:69-78), with
+// texel coordinates clamped to the image.  Two 32-bit loads (the quad's two
+// rows); the mosaic width is even.
+__device__ __forceinline__ float scale_texel(uint32_t v16, int site, float4 smul, float black) {
+    const float sm = site == 0 ? smul.x : site == 1 ? smul.y : site == 2 ? smul.z : smul.w;
+    const float s = sm * ((float)v16 / 65535.0f - black);
+    return s > 0.0f ? s : 0.0f;
+}
+
+struct RawQuadLoader {
+    Img raw;
+    Bayer b;
+    float4 smul;
+    float black;
+    // returns the raw pair (x even, x+1) of row y, with both clamped
+    __device__ __forceinline__ uint32_t pair(int qx, int y) const {
+        const int yc = clampi(y, 0, raw.h - 1);
+        const int qc = clampi(qx, 0, raw.w / 2 - 1);
+        uint32_t w = reinterpret_cast<const uint32_t*>(raw.p + (size_t)yc * raw.pitch)[qc];
+        if (qx < 0) w = (w & 0xffffu) * 0x10001u;          // both clamp to x = 0
+        if (qx > qc) w = (w >> 16) * 0x10001u;              // both clamp to x = W-1
+        return w;
+    }
+    // the quad's two rows (loads only)
+    __device__ __forceinline__ uint2 words(int qx, int qy) const {
+        return make_uint2(pair(qx, 2 * qy), pair(qx, 2 * qy + 1));
+    }
+    // (R, G, B, G2) of quad (qx, qy), scaled
+    __device__ __forceinline__ float4 quad(int qx, int qy) const { return quad(words(qx, qy), qx, qy); }
+    __device__ __forceinline__ float4 quad(uint2 wds, int qx, int qy) const {
+        const uint32_t r0 = wds.x, r1 = wds.y;
+        // site parity of the clamped rows: row 2qy clamps to 0 or H-1 only
+        // at the borders, where both rows clamp to the same row
+        const int y0 = clampi(2 * qy, 0, raw.h - 1), y1 = clampi(2 * qy + 1, 0, raw.h - 1);
+        const int x0 = qx < 0 ? 0 : (qx >= raw.w / 2 ? raw.w - 1 : 2 * qx);
+        const int x1 = qx < 0 ? 0 : (qx >= raw.w / 2 ? raw.w - 1 : 2 * qx + 1);
+        float v[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t row = b.oy[c] ? r1 : r0;
+            const uint32_t v16 = b.ox[c] ? (row >> 16) : (row & 0xffffu);
+            const int xs = b.ox[c] ? x1 : x0, ys = b.oy[c] ? y1 : y0;
+            v[c] = scale_texel(v16, b.site(xs, ys), smul, black);
+        }
+        return make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+
 // 128-bit fixed-point helpers for wave reductions
 __device__ __forceinline__ clim_fx128 fx_shfl_xor(clim_fx128 a, int m) {
     clim_fx128 r;

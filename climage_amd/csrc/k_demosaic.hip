@@ -7,6 +7,7 @@
 // Reference: cl/demosaic.cl (cl/ = CLImage/app/src/main/cpp/OpenCL/); line
 // numbers are cited per kernel.  All loads clamp to the image.
 #include "kernels.h"
+#include "tiles.h"
 
 namespace clim {
 
@@ -41,42 +42,28 @@ void launch_scale_raw(const Img& raw, const Img& out, const Bayer& b, const floa
 // float4 (one quad per texel).
 __global__ void __launch_bounds__(256) k_raw_stats(Img scaled, Bayer b, Img mean, Img var) {
     constexpr int T = 16, R = 2, S = T + 2 * R;
-    __shared__ float4 q[S][S];
+    __shared__ QuadTile<T> sm;
     const int qx0 = blockIdx.x * T, qy0 = blockIdx.y * T;
     const int tid = threadIdx.y * T + threadIdx.x;
-    for (int i = tid; i < S * S; i += 256) {
-        const int sy = i / S, sx = i % S;
-        const int qx = qx0 - R + sx, qy = qy0 - R + sy;
-        float4 v;
-        v.x = ld1(scaled, 2 * qx + b.ox[0], 2 * qy + b.oy[0]);
-        v.y = ld1(scaled, 2 * qx + b.ox[1], 2 * qy + b.oy[1]);
-        v.z = ld1(scaled, 2 * qx + b.ox[2], 2 * qy + b.oy[2]);
-        v.w = ld1(scaled, 2 * qx + b.ox[3], 2 * qy + b.oy[3]);
-        q[sy][sx] = v;
-    }
+    stage<S * S, 256>(
+        tid,
+        [&](int i) {
+            const int qx = qx0 - R + i % S, qy = qy0 - R + i / S;
+            return make_float4(ld1(scaled, 2 * qx + b.ox[0], 2 * qy + b.oy[0]),
+                               ld1(scaled, 2 * qx + b.ox[1], 2 * qy + b.oy[1]),
+                               ld1(scaled, 2 * qx + b.ox[2], 2 * qy + b.oy[2]),
+                               ld1(scaled, 2 * qx + b.ox[3], 2 * qy + b.oy[3]));
+        },
+        [&](int i, float4 v) { sm.q[i / S][i % S] = v; });
+    __syncthreads();
+    sm.hsums(tid);
     __syncthreads();
     const int qx = qx0 + threadIdx.x, qy = qy0 + threadIdx.y;
     if (qx >= mean.w || qy >= mean.h) return;
-    float s0 = 0, s1 = 0, s2 = 0, s3 = 0, q0 = 0, q1 = 0, q2 = 0, q3 = 0;
-#pragma unroll
-    for (int j = 0; j < 5; j++) {
-#pragma unroll
-        for (int i = 0; i < 5; i++) {
-            const float4 v = q[threadIdx.y + j][threadIdx.x + i];
-            s0 += v.x; q0 += v.x * v.x;
-            s1 += v.y; q1 += v.y * v.y;
-            s2 += v.z; q2 += v.z * v.z;
-            s3 += v.w; q3 += v.w * v.w;
-        }
-    }
-    float4 m, vv;
-    m.x = s0 / 25.0f; m.y = s1 / 25.0f; m.z = s2 / 25.0f; m.w = s3 / 25.0f;
-    vv.x = (q0 - (s0 * s0) / 25.0f) / 25.0f;
-    vv.y = (q1 - (s1 * s1) / 25.0f) / 25.0f;
-    vv.z = (q2 - (s2 * s2) / 25.0f) / 25.0f;
-    vv.w = (q3 - (s3 * s3) / 25.0f) / 25.0f;
-    wrow_f4(mean, qy)[qx] = m;
-    wrow_f4(var, qy)[qx] = vv;
+    float m[4], v[4];
+    sm.stats(threadIdx.x, threadIdx.y, m, v);
+    wrow_f4(mean, qy)[qx] = make_float4(m[0], m[1], m[2], m[3]);
+    wrow_f4(var, qy)[qx] = make_float4(v[0], v[1], v[2], v[3]);
 }
 
 void launch_raw_stats(const Img& scaled, const Bayer& b, const Img& mean, const Img& var, hipStream_t st) {
@@ -101,10 +88,9 @@ __global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int 
     __shared__ float sgl[GS_Y][GS_X + 1];
     const int x0 = blockIdx.x * GT_X, y0 = blockIdx.y * GT_Y;
     const int tid = threadIdx.y * 32 + threadIdx.x;
-    for (int i = tid; i < GR_X * GR_Y; i += 256) {
-        const int sy = i / GR_X, sx = i % GR_X;
-        r[sy][sx] = ld1(raw, x0 - 3 + sx, y0 - 3 + sy);
-    }
+    stage<GR_X * GR_Y, 256>(
+        tid, [&](int i) { return ld1(raw, x0 - 3 + i % GR_X, y0 - 3 + i / GR_X); },
+        [&](int i, float v) { r[i / GR_X][i % GR_X] = v; });
     __syncthreads();
     // sobel(), cl/demosaic.cl:96-112, value / sqrt(4.5) then abs()
     for (int i = tid; i < GS_X * GS_Y; i += 256) {
@@ -121,7 +107,7 @@ __global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int 
                 vy += ky[j][k] * s;
             }
         }
-        const float ax = fabsf(vx / 2.12132034f), ay = fabsf(vy / 2.12132034f);
+        const float ax = fabsf(vx * 0.471404521f), ay = fabsf(vy * 0.471404521f);
         sgx[sy][sx] = ax;
         sgy[sy][sx] = ay;
         sgl[sy][sx] = len2(ax, ay);
@@ -151,22 +137,23 @@ __global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int 
             // bilateralSobel(), cl/demosaic.cl:114-129
             const int SX = tx + 2, SY = ty + 2;  // sobel tile coordinates
             const float cl = sgl[SY][SX];
-            const float e0 = 4.0f * lumaStdDev, e1 = 16.0f * lumaStdDev;
-            float sx = 0.0f, sy = 0.0f, wx = 0.0f, wy = 0.0f;
+            // smoothstep(4 sigma, 16 sigma, d) with its division hoisted as a reciprocal
+            const float e0 = 4.0f * lumaStdDev, inv = 1.0f / (16.0f * lumaStdDev - 4.0f * lumaStdDev);
+            float sx = 0.0f, sy = 0.0f, wsum = 0.0f;
 #pragma unroll
             for (int j = -2; j <= 2; j++) {
 #pragma unroll
                 for (int i = -2; i <= 2; i++) {
                     const float gx = sgx[SY + j][SX + i], gy = sgy[SY + j][SX + i];
                     const float d = 0.05f * fabsf(sgl[SY + j][SX + i] - cl);
-                    const float wt = 1.0f - smoothstep(e0, e1, d);
+                    const float t = clampf((d - e0) * inv, 0.0f, 1.0f);
+                    const float wt = 1.0f - t * t * (3.0f - 2.0f * t);
                     sx += wt * gx;
                     sy += wt * gy;
-                    wx += wt;
-                    wy += wt;
+                    wsum += wt;  // the x and y weight sums are equal
                 }
             }
-            const float gradx = sx / wx, grady = sy / wy;
+            const float gradx = sx / wsum, grady = sy / wsum;
 
             const float g_lf_x = (g_left + g_right) / 2.0f, g_lf_y = (g_up + g_down) / 2.0f;
             const float g_hf_x = (2.0f * c_xy - (c_left + c_right)) / 4.0f;

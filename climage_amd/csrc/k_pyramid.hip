@@ -5,6 +5,7 @@
 //
 // Reference: cl/demosaic.cl (cl/ = CLImage/app/src/main/cpp/OpenCL/).
 #include "kernels.h"
+#include "tiles.h"
 
 namespace clim {
 
@@ -57,49 +58,30 @@ void launch_downsample(const Img& in, const Img& out, hipStream_t st) {
 // 5x5 stencils over an RGBA level: a 32x16 output tile plus a 2-texel halo in
 // LDS (xyz only; alpha never enters these filters).
 namespace {
-constexpr int TX = 32, TY = 16, HR = 2, SX = TX + 2 * HR, SY = TY + 2 * HR;
+constexpr int TX = 32, TY = 16, HR = 2;
 }
 
-struct Tile3 {
-    float x[SY][SX + 1], y[SY][SX + 1], z[SY][SX + 1];
+struct StatsTile {
+    LevelTile<TX, TY> t;
+    StatsRows<TX, TY> r;
 };
-
-__device__ __forceinline__ void load_tile3(Tile3& t, const Img& in, int x0, int y0) {
-    const int tid = threadIdx.y * 32 + threadIdx.x;
-    for (int i = tid; i < SX * SY; i += 256) {
-        const int sy = i / SX, sx = i % SX;
-        const float4 p = ld4(in, x0 - HR + sx, y0 - HR + sy);
-        t.x[sy][sx] = p.x;
-        t.y[sy][sx] = p.y;
-        t.z[sy][sx] = p.z;
-    }
-}
 
 // ------------------------------------------------------------ noiseStatistics
 // cl/demosaic.cl:1267-1286: (mean Y, var Y, var Cb, var Cr) over 5x5.
 __global__ void __launch_bounds__(256) k_noise_stats(Img in, Img out) {
-    __shared__ Tile3 t;
+    __shared__ StatsTile sm;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    load_tile3(t, in, x0, y0);
+    const int tid = threadIdx.y * 32 + threadIdx.x;
+    sm.t.load(in, x0, y0, tid);
     __syncthreads();
-#pragma unroll 1
+    sm.r.compute(sm.t, tid);
+    __syncthreads();
+#pragma unroll
     for (int rr = 0; rr < 2; rr++) {
         const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
         const int x = x0 + tx, y = y0 + ty;
         if (x >= out.w || y >= out.h) continue;
-        float s0 = 0, s1 = 0, s2 = 0, q0 = 0, q1 = 0, q2 = 0;
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-#pragma unroll
-            for (int i = 0; i < 5; i++) {
-                const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
-                s0 += a; q0 += a * a;
-                s1 += b; q1 += b * b;
-                s2 += c; q2 += c * c;
-            }
-        }
-        wrow_f4(out, y)[x] = make_float4(s0 / 25.0f, (q0 - (s0 * s0) / 25.0f) / 25.0f,
-                                         (q1 - (s1 * s1) / 25.0f) / 25.0f, (q2 - (s2 * s2) / 25.0f) / 25.0f);
+        wrow_f4(out, y)[x] = sm.r.stats(tx, ty);
     }
 }
 
@@ -109,12 +91,62 @@ void launch_noise_stats(const Img& in, const Img& out, hipStream_t st) {
 }
 
 // --------------------------------------------------------------- denoiseImage
-// cl/demosaic.cl:735-765.  nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr.
+// cl/demosaic.cl:735-765.  nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr.  The per-tap
+// division by sigma is a multiplication by 1/sigma and length(d) < edge is
+// dot(d, d) < edge^2 (the FP32 oracle evaluates the same expressions).
+__device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int tx, int ty, float va0, float va1,
+                                               float va2, float vb0, float vb1, float vb2, float chromaBoost,
+                                               float gradientBoost) {
+    const int X = tx + HR, Y = ty + HR;
+    const float cY = t.x[Y][X], cCb = t.y[Y][X], cCr = t.z[Y][X];
+    const float sig0 = sqrtf(va0 + vb0 * cY), sig1 = sqrtf(va1 + vb1 * cY), sig2 = sqrtf(va2 + vb2 * cY);
+    float lumaBoost = 1.0f;
+    if (gradientBoost > 1.0f) {
+        // sobel() of the luma channel, cl/demosaic.cl:96-112
+        const float kx[3][3] = {{1, 0, -1}, {2, 0, -2}, {1, 0, -1}};
+        const float ky[3][3] = {{1, 2, 1}, {0, 0, 0}, {-1, -2, -1}};
+        float vx = 0.0f, vy = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const float s = t.x[Y - 1 + j][X - 1 + i];
+                vx += kx[j][i] * s;
+                vy += ky[j][i] * s;
+            }
+        vx = vx * 0.471404521f;
+        vy = vy * 0.471404521f;
+        const float is0 = 1.0f / sig0;
+        lumaBoost = gradientBoost - (gradientBoost - 1.0f) * smoothstep(0.5f, 2.0f, len2(vx * is0, vy * is0));
+    }
+    float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
+    e0 = e0 * e0;
+    e1 = e1 * e1;
+    const float i0 = 1.0f / sig0, i1 = 1.0f / sig1, i2 = 1.0f / sig2;
+    float f0 = 0, f1 = 0, f2 = 0, n0 = 0, n1 = 0;
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
+            const float d0 = (a - cY) * i0, d1 = (b - cCb) * i1, d2 = (c - cCr) * i2;
+            const float l = d0 * d0 + d1 * d1 + d2 * d2;
+            const float w0 = 1.0f - (l < e0 ? 0.0f : 1.0f);
+            const float w1 = 1.0f - (l < e1 ? 0.0f : 1.0f);
+            f0 += w0 * a; n0 += w0;
+            f1 += w1 * b; n1 += w1;
+            f2 += w1 * c;
+        }
+    }
+    // the Cr weight sum equals the Cb one (same threshold, same weights)
+    return make_float4(f0 / n0, f1 / n1, f2 / n1, 0.0f);
+}
+
 __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
                                                    float gradientBoost) {
-    __shared__ Tile3 t;
+    __shared__ LevelTile<TX, TY> t;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    load_tile3(t, in, x0, y0);
+    t.load(in, x0, y0, threadIdx.y * 32 + threadIdx.x);
     __syncthreads();
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
     const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
@@ -123,45 +155,7 @@ __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf
         const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
         const int x = x0 + tx, y = y0 + ty;
         if (x >= out.w || y >= out.h) continue;
-        const int X = tx + HR, Y = ty + HR;
-        const float cY = t.x[Y][X], cCb = t.y[Y][X], cCr = t.z[Y][X];
-        const float sig0 = sqrtf(va0 + vb0 * cY), sig1 = sqrtf(va1 + vb1 * cY), sig2 = sqrtf(va2 + vb2 * cY);
-        float lumaBoost = 1.0f;
-        if (gradientBoost > 1.0f) {
-            // sobel() of the luma channel, cl/demosaic.cl:96-112
-            const float kx[3][3] = {{1, 0, -1}, {2, 0, -2}, {1, 0, -1}};
-            const float ky[3][3] = {{1, 2, 1}, {0, 0, 0}, {-1, -2, -1}};
-            float vx = 0.0f, vy = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-#pragma unroll
-                for (int i = 0; i < 3; i++) {
-                    const float s = t.x[Y - 1 + j][X - 1 + i];
-                    vx += kx[j][i] * s;
-                    vy += ky[j][i] * s;
-                }
-            vx = vx / 2.12132034f;
-            vy = vy / 2.12132034f;
-            lumaBoost = gradientBoost -
-                        (gradientBoost - 1.0f) * smoothstep(0.5f, 2.0f, len2(vx / sig0, vy / sig0));
-        }
-        const float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
-        float f0 = 0, f1 = 0, f2 = 0, n0 = 0, n1 = 0, n2 = 0;
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-#pragma unroll
-            for (int i = 0; i < 5; i++) {
-                const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
-                const float d0 = (a - cY) / sig0, d1 = (b - cCb) / sig1, d2 = (c - cCr) / sig2;
-                const float l = len3(d0, d1, d2);
-                const float w0 = 1.0f - (l < e0 ? 0.0f : 1.0f);
-                const float w1 = 1.0f - (l < e1 ? 0.0f : 1.0f);
-                f0 += w0 * a; n0 += w0;
-                f1 += w1 * b; n1 += w1;
-                f2 += w1 * c; n2 += w1;
-            }
-        }
-        wrow_f4(out, y)[x] = make_float4(f0 / n0, f1 / n1, f2 / n2, 0.0f);
+        wrow_f4(out, y)[x] = bilateral_px(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost);
     }
 }
 
@@ -172,11 +166,38 @@ void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float 
 }
 
 // --------------------------------------------------------- denoiseImageGuided
-// cl/demosaic.cl:767-851: one-pass 5x5 self-guided colour guided filter.
+// cl/demosaic.cl:767-851: one-pass 5x5 self-guided colour guided filter.  The
+// nine box sums (I and its six products) use the separable order.
+struct GuidedTile {
+    LevelTile<TX, TY> t;
+    float h[9][TY + 4][TX + 1];
+};
+
 __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef nlf) {
-    __shared__ Tile3 t;
+    __shared__ GuidedTile sm;
     const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    load_tile3(t, in, x0, y0);
+    const int tid = threadIdx.y * 32 + threadIdx.x;
+    sm.t.load(in, x0, y0, tid);
+    __syncthreads();
+    {
+        constexpr int N = (TY + 4) * TX, IT = (N + 255) / 256;
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int i = tid + k * 256;
+            if (i >= N) break;
+            const int r = i / TX, c = i % TX;
+            float hs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < 5; d++) {
+                const float a = sm.t.x[r][c + d], b = sm.t.y[r][c + d], e = sm.t.z[r][c + d];
+                const float v[9] = {a, b, e, a * a, a * b, a * e, b * b, b * e, e * e};
+#pragma unroll
+                for (int q = 0; q < 9; q++) hs[q] += v[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 9; q++) sm.h[q][r][c] = hs[q];
+        }
+    }
     __syncthreads();
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
     const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
@@ -187,19 +208,18 @@ __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef n
         const int x = x0 + tx, y = y0 + ty;
         if (x >= out.w || y >= out.h) continue;
         const int X = tx + HR, Y = ty + HR;
-        const float ix = t.x[Y][X], iy = t.y[Y][X], iz = t.z[Y][X];
+        const float ix = sm.t.x[Y][X], iy = sm.t.y[Y][X], iz = sm.t.z[Y][X];
         const float eps0 = va0 + vb0 * ix, eps1 = va1 + vb1 * ix, eps2 = va2 + vb2 * ix;
-        float m0 = 0, m1 = 0, m2 = 0, rr_ = 0, rg = 0, rb = 0, gg = 0, gb = 0, bb = 0;
+        float acc[9];
 #pragma unroll
-        for (int j = 0; j < 5; j++) {
+        for (int q = 0; q < 9; q++) {
+            float a = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 5; i++) {
-                const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
-                m0 += a; m1 += b; m2 += c;
-                rr_ += a * a; rg += a * b; rb += a * c;
-                gg += b * b; gb += b * c; bb += c * c;
-            }
+            for (int j = 0; j < 5; j++) a += sm.h[q][ty + j][tx];
+            acc[q] = a;
         }
+        float m0 = acc[0], m1 = acc[1], m2 = acc[2];
+        float rr_ = acc[3], rg = acc[4], rb = acc[5], gg = acc[6], gb = acc[7], bb = acc[8];
         m0 *= norm; m1 *= norm; m2 *= norm;
         rr_ *= norm; rg *= norm; rb *= norm; gg *= norm; gb *= norm; bb *= norm;
         const float vrr = rr_ - m0 * m0, vrg = rg - m0 * m1, vrb = rb - m0 * m2;

@@ -15,23 +15,30 @@ struct FitState {
     float cal[8];          // out * mult (calibrated per-level model)
 };
 
+// what a fit reads: the uint16 mosaic (scaled on the fly), the scaled FP32
+// mosaic, or an RGBA32F pyramid level
+enum FitKind { FIT_RAW_U16 = 0, FIT_RAW_F32 = 1, FIT_LEVEL = 2 };
+
+constexpr int kFitBlocksPerJob = 1024;
+
 struct FitJob {
-    const char* m;        // mean samples (float4 texels)
-    const char* v;        // variance samples (float4 texels)
-    int w, h, pitch_m, pitch_v;
-    int m_elem, v_elem;   // float offset of the first mean / variance inside a texel
-    int m_nch;            // 1: one mean shared by all channels, else per channel
+    Img src;
+    Bayer b;              // raw kinds
+    float4 smul;          // FIT_RAW_U16: scale_mul
+    float black;          // FIT_RAW_U16: normalised black level
+    int tiles_x, tiles_y, nblocks, block_base;  // set by launch_fit
     FitState* st;
-    clim_fx128* partials;
+    clim_fx128* partials; // nblocks x (4*nch + 1)
     float nlf_in[8];      // fallback model (written to out when no sample is in range)
     float mult[8];        // calibration multipliers
 };
 
 struct FitJobs {
     FitJob j[5];
+    int njobs;
 };
 
-void launch_fit(const FitJobs& jobs, int njobs, int nch, int nblocks, int pass, hipStream_t st);
+void launch_fit(FitJobs& jobs, int kind, hipStream_t st);
 
 // ------------------------------------------------------------ demosaic stages
 void launch_scale_raw(const Img& raw, const Img& out, const Bayer& b, const float smul[4], float black,

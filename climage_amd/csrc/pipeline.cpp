@@ -21,9 +21,8 @@
 using clim::Img;
 using clim::ParamRef;
 
-clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, int njobs, int nch);
-clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& m, const Img& v, int m_elem, int v_elem, int m_nch,
-                          const float* nlf_in, const float* mult);
+clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, int kind);
+clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& src, const float* nlf_in, const float* mult);
 clim_status guided_scratch(clim_ctx* ctx, OwnedImg* imgs, int w, int h, clim::GuidedScratch* g);
 int inverse3x3_scaled(const float mf[9], double scale, float out[9]);
 
@@ -39,15 +38,12 @@ static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int
     CLIM_TRY(owned_alloc(ctx, ws.green, w, h, CLIM_FMT_R32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbA, w, h, CLIM_FMT_RGBA32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbB, w, h, CLIM_FMT_RGBA32F));
-    CLIM_TRY(owned_alloc(ctx, ws.raw_mean, w / 2, h / 2, CLIM_FMT_RGBA32F));
-    CLIM_TRY(owned_alloc(ctx, ws.raw_var, w / 2, h / 2, CLIM_FMT_RGBA32F));
     for (int i = 0, scale = 1; i < levels; i++, scale *= 2) {
         const int lw = w / scale, lh = h / scale;
         if (i > 0) {
             CLIM_TRY(owned_alloc(ctx, ws.pyr[i], lw, lh, CLIM_FMT_RGBA32F));
             CLIM_TRY(owned_alloc(ctx, ws.den[i], lw, lh, CLIM_FMT_RGBA32F));
         }
-        CLIM_TRY(owned_alloc(ctx, ws.stats[i], lw, lh, CLIM_FMT_RGBA32F));
     }
     if (algorithm == CLIM_DENOISE_GUIDED_PRECISE) {
         OwnedImg* g[8] = {&ws.g_meanI, &ws.g_s1, &ws.g_ipr, &ws.g_ipg, &ws.g_ipb, &ws.g_tmp4, &ws.g_tmp2, &ws.g_s2};
@@ -84,22 +80,19 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
             ProfScope ps(ctx, ST_DOWNSAMPLE, (long long)lvl[i + 1].w * lvl[i + 1].h);
             clim::launch_downsample(lvl[i], lvl[i + 1], st);
         }
-        {
-            ProfScope ps(ctx, ST_LEVEL_STATS, (long long)lvl[i].w * lvl[i].h);
-            clim::launch_noise_stats(lvl[i], ws.stats[i].img, st);
-        }
         const float lm = dp[i].luma * dp[i].luma, cm = dp[i].chroma * dp[i].chroma;
         const float mult[8] = {lm, cm, cm, lm, cm, cm, 1.0f, 1.0f};
         float in8[8] = {0};
         std::memcpy(in8, nlf_in[i], 6 * sizeof(float));
-        jobs.j[i] = make_fit_job(ctx, 1 + i, ws.stats[i].img, ws.stats[i].img, 0, 1, 1, in8, mult);
+        jobs.j[i] = make_fit_job(ctx, 1 + i, lvl[i], in8, mult);
     }
+    jobs.njobs = levels;
     CLIM_TRY_HIP(ctx, hipGetLastError());
     {
         long long n = 0;
         for (int i = 0; i < levels; i++) n += (long long)lvl[i].w * lvl[i].h;
         ProfScope ps(ctx, ST_LEVEL_FIT, n);
-        CLIM_TRY(run_fit(ctx, jobs, levels, 3));
+        CLIM_TRY(run_fit(ctx, jobs, clim::FIT_LEVEL));
     }
 
     // synthesis, coarse to fine
@@ -215,18 +208,18 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
     }
     // computeRawNoiseStatistics (cpp/raw_converter.cpp:108-109): fit slot 0,
     // rawNlf = B; the input rawNlf is the fallback when no quad is in range
-    {
-        ProfScope ps(ctx, ST_RAW_STATS, npx / 4);
-        clim::launch_raw_stats(ws.scaled.img, b, ws.raw_mean.img, ws.raw_var.img, st);
-    }
     CLIM_TRY_HIP(ctx, hipGetLastError());
     {
         ProfScope ps(ctx, ST_RAW_FIT, npx / 4);
         clim::FitJobs jobs{};
         float in8[8] = {0, 0, 0, 0, p->noiseModel.rawNlf[0], p->noiseModel.rawNlf[1], p->noiseModel.rawNlf[2],
                         p->noiseModel.rawNlf[3]};
-        jobs.j[0] = make_fit_job(ctx, 0, ws.raw_mean.img, ws.raw_var.img, 0, 0, 4, in8, nullptr);
-        CLIM_TRY(run_fit(ctx, jobs, 1, 4));
+        jobs.njobs = 1;
+        jobs.j[0] = make_fit_job(ctx, 0, view(raw), in8, nullptr);
+        jobs.j[0].b = b;
+        jobs.j[0].smul = make_float4(p->scale_mul[0], p->scale_mul[1], p->scale_mul[2], p->scale_mul[3]);
+        jobs.j[0].black = p->black_level / 65535.0f;
+        CLIM_TRY(run_fit(ctx, jobs, clim::FIT_RAW_U16));
     }
     const float* rawfit = ctx->fit[0].out;  // device: A[4], B[4]
     // interpolateGreen(rawNlf[1]) and interpolateRedBlue((rawNlf[0]+rawNlf[2])/2)

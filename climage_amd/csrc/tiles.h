@@ -1,0 +1,122 @@
+// tiles.h - LDS tiles with 5x5 window statistics, shared by the statistics,
+// regression and filter kernels.
+//
+// Window sums follow the path's definition: the vertical sum of five
+// horizontal 5-tap sums, each a sequential sum starting from zero,
+//   S = 0 + H(-2) + H(-1) + H(0) + H(1) + H(2),  H(j) = 0 + x(j,-2) + ... + x(j,2),
+// so a horizontal sum is computed once per tile row and reused by the five
+// output rows that need it.
+#pragma once
+
+#include "device_common.h"
+
+namespace clim {
+
+// ------------------------------------------------------------ RGBA levels
+// TXxTY outputs, 256 threads, a 2-texel halo; only x, y, z enter the filters.
+template <int TX, int TY>
+struct LevelTile {
+    static constexpr int SX = TX + 4, SY = TY + 4;
+    float x[SY][SX + 1], y[SY][SX + 1], z[SY][SX + 1];
+
+    __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
+        stage<SX * SY, 256>(
+            tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
+            [&](int i, float4 p) {
+                x[i / SX][i % SX] = p.x;
+                y[i / SX][i % SX] = p.y;
+                z[i / SX][i % SX] = p.z;
+            });
+    }
+};
+
+// Horizontal 5-tap sums of the level tile for all SY rows: h[0..2] the
+// channel sums, h[3..5] the sums of squares (noiseStatistics, cl/demosaic.cl:1267-1286).
+template <int TX, int TY>
+struct StatsRows {
+    static constexpr int SY = TY + 4;
+    float h[6][SY][TX + 1];
+
+    __device__ __forceinline__ void compute(const LevelTile<TX, TY>& t, int tid) {
+        constexpr int N = SY * TX, IT = (N + 255) / 256;
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int i = tid + k * 256;
+            if (i >= N) break;
+            const int r = i / TX, c = i % TX;
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 5; d++) {
+                const float a = t.x[r][c + d], b = t.y[r][c + d], e = t.z[r][c + d];
+                s0 += a; q0 += a * a;
+                s1 += b; q1 += b * b;
+                s2 += e; q2 += e * e;
+            }
+            h[0][r][c] = s0; h[1][r][c] = s1; h[2][r][c] = s2;
+            h[3][r][c] = q0; h[4][r][c] = q1; h[5][r][c] = q2;
+        }
+    }
+
+    // (mean Y, var Y, var Cb, var Cr) of output (lx, ly)
+    __device__ __forceinline__ float4 stats(int lx, int ly) const {
+        float s[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 5; j++) a += h[k][ly + j][lx];
+            s[k] = a;
+        }
+        return make_float4(s[0] / 25.0f, (s[3] - (s[0] * s[0]) / 25.0f) / 25.0f,
+                           (s[4] - (s[1] * s[1]) / 25.0f) / 25.0f, (s[5] - (s[2] * s[2]) / 25.0f) / 25.0f);
+    }
+};
+
+// ------------------------------------------------------------ raw quads
+// RTxRT output quads, 256 threads, 2-quad halo; texels are (R, G, B, G2).
+template <int RT>
+struct QuadTile {
+    static constexpr int S = RT + 4;
+    float4 q[S][S];
+    float h[8][S][RT + 1];  // horizontal sums: 4 channel sums, 4 sums of squares
+
+    __device__ __forceinline__ void hsums(int tid) {
+        constexpr int N = S * RT, IT = (N + 255) / 256;
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int i = tid + k * 256;
+            if (i >= N) break;
+            const int r = i / RT, c = i % RT;
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, q0 = 0.0f, q1 = 0.0f, q2 = 0.0f, q3 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 5; d++) {
+                const float4 v = q[r][c + d];
+                s0 += v.x; q0 += v.x * v.x;
+                s1 += v.y; q1 += v.y * v.y;
+                s2 += v.z; q2 += v.z * v.z;
+                s3 += v.w; q3 += v.w * v.w;
+            }
+            h[0][r][c] = s0; h[1][r][c] = s1; h[2][r][c] = s2; h[3][r][c] = s3;
+            h[4][r][c] = q0; h[5][r][c] = q1; h[6][r][c] = q2; h[7][r][c] = q3;
+        }
+    }
+
+    // rawNoiseStatistics (cl/demosaic.cl:1302-1324) of quad (lx, ly): mean, var
+    __device__ __forceinline__ void stats(int lx, int ly, float* m, float* v) const {
+        float s[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float a = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 5; j++) a += h[k][ly + j][lx];
+            s[k] = a;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            m[c] = s[c] / 25.0f;
+            v[c] = (s[4 + c] - (s[c] * s[c]) / 25.0f) / 25.0f;
+        }
+    }
+};
+
+}  // namespace clim

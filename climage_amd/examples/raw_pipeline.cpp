@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 
 #include "raw_converter.hpp"
@@ -105,6 +106,7 @@ int main(int argc, char** argv) {
         write_rgb(out + ".stages.rgb", *rgb2);
 
         std::ofstream f(out + ".nlf");
+        f.precision(9);
         for (const auto* p : {&p1, &p2}) {
             for (float v : p->noiseModel.rawNlf) f << v << ' ';
             for (const auto& row : p->noiseModel.pyramidNlf)

@@ -210,12 +210,105 @@ typedef struct { uint64_t hi, lo; } clim_fx128;
 
 CLIM_HD clim_fx128 clim_fx_from_double(double t) {
     clim_fx128 r;
-    if (!(t > 0.0)) { r.hi = 0; r.lo = 0; return r; }
+    /* NaN, non-positive and >= 2^37 (never a regression term) count as 0 */
+    if (!(t > 0.0) || !(t < 137438953472.0)) { r.hi = 0; r.lo = 0; return r; }
     double y = t * 1237940039285380274899124224.0;          /* 2^90 */
     double h = floor(y * 5.421010862427522170037264e-20);  /* 2^-64 */
     r.hi = (uint64_t)h;
     r.lo = (uint64_t)(y - h * 18446744073709551616.0);      /* exact */
     return r;
+}
+
+/* m * 2^s as a 128-bit integer, truncated (s may be negative) */
+CLIM_HD clim_fx128 clim_fx_shift(uint64_t m, int s) {
+    clim_fx128 r;
+    if (s >= 0) {
+        if (s >= 128) { r.hi = 0; r.lo = 0; }               /* out of range: not reached for |t| < 2^37 */
+        else if (s >= 64) { r.hi = m << (s - 64); r.lo = 0; }
+        else if (s == 0) { r.hi = 0; r.lo = m; }
+        else { r.hi = m >> (64 - s); r.lo = m << s; }
+    } else {
+        r.hi = 0;
+        r.lo = -s >= 64 ? 0 : m >> (-s);
+    }
+    return r;
+}
+
+/* floor(f * 2^90) for a float f >= 0, by integer arithmetic: equal to
+ * clim_fx_from_double((double)f) */
+CLIM_HD clim_fx128 clim_fx_from_float(float f) {
+    uint32_t b = clim_f2u(f) & 0x7fffffffu;
+    int e = (int)(b >> 23);
+    uint64_t m = b & 0x7fffffu;
+    if ((clim_f2u(f) >> 31) || e >= 127 + 37) { clim_fx128 z = {0, 0}; return z; }
+    if (e) m |= 0x800000u; else e = 1;
+    return clim_fx_shift(m, e - 150 + CLIM_FX_FRAC_BITS);
+}
+
+/* floor(f * g * 2^90) for floats f, g >= 0: the 48-bit product of the
+ * significands is exact, so this equals clim_fx_from_double((double)f * g) */
+CLIM_HD clim_fx128 clim_fx_from_product(float f, float g) {
+    uint32_t bf = clim_f2u(f) & 0x7fffffffu, bg = clim_f2u(g) & 0x7fffffffu;
+    int ef = (int)(bf >> 23), eg = (int)(bg >> 23);
+    if (((clim_f2u(f) ^ clim_f2u(g)) >> 31 && bf && bg) || ef == 255 || eg == 255 || ef + eg >= 254 + 37) {
+        clim_fx128 z = {0, 0};
+        return z;
+    }
+    uint64_t mf = bf & 0x7fffffu, mg = bg & 0x7fffffu;
+    if (ef) mf |= 0x800000u; else ef = 1;
+    if (eg) mg |= 0x800000u; else eg = 1;
+    clim_fx128 r = clim_fx_shift(mf * mg, ef + eg - 300 + CLIM_FX_FRAC_BITS);
+    if (r.hi >> 63) { r.hi = 0; r.lo = 0; }  /* product >= 2^37 */
+    return r;
+}
+
+/* floor(t * 2^90) for a double t >= 0 from its bits: equal to clim_fx_from_double(t) */
+CLIM_HD clim_fx128 clim_fx_from_double_bits(double t) {
+    uint64_t b;
+    __builtin_memcpy(&b, &t, 8);
+    int e = (int)((b & 0x7fffffffffffffffull) >> 52);
+    if ((b >> 63) || e >= 1023 + 37) { clim_fx128 z = {0, 0}; return z; }
+    uint64_t m = b & 0xfffffffffffffull;
+    if (e) m |= 0x10000000000000ull; else e = 1;
+    return clim_fx_shift(m, e - 1075 + CLIM_FX_FRAC_BITS);
+}
+
+/* ---- regression terms as exact integers
+ * An in-range mean m lies in [0.001, 0.5]; every float there is a multiple
+ * of 2^-33 (0.001 > 2^-10), so m * 2^33 is an exact uint32 and its square an
+ * exact uint64.  A variance v in [0, 0.001] is quantised to floor(v * 2^64)
+ * (at most 5.4e-20 below v).  All regression sums are then sums of exact
+ * integers: s_x = sum(mq) 2^-33, s_xx = sum(mq^2) 2^-66, s_y = sum(vq) 2^-64,
+ * s_xy = sum(mq vq) 2^-97. */
+CLIM_HD uint32_t clim_q_mean(float m) { return (uint32_t)(m * 8589934592.0f); }
+
+CLIM_HD uint64_t clim_q_var(float v) {
+    float x = v * 18446744073709551616.0f;               /* exact: v * 2^64 < 2^55 */
+    uint32_t hi = (uint32_t)(x * 2.3283064365386962890625e-10f); /* floor(x / 2^32) */
+    uint32_t lo = (uint32_t)(x - (float)hi * 4294967296.0f);      /* exact remainder */
+    return ((uint64_t)hi << 32) | lo;
+}
+
+CLIM_HD clim_fx128 clim_fx_u64(uint64_t a) {
+    clim_fx128 r;
+    r.hi = 0;
+    r.lo = a;
+    return r;
+}
+
+/* a * b for a < 2^32, b < 2^64 as 128 bits */
+CLIM_HD clim_fx128 clim_fx_mul32x64(uint32_t a, uint64_t b) {
+    uint64_t lo = (uint64_t)a * (uint32_t)b;
+    uint64_t hp = (uint64_t)a * (uint32_t)(b >> 32);
+    clim_fx128 r;
+    r.lo = lo + (hp << 32);
+    r.hi = (hp >> 32) + (r.lo < lo ? 1u : 0u);
+    return r;
+}
+
+/* (hi 2^64 + lo) * scale, scale a power of two */
+CLIM_HD double clim_fx_to_double_scaled(clim_fx128 a, double scale) {
+    return ((double)a.hi * 18446744073709551616.0 + (double)a.lo) * scale;
 }
 
 CLIM_HD clim_fx128 clim_fx_add(clim_fx128 a, clim_fx128 b) {

@@ -123,7 +123,12 @@ struct PyramidalDenoise {
         for (size_t i = 0; i < levels; i++) {
             if (i < levels - 1) resampleImage(ctx, "downsampleImage", i == 0 ? *image : *imagePyramid[i - 1],
                                               imagePyramid[i].get());
-            (*nlfParameters)[i] = computeNoiseStatistics(ctx, i == 0 ? *image : *imagePyramid[i - 1]);
+            try {
+                (*nlfParameters)[i] = computeNoiseStatistics(ctx, i == 0 ? *image : *imagePyramid[i - 1]);
+            } catch (const gls::clim_error& e) {
+                // no sample in range: keep the caller's model for this level
+                if (e.status != CLIM_ENUMERIC) throw;
+            }
             const DenoiseParameters& d = (*denoiseParameters)[i];
             const float lm = d.luma * d.luma, cm = d.chroma * d.chroma;
             const float mul[6] = {lm, cm, cm, lm, cm, cm};

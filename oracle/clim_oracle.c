@@ -50,6 +50,7 @@ typedef double real;
 #define ATAN2PI(y, x) (atan2(y, x) / 3.14159265358979323846)
 #define TANH(x) tanh(x)
 #define EXP(x) exp(x)
+#define INV_SQRT4_5 0.47140452079103168
 #else
 typedef float real;
 #define K(x) x##f
@@ -62,6 +63,7 @@ typedef float real;
 #define ATAN2PI(y, x) clim_atan2pif(y, x)
 #define TANH(x) clim_tanhf(x)
 #define EXP(x) clim_expf(x)
+#define INV_SQRT4_5 0.471404521f
 #endif
 
 /* OpenCL built-ins (OpenCL C 2.0, 6.13.2 / 6.13.4) */
@@ -71,6 +73,11 @@ static inline real smoothstepr(real e0, real e1, real x) {
     return t * t * (K(3.0) - K(2.0) * t);
 }
 static inline real mixr(real a, real b, real t) { return a + (b - a) * t; }
+/* smoothstep with the division by (e1 - e0) given as a reciprocal `inv` */
+static inline real smoothstep_inv(real e0, real inv, real x) {
+    real t = clampr((x - e0) * inv, K(0.0), K(1.0));
+    return t * t * (K(3.0) - K(2.0) * t);
+}
 /* the kernel file's own min/max macros, cl/demosaic.cl:57-63 */
 static inline real mmin(real a, real b) { return a < b ? a : b; }
 static inline real mmax(real a, real b) { return a > b ? a : b; }
@@ -115,6 +122,13 @@ void P(scale_raw)(const uint16_t* raw, int w, int h, int bayer, const float scal
     }
 }
 
+/* 5x5 window sums, evaluated as the vertical sum of five horizontal 5-tap
+ * sums: S = ((((H(-2) + H(-1)) + H(0)) + H(1)) + H(2)) with
+ * H(j) = ((((x(j,-2) + x(j,-1)) + x(j,0)) + x(j,1)) + x(j,2)).  The
+ * reference writes one sequential 25-term loop; -cl-fast-relaxed-math leaves
+ * the association to the compiler, and this one lets a GPU share the
+ * horizontal sums between the five rows that use them. */
+
 /* ------------------------------------------------------ rawNoiseStatistics
  * cl/demosaic.cl:1288-1324: 5x5-quad mean and variance of (R, G, B, G2). */
 void P(raw_noise_statistics)(const real* raw, int w, int h, int bayer, real* mean, real* var) {
@@ -124,13 +138,18 @@ void P(raw_noise_statistics)(const real* raw, int w, int h, int bayer, real* mea
         for (int qx = 0; qx < qw; qx++) {
             real sum[4] = {0, 0, 0, 0}, sumSq[4] = {0, 0, 0, 0};
             for (int j = -2; j <= 2; j++) {
+                real hs[4] = {0, 0, 0, 0}, hq[4] = {0, 0, 0, 0};
                 for (int i = -2; i <= 2; i++) {
                     for (int c = 0; c < 4; c++) {
                         real s = rd1(raw, w, h, 2 * (qx + i) + kBayerOffsets[bayer][c][0],
                                      2 * (qy + j) + kBayerOffsets[bayer][c][1]);
-                        sum[c] += s;
-                        sumSq[c] += s * s;
+                        hs[c] += s;
+                        hq[c] += s * s;
                     }
+                }
+                for (int c = 0; c < 4; c++) {
+                    sum[c] += hs[c];
+                    sumSq[c] += hq[c];
                 }
             }
             real* m = mean + ((size_t)qy * qw + qx) * 4;
@@ -153,7 +172,10 @@ void P(raw_noise_statistics)(const real* raw, int w, int h, int bayer, real* mea
  * of that fit selects inliers (squared residual <= RMS in every channel)
  * for a second fit.  Returns 0 on success, 1 if no sample is in range
  * (out untouched), 2 if the refit had no inliers (first fit returned).
- * out = A[0..nch), B[0..nch). */
+ * out = A[0..nch), B[0..nch).
+ * Sums are exact: in the FP32 build of integer terms (means * 2^33 and
+ * variances quantised to 2^-64, see clim_q_mean / clim_q_var), in the FP64
+ * build of the 2^-90 fixed-point values of the FP64 terms. */
 #define FIT_MAXCH 4
 
 /* loads sample i; returns 1 when every mean is in [0.001, 0.5] and every
@@ -195,10 +217,20 @@ static long fit_sums(int nch, int m_nch, const real* m, int m_step, const real* 
                 if (!inlier) continue;
             }
             for (int c = 0; c < nch; c++) {
+#if ORACLE_DOUBLE
                 l[4 * c + 0] = clim_fx_add(l[4 * c + 0], clim_fx_from_double(mm[c]));
                 l[4 * c + 1] = clim_fx_add(l[4 * c + 1], clim_fx_from_double(vv[c]));
                 l[4 * c + 2] = clim_fx_add(l[4 * c + 2], clim_fx_from_double(mm[c] * mm[c]));
                 l[4 * c + 3] = clim_fx_add(l[4 * c + 3], clim_fx_from_double(mm[c] * vv[c]));
+#else
+                /* exact integer terms (clim_q_mean, clim_q_var) */
+                const uint32_t mq = clim_q_mean((float)mm[c]);
+                const uint64_t vq = clim_q_var((float)vv[c]);
+                l[4 * c + 0] = clim_fx_add(l[4 * c + 0], clim_fx_u64(mq));
+                l[4 * c + 1] = clim_fx_add(l[4 * c + 1], clim_fx_u64(vq));
+                l[4 * c + 2] = clim_fx_add(l[4 * c + 2], clim_fx_u64((uint64_t)mq * mq));
+                l[4 * c + 3] = clim_fx_add(l[4 * c + 3], clim_fx_mul32x64(mq, vq));
+#endif
             }
             ln++;
         }
@@ -240,10 +272,17 @@ static void fit_residuals(int nch, int m_nch, const real* m, int m_step, const r
 static void fit_solve(int nch, long n, const clim_fx128* acc, double* A, double* B) {
     double N = (double)n;
     for (int c = 0; c < nch; c++) {
+#if ORACLE_DOUBLE
         double s_x = clim_fx_to_double(acc[4 * c + 0]);
         double s_y = clim_fx_to_double(acc[4 * c + 1]);
         double s_xx = clim_fx_to_double(acc[4 * c + 2]);
         double s_xy = clim_fx_to_double(acc[4 * c + 3]);
+#else
+        double s_x = clim_fx_to_double_scaled(acc[4 * c + 0], 0x1p-33);
+        double s_y = clim_fx_to_double_scaled(acc[4 * c + 1], 0x1p-64);
+        double s_xx = clim_fx_to_double_scaled(acc[4 * c + 2], 0x1p-66);
+        double s_xy = clim_fx_to_double_scaled(acc[4 * c + 3], 0x1p-97);
+#endif
         B[c] = fmax((N * s_xy - s_x * s_y) / (N * s_xx - s_x * s_x), 1e-8);
         A[c] = fmax((s_y - B[c] * s_x) / N, 1e-8);
     }
@@ -285,9 +324,9 @@ static inline void sobel(const real* img, int w, int h, int x, int y, real* gx, 
             vy += kSobel[j + 1][i + 1][1] * s;
         }
     }
-    /* value / sqrt(4.5), sqrt(4.5) folded to a single-precision constant */
-    *gx = vx / K(2.12132034);
-    *gy = vy / K(2.12132034);
+    /* value / sqrt(4.5), as a multiplication by the reciprocal */
+    *gx = vx * INV_SQRT4_5;
+    *gy = vy * INV_SQRT4_5;
 }
 
 static inline void bilateral_sobel(const real* img, int w, int h, int x, int y, real sigma,
@@ -297,6 +336,9 @@ static inline void bilateral_sobel(const real* img, int w, int h, int x, int y, 
     sobel(img, w, h, x, y, &cx, &cy);
     cx = RABS(cx);
     cy = RABS(cy);
+    /* smoothstep(4 sigma, 16 sigma, d) with the division by the (per-pixel)
+     * edge distance hoisted out of the 5x5 loop as a reciprocal */
+    const real e0 = K(4.0) * sigma, inv = K(1.0) / (K(16.0) * sigma - K(4.0) * sigma);
     for (int j = -2; j <= 2; j++) {
         for (int i = -2; i <= 2; i++) {
             real gx, gy;
@@ -304,7 +346,7 @@ static inline void bilateral_sobel(const real* img, int w, int h, int x, int y, 
             gx = RABS(gx);
             gy = RABS(gy);
             real d = K(0.05) * RABS(len2(gx, gy) - len2(cx, cy));
-            real wt = K(1.0) - smoothstepr(K(4.0) * sigma, K(16.0) * sigma, d);
+            real wt = K(1.0) - smoothstep_inv(e0, inv, d);
             sx += wt * gx;
             sy += wt * gy;
             wx += wt;
@@ -585,12 +627,17 @@ void P(noise_statistics)(const real* in, int w, int h, real* out) {
         for (int x = 0; x < w; x++) {
             real sum[3] = {0, 0, 0}, sumSq[3] = {0, 0, 0};
             for (int j = -2; j <= 2; j++) {
+                real hs[3] = {0, 0, 0}, hq[3] = {0, 0, 0};
                 for (int i = -2; i <= 2; i++) {
                     const real* p = rd4(in, w, h, x + i, y + j);
                     for (int c = 0; c < 3; c++) {
-                        sum[c] += p[c];
-                        sumSq[c] += p[c] * p[c];
+                        hs[c] += p[c];
+                        hq[c] += p[c] * p[c];
                     }
+                }
+                for (int c = 0; c < 3; c++) {
+                    sum[c] += hs[c];
+                    sumSq[c] += hq[c];
                 }
             }
             real* o = out + ((size_t)y * w + x) * 4;
@@ -601,7 +648,9 @@ void P(noise_statistics)(const real* in, int w, int h, real* out) {
 }
 
 /* --------------------------------------------------------------- denoiseImage
- * cl/demosaic.cl:735-765: 5x5 hard-threshold sigma filter. */
+ * cl/demosaic.cl:735-765: 5x5 hard-threshold sigma filter.  The per-tap
+ * division by sigma is a multiplication by 1/sigma (computed once per pixel),
+ * and length(d) < edge is evaluated as dot(d, d) < edge * edge. */
 void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], const float var_b[3],
                           float chroma_boost, float gradient_boost, real* out) {
 #pragma omp parallel for schedule(static)
@@ -621,21 +670,24 @@ void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], co
                         vx += kSobel[j + 1][i + 1][0] * s;
                         vy += kSobel[j + 1][i + 1][1] * s;
                     }
-                vx = vx / K(2.12132034);
-                vy = vy / K(2.12132034);
+                vx = vx * INV_SQRT4_5;
+                vy = vy * INV_SQRT4_5;
                 real gb = (real)gradient_boost;
-                lumaBoost = gb - (gb - K(1.0)) * smoothstepr(K(0.5), K(2.0), len2(vx / sigma[0], vy / sigma[0]));
+                real is0 = K(1.0) / sigma[0];
+                lumaBoost = gb - (gb - K(1.0)) * smoothstepr(K(0.5), K(2.0), len2(vx * is0, vy * is0));
             }
-            const real edge[3] = {lumaBoost * K(1.41421356), (real)chroma_boost * K(1.41421356),
-                                  (real)chroma_boost * K(1.41421356)};
+            real edge[3] = {lumaBoost * K(1.41421356), (real)chroma_boost * K(1.41421356),
+                            (real)chroma_boost * K(1.41421356)};
+            for (int c = 0; c < 3; c++) edge[c] = edge[c] * edge[c];
+            const real isig[3] = {K(1.0) / sigma[0], K(1.0) / sigma[1], K(1.0) / sigma[2]};
             real fp[3] = {0, 0, 0}, kn[3] = {0, 0, 0};
             for (int j = -2; j <= 2; j++) {
                 for (int i = -2; i <= 2; i++) {
                     const real* s = rd4(in, w, h, x + i, y + j);
-                    real d0 = (s[0] - cY) / sigma[0];
-                    real d1 = (s[1] - cCb) / sigma[1];
-                    real d2 = (s[2] - cCr) / sigma[2];
-                    real l = len3(d0, d1, d2);
+                    real d0 = (s[0] - cY) * isig[0];
+                    real d1 = (s[1] - cCb) * isig[1];
+                    real d2 = (s[2] - cCr) * isig[2];
+                    real l = d0 * d0 + d1 * d1 + d2 * d2;
                     for (int c = 0; c < 3; c++) {
                         real wt = K(1.0) - (l < edge[c] ? K(0.0) : K(1.0));
                         fp[c] += wt * s[c];
@@ -662,15 +714,20 @@ void P(denoise_guided_fast)(const real* in, int w, int h, const float var_a[3], 
             real ix = c0[0], iy = c0[1], iz = c0[2];
             real eps[3];
             for (int c = 0; c < 3; c++) eps[c] = (real)var_a[c] + (real)var_b[c] * ix;
-            real mI[3] = {0, 0, 0}, rr = 0, rg = 0, rb = 0, gg = 0, gb = 0, bb = 0;
+            /* 9 box sums: mean of I and of its 6 products (separable order) */
+            real acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             for (int j = -2; j <= 2; j++) {
+                real hsum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
                 for (int i = -2; i <= 2; i++) {
                     const real* s = rd4(in, w, h, x + i, y + j);
-                    mI[0] += s[0]; mI[1] += s[1]; mI[2] += s[2];
-                    rr += s[0] * s[0]; rg += s[0] * s[1]; rb += s[0] * s[2];
-                    gg += s[1] * s[1]; gb += s[1] * s[2]; bb += s[2] * s[2];
+                    const real v[9] = {s[0], s[1], s[2], s[0] * s[0], s[0] * s[1], s[0] * s[2],
+                                       s[1] * s[1], s[1] * s[2], s[2] * s[2]};
+                    for (int k = 0; k < 9; k++) hsum[k] += v[k];
                 }
+                for (int k = 0; k < 9; k++) acc[k] += hsum[k];
             }
+            real mI[3] = {acc[0], acc[1], acc[2]};
+            real rr = acc[3], rg = acc[4], rb = acc[5], gg = acc[6], gb = acc[7], bb = acc[8];
             mI[0] *= norm; mI[1] *= norm; mI[2] *= norm;
             rr *= norm; rg *= norm; rb *= norm; gg *= norm; gb *= norm; bb *= norm;
             real vrr = rr - mI[0] * mI[0], vrg = rg - mI[0] * mI[1], vrb = rb - mI[0] * mI[2];

@@ -46,3 +46,24 @@ double om_fx_sum(const double* t, long n, uint64_t* hi, uint64_t* lo) {
     *lo = s.lo;
     return clim_fx_to_double(s);
 }
+
+/* the integer conversions must equal clim_fx_from_double on the exact value:
+ * out[4*i .. 4*i+3] = (hi, lo) of fx_from_float(f[i]) and fx_from_double(f[i]) */
+void om_fx_float_check(const float* f, uint64_t* out, long n) {
+    for (long i = 0; i < n; i++) {
+        clim_fx128 a = clim_fx_from_float(f[i]), b = clim_fx_from_double((double)f[i]);
+        out[4 * i] = a.hi; out[4 * i + 1] = a.lo; out[4 * i + 2] = b.hi; out[4 * i + 3] = b.lo;
+    }
+}
+void om_fx_product_check(const float* f, const float* g, uint64_t* out, long n) {
+    for (long i = 0; i < n; i++) {
+        clim_fx128 a = clim_fx_from_product(f[i], g[i]), b = clim_fx_from_double((double)f[i] * (double)g[i]);
+        out[4 * i] = a.hi; out[4 * i + 1] = a.lo; out[4 * i + 2] = b.hi; out[4 * i + 3] = b.lo;
+    }
+}
+void om_fx_double_bits_check(const double* t, uint64_t* out, long n) {
+    for (long i = 0; i < n; i++) {
+        clim_fx128 a = clim_fx_from_double_bits(t[i]), b = clim_fx_from_double(t[i]);
+        out[4 * i] = a.hi; out[4 * i + 1] = a.lo; out[4 * i + 2] = b.hi; out[4 * i + 3] = b.lo;
+    }
+}

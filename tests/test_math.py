@@ -109,3 +109,24 @@ def test_fixed_point_is_exact_truncation(L):
     s2 = L.om_fx_sum(b.ctypes.data_as(C.c_void_p), C.c_long(b.size), C.byref(hi2), C.byref(lo2))
     assert (hi1.value, lo1.value) == (hi2.value, lo2.value) and s1 == s2
     assert abs(s1 - float(np.sum(np.sort(t)))) <= 1e-12 * s1
+
+
+def test_integer_fixed_point_conversions_match_double_path(L):
+    rng = np.random.default_rng(4)
+    f = np.concatenate([rng.uniform(0, 0.5, 20000), rng.uniform(0, 1e-3, 20000), 10.0 ** rng.uniform(-45, 10, 20000),
+                        [0.0, -0.0, 1e-45, 1.17549435e-38, 0.5, 0.001]]).astype(np.float32)
+    g = rng.permutation(f).astype(np.float32)
+    out = np.empty(4 * f.size, np.uint64)
+    L.om_fx_float_check(f.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), C.c_long(f.size))
+    o = out.reshape(-1, 4)
+    np.testing.assert_array_equal(o[:, :2], o[:, 2:])
+    L.om_fx_product_check(f.ctypes.data_as(C.c_void_p), g.ctypes.data_as(C.c_void_p),
+                          out.ctypes.data_as(C.c_void_p), C.c_long(f.size))
+    o = out.reshape(-1, 4)
+    np.testing.assert_array_equal(o[:, :2], o[:, 2:])
+    t = np.concatenate([rng.uniform(0, 1e-6, 20000) ** 2, 10.0 ** rng.uniform(-300, 10, 20000),
+                        [0.0, -0.0, 5e-324, 2.0 ** -90, 2.0 ** -91, np.nan]])
+    out = np.empty(4 * t.size, np.uint64)
+    L.om_fx_double_bits_check(t.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), C.c_long(t.size))
+    o = out.reshape(-1, 4)
+    np.testing.assert_array_equal(o[:, :2], o[:, 2:])
