@@ -177,6 +177,9 @@ extern "C" void clim_destroy(clim_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_workspace(ctx->ws);
     for (auto& o : ctx->op_guided_imgs) owned_free(o);
+    owned_free(ctx->op_mean);
+    owned_free(ctx->op_var);
+    owned_free(ctx->op_stats);
     if (ctx->fit) (void)hipFree(ctx->fit);
     if (ctx->partials) (void)hipFree(ctx->partials);
     if (ctx->fit_host) (void)hipHostFree(ctx->fit_host);
@@ -271,16 +274,18 @@ extern "C" clim_status clim_raw_noise_statistics(clim_ctx* ctx, const clim_image
     return CLIM_OK;
 }
 
-// runs the three regression passes of the jobs (one kind per launch)
-clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, int kind) {
-    clim::launch_fit(jobs, kind, ctx->stream);
+// runs the three regression passes of the jobs (all raw or all level fits)
+clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw) {
+    clim::launch_fit(jobs, raw, ctx->stream);
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
 }
 
-clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& src, const float* nlf_in, const float* mult) {
+clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& m, const Img& v, const float* nlf_in,
+                          const float* mult) {
     clim::FitJob jb{};
-    jb.src = src;
+    jb.m = m;
+    jb.v = v;
     jb.st = ctx->fit + slot;
     jb.partials = ctx->partials + (size_t)slot * clim::kFitBlocksPerJob * kFitK;
     for (int k = 0; k < 8; k++) {
@@ -302,11 +307,17 @@ extern "C" clim_status clim_compute_raw_noise_statistics(clim_ctx* ctx, const cl
     if (!ctx || !nlf_out) return ctx_fail(ctx, CLIM_EINVAL, "computeRawNoiseStatistics: null argument");
     if (bayer < 0 || bayer > 3) return ctx_fail(ctx, CLIM_EINVAL, "computeRawNoiseStatistics: bad bayer");
     CLIM_TRY(check_img(ctx, scaled, CLIM_FMT_R32F, "scaled"));
+    if ((scaled->width & 1) || (scaled->height & 1))
+        return ctx_fail(ctx, CLIM_EINVAL, "computeRawNoiseStatistics: odd mosaic size");
+    // rawNoiseStatistics into the context's mean/var images, then the fit
+    const int qw = scaled->width / 2, qh = scaled->height / 2;
+    CLIM_TRY(owned_alloc(ctx, ctx->op_mean, qw, qh, CLIM_FMT_RGBA32F));
+    CLIM_TRY(owned_alloc(ctx, ctx->op_var, qw, qh, CLIM_FMT_RGBA32F));
+    clim::launch_raw_stats(view(scaled), make_bayer(bayer), ctx->op_mean.img, ctx->op_var.img, ctx->stream);
     clim::FitJobs jobs{};
     jobs.njobs = 1;
-    jobs.j[0] = make_fit_job(ctx, 0, view(scaled), nullptr, nullptr);
-    jobs.j[0].b = make_bayer(bayer);
-    CLIM_TRY(run_fit(ctx, jobs, clim::FIT_RAW_F32));
+    jobs.j[0] = make_fit_job(ctx, 0, ctx->op_mean.img, ctx->op_var.img, nullptr, nullptr);
+    CLIM_TRY(run_fit(ctx, jobs, true));
     CLIM_TRY(read_fit(ctx, 0, 1));
     if (ctx->fit_host[0].status == 1)
         return ctx_fail(ctx, CLIM_ENUMERIC, "computeRawNoiseStatistics: no sample in range");
@@ -395,10 +406,12 @@ extern "C" clim_status clim_noise_statistics(clim_ctx* ctx, const clim_image* in
 extern "C" clim_status clim_compute_noise_statistics(clim_ctx* ctx, const clim_image* in, float nlf_out[6]) {
     if (!ctx || !nlf_out) return ctx_fail(ctx, CLIM_EINVAL, "computeNoiseStatistics: null argument");
     CLIM_TRY(check_img(ctx, in, CLIM_FMT_RGBA32F, "in"));
+    CLIM_TRY(owned_alloc(ctx, ctx->op_stats, in->width, in->height, CLIM_FMT_RGBA32F));
+    clim::launch_noise_stats(view(in), ctx->op_stats.img, ctx->stream);
     clim::FitJobs jobs{};
     jobs.njobs = 1;
-    jobs.j[0] = make_fit_job(ctx, 1, view(in), nullptr, nullptr);
-    CLIM_TRY(run_fit(ctx, jobs, clim::FIT_LEVEL));
+    jobs.j[0] = make_fit_job(ctx, 1, ctx->op_stats.img, ctx->op_stats.img, nullptr, nullptr);
+    CLIM_TRY(run_fit(ctx, jobs, false));
     CLIM_TRY(read_fit(ctx, 1, 1));
     if (ctx->fit_host[1].status == 1)
         return ctx_fail(ctx, CLIM_ENUMERIC, "computeNoiseStatistics: no sample in range");

@@ -30,7 +30,7 @@ struct Workspace {
 };
 
 constexpr int kFitJobs = 1 + CLIM_MAX_LEVELS;  // raw + levels
-constexpr int kFitK = 17;                       // largest accumulator count (raw: 4 sums x 4 channels + count)
+constexpr int kFitK = 21;                       // largest accumulator count (raw first pass: 5 sums x 4 channels + count)
 
 enum Stage {
     ST_SCALE = 0, ST_RAW_STATS, ST_RAW_FIT, ST_GREEN, ST_RED_BLUE, ST_DESPECKLE,
@@ -56,6 +56,8 @@ struct clim_ctx {
     // scratch of the stand-alone guided filter
     clim::GuidedScratch op_guided{};
     OwnedImg op_guided_imgs[8];
+    // statistics images of the stand-alone regression operators
+    OwnedImg op_mean, op_var, op_stats;
     // stage profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;

@@ -10,27 +10,22 @@ struct FitState {
     double A[4], B[4], err2[4];
     unsigned long long n0;
     int status;            // 0 fitted, 1 no sample in range (fallback), 2 no inliers (first fit)
-    unsigned int counter;  // blocks finished in the current pass
     float out[8];          // effective model: A[0..nch), B[0..nch)
     float cal[8];          // out * mult (calibrated per-level model)
 };
 
-// what a fit reads: the uint16 mosaic (scaled on the fly), the scaled FP32
-// mosaic, or an RGBA32F pyramid level
-enum FitKind { FIT_RAW_U16 = 0, FIT_RAW_F32 = 1, FIT_LEVEL = 2 };
-
 constexpr int kFitBlocksPerJob = 1024;
 
+// A fit streams over statistics images: raw fits over the mean and var images
+// of rawNoiseStatistics (float4 R, G, B, G2 each), level fits over the
+// noiseStatistics image (mean Y, var Y, var Cb, var Cr; v unused).
 struct FitJob {
-    Img src;
-    Bayer b;              // raw kinds
-    float4 smul;          // FIT_RAW_U16: scale_mul
-    float black;          // FIT_RAW_U16: normalised black level
-    int tiles_x, tiles_y, nblocks, block_base;  // set by launch_fit
+    Img m, v;
+    int nblocks, block_base;  // set by launch_fit
     FitState* st;
-    clim_fx128* partials; // nblocks x (4*nch + 1)
-    float nlf_in[8];      // fallback model (written to out when no sample is in range)
-    float mult[8];        // calibration multipliers
+    clim_fx128* partials;     // nblocks x accumulator count
+    float nlf_in[8];          // fallback model (written to out when no sample is in range)
+    float mult[8];            // calibration multipliers
 };
 
 struct FitJobs {
@@ -38,7 +33,7 @@ struct FitJobs {
     int njobs;
 };
 
-void launch_fit(FitJobs& jobs, int kind, hipStream_t st);
+void launch_fit(FitJobs& jobs, bool raw, hipStream_t st);
 
 // ------------------------------------------------------------ demosaic stages
 void launch_scale_raw(const Img& raw, const Img& out, const Bayer& b, const float smul[4], float black,

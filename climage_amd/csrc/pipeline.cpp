@@ -21,8 +21,9 @@
 using clim::Img;
 using clim::ParamRef;
 
-clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, int kind);
-clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& src, const float* nlf_in, const float* mult);
+clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw);
+clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& m, const Img& v, const float* nlf_in,
+                          const float* mult);
 clim_status guided_scratch(clim_ctx* ctx, OwnedImg* imgs, int w, int h, clim::GuidedScratch* g);
 int inverse3x3_scaled(const float mf[9], double scale, float out[9]);
 
@@ -38,8 +39,11 @@ static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int
     CLIM_TRY(owned_alloc(ctx, ws.green, w, h, CLIM_FMT_R32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbA, w, h, CLIM_FMT_RGBA32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbB, w, h, CLIM_FMT_RGBA32F));
+    CLIM_TRY(owned_alloc(ctx, ws.raw_mean, w / 2, h / 2, CLIM_FMT_RGBA32F));
+    CLIM_TRY(owned_alloc(ctx, ws.raw_var, w / 2, h / 2, CLIM_FMT_RGBA32F));
     for (int i = 0, scale = 1; i < levels; i++, scale *= 2) {
         const int lw = w / scale, lh = h / scale;
+        CLIM_TRY(owned_alloc(ctx, ws.stats[i], lw, lh, CLIM_FMT_RGBA32F));
         if (i > 0) {
             CLIM_TRY(owned_alloc(ctx, ws.pyr[i], lw, lh, CLIM_FMT_RGBA32F));
             CLIM_TRY(owned_alloc(ctx, ws.den[i], lw, lh, CLIM_FMT_RGBA32F));
@@ -84,15 +88,20 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         const float mult[8] = {lm, cm, cm, lm, cm, cm, 1.0f, 1.0f};
         float in8[8] = {0};
         std::memcpy(in8, nlf_in[i], 6 * sizeof(float));
-        jobs.j[i] = make_fit_job(ctx, 1 + i, lvl[i], in8, mult);
+        const Img sv = sub_view(ws.stats[i].img, lvl[i].w, lvl[i].h);
+        jobs.j[i] = make_fit_job(ctx, 1 + i, sv, sv, in8, mult);
     }
     jobs.njobs = levels;
     CLIM_TRY_HIP(ctx, hipGetLastError());
+    long long nlev = 0;
+    for (int i = 0; i < levels; i++) nlev += (long long)lvl[i].w * lvl[i].h;
     {
-        long long n = 0;
-        for (int i = 0; i < levels; i++) n += (long long)lvl[i].w * lvl[i].h;
-        ProfScope ps(ctx, ST_LEVEL_FIT, n);
-        CLIM_TRY(run_fit(ctx, jobs, clim::FIT_LEVEL));
+        ProfScope ps(ctx, ST_LEVEL_STATS, nlev);
+        for (int i = 0; i < levels; i++) clim::launch_noise_stats(lvl[i], jobs.j[i].m, st);
+    }
+    {
+        ProfScope ps(ctx, ST_LEVEL_FIT, nlev);
+        CLIM_TRY(run_fit(ctx, jobs, false));
     }
 
     // synthesis, coarse to fine
@@ -210,16 +219,17 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
     // rawNlf = B; the input rawNlf is the fallback when no quad is in range
     CLIM_TRY_HIP(ctx, hipGetLastError());
     {
+        ProfScope ps(ctx, ST_RAW_STATS, npx / 4);
+        clim::launch_raw_stats(ws.scaled.img, b, ws.raw_mean.img, ws.raw_var.img, st);
+    }
+    {
         ProfScope ps(ctx, ST_RAW_FIT, npx / 4);
         clim::FitJobs jobs{};
         float in8[8] = {0, 0, 0, 0, p->noiseModel.rawNlf[0], p->noiseModel.rawNlf[1], p->noiseModel.rawNlf[2],
                         p->noiseModel.rawNlf[3]};
         jobs.njobs = 1;
-        jobs.j[0] = make_fit_job(ctx, 0, view(raw), in8, nullptr);
-        jobs.j[0].b = b;
-        jobs.j[0].smul = make_float4(p->scale_mul[0], p->scale_mul[1], p->scale_mul[2], p->scale_mul[3]);
-        jobs.j[0].black = p->black_level / 65535.0f;
-        CLIM_TRY(run_fit(ctx, jobs, clim::FIT_RAW_U16));
+        jobs.j[0] = make_fit_job(ctx, 0, ws.raw_mean.img, ws.raw_var.img, in8, nullptr);
+        CLIM_TRY(run_fit(ctx, jobs, true));
     }
     const float* rawfit = ctx->fit[0].out;  // device: A[4], B[4]
     // interpolateGreen(rawNlf[1]) and interpolateRedBlue((rawNlf[0]+rawNlf[2])/2)

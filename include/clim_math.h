@@ -206,8 +206,6 @@ CLIM_HD float clim_u01_open0(uint32_t r) { return (float)((r >> 8) + 1u) * 5.960
  * reduction order: GPU block trees, CPU threads or a sequential loop. */
 typedef struct { uint64_t hi, lo; } clim_fx128;
 
-#define CLIM_FX_FRAC_BITS 90
-
 CLIM_HD clim_fx128 clim_fx_from_double(double t) {
     clim_fx128 r;
     /* NaN, non-positive and >= 2^37 (never a regression term) count as 0 */
@@ -219,96 +217,11 @@ CLIM_HD clim_fx128 clim_fx_from_double(double t) {
     return r;
 }
 
-/* m * 2^s as a 128-bit integer, truncated (s may be negative) */
-CLIM_HD clim_fx128 clim_fx_shift(uint64_t m, int s) {
-    clim_fx128 r;
-    if (s >= 0) {
-        if (s >= 128) { r.hi = 0; r.lo = 0; }               /* out of range: not reached for |t| < 2^37 */
-        else if (s >= 64) { r.hi = m << (s - 64); r.lo = 0; }
-        else if (s == 0) { r.hi = 0; r.lo = m; }
-        else { r.hi = m >> (64 - s); r.lo = m << s; }
-    } else {
-        r.hi = 0;
-        r.lo = -s >= 64 ? 0 : m >> (-s);
-    }
-    return r;
-}
-
-/* floor(f * 2^90) for a float f >= 0, by integer arithmetic: equal to
- * clim_fx_from_double((double)f) */
-CLIM_HD clim_fx128 clim_fx_from_float(float f) {
-    uint32_t b = clim_f2u(f) & 0x7fffffffu;
-    int e = (int)(b >> 23);
-    uint64_t m = b & 0x7fffffu;
-    if ((clim_f2u(f) >> 31) || e >= 127 + 37) { clim_fx128 z = {0, 0}; return z; }
-    if (e) m |= 0x800000u; else e = 1;
-    return clim_fx_shift(m, e - 150 + CLIM_FX_FRAC_BITS);
-}
-
-/* floor(f * g * 2^90) for floats f, g >= 0: the 48-bit product of the
- * significands is exact, so this equals clim_fx_from_double((double)f * g) */
-CLIM_HD clim_fx128 clim_fx_from_product(float f, float g) {
-    uint32_t bf = clim_f2u(f) & 0x7fffffffu, bg = clim_f2u(g) & 0x7fffffffu;
-    int ef = (int)(bf >> 23), eg = (int)(bg >> 23);
-    if (((clim_f2u(f) ^ clim_f2u(g)) >> 31 && bf && bg) || ef == 255 || eg == 255 || ef + eg >= 254 + 37) {
-        clim_fx128 z = {0, 0};
-        return z;
-    }
-    uint64_t mf = bf & 0x7fffffu, mg = bg & 0x7fffffu;
-    if (ef) mf |= 0x800000u; else ef = 1;
-    if (eg) mg |= 0x800000u; else eg = 1;
-    clim_fx128 r = clim_fx_shift(mf * mg, ef + eg - 300 + CLIM_FX_FRAC_BITS);
-    if (r.hi >> 63) { r.hi = 0; r.lo = 0; }  /* product >= 2^37 */
-    return r;
-}
-
-/* floor(t * 2^90) for a double t >= 0 from its bits: equal to clim_fx_from_double(t) */
-CLIM_HD clim_fx128 clim_fx_from_double_bits(double t) {
-    uint64_t b;
-    __builtin_memcpy(&b, &t, 8);
-    int e = (int)((b & 0x7fffffffffffffffull) >> 52);
-    if ((b >> 63) || e >= 1023 + 37) { clim_fx128 z = {0, 0}; return z; }
-    uint64_t m = b & 0xfffffffffffffull;
-    if (e) m |= 0x10000000000000ull; else e = 1;
-    return clim_fx_shift(m, e - 1075 + CLIM_FX_FRAC_BITS);
-}
-
-/* ---- regression terms as exact integers
- * An in-range mean m lies in [0.001, 0.5]; every float there is a multiple
- * of 2^-33 (0.001 > 2^-10), so m * 2^33 is an exact uint32 and its square an
- * exact uint64.  A variance v in [0, 0.001] is quantised to floor(v * 2^64)
- * (at most 5.4e-20 below v).  All regression sums are then sums of exact
- * integers: s_x = sum(mq) 2^-33, s_xx = sum(mq^2) 2^-66, s_y = sum(vq) 2^-64,
- * s_xy = sum(mq vq) 2^-97. */
-CLIM_HD uint32_t clim_q_mean(float m) { return (uint32_t)(m * 8589934592.0f); }
-
-CLIM_HD uint64_t clim_q_var(float v) {
-    float x = v * 18446744073709551616.0f;               /* exact: v * 2^64 < 2^55 */
-    uint32_t hi = (uint32_t)(x * 2.3283064365386962890625e-10f); /* floor(x / 2^32) */
-    uint32_t lo = (uint32_t)(x - (float)hi * 4294967296.0f);      /* exact remainder */
-    return ((uint64_t)hi << 32) | lo;
-}
-
 CLIM_HD clim_fx128 clim_fx_u64(uint64_t a) {
     clim_fx128 r;
     r.hi = 0;
     r.lo = a;
     return r;
-}
-
-/* a * b for a < 2^32, b < 2^64 as 128 bits */
-CLIM_HD clim_fx128 clim_fx_mul32x64(uint32_t a, uint64_t b) {
-    uint64_t lo = (uint64_t)a * (uint32_t)b;
-    uint64_t hp = (uint64_t)a * (uint32_t)(b >> 32);
-    clim_fx128 r;
-    r.lo = lo + (hp << 32);
-    r.hi = (hp >> 32) + (r.lo < lo ? 1u : 0u);
-    return r;
-}
-
-/* (hi 2^64 + lo) * scale, scale a power of two */
-CLIM_HD double clim_fx_to_double_scaled(clim_fx128 a, double scale) {
-    return ((double)a.hi * 18446744073709551616.0 + (double)a.lo) * scale;
 }
 
 CLIM_HD clim_fx128 clim_fx_add(clim_fx128 a, clim_fx128 b) {
@@ -318,12 +231,63 @@ CLIM_HD clim_fx128 clim_fx_add(clim_fx128 a, clim_fx128 b) {
     return r;
 }
 
+/* (hi 2^64 + lo) * scale, scale a power of two */
+CLIM_HD double clim_fx_to_double_scaled(clim_fx128 a, double scale) {
+    return ((double)a.hi * 18446744073709551616.0 + (double)a.lo) * scale;
+}
+
 CLIM_HD double clim_fx_to_double(clim_fx128 a) {
     /* hi*2^64 + lo, scaled by 2^-90; the two parts are converted separately,
      * so the result is the correctly rounded sum of two exact-ish doubles */
     double hi = (double)a.hi * 1.490116119384765625e-08;    /* hi*2^64*2^-90 = hi*2^-26 */
     double lo = (double)a.lo * 8.077935669463160887416100e-28; /* 2^-90 */
     return hi + lo;
+}
+
+/* ------------------------------------- noise-model regression, FP32 path
+ * The least-squares fits of the noise model (var = A + B * mean) take their
+ * samples on a fixed grid: an in-range mean m in [0.001, 0.5] as
+ * mq = floor(m * 2^32) <= 2^31 and a variance v in [0, 0.001] as
+ * vq = floor(v * 2^41) < 2^32 (grid errors below 2.4e-10 and 4.6e-13, far
+ * below the FP32 error of the statistics themselves).  Every regression term
+ * (mq, vq, mq^2, mq vq, vq^2) is then an integer below 2^63 and every sum an
+ * exact 128-bit integer, whatever the summation order. */
+CLIM_HD uint32_t clim_q32_mean(float m) { return (uint32_t)(m * 4294967296.0f); }
+CLIM_HD uint32_t clim_q32_var(float v) { return (uint32_t)(v * 2199023255552.0f); }
+
+/* the exact sums of one channel */
+typedef struct { clim_fx128 sx, sy, sxx, sxy, svv; } clim_qsums;
+
+/* B = max((N s_xy - s_x s_y) / (N s_xx - s_x^2), 1e-8), A = max((s_y - B s_x) / N, 1e-8)
+ * with s_x = sum(m), s_y = sum(v), s_xx = sum(m^2), s_xy = sum(m v) of the grid values */
+CLIM_HD void clim_fit_solve_q(const clim_qsums* q, unsigned long long n, double* A, double* B) {
+    const double N = (double)n;
+    const double s_x = clim_fx_to_double_scaled(q->sx, 0x1p-32);
+    const double s_y = clim_fx_to_double_scaled(q->sy, 0x1p-41);
+    const double s_xx = clim_fx_to_double_scaled(q->sxx, 0x1p-64);
+    const double s_xy = clim_fx_to_double_scaled(q->sxy, 0x1p-73);
+    const double b = fmax((N * s_xy - s_x * s_y) / (N * s_xx - s_x * s_x), 1e-8);
+    *B = b;
+    *A = fmax((s_y - b * s_x) / N, 1e-8);
+}
+
+/* sum over the samples of (A + B m - v)^2 of the grid values, expanded into
+ * the exact moment sums:  N A^2 + B^2 s_xx + s_vv + 2AB s_x - 2A s_y - 2B s_xy
+ * (evaluated in this order in FP64, clamped at 0) */
+CLIM_HD double clim_fit_rss_q(const clim_qsums* q, unsigned long long n, double A, double B) {
+    const double s_x = clim_fx_to_double_scaled(q->sx, 0x1p-32);
+    const double s_y = clim_fx_to_double_scaled(q->sy, 0x1p-41);
+    const double s_xx = clim_fx_to_double_scaled(q->sxx, 0x1p-64);
+    const double s_xy = clim_fx_to_double_scaled(q->sxy, 0x1p-73);
+    const double s_vv = clim_fx_to_double_scaled(q->svv, 0x1p-82);
+    const double a2 = 2.0 * A, b2 = 2.0 * B;
+    double r = (double)n * (A * A);
+    r = r + (B * B) * s_xx;
+    r = r + s_vv;
+    r = r + (a2 * B) * s_x;
+    r = r - a2 * s_y;
+    r = r - b2 * s_xy;
+    return r > 0.0 ? r : 0.0;
 }
 
 #endif /* CLIM_MATH_H */

@@ -173,9 +173,12 @@ void P(raw_noise_statistics)(const real* raw, int w, int h, int bayer, real* mea
  * for a second fit.  Returns 0 on success, 1 if no sample is in range
  * (out untouched), 2 if the refit had no inliers (first fit returned).
  * out = A[0..nch), B[0..nch).
- * Sums are exact: in the FP32 build of integer terms (means * 2^33 and
- * variances quantised to 2^-64, see clim_q_mean / clim_q_var), in the FP64
- * build of the 2^-90 fixed-point values of the FP64 terms. */
+ * Sums are exact.  The FP64 build follows the reference's three passes
+ * (sums, squared residuals, sums over inliers) on the 2^-90 fixed-point values
+ * of the FP64 terms.  The FP32 build (the definition the HIP kernels follow)
+ * takes the sums over the grid values of clim_q32_mean / clim_q32_var, adds
+ * the sum of squared variances, and gets the sum of squared residuals from
+ * those moments (clim_fit_rss_q) instead of a second pass over the samples. */
 #define FIT_MAXCH 4
 
 /* loads sample i; returns 1 when every mean is in [0.001, 0.5] and every
@@ -192,9 +195,19 @@ static inline int fit_sample(int nch, int m_nch, const real* m, int m_step, cons
     return ok;
 }
 
+/* 1 when (A + B*m - v)^2 <= err2 in every channel (cpp/pyramidal_denoise.cpp:324-343) */
+static inline int fit_inlier(int nch, const double* mm, const double* vv, const double* A, const double* B,
+                             const double* err2) {
+    for (int c = 0; c < nch; c++) {
+        double diff = (A[c] + B[c] * mm[c]) - vv[c];
+        if (!(diff * diff <= err2[c])) return 0;
+    }
+    return 1;
+}
+
+#if ORACLE_DOUBLE
 /* sums (s_x, s_y, s_xx, s_xy) per channel over in-range samples; with
- * err2 != NULL only over inliers, (A + B*m - v)^2 <= err2 in every channel
- * (cpp/pyramidal_denoise.cpp:324-343) */
+ * err2 != NULL only over inliers */
 static long fit_sums(int nch, int m_nch, const real* m, int m_step, const real* v, int v_step, long count,
                      const double* A, const double* B, const double* err2, clim_fx128* acc) {
     long n = 0;
@@ -208,29 +221,12 @@ static long fit_sums(int nch, int m_nch, const real* m, int m_step, const real* 
         for (long i = 0; i < count; i++) {
             double mm[FIT_MAXCH], vv[FIT_MAXCH];
             if (!fit_sample(nch, m_nch, m, m_step, v, v_step, i, mm, vv)) continue;
-            if (err2) {
-                int inlier = 1;
-                for (int c = 0; c < nch; c++) {
-                    double diff = (A[c] + B[c] * mm[c]) - vv[c];
-                    if (!(diff * diff <= err2[c])) inlier = 0;
-                }
-                if (!inlier) continue;
-            }
+            if (err2 && !fit_inlier(nch, mm, vv, A, B, err2)) continue;
             for (int c = 0; c < nch; c++) {
-#if ORACLE_DOUBLE
                 l[4 * c + 0] = clim_fx_add(l[4 * c + 0], clim_fx_from_double(mm[c]));
                 l[4 * c + 1] = clim_fx_add(l[4 * c + 1], clim_fx_from_double(vv[c]));
                 l[4 * c + 2] = clim_fx_add(l[4 * c + 2], clim_fx_from_double(mm[c] * mm[c]));
                 l[4 * c + 3] = clim_fx_add(l[4 * c + 3], clim_fx_from_double(mm[c] * vv[c]));
-#else
-                /* exact integer terms (clim_q_mean, clim_q_var) */
-                const uint32_t mq = clim_q_mean((float)mm[c]);
-                const uint64_t vq = clim_q_var((float)vv[c]);
-                l[4 * c + 0] = clim_fx_add(l[4 * c + 0], clim_fx_u64(mq));
-                l[4 * c + 1] = clim_fx_add(l[4 * c + 1], clim_fx_u64(vq));
-                l[4 * c + 2] = clim_fx_add(l[4 * c + 2], clim_fx_u64((uint64_t)mq * mq));
-                l[4 * c + 3] = clim_fx_add(l[4 * c + 3], clim_fx_mul32x64(mq, vq));
-#endif
             }
             ln++;
         }
@@ -272,17 +268,10 @@ static void fit_residuals(int nch, int m_nch, const real* m, int m_step, const r
 static void fit_solve(int nch, long n, const clim_fx128* acc, double* A, double* B) {
     double N = (double)n;
     for (int c = 0; c < nch; c++) {
-#if ORACLE_DOUBLE
         double s_x = clim_fx_to_double(acc[4 * c + 0]);
         double s_y = clim_fx_to_double(acc[4 * c + 1]);
         double s_xx = clim_fx_to_double(acc[4 * c + 2]);
         double s_xy = clim_fx_to_double(acc[4 * c + 3]);
-#else
-        double s_x = clim_fx_to_double_scaled(acc[4 * c + 0], 0x1p-33);
-        double s_y = clim_fx_to_double_scaled(acc[4 * c + 1], 0x1p-64);
-        double s_xx = clim_fx_to_double_scaled(acc[4 * c + 2], 0x1p-66);
-        double s_xy = clim_fx_to_double_scaled(acc[4 * c + 3], 0x1p-97);
-#endif
         B[c] = fmax((N * s_xy - s_x * s_y) / (N * s_xx - s_x * s_x), 1e-8);
         A[c] = fmax((s_y - B[c] * s_x) / N, 1e-8);
     }
@@ -306,6 +295,69 @@ int P(nlf_fit)(int nch, int m_nch, const real* m, int m_step, const real* v, int
     for (int c = 0; c < nch; c++) { out[c] = (float)A[c]; out[nch + c] = (float)B[c]; }
     return rc;
 }
+#else
+/* exact sums of the grid terms per channel over in-range samples; with
+ * err2 != NULL only over inliers (the inlier test uses the float values) */
+static long fit_qsums(int nch, int m_nch, const real* m, int m_step, const real* v, int v_step, long count,
+                      const double* A, const double* B, const double* err2, clim_qsums* acc) {
+    long n = 0;
+    const clim_fx128 z = {0, 0};
+    for (int c = 0; c < FIT_MAXCH; c++) acc[c].sx = acc[c].sy = acc[c].sxx = acc[c].sxy = acc[c].svv = z;
+#pragma omp parallel
+    {
+        clim_qsums l[FIT_MAXCH];
+        long ln = 0;
+        for (int c = 0; c < FIT_MAXCH; c++) l[c].sx = l[c].sy = l[c].sxx = l[c].sxy = l[c].svv = z;
+#pragma omp for schedule(static)
+        for (long i = 0; i < count; i++) {
+            double mm[FIT_MAXCH], vv[FIT_MAXCH];
+            if (!fit_sample(nch, m_nch, m, m_step, v, v_step, i, mm, vv)) continue;
+            if (err2 && !fit_inlier(nch, mm, vv, A, B, err2)) continue;
+            for (int c = 0; c < nch; c++) {
+                const uint64_t mq = clim_q32_mean((float)mm[c]), vq = clim_q32_var((float)vv[c]);
+                l[c].sx = clim_fx_add(l[c].sx, clim_fx_u64(mq));
+                l[c].sy = clim_fx_add(l[c].sy, clim_fx_u64(vq));
+                l[c].sxx = clim_fx_add(l[c].sxx, clim_fx_u64(mq * mq));
+                l[c].sxy = clim_fx_add(l[c].sxy, clim_fx_u64(mq * vq));
+                l[c].svv = clim_fx_add(l[c].svv, clim_fx_u64(vq * vq));
+            }
+            ln++;
+        }
+#pragma omp critical
+        {
+            for (int c = 0; c < FIT_MAXCH; c++) {
+                acc[c].sx = clim_fx_add(acc[c].sx, l[c].sx);
+                acc[c].sy = clim_fx_add(acc[c].sy, l[c].sy);
+                acc[c].sxx = clim_fx_add(acc[c].sxx, l[c].sxx);
+                acc[c].sxy = clim_fx_add(acc[c].sxy, l[c].sxy);
+                acc[c].svv = clim_fx_add(acc[c].svv, l[c].svv);
+            }
+            n += ln;
+        }
+    }
+    return n;
+}
+
+int P(nlf_fit)(int nch, int m_nch, const real* m, int m_step, const real* v, int v_step, long count,
+               float* out) {
+    double A[FIT_MAXCH], B[FIT_MAXCH], err2[FIT_MAXCH];
+    clim_qsums acc[FIT_MAXCH];
+    long n0 = fit_qsums(nch, m_nch, m, m_step, v, v_step, count, NULL, NULL, NULL, acc);
+    if (n0 == 0) return 1;
+    for (int c = 0; c < nch; c++) {
+        clim_fit_solve_q(&acc[c], (unsigned long long)n0, &A[c], &B[c]);
+        err2[c] = sqrt(clim_fit_rss_q(&acc[c], (unsigned long long)n0, A[c], B[c]) / (double)n0);
+    }
+    long n1 = fit_qsums(nch, m_nch, m, m_step, v, v_step, count, A, B, err2, acc);
+    int rc = 2;
+    if (n1 > 0) {
+        for (int c = 0; c < nch; c++) clim_fit_solve_q(&acc[c], (unsigned long long)n1, &A[c], &B[c]);
+        rc = 0;
+    }
+    for (int c = 0; c < nch; c++) { out[c] = (float)A[c]; out[nch + c] = (float)B[c]; }
+    return rc;
+}
+#endif
 
 /* --------------------------------------------------------- interpolateGreen
  * cl/demosaic.cl:96-129 (sobel, bilateralSobel) and :160-227. */

@@ -47,23 +47,20 @@ double om_fx_sum(const double* t, long n, uint64_t* hi, uint64_t* lo) {
     return clim_fx_to_double(s);
 }
 
-/* the integer conversions must equal clim_fx_from_double on the exact value:
- * out[4*i .. 4*i+3] = (hi, lo) of fx_from_float(f[i]) and fx_from_double(f[i]) */
-void om_fx_float_check(const float* f, uint64_t* out, long n) {
+/* the grid terms of one (mean, variance) pair: out[2*i ..] = mq, vq */
+void om_q32_terms(const float* m, const float* v, uint32_t* out, long n) {
     for (long i = 0; i < n; i++) {
-        clim_fx128 a = clim_fx_from_float(f[i]), b = clim_fx_from_double((double)f[i]);
-        out[4 * i] = a.hi; out[4 * i + 1] = a.lo; out[4 * i + 2] = b.hi; out[4 * i + 3] = b.lo;
+        out[2 * i] = clim_q32_mean(m[i]);
+        out[2 * i + 1] = clim_q32_var(v[i]);
     }
 }
-void om_fx_product_check(const float* f, const float* g, uint64_t* out, long n) {
-    for (long i = 0; i < n; i++) {
-        clim_fx128 a = clim_fx_from_product(f[i], g[i]), b = clim_fx_from_double((double)f[i] * (double)g[i]);
-        out[4 * i] = a.hi; out[4 * i + 1] = a.lo; out[4 * i + 2] = b.hi; out[4 * i + 3] = b.lo;
-    }
-}
-void om_fx_double_bits_check(const double* t, uint64_t* out, long n) {
-    for (long i = 0; i < n; i++) {
-        clim_fx128 a = clim_fx_from_double_bits(t[i]), b = clim_fx_from_double(t[i]);
-        out[4 * i] = a.hi; out[4 * i + 1] = a.lo; out[4 * i + 2] = b.hi; out[4 * i + 3] = b.lo;
-    }
+
+/* clim_fit_solve_q and clim_fit_rss_q of sums given as (hi, lo) pairs in the
+ * order sx, sy, sxx, sxy, svv; out = A, B, rss */
+void om_fit_q(const uint64_t* sums, unsigned long long n, double* out) {
+    clim_qsums q;
+    clim_fx128* f[5] = {&q.sx, &q.sy, &q.sxx, &q.sxy, &q.svv};
+    for (int k = 0; k < 5; k++) { f[k]->hi = sums[2 * k]; f[k]->lo = sums[2 * k + 1]; }
+    clim_fit_solve_q(&q, n, &out[0], &out[1]);
+    out[2] = clim_fit_rss_q(&q, n, out[0], out[1]);
 }

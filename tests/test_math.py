@@ -111,22 +111,57 @@ def test_fixed_point_is_exact_truncation(L):
     assert abs(s1 - float(np.sum(np.sort(t)))) <= 1e-12 * s1
 
 
-def test_integer_fixed_point_conversions_match_double_path(L):
-    rng = np.random.default_rng(4)
-    f = np.concatenate([rng.uniform(0, 0.5, 20000), rng.uniform(0, 1e-3, 20000), 10.0 ** rng.uniform(-45, 10, 20000),
-                        [0.0, -0.0, 1e-45, 1.17549435e-38, 0.5, 0.001]]).astype(np.float32)
-    g = rng.permutation(f).astype(np.float32)
-    out = np.empty(4 * f.size, np.uint64)
-    L.om_fx_float_check(f.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), C.c_long(f.size))
-    o = out.reshape(-1, 4)
-    np.testing.assert_array_equal(o[:, :2], o[:, 2:])
-    L.om_fx_product_check(f.ctypes.data_as(C.c_void_p), g.ctypes.data_as(C.c_void_p),
-                          out.ctypes.data_as(C.c_void_p), C.c_long(f.size))
-    o = out.reshape(-1, 4)
-    np.testing.assert_array_equal(o[:, :2], o[:, 2:])
-    t = np.concatenate([rng.uniform(0, 1e-6, 20000) ** 2, 10.0 ** rng.uniform(-300, 10, 20000),
-                        [0.0, -0.0, 5e-324, 2.0 ** -90, 2.0 ** -91, np.nan]])
-    out = np.empty(4 * t.size, np.uint64)
-    L.om_fx_double_bits_check(t.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), C.c_long(t.size))
-    o = out.reshape(-1, 4)
-    np.testing.assert_array_equal(o[:, :2], o[:, 2:])
+def test_q32_terms_are_floors_on_the_grid(L):
+    """mq = floor(m 2^32) and vq = floor(v 2^41) against exact rationals, over
+    the in-range domain and its ends (mq <= 2^31, vq < 2^32)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(9)
+    m_min = np.float32(0.001)
+    if m_min < 0.001:
+        m_min = np.nextafter(m_min, np.float32(1))
+    v_max = np.float32(0.001)
+    if v_max > 0.001:
+        v_max = np.nextafter(v_max, np.float32(0))
+    m = np.concatenate([rng.uniform(0.001, 0.5, 3000), 10.0 ** rng.uniform(-3, np.log10(0.5), 3000)])
+    m = np.concatenate([np.clip(m, m_min, 0.5), [m_min, 0.5, np.nextafter(np.float32(0.5), np.float32(0)), 0.25]])
+    v = np.concatenate([rng.uniform(0, 1e-3, 3000), 10.0 ** rng.uniform(-40, -3, 3000)])
+    v = np.concatenate([np.clip(v, 0, v_max), [0.0, v_max, 1e-45, 2.0 ** -41]])
+    m, v = m.astype(np.float32), v.astype(np.float32)
+    out = np.empty(2 * m.size, np.uint32)
+    L.om_q32_terms(m.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+                   C.c_long(m.size))
+    o = out.reshape(-1, 2)
+    for i in range(m.size):
+        assert int(o[i, 0]) == int(Fraction(float(m[i])) * 2 ** 32), m[i]
+        assert int(o[i, 1]) == int(Fraction(float(v[i])) * 2 ** 41), v[i]
+    assert int(o[np.flatnonzero(m == 0.5)[0], 0]) == 2 ** 31
+    assert int(o[np.flatnonzero(v == v_max)[0], 1]) < 2 ** 32
+
+
+def test_fit_q_solve_and_moment_residuals(L):
+    """clim_fit_solve_q against least squares on the grid values, and
+    clim_fit_rss_q (moment expansion) against the residuals summed exactly."""
+    from fractions import Fraction
+    rng = np.random.default_rng(10)
+    for trial in range(6):
+        n = int(rng.integers(2, 3000))
+        mq = rng.integers(2 ** 22, 2 ** 31, n, dtype=np.int64)
+        slope, icpt = rng.uniform(1e-4, 5e-3), rng.uniform(1e-6, 5e-5)
+        vf = icpt + slope * mq / 2.0 ** 32 + rng.normal(0, 10.0 ** rng.uniform(-7, -5), n)
+        vq = np.clip(np.floor(vf * 2.0 ** 41), 0, 2 ** 32 - 1).astype(np.int64)
+        if trial == 5:  # samples exactly on a line: residuals vanish
+            vq = 1000 + 3 * mq // 2 ** 11
+        mi, vi = [int(x) for x in mq], [int(x) for x in vq]
+        sums = [sum(mi), sum(vi), sum(x * x for x in mi), sum(x * y for x, y in zip(mi, vi)), sum(y * y for y in vi)]
+        flat = np.array([w for t in sums for w in (t >> 64, t & (2 ** 64 - 1))], np.uint64)
+        out = np.empty(3, np.float64)
+        L.om_fit_q(flat.ctypes.data_as(C.c_void_p), C.c_ulonglong(n), out.ctypes.data_as(C.c_void_p))
+        A, B, rss = out
+        mg, vg = mq / 2.0 ** 32, vq / 2.0 ** 41
+        Bl, Al = np.polyfit(mg, vg, 1)
+        if Bl > 1e-8 and Al > 1e-8:
+            np.testing.assert_allclose([A, B], [Al, Bl], rtol=1e-8, atol=1e-14)
+        exact = sum((Fraction(A) + Fraction(B) * Fraction(x, 2 ** 32) - Fraction(y, 2 ** 41)) ** 2
+                    for x, y in zip(mi, vi))
+        scale = sum(Fraction(y, 2 ** 41) ** 2 for y in vi)
+        assert abs(Fraction(rss) - exact) <= Fraction(1, 10 ** 12) * scale + Fraction(1, 10 ** 30), trial

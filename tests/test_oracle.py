@@ -149,6 +149,26 @@ def test_fit_matches_numpy_three_pass(o64):
     np.testing.assert_allclose(fit[3:], B, rtol=2e-6)
 
 
+def test_fit_with_flat_half_patches(o32, o64):
+    """Flat patches at 0.5 give window means of exactly 0.5 with zero variance:
+    in range, and the largest mean term (2^32 at 2^-33 resolution)."""
+    rng = np.random.default_rng(6)
+    n = 4000
+    m = rng.uniform(0.01, 0.45, (n, 4))
+    v = 2e-5 + 1e-3 * m + rng.normal(0, 2e-6, (n, 4))
+    m[: n // 4] = 0.5
+    v[: n // 4] = 0.0
+    mean = m[None].astype(np.float32).astype(np.float64)
+    var = v[None].astype(np.float32).astype(np.float64)
+    rc64, f64 = o64.nlf_fit_raw(mean, var)
+    rc32, f32 = o32.nlf_fit_raw(mean.astype(np.float32), var.astype(np.float32))
+    A, B = fit3_numpy(mean[0], var[0])
+    assert rc64 == 0 and rc32 == 0
+    np.testing.assert_allclose(f64[:4], A, rtol=1e-5)
+    np.testing.assert_allclose(f64[4:], B, rtol=1e-5)
+    np.testing.assert_allclose(f32, f64, rtol=1e-5)
+
+
 def test_fit_empty_sample_reports_fallback(o64):
     mean = np.full((4, 4, 4), 0.9)  # every mean above 0.5
     var = np.full((4, 4, 4), 1e-4)
