@@ -7,9 +7,9 @@ full path (scale, raw noise fit, demosaic, highlights, YCbCr, despeckle,
 5-level bilateral pyramid with per-level noise fits, tone map to RGBA8) on
 all six frames; inputs are resident in HBM before timing starts.
 
-Prints one JSON line.  `roofline` is measured for the dominant stage with
-HIP events on the library's stream during the timed steps; `cpu_baseline`
-times the FP32 CPU restatement (the oracle) on one of the frames.
+Prints one JSON line.  `roofline` is measured for the dominant single-kernel
+stage with HIP events on the library's stream during the timed steps;
+`cpu_baseline` times the FP32 CPU restatement (the oracle) on the same frames.
 """
 import argparse
 import json
@@ -34,13 +34,13 @@ def stage_bytes(stage, texels, levels_px=None):
     per = {
         "scale_raw": 2 + 4,             # u16 in, f32 out (per pixel)
         "raw_stats": 16 + 32,           # 4 f32 texels in, mean+var RGBA out (per quad)
-        "raw_fit": 3 * 32,              # three passes over mean+var (per quad)
+        "raw_fit": 2 * 32,              # two passes over mean+var (per quad)
         "green": 4 + 4,
         "red_blue_ycc": 4 + 4 + 16,
         "despeckle": 16 + 16,
         "downsample": 4 * 16 + 16,      # per output texel: 4 input texels + 1 output
         "level_stats": 16 + 16,
-        "level_fit": 3 * 16,
+        "level_fit": 2 * 16,            # two passes over the statistics image
         "denoise": 16 + 16,
         "reassemble": 16 + 16 + 16 / 4 + 16 / 4,
         "tone": 16 + 4,
@@ -48,20 +48,64 @@ def stage_bytes(stage, texels, levels_px=None):
     return per[stage] * texels
 
 
-def cpu_baseline(w, h, bayer, a, seconds_cap=30.0):
+# stages that are one kernel (launched once per frame, or once per pyramid
+# level), so that their event timing is that kernel's launch duration
+SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green": "k_green",
+                 "red_blue_ycc": "k_red_blue", "despeckle": "k_despeckle", "downsample": "k_downsample",
+                 "level_stats": "k_noise_stats", "denoise": "k_bilateral", "reassemble": "k_reassemble",
+                 "tone": "k_tone"}
+
+
+def cpu_baseline(w, h, bayer, seconds_min=10.0):
+    """The FP32 CPU restatement on the bench frames (same sizes and noise
+    levels), as many as fit in about `seconds_min` seconds (at least one)."""
     from oracle import oracle as O
     import subprocess
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    raw = O.generate_bayer(w, h, bayer, 1000, 2000, a, a / 200.0, 64.0, 4095.0)
     o = O.Oracle(double=False)
     p = O.default_params(bayer, 64.0, 4095.0, levels=5)
-    t0 = time.perf_counter()
-    o.raw_to_rgb(raw, p)
-    dt = time.perf_counter() - t0
+    done, dt = 0, 0.0
+    for i, a in enumerate(NOISE_A):
+        raw = O.generate_bayer(w, h, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
+        t0 = time.perf_counter()
+        o.raw_to_rgb(raw, p)
+        dt += time.perf_counter() - t0
+        done += 1
+        if dt >= seconds_min:
+            break
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": w * h / dt / 1e6, "unit": "Mpix/s", "cores": cores, "kind": "port",
-            "sample": f"one {w}x{h} RGGB frame (a={a}) through the FP32 CPU restatement, "
-                      f"{dt:.2f} s, OpenMP row bands"}
+    return {"value": round(done * w * h / dt / 1e6, 2), "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "sample": f"{done} of the {len(NOISE_A)} bench frames ({w}x{h} RGGB) through the FP32 CPU "
+                      f"restatement in {dt:.1f} s, OpenMP row bands"}
+
+
+def measured_traffic(kernel, w, h):
+    """HBM bytes per dispatch of `kernel` (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy calibration) from the committed counter summary of the default
+    workload, or None for another workload or an unlisted kernel."""
+    path = os.path.join(ROOT, "profiles", "traffic_raw_to_rgb_12mp.json")
+    if (w, h) != (4032, 3024) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["kernels"].get(kernel)
+
+
+def copy_calibration(nbytes):
+    """hipMemset then hipMemcpy (device to device) of nbytes: the runtime's fill
+    kernel writes nbytes, its copy kernel reads and writes nbytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0 or \
+                hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("copy calibration failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
 
 
 def main():
@@ -73,6 +117,9 @@ def main():
     ap.add_argument("--height", type=int, default=3024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device fill and device-to-device copy (reference kernels "
+                         "of known traffic for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -127,23 +174,28 @@ def main():
                 stages[name] = {"ms_total": pr.ms[i], "launches": int(pr.launches[i]),
                                 "texels": int(pr.texels[i])}
 
+    if args.copy_calibration:
+        copy_calibration(256 << 20)
+
     frames = args.steps * len(raws)
     mpix = frames * w * h / dt / 1e6
     roofline = None
     if stages:
-        dom = max(stages, key=lambda k: stages[k]["ms_total"])
+        dom = max((k for k in stages if k in SINGLE_KERNEL), key=lambda k: stages[k]["ms_total"])
         s = stages[dom]
         avg_s = s["ms_total"] / s["launches"] / 1e3
         bytes_per_launch = stage_bytes(dom, s["texels"] / s["launches"])
         ach = bytes_per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                    "avg_launch_us": round(avg_s * 1e6, 2), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": SINGLE_KERNEL[dom],
+                    "stage": dom, "avg_launch_us": round(avg_s * 1e6, 2),
+                    "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+        roofline["traffic"] = measured_traffic(SINGLE_KERNEL[dom], w, h)
         for k, v in stages.items():
             v["us_per_frame"] = round(v["ms_total"] * 1e3 / frames, 2)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(w, h, bayer, 0.002)
+        cpu = cpu_baseline(w, h, bayer)
 
     res = {
         "metric": "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP",

@@ -585,7 +585,14 @@ static const char* kStageNames[CLIM_PROFILE_STAGES] = {
 static constexpr size_t kProfEvents = 4096;
 
 ProfScope::ProfScope(clim_ctx* c, int stage, long long texels) : ctx(c) {
-    if (!ctx->prof || ctx->ev_next + 2 > ctx->ev_pool.size()) return;
+    if (!ctx->prof) return;
+    // the pool grows while a profile is being collected; events are reused
+    // after clim_profile_read
+    while (ctx->ev_next + 2 > ctx->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        ctx->ev_pool.push_back(e);
+    }
     ProfRec r{stage, texels, ctx->ev_pool[ctx->ev_next], ctx->ev_pool[ctx->ev_next + 1]};
     ctx->ev_next += 2;
     (void)hipEventRecord(r.a, ctx->stream);

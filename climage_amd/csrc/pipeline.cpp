@@ -95,9 +95,9 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
     CLIM_TRY_HIP(ctx, hipGetLastError());
     long long nlev = 0;
     for (int i = 0; i < levels; i++) nlev += (long long)lvl[i].w * lvl[i].h;
-    {
-        ProfScope ps(ctx, ST_LEVEL_STATS, nlev);
-        for (int i = 0; i < levels; i++) clim::launch_noise_stats(lvl[i], jobs.j[i].m, st);
+    for (int i = 0; i < levels; i++) {
+        ProfScope ps(ctx, ST_LEVEL_STATS, (long long)lvl[i].w * lvl[i].h);
+        clim::launch_noise_stats(lvl[i], jobs.j[i].m, st);
     }
     {
         ProfScope ps(ctx, ST_LEVEL_FIT, nlev);
