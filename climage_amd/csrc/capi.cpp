@@ -585,7 +585,7 @@ static const char* kStageNames[CLIM_PROFILE_STAGES] = {
 static constexpr size_t kProfEvents = 4096;
 
 ProfScope::ProfScope(clim_ctx* c, int stage, long long texels) : ctx(c) {
-    if (!ctx->prof) return;
+    if (!ctx->prof || !((ctx->prof_mask >> stage) & 1u)) return;
     // the pool grows while a profile is being collected; events are reused
     // after clim_profile_read
     while (ctx->ev_next + 2 > ctx->ev_pool.size()) {
@@ -612,6 +612,12 @@ extern "C" clim_status clim_profile_enable(clim_ctx* ctx, int32_t enable) {
         ctx->prof_recs.reserve(kProfEvents / 2);
     }
     ctx->prof = enable != 0;
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_profile_select(clim_ctx* ctx, uint32_t stage_mask) {
+    if (!ctx) return CLIM_EINVAL;
+    ctx->prof_mask = stage_mask;
     return CLIM_OK;
 }
 

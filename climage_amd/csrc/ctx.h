@@ -60,6 +60,7 @@ struct clim_ctx {
     OwnedImg op_mean, op_var, op_stats;
     // stage profiling
     bool prof = false;
+    uint32_t prof_mask = 0xffffffffu;  // stages that record events
     std::vector<hipEvent_t> ev_pool;
     std::vector<ProfRec> prof_recs;
     size_t ev_next = 0;

@@ -75,6 +75,23 @@ struct ParamRef {
     __device__ __forceinline__ float get(int i) const { return dev ? dev[i] : host[i]; }
 };
 
+// Workgroup -> tile order of a 2-D grid.  The dispatcher hands workgroups to
+// the device's 8 XCDs round robin (linear id mod 8) and each XCD has its own
+// L2, so in launch order horizontally adjacent tiles run on different XCDs and
+// every tile fetches its halo from HBM.  Renumbered, XCD x takes the x-th
+// contiguous run of tiles in row-major order: neighbouring tiles, and the halo
+// rows they share, meet in the same L2.  A bijection of [0, tiles).
+struct TileId {
+    int x, y;
+};
+__device__ __forceinline__ TileId xcd_tile() {
+    const int gx = gridDim.x, n = gx * gridDim.y;
+    const int lin = blockIdx.x + blockIdx.y * gx;
+    const int q = n >> 3, r = n & 7, xcd = lin & 7;
+    const int t = xcd * q + min(xcd, r) + (lin >> 3);
+    return TileId{t % gx, t / gx};
+}
+
 // Stages N elements into LDS with NT threads: every thread issues all of its
 // global loads before it waits for any (a loop with a run-time trip count
 // would wait for each load before issuing the next).

@@ -43,7 +43,8 @@ void launch_scale_raw(const Img& raw, const Img& out, const Bayer& b, const floa
 __global__ void __launch_bounds__(256) k_raw_stats(Img scaled, Bayer b, Img mean, Img var) {
     constexpr int T = 16, R = 2, S = T + 2 * R;
     __shared__ QuadTile<T> sm;
-    const int qx0 = blockIdx.x * T, qy0 = blockIdx.y * T;
+    const TileId tl = xcd_tile();
+    const int qx0 = tl.x * T, qy0 = tl.y * T;
     const int tid = threadIdx.y * T + threadIdx.x;
     stage<S * S, 256>(
         tid,
@@ -86,7 +87,8 @@ __global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int 
     __shared__ float sgx[GS_Y][GS_X + 1];
     __shared__ float sgy[GS_Y][GS_X + 1];
     __shared__ float sgl[GS_Y][GS_X + 1];
-    const int x0 = blockIdx.x * GT_X, y0 = blockIdx.y * GT_Y;
+    const TileId tl = xcd_tile();
+    const int x0 = tl.x * GT_X, y0 = tl.y * GT_Y;
     const int tid = threadIdx.y * 32 + threadIdx.x;
     stage<GR_X * GR_Y, 256>(
         tid, [&](int i) { return ld1(raw, x0 - 3 + i % GR_X, y0 - 3 + i / GR_X); },
@@ -114,67 +116,76 @@ __global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int 
     }
     __syncthreads();
     const float lumaVariance = mmax(lv.get(0), 1e-4f);
-#pragma unroll 1
-    for (int rr = 0; rr < 2; rr++) {
-        const int ty = threadIdx.y + rr * 8, tx = threadIdx.x;
-        const int x = x0 + tx, y = y0 + ty;
-        if (x >= green.w || y >= green.h) continue;
-        const int X = tx + 3, Y = ty + 3;  // raw tile coordinates
-        const int xg = (y & 1) == (gyo & 1) ? gxo + 1 : gxo;
-        float out;
-        if ((xg & 1) != (x & 1)) {
-            out = r[Y][X];
-        } else {
-            const float g_left = r[Y][X - 1], g_right = r[Y][X + 1];
-            const float g_up = r[Y - 1][X], g_down = r[Y + 1][X];
-            const float c_xy = r[Y][X];
-            const float c_left = r[Y][X - 2], c_right = r[Y][X + 2];
-            const float c_up = r[Y - 2][X], c_down = r[Y + 2][X];
-            const float c2_ave = (r[Y - 1][X - 1] + r[Y - 1][X + 1] + r[Y + 1][X - 1] + r[Y + 1][X + 1]) / 4.0f;
-            const float g_ave = (g_left + g_right + g_up + g_down) / 4.0f;
-            const float lumaStdDev = sqrtf(lumaVariance * g_ave);
+    // Thread t takes the horizontal pixel pair k = t % 16 of tile row t / 16:
+    // it interpolates the pair's non-green pixel (so every lane of a wave runs
+    // the same code) and copies the green one.
+    const int ty = tid >> 4, k = tid & 15;
+    const int y = y0 + ty;
+    if (y >= green.h) return;
+    const int cpar = ((y & 1) == (gyo & 1) ? gxo + 1 : gxo) & 1;  // x parity of the non-green site
+    const int tx = 2 * k + cpar;                                    // x0 is even
+    const int x = x0 + tx;
+    const int Y = ty + 3;  // raw tile coordinates
+    float out = 0.0f;
+    if (x < green.w) {
+        const int X = tx + 3;
+        const float g_left = r[Y][X - 1], g_right = r[Y][X + 1];
+        const float g_up = r[Y - 1][X], g_down = r[Y + 1][X];
+        const float c_xy = r[Y][X];
+        const float c_left = r[Y][X - 2], c_right = r[Y][X + 2];
+        const float c_up = r[Y - 2][X], c_down = r[Y + 2][X];
+        const float c2_ave = (r[Y - 1][X - 1] + r[Y - 1][X + 1] + r[Y + 1][X - 1] + r[Y + 1][X + 1]) / 4.0f;
+        const float g_ave = (g_left + g_right + g_up + g_down) / 4.0f;
+        const float lumaStdDev = sqrtf(lumaVariance * g_ave);
 
-            // bilateralSobel(), cl/demosaic.cl:114-129
-            const int SX = tx + 2, SY = ty + 2;  // sobel tile coordinates
-            const float cl = sgl[SY][SX];
-            // smoothstep(4 sigma, 16 sigma, d) with its division hoisted as a reciprocal
-            const float e0 = 4.0f * lumaStdDev, inv = 1.0f / (16.0f * lumaStdDev - 4.0f * lumaStdDev);
-            float sx = 0.0f, sy = 0.0f, wsum = 0.0f;
+        // bilateralSobel(), cl/demosaic.cl:114-129
+        const int SX = tx + 2, SY = ty + 2;  // sobel tile coordinates
+        const float cl = sgl[SY][SX];
+        // smoothstep(4 sigma, 16 sigma, d) with its division hoisted as a reciprocal
+        const float e0 = 4.0f * lumaStdDev, inv = 1.0f / (16.0f * lumaStdDev - 4.0f * lumaStdDev);
+        float sx = 0.0f, sy = 0.0f, wsum = 0.0f;
 #pragma unroll
-            for (int j = -2; j <= 2; j++) {
+        for (int j = -2; j <= 2; j++) {
 #pragma unroll
-                for (int i = -2; i <= 2; i++) {
-                    const float gx = sgx[SY + j][SX + i], gy = sgy[SY + j][SX + i];
-                    const float d = 0.05f * fabsf(sgl[SY + j][SX + i] - cl);
-                    const float t = clampf((d - e0) * inv, 0.0f, 1.0f);
-                    const float wt = 1.0f - t * t * (3.0f - 2.0f * t);
-                    sx += wt * gx;
-                    sy += wt * gy;
-                    wsum += wt;  // the x and y weight sums are equal
-                }
+            for (int i = -2; i <= 2; i++) {
+                const float gx = sgx[SY + j][SX + i], gy = sgy[SY + j][SX + i];
+                const float d = 0.05f * fabsf(sgl[SY + j][SX + i] - cl);
+                const float t = clampf((d - e0) * inv, 0.0f, 1.0f);
+                const float wt = 1.0f - t * t * (3.0f - 2.0f * t);
+                sx += wt * gx;
+                sy += wt * gy;
+                wsum += wt;  // the x and y weight sums are equal
             }
-            const float gradx = sx / wsum, grady = sy / wsum;
-
-            const float g_lf_x = (g_left + g_right) / 2.0f, g_lf_y = (g_up + g_down) / 2.0f;
-            const float g_hf_x = (2.0f * c_xy - (c_left + c_right)) / 4.0f;
-            const float g_hf_y = (2.0f * c_xy - (c_up + c_down)) / 4.0f;
-            const float whiteness =
-                clampf(mmin(c_xy, mmin(g_ave, c2_ave)) / mmax(c_xy, mmax(g_ave, c2_ave)), 0.0f, 1.0f);
-            const float glen = len2(gradx, grady);
-            const float lgt = smoothstep(lumaStdDev, 4.0f * lumaStdDev, glen);
-            const float hf_gain =
-                lgt * mmin(0.5f * whiteness + smoothstep(0.0f, 0.3f, glen / 1.41421356f), 1.0f);
-            const float g_est_x = g_lf_x + hf_gain * g_hf_x;
-            const float g_est_y = g_lf_y + hf_gain * g_hf_y;
-            float direction = 2.0f * clim_atan2pif(grady, gradx);
-            direction = direction < 0.5f
-                            ? mixf(direction, 0.0f, 1.0f - smoothstep(0.3f * lgt, 0.45f * lgt, direction))
-                            : mixf(direction, 1.0f,
-                                   smoothstep((1.0f - 0.45f) * lgt, (1.0f - 0.3f) * lgt, direction));
-            direction = mixf(0.5f, direction, lgt);
-            out = mixf(g_est_y, g_est_x, direction);
         }
-        wrow_f(green, y)[x] = out;
+        const float gradx = sx / wsum, grady = sy / wsum;
+
+        const float g_lf_x = (g_left + g_right) / 2.0f, g_lf_y = (g_up + g_down) / 2.0f;
+        const float g_hf_x = (2.0f * c_xy - (c_left + c_right)) / 4.0f;
+        const float g_hf_y = (2.0f * c_xy - (c_up + c_down)) / 4.0f;
+        const float whiteness =
+            clampf(mmin(c_xy, mmin(g_ave, c2_ave)) / mmax(c_xy, mmax(g_ave, c2_ave)), 0.0f, 1.0f);
+        const float glen = len2(gradx, grady);
+        const float lgt = smoothstep(lumaStdDev, 4.0f * lumaStdDev, glen);
+        const float hf_gain = lgt * mmin(0.5f * whiteness + smoothstep(0.0f, 0.3f, glen / 1.41421356f), 1.0f);
+        const float g_est_x = g_lf_x + hf_gain * g_hf_x;
+        const float g_est_y = g_lf_y + hf_gain * g_hf_y;
+        float direction = 2.0f * clim_atan2pif(grady, gradx);
+        direction = direction < 0.5f
+                        ? mixf(direction, 0.0f, 1.0f - smoothstep(0.3f * lgt, 0.45f * lgt, direction))
+                        : mixf(direction, 1.0f, smoothstep((1.0f - 0.45f) * lgt, (1.0f - 0.3f) * lgt, direction));
+        direction = mixf(0.5f, direction, lgt);
+        out = mixf(g_est_y, g_est_x, direction);
+    }
+    // green sites are copied
+    const int xe = x0 + 2 * k;  // even pixel of the pair
+    const float copy = r[Y][2 * k + (cpar ^ 1) + 3];
+    const float ev = cpar ? copy : out, od = cpar ? out : copy;
+    float* row = wrow_f(green, y);
+    if (xe + 1 < green.w && (green.pitch & 7) == 0) {
+        *reinterpret_cast<float2*>(row + xe) = make_float2(ev, od);
+    } else {
+        if (xe < green.w) row[xe] = ev;
+        if (xe + 1 < green.w) row[xe + 1] = od;
     }
 }
 
@@ -214,53 +225,76 @@ __device__ __forceinline__ void blend_highlights_px(float& px, float& py, float&
     }
 }
 
+// One 2x2 quad per thread: the 4x4 windows of the mosaic and of the green
+// plane around it are loaded once, and the quad's four sites are computed in
+// turn.  Which of them is R, G, B or G2 is the same for every quad, so every
+// lane of a wave runs the same code.
 template <bool kFuse>
 __global__ void __launch_bounds__(256) k_red_blue(Img raw, Img green, Img out, Bayer b, int red_is_c1_site,
                                                   ParamRef cv, int rotate_180, float clip, Mat3 m) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
-    if (x >= out.w || y >= out.h) return;
+    const TileId tl = xcd_tile();
+    const int qx = tl.x * 64 + threadIdx.x, qy = tl.y * 4 + threadIdx.y;
+    const int x0 = 2 * qx, y0 = 2 * qy;
+    if (x0 >= out.w || y0 >= out.h) return;
     float chromaVariance = cv.dev ? (cv.dev[4] + cv.dev[6]) / 2.0f : cv.host[0];
     chromaVariance = mmax(chromaVariance, 1e-6f);
-    const int color = b.site(x, y);
-    const float g = ld1(green, x, y);
-    float red, blue;
-    if (color == 0 || color == 2) {
-        const float c1 = ld1(raw, x, y);
-        const float c2_tl = ld1(green, x - 1, y - 1) - ld1(raw, x - 1, y - 1);
-        const float c2_tr = ld1(green, x + 1, y - 1) - ld1(raw, x + 1, y - 1);
-        const float c2_bl = ld1(green, x - 1, y + 1) - ld1(raw, x - 1, y + 1);
-        const float c2_br = ld1(green, x + 1, y + 1) - ld1(raw, x + 1, y + 1);
-        const float chromaStdDev = sqrtf(chromaVariance * g);
-        const float dvx = fabsf(c2_tl - c2_br), dvy = fabsf(c2_tr - c2_bl);
-        const float flatness = 1.0f - smoothstep(0.25f * chromaStdDev, chromaStdDev, len2(dvx, dvy));
-        const float alpha = mixf(2.0f * clim_atan2pif(dvy, dvx), 0.5f, flatness);
-        const float c2 = g - mixf((c2_tr + c2_bl) / 2.0f, (c2_tl + c2_br) / 2.0f, alpha);
-        if (color == 0) { red = c1; blue = c2; } else { blue = c1; red = c2; }
-    } else {
-        const float c1_left = ld1(green, x - 1, y) - ld1(raw, x - 1, y);
-        const float c1_right = ld1(green, x + 1, y) - ld1(raw, x + 1, y);
-        const float c2_up = ld1(green, x, y - 1) - ld1(raw, x, y - 1);
-        const float c2_down = ld1(green, x, y + 1) - ld1(raw, x, y + 1);
-        const float c1 = g - (c1_left + c1_right) / 2.0f;
-        const float c2 = g - (c2_up + c2_down) / 2.0f;
-        if (color == red_is_c1_site) { red = c1; blue = c2; } else { blue = c1; red = c2; }
+    // R[j][i], G[j][i]: mosaic and green at (x0 - 1 + i, y0 - 1 + j), clamped
+    float R[4][4], G[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            R[j][i] = ld1(raw, x0 - 1 + i, y0 - 1 + j);
+            G[j][i] = ld1(green, x0 - 1 + i, y0 - 1 + j);
+        }
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+            const int x = x0 + dx, y = y0 + dy;
+            const int color = b.site(dx, dy);  // x0, y0 are even
+            const int X = 1 + dx, Y = 1 + dy;
+            const float g = G[Y][X];
+            float red, blue;
+            if (color == 0 || color == 2) {
+                const float c1 = R[Y][X];
+                const float c2_tl = G[Y - 1][X - 1] - R[Y - 1][X - 1];
+                const float c2_tr = G[Y - 1][X + 1] - R[Y - 1][X + 1];
+                const float c2_bl = G[Y + 1][X - 1] - R[Y + 1][X - 1];
+                const float c2_br = G[Y + 1][X + 1] - R[Y + 1][X + 1];
+                const float chromaStdDev = sqrtf(chromaVariance * g);
+                const float dvx = fabsf(c2_tl - c2_br), dvy = fabsf(c2_tr - c2_bl);
+                const float flatness = 1.0f - smoothstep(0.25f * chromaStdDev, chromaStdDev, len2(dvx, dvy));
+                const float alpha = mixf(2.0f * clim_atan2pif(dvy, dvx), 0.5f, flatness);
+                const float c2 = g - mixf((c2_tr + c2_bl) / 2.0f, (c2_tl + c2_br) / 2.0f, alpha);
+                if (color == 0) { red = c1; blue = c2; } else { blue = c1; red = c2; }
+            } else {
+                const float c1_left = G[Y][X - 1] - R[Y][X - 1];
+                const float c1_right = G[Y][X + 1] - R[Y][X + 1];
+                const float c2_up = G[Y - 1][X] - R[Y - 1][X];
+                const float c2_down = G[Y + 1][X] - R[Y + 1][X];
+                const float c1 = g - (c1_left + c1_right) / 2.0f;
+                const float c2 = g - (c2_up + c2_down) / 2.0f;
+                if (color == red_is_c1_site) { red = c1; blue = c2; } else { blue = c1; red = c2; }
+            }
+            float px = red, py = g, pz = blue;
+            if (kFuse) {
+                blend_highlights_px(px, py, pz, clip);
+                const float tx = m.m[0] * px + m.m[1] * py + m.m[2] * pz;
+                const float ty = m.m[3] * px + m.m[4] * py + m.m[5] * pz;
+                const float tz = m.m[6] * px + m.m[7] * py + m.m[8] * pz;
+                px = tx; py = ty; pz = tz;
+            }
+            if (x >= out.w || y >= out.h) continue;
+            int ox = x, oy = y;
+            if (rotate_180) {
+                ox = out.w - x;
+                oy = out.h - y;
+                if (ox >= out.w || oy >= out.h) continue;
+            }
+            wrow_f4(out, oy)[ox] = make_float4(px, py, pz, 0.0f);
+        }
     }
-    float px = red, py = g, pz = blue;
-    if (kFuse) {
-        blend_highlights_px(px, py, pz, clip);
-        const float tx = m.m[0] * px + m.m[1] * py + m.m[2] * pz;
-        const float ty = m.m[3] * px + m.m[4] * py + m.m[5] * pz;
-        const float tz = m.m[6] * px + m.m[7] * py + m.m[8] * pz;
-        px = tx; py = ty; pz = tz;
-    }
-    int ox = x, oy = y;
-    if (rotate_180) {
-        ox = out.w - x;
-        oy = out.h - y;
-        if (ox >= out.w || oy >= out.h) return;
-    }
-    wrow_f4(out, oy)[ox] = make_float4(px, py, pz, 0.0f);
 }
 
 void launch_red_blue(const Img& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
@@ -271,7 +305,7 @@ void launch_red_blue(const Img& raw, const Img& green, const Img& out, int bayer
     const int red_is_c1 = (bayer == 3 || bayer == 0) ? 1 : 3;
     Mat3 m;
     for (int i = 0; i < 9; i++) m.m[i] = c2y ? c2y[i] : (i % 4 == 0 ? 1.0f : 0.0f);
-    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    dim3 blk(64, 4), grd((out.w + 127) / 128, (out.h + 7) / 8);  // one quad per thread
     if (fuse)
         hipLaunchKernelGGL(k_red_blue<true>, grd, blk, 0, st, raw, green, out, b, red_is_c1, cv, 0, clip, m);
     else
@@ -331,8 +365,9 @@ __device__ __forceinline__ float median9(float v0, float v1, float v2, float v3,
 }
 
 __global__ void __launch_bounds__(256) k_despeckle(Img in, Img out, float var_a, float var_b) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x;
+    const int y = tl.y * 4 + threadIdx.y;
     if (x >= out.w || y >= out.h) return;
     // read order of fast_median3x3
     const int ox[9] = {0, 1, 0, 1, 0, 1, -1, -1, -1};

@@ -20,7 +20,8 @@ __device__ __forceinline__ float* wpx(const Img& im, int x, int y) {
 // boxBlurX pass along rows (cl/guided_filter.cl:16-28)
 template <int NC>
 __global__ void __launch_bounds__(256) k_hblur(Img in, Img out, int fs) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x, y = tl.y * 4 + threadIdx.y;
     if (x >= in.w || y >= in.h) return;
     float s[NC];
 #pragma unroll
@@ -38,7 +39,8 @@ __global__ void __launch_bounds__(256) k_hblur(Img in, Img out, int fs) {
 // the second boxBlurX pass: along columns of the row-blurred image
 template <int NC>
 __global__ void __launch_bounds__(256) k_vblur(Img in, Img out, int fs) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x, y = tl.y * 4 + threadIdx.y;
     if (x >= in.w || y >= in.h) return;
     float s[NC];
 #pragma unroll
@@ -55,7 +57,8 @@ __global__ void __launch_bounds__(256) k_vblur(Img in, Img out, int fs) {
 
 // covMatrixProducts, cl/guided_filter.cl:30-44
 __global__ void __launch_bounds__(256) k_cov_products(Img in, Img s1, Img s2) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x, y = tl.y * 4 + threadIdx.y;
     if (x >= in.w || y >= in.h) return;
     const float* p = px<4>(in, x, y);
     float* a = wpx<4>(s1, x, y);
@@ -66,7 +69,8 @@ __global__ void __launch_bounds__(256) k_cov_products(Img in, Img s1, Img s2) {
 
 // invSigma, cl/guided_filter.cl:46-83 (in place on s1, s2)
 __global__ void __launch_bounds__(256) k_inv_sigma(Img meanI, Img s1, Img s2, ParamRef eps) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x, y = tl.y * 4 + threadIdx.y;
     if (x >= meanI.w || y >= meanI.h) return;
     const float e0 = eps.get(0), e1 = eps.get(1), e2 = eps.get(2);
     const float* mI = px<4>(meanI, x, y);
@@ -91,7 +95,8 @@ __global__ void __launch_bounds__(256) k_inv_sigma(Img meanI, Img s1, Img s2, Pa
 
 // meanIpProducts, cl/guided_filter.cl:85-95
 __global__ void __launch_bounds__(256) k_mean_ip(Img in, Img ipr, Img ipg, Img ipb) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x, y = tl.y * 4 + threadIdx.y;
     if (x >= in.w || y >= in.h) return;
     const float* p = px<4>(in, x, y);
     float* r = wpx<4>(ipr, x, y);
@@ -108,7 +113,8 @@ __global__ void __launch_bounds__(256) k_mean_ip(Img in, Img ipr, Img ipg, Img i
 
 // computeAb, cl/guided_filter.cl:97-137 (in place on ipr, ipg, ipb)
 __global__ void __launch_bounds__(256) k_compute_ab(Img meanI, Img ipr, Img ipg, Img ipb, Img s1, Img s2) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x, y = tl.y * 4 + threadIdx.y;
     if (x >= meanI.w || y >= meanI.h) return;
     const float* mp = px<4>(meanI, x, y);
     float* r = wpx<4>(ipr, x, y);
@@ -140,7 +146,8 @@ __global__ void __launch_bounds__(256) k_compute_ab(Img meanI, Img ipr, Img ipg,
 
 // computeResult, cl/guided_filter.cl:139-154
 __global__ void __launch_bounds__(256) k_guided_result(Img in, Img ipr, Img ipg, Img ipb, Img out) {
-    const int x = blockIdx.x * 64 + threadIdx.x, y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x, y = tl.y * 4 + threadIdx.y;
     if (x >= in.w || y >= in.h) return;
     const float* p = px<4>(in, x, y);
     const float* r = px<4>(ipr, x, y);

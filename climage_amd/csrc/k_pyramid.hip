@@ -34,8 +34,9 @@ __device__ __forceinline__ float4 bilinear4(const Img& im, float un, float vn) {
 // ------------------------------------------------------------ downsampleImage
 // cl/demosaic.cl:853-865: four linear taps at +-1/2 output texel, averaged.
 __global__ void __launch_bounds__(256) k_downsample(Img in, Img out) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x;
+    const int y = tl.y * 4 + threadIdx.y;
     if (x >= out.w || y >= out.h) return;
     const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
     const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
@@ -70,7 +71,8 @@ struct StatsTile {
 // cl/demosaic.cl:1267-1286: (mean Y, var Y, var Cb, var Cr) over 5x5.
 __global__ void __launch_bounds__(256) k_noise_stats(Img in, Img out) {
     __shared__ StatsTile sm;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const TileId tl = xcd_tile();
+    const int x0 = tl.x * TX, y0 = tl.y * TY;
     const int tid = threadIdx.y * 32 + threadIdx.x;
     sm.t.load(in, x0, y0, tid);
     __syncthreads();
@@ -145,7 +147,8 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
 __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
                                                    float gradientBoost) {
     __shared__ LevelTile<TX, TY> t;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const TileId tl = xcd_tile();
+    const int x0 = tl.x * TX, y0 = tl.y * TY;
     t.load(in, x0, y0, threadIdx.y * 32 + threadIdx.x);
     __syncthreads();
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
@@ -175,7 +178,8 @@ struct GuidedTile {
 
 __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef nlf) {
     __shared__ GuidedTile sm;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const TileId tl = xcd_tile();
+    const int x0 = tl.x * TX, y0 = tl.y * TY;
     const int tid = threadIdx.y * 32 + threadIdx.x;
     sm.t.load(in, x0, y0, tid);
     __syncthreads();
@@ -262,8 +266,9 @@ void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipS
 // out may alias d0 (each texel is read, then written, by one thread).
 __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img out, float sharpening0,
                                                     ParamRef nlf) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x;
+    const int y = tl.y * 4 + threadIdx.y;
     if (x >= out.w || y >= out.h) return;
     const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
     const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;

@@ -109,20 +109,22 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         const ParamRef cal = dev_params(ctx->fit[1 + i].cal);
         const bool bilateral = algorithm == CLIM_DENOISE_BILATERAL ||
                                (algorithm == CLIM_DENOISE_GUIDED_FAST && i == levels - 1);
-        ProfScope ps(ctx, ST_DENOISE, (long long)lvl[i].w * lvl[i].h);
-        if (algorithm == CLIM_DENOISE_GUIDED_PRECISE) {
-            clim::GuidedScratch g{ws.g_meanI.img, ws.g_s1.img, ws.g_s2.img, ws.g_ipr.img,
-                                  ws.g_ipg.img,   ws.g_ipb.img, ws.g_tmp4.img, ws.g_tmp2.img};
-            clim::GuidedScratch gv{sub_view(g.mean_I, lvl[i].w, lvl[i].h), sub_view(g.s1, lvl[i].w, lvl[i].h),
-                                   sub_view(g.s2, lvl[i].w, lvl[i].h),     sub_view(g.ipr, lvl[i].w, lvl[i].h),
-                                   sub_view(g.ipg, lvl[i].w, lvl[i].h),    sub_view(g.ipb, lvl[i].w, lvl[i].h),
-                                   sub_view(g.tmp4, lvl[i].w, lvl[i].h),   sub_view(g.tmp2, lvl[i].w, lvl[i].h)};
-            // GuidedPreciseDenoiser passes var_b as eps (cpp/pyramidal_denoise.cpp:65)
-            clim::launch_guided_filter(lvl[i], den[i], 5, dev_params(ctx->fit[1 + i].cal + 3), gv, st);
-        } else if (bilateral) {
-            clim::launch_bilateral(lvl[i], den[i], cal, dp[i].chromaBoost, dp[i].gradientBoost, st);
-        } else {
-            clim::launch_guided_fast(lvl[i], den[i], cal, st);
+        {
+            ProfScope ps(ctx, ST_DENOISE, (long long)lvl[i].w * lvl[i].h);
+            if (algorithm == CLIM_DENOISE_GUIDED_PRECISE) {
+                clim::GuidedScratch g{ws.g_meanI.img, ws.g_s1.img, ws.g_s2.img, ws.g_ipr.img,
+                                      ws.g_ipg.img,   ws.g_ipb.img, ws.g_tmp4.img, ws.g_tmp2.img};
+                clim::GuidedScratch gv{sub_view(g.mean_I, lvl[i].w, lvl[i].h), sub_view(g.s1, lvl[i].w, lvl[i].h),
+                                       sub_view(g.s2, lvl[i].w, lvl[i].h),     sub_view(g.ipr, lvl[i].w, lvl[i].h),
+                                       sub_view(g.ipg, lvl[i].w, lvl[i].h),    sub_view(g.ipb, lvl[i].w, lvl[i].h),
+                                       sub_view(g.tmp4, lvl[i].w, lvl[i].h),   sub_view(g.tmp2, lvl[i].w, lvl[i].h)};
+                // GuidedPreciseDenoiser passes var_b as eps (cpp/pyramidal_denoise.cpp:65)
+                clim::launch_guided_filter(lvl[i], den[i], 5, dev_params(ctx->fit[1 + i].cal + 3), gv, st);
+            } else if (bilateral) {
+                clim::launch_bilateral(lvl[i], den[i], cal, dp[i].chromaBoost, dp[i].gradientBoost, st);
+            } else {
+                clim::launch_guided_fast(lvl[i], den[i], cal, st);
+            }
         }
         if (i < levels - 1) {
             ProfScope pr(ctx, ST_REASSEMBLE, (long long)lvl[i].w * lvl[i].h);

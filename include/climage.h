@@ -282,6 +282,10 @@ typedef struct {
     int64_t texels[CLIM_PROFILE_STAGES];    /* output texels (or fitted samples) per stage */
 } clim_profile;
 clim_status clim_profile_enable(clim_ctx* ctx, int32_t enable);
+/* Restricts event recording to the stages whose bit (1 << stage) is set in
+ * stage_mask (default: all stages), so that a timed run can measure one
+ * kernel while recording as few events as possible. */
+clim_status clim_profile_select(clim_ctx* ctx, uint32_t stage_mask);
 clim_status clim_profile_read(clim_ctx* ctx, clim_profile* out);
 const char* clim_profile_stage_name(int32_t stage);
 
