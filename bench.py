@@ -46,17 +46,19 @@ def stage_bytes(stage, texels, levels_px=None):
         "level_fit": 2 * 16,            # two passes over the statistics image
         "denoise": 16 + 16,
         "reassemble": 16 + 16 + 16 / 4 + 16 / 4,
-        "tone": 16 + 4,
+        "tone": 16 + 16 / 4 + 16 / 4 + 4,  # level-0 reassembly fused with the tone map: RGBA8 out
     }
     return per[stage] * texels
 
 
 # stages that are one kernel (launched once per frame, or once per pyramid
-# level), so that their event timing is that kernel's launch duration
+# level), so that their event timing is that kernel's launch duration; the
+# names are the kernel instances this workload runs (as rocprofv3 lists them)
 SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green": "k_green",
-                 "red_blue_ycc": "k_red_blue", "despeckle": "k_despeckle", "downsample": "k_downsample",
-                 "level_stats": "k_noise_stats", "denoise": "k_bilateral", "reassemble": "k_reassemble",
-                 "tone": "k_tone"}
+                 "red_blue_ycc": "k_red_blue<true>", "despeckle": "k_despeckle",
+                 "downsample": "k_downsample<true>", "level_stats": "k_noise_stats",
+                 "denoise": "k_bilateral<true>", "reassemble": "k_reassemble<true, false>",
+                 "tone": "k_reassemble<true, true>"}
 
 
 def cpu_baseline(w, h, bayer, seconds_min=10.0):

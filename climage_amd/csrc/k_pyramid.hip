@@ -11,14 +11,8 @@ namespace clim {
 
 // CL_FILTER_LINEAR | CLK_ADDRESS_CLAMP_TO_EDGE with normalised coordinates:
 // u = x*W - 0.5, i0 = floor(u), weight frac(u) rounded to 8 fractional bits.
-__device__ __forceinline__ float4 bilinear4(const Img& im, float un, float vn) {
-    const float u = un * (float)im.w - 0.5f;
-    const float v = vn * (float)im.h - 0.5f;
-    const float fu = floorf(u), fv = floorf(v);
-    float a = u - fu, b = v - fv;
-    a = floorf(a * 256.0f + 0.5f) * 0.00390625f;
-    b = floorf(b * 256.0f + 0.5f) * 0.00390625f;
-    const int i0 = (int)fu, j0 = (int)fv;
+// The sampling half, given the texel (i0, j0) and the rounded weights (a, b):
+__device__ __forceinline__ float4 bilinear4_at(const Img& im, int i0, int j0, float a, float b) {
     const float4 t00 = ld4(im, i0, j0), t10 = ld4(im, i0 + 1, j0);
     const float4 t01 = ld4(im, i0, j0 + 1), t11 = ld4(im, i0 + 1, j0 + 1);
     const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b);
@@ -31,29 +25,63 @@ __device__ __forceinline__ float4 bilinear4(const Img& im, float un, float vn) {
     return r;
 }
 
+__device__ __forceinline__ float4 bilinear4(const Img& im, float un, float vn) {
+    const float u = un * (float)im.w - 0.5f;
+    const float v = vn * (float)im.h - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    float a = u - fu, b = v - fv;
+    a = floorf(a * 256.0f + 0.5f) * 0.00390625f;
+    b = floorf(b * 256.0f + 0.5f) * 0.00390625f;
+    return bilinear4_at(im, (int)fu, (int)fv, a, b);
+}
+
+// Exact 2x ratios.  For an output of at most kDown2xMax texels per axis
+// downsampled from exactly twice that, the float coordinate arithmetic of
+// downsampleImage puts every tap at texel 2x-1 or 2x+1 with weight exactly
+// 1/2; for an output of at most kUp2xMax texels per axis sampling an image of
+// exactly half that (reassembleImage), even x lands on texel x/2-1 with
+// weight 3/4, odd x on (x-1)/2 with 1/4.  (Checked exhaustively for every
+// size up to these limits; the general path serves larger images.)  The
+// kernels then skip the coordinate arithmetic and call the same sampling code.
+constexpr int kDown2xMax = 6144, kUp2xMax = 16384;
+
 // ------------------------------------------------------------ downsampleImage
 // cl/demosaic.cl:853-865: four linear taps at +-1/2 output texel, averaged.
+template <bool EXACT2X>
 __global__ void __launch_bounds__(256) k_downsample(Img in, Img out) {
     const TileId tl = xcd_tile();
     const int x = tl.x * 64 + threadIdx.x;
     const int y = tl.y * 4 + threadIdx.y;
     if (x >= out.w || y >= out.h) return;
-    const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
-    const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
-    const float sx = 0.5f * nx, sy = 0.5f * ny;
-    float4 acc = bilinear4(in, px - sx, py - sy);
-    float4 t = bilinear4(in, px + sx, py - sy);
-    acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
-    t = bilinear4(in, px - sx, py + sy);
-    acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
-    t = bilinear4(in, px + sx, py + sy);
-    acc.x += t.x; acc.y += t.y; acc.z += t.z; acc.w += t.w;
+    float4 t0, t1, t2, t3;
+    if (EXACT2X) {
+        t0 = bilinear4_at(in, 2 * x - 1, 2 * y - 1, 0.5f, 0.5f);
+        t1 = bilinear4_at(in, 2 * x + 1, 2 * y - 1, 0.5f, 0.5f);
+        t2 = bilinear4_at(in, 2 * x - 1, 2 * y + 1, 0.5f, 0.5f);
+        t3 = bilinear4_at(in, 2 * x + 1, 2 * y + 1, 0.5f, 0.5f);
+    } else {
+        const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
+        const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
+        const float sx = 0.5f * nx, sy = 0.5f * ny;
+        t0 = bilinear4(in, px - sx, py - sy);
+        t1 = bilinear4(in, px + sx, py - sy);
+        t2 = bilinear4(in, px - sx, py + sy);
+        t3 = bilinear4(in, px + sx, py + sy);
+    }
+    float4 acc = t0;
+    acc.x += t1.x; acc.y += t1.y; acc.z += t1.z; acc.w += t1.w;
+    acc.x += t2.x; acc.y += t2.y; acc.z += t2.z; acc.w += t2.w;
+    acc.x += t3.x; acc.y += t3.y; acc.z += t3.z; acc.w += t3.w;
     wrow_f4(out, y)[x] = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.25f * acc.w);
 }
 
 void launch_downsample(const Img& in, const Img& out, hipStream_t st) {
     dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
-    hipLaunchKernelGGL(k_downsample, grd, blk, 0, st, in, out);
+    const bool exact = in.w == 2 * out.w && in.h == 2 * out.h && out.w <= kDown2xMax && out.h <= kDown2xMax;
+    if (exact)
+        hipLaunchKernelGGL(k_downsample<true>, grd, blk, 0, st, in, out);
+    else
+        hipLaunchKernelGGL(k_downsample<false>, grd, blk, 0, st, in, out);
 }
 
 // 5x5 stencils over an RGBA level: a 32x16 output tile plus a 2-texel halo in
@@ -96,6 +124,9 @@ void launch_noise_stats(const Img& in, const Img& out, hipStream_t st) {
 // cl/demosaic.cl:735-765.  nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr.  The per-tap
 // division by sigma is a multiplication by 1/sigma and length(d) < edge is
 // dot(d, d) < edge^2 (the FP32 oracle evaluates the same expressions).
+// SAME_EDGE: gradientBoost <= 1 and the luma and chroma edges are equal, so
+// every tap's luma and chroma weights are equal (one test, one weight sum).
+template <bool SAME_EDGE>
 __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int tx, int ty, float va0, float va1,
                                                float va2, float vb0, float vb1, float vb2, float chromaBoost,
                                                float gradientBoost) {
@@ -103,7 +134,7 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
     const float cY = t.x[Y][X], cCb = t.y[Y][X], cCr = t.z[Y][X];
     const float sig0 = sqrtf(va0 + vb0 * cY), sig1 = sqrtf(va1 + vb1 * cY), sig2 = sqrtf(va2 + vb2 * cY);
     float lumaBoost = 1.0f;
-    if (gradientBoost > 1.0f) {
+    if (!SAME_EDGE && gradientBoost > 1.0f) {
         // sobel() of the luma channel, cl/demosaic.cl:96-112
         const float kx[3][3] = {{1, 0, -1}, {2, 0, -2}, {1, 0, -1}};
         const float ky[3][3] = {{1, 2, 1}, {0, 0, 0}, {-1, -2, -1}};
@@ -134,16 +165,23 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
             const float d0 = (a - cY) * i0, d1 = (b - cCb) * i1, d2 = (c - cCr) * i2;
             const float l = d0 * d0 + d1 * d1 + d2 * d2;
             const float w0 = 1.0f - (l < e0 ? 0.0f : 1.0f);
-            const float w1 = 1.0f - (l < e1 ? 0.0f : 1.0f);
             f0 += w0 * a; n0 += w0;
-            f1 += w1 * b; n1 += w1;
-            f2 += w1 * c;
+            if (SAME_EDGE) {
+                f1 += w0 * b;
+                f2 += w0 * c;
+            } else {
+                const float w1 = 1.0f - (l < e1 ? 0.0f : 1.0f);
+                f1 += w1 * b; n1 += w1;
+                f2 += w1 * c;
+            }
         }
     }
+    if (SAME_EDGE) n1 = n0;
     // the Cr weight sum equals the Cb one (same threshold, same weights)
     return make_float4(f0 / n0, f1 / n1, f2 / n1, 0.0f);
 }
 
+template <bool SAME_EDGE>
 __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
                                                    float gradientBoost) {
     __shared__ LevelTile<TX, TY> t;
@@ -158,14 +196,22 @@ __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf
         const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
         const int x = x0 + tx, y = y0 + ty;
         if (x >= out.w || y >= out.h) continue;
-        wrow_f4(out, y)[x] = bilateral_px(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost);
+        wrow_f4(out, y)[x] =
+            bilateral_px<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost);
     }
 }
 
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
                       hipStream_t st) {
     dim3 blk(32, 8), grd((out.w + TX - 1) / TX, (out.h + TY - 1) / TY);
-    hipLaunchKernelGGL(k_bilateral, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost);
+    // the edges as the kernel computes them (lumaBoost = 1 without gradient boost)
+    float e0 = 1.0f * 1.41421356f, e1 = chromaBoost * 1.41421356f;
+    e0 = e0 * e0;
+    e1 = e1 * e1;
+    if (!(gradientBoost > 1.0f) && e0 == e1)
+        hipLaunchKernelGGL(k_bilateral<true>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost);
+    else
+        hipLaunchKernelGGL(k_bilateral<false>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost);
 }
 
 // --------------------------------------------------------- denoiseImageGuided
@@ -261,44 +307,6 @@ void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipS
     hipLaunchKernelGGL(k_guided_fast, grd, blk, 0, st, in, out, nlf);
 }
 
-// ------------------------------------------------------------ reassembleImage
-// cl/demosaic.cl:867-896.  nlf: (A_Y, B_Y) at nlf.get(0), nlf.get(3).
-// out may alias d0 (each texel is read, then written, by one thread).
-__global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img out, float sharpening0,
-                                                    ParamRef nlf) {
-    const TileId tl = xcd_tile();
-    const int x = tl.x * 64 + threadIdx.x;
-    const int y = tl.y * 4 + threadIdx.y;
-    if (x >= out.w || y >= out.h) return;
-    const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
-    const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
-    const float4 p0 = row_f4(d0, y)[x];
-    const float4 a1 = bilinear4(n1, px, py);
-    const float4 b1 = bilinear4(d1, px, py);
-    float4 d;
-    d.x = p0.x - (a1.x - b1.x);
-    d.y = p0.y - (a1.y - b1.y);
-    d.z = p0.z - (a1.z - b1.z);
-    d.w = p0.w - (a1.w - b1.w);
-    float sharpening = sharpening0;
-    if (sharpening > 1.0f) {
-        const float dx = (bilinear4(d1, px + nx, py).x - bilinear4(d1, px - nx, py).x) / 2.0f;
-        const float dy = (bilinear4(d1, px, py + ny).x - bilinear4(d1, px, py - ny).x) / 2.0f;
-        const float threshold = 0.25f * sqrtf(nlf.get(0) + nlf.get(3) * b1.x);
-        const float detail = smoothstep(0.25f * threshold, threshold, len2(dx, dy)) *
-                             (1.0f - smoothstep(0.95f, 1.0f, d.x)) * (0.6f + 0.4f * smoothstep(0.0f, 0.1f, d.x));
-        sharpening = 1.0f + (sharpening - 1.0f) * detail;
-    }
-    d.x = mixf(b1.x, d.x, sharpening);
-    wrow_f4(out, y)[x] = d;
-}
-
-void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& out, float sharpening,
-                       const ParamRef& nlf, hipStream_t st) {
-    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
-    hipLaunchKernelGGL(k_reassemble, grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf);
-}
-
 // -------------------------------------------------------------- convertTosRGB
 // cl/demosaic.cl:1391-1474 (localToneMapping = 0), optionally preceded by
 // transformImage with `pre` (the YCbCr->camera matrix of
@@ -313,11 +321,8 @@ struct ToneArgs {
 
 __device__ __forceinline__ float sigmoidf(float x, float s) { return 0.5f * (clim_tanhf(s * x - 0.3f * s) + 1.0f); }
 
-__global__ void __launch_bounds__(256) k_tone(Img in, Img out8, Img outf, ToneArgs a) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * 4 + threadIdx.y;
-    if (x >= out8.w || y >= out8.h) return;
-    const float4 p = row_f4(in, y)[x];
+// one pixel: RGBA8 (packed) and the FP32 RGB before quantisation
+__device__ __forceinline__ uint32_t tone_px(float4 p, const ToneArgs& a, float rgb[3]) {
     float v[3] = {p.x, p.y, p.z};
     if (a.has_pre) {
         const float t0 = a.pre[0] * v[0] + a.pre[1] * v[1] + a.pre[2] * v[2];
@@ -337,7 +342,6 @@ __global__ void __launch_bounds__(256) k_tone(Img in, Img out8, Img outf, ToneAr
 #pragma unroll
         for (int c = 0; c < 3; c++) v[c] = mixf(0.2f, v[c], mixf(a.contrast, 1.0f, smoothstep(0.9f, 2.0f, v[c])));
     }
-    float rgb[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) rgb[r] = a.cam[r * 3 + 0] * v[0] + a.cam[r * 3 + 1] * v[1] + a.cam[r * 3 + 2] * v[2];
     uint32_t packed = 0;
@@ -350,12 +354,26 @@ __global__ void __launch_bounds__(256) k_tone(Img in, Img out8, Img outf, ToneAr
         const uint32_t q = (uint32_t)rintf(clampf(rgb[c], 0.0f, 1.0f) * 255.0f);
         packed |= q << (8 * c);
     }
+    return packed;
+}
+
+__device__ __forceinline__ void tone_store(const Img& out8, const Img& outf, int x, int y, float4 p,
+                                           const ToneArgs& a) {
+    float rgb[3];
+    const uint32_t packed = tone_px(p, a, rgb);
     reinterpret_cast<uint32_t*>(out8.p + (size_t)y * out8.pitch)[x] = packed;
     if (outf.p) wrow_f4(outf, y)[x] = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
 }
 
-void launch_tone(const Img& in, const Img& out8, const Img& outf, const float* pre, const float cam[9],
-                 float contrast, float saturation, float slope, float exposureBias, float blacks, hipStream_t st) {
+__global__ void __launch_bounds__(256) k_tone(Img in, Img out8, Img outf, ToneArgs a) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (x >= out8.w || y >= out8.h) return;
+    tone_store(out8, outf, x, y, row_f4(in, y)[x], a);
+}
+
+static ToneArgs make_tone_args(const float* pre, const float cam[9], float contrast, float saturation, float slope,
+                               float exposureBias, float blacks) {
     ToneArgs a;
     a.has_pre = pre != nullptr;
     for (int i = 0; i < 9; i++) {
@@ -370,8 +388,84 @@ void launch_tone(const Img& in, const Img& out8, const Img& outf, const float* p
     a.blacks = blacks;
     a.sig0 = 0.5f * (clim_tanhf(slope * 0.0f - 0.3f * slope) + 1.0f);
     a.sig1 = 0.5f * (clim_tanhf(slope * 1.0f - 0.3f * slope) + 1.0f);
+    return a;
+}
+
+void launch_tone(const Img& in, const Img& out8, const Img& outf, const float* pre, const float cam[9],
+                 float contrast, float saturation, float slope, float exposureBias, float blacks, hipStream_t st) {
+    const ToneArgs a = make_tone_args(pre, cam, contrast, saturation, slope, exposureBias, blacks);
     dim3 blk(64, 4), grd((out8.w + 63) / 64, (out8.h + 3) / 4);
     hipLaunchKernelGGL(k_tone, grd, blk, 0, st, in, out8, outf, a);
+}
+
+// ------------------------------------------------------------ reassembleImage
+// cl/demosaic.cl:867-896.  nlf: (A_Y, B_Y) at nlf.get(0), nlf.get(3).
+// out may alias d0 (each texel is read, then written, by one thread).
+// TONE: the level-0 pass of the pipeline, which hands its result straight to
+// convertTosRGB (out8 / outf) instead of writing it.
+template <bool EXACT2X, bool TONE>
+__global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img out, float sharpening0,
+                                                    ParamRef nlf, Img out8, Img outf, ToneArgs ta) {
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x;
+    const int y = tl.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
+    const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
+    const float4 p0 = row_f4(d0, y)[x];
+    float4 a1, b1;
+    if (EXACT2X) {
+        const int i0 = (x & 1) ? (x - 1) >> 1 : (x >> 1) - 1, j0 = (y & 1) ? (y - 1) >> 1 : (y >> 1) - 1;
+        const float a = (x & 1) ? 0.25f : 0.75f, b = (y & 1) ? 0.25f : 0.75f;
+        a1 = bilinear4_at(n1, i0, j0, a, b);
+        b1 = bilinear4_at(d1, i0, j0, a, b);
+    } else {
+        a1 = bilinear4(n1, px, py);
+        b1 = bilinear4(d1, px, py);
+    }
+    float4 d;
+    d.x = p0.x - (a1.x - b1.x);
+    d.y = p0.y - (a1.y - b1.y);
+    d.z = p0.z - (a1.z - b1.z);
+    d.w = p0.w - (a1.w - b1.w);
+    float sharpening = sharpening0;
+    if (sharpening > 1.0f) {
+        const float dx = (bilinear4(d1, px + nx, py).x - bilinear4(d1, px - nx, py).x) / 2.0f;
+        const float dy = (bilinear4(d1, px, py + ny).x - bilinear4(d1, px, py - ny).x) / 2.0f;
+        const float threshold = 0.25f * sqrtf(nlf.get(0) + nlf.get(3) * b1.x);
+        const float detail = smoothstep(0.25f * threshold, threshold, len2(dx, dy)) *
+                             (1.0f - smoothstep(0.95f, 1.0f, d.x)) * (0.6f + 0.4f * smoothstep(0.0f, 0.1f, d.x));
+        sharpening = 1.0f + (sharpening - 1.0f) * detail;
+    }
+    d.x = mixf(b1.x, d.x, sharpening);
+    if (TONE)
+        tone_store(out8, outf, x, y, d, ta);
+    else
+        wrow_f4(out, y)[x] = d;
+}
+
+void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& out, float sharpening,
+                       const ParamRef& nlf, hipStream_t st, const ToneLaunch* tone) {
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    const bool exact = out.w == 2 * n1.w && out.h == 2 * n1.h && n1.w == d1.w && n1.h == d1.h &&
+                       out.w <= kUp2xMax && out.h <= kUp2xMax;
+    ToneArgs ta{};
+    Img o8{nullptr, 0, 0, 0}, of{nullptr, 0, 0, 0};
+    if (tone) {
+        ta = make_tone_args(tone->pre, tone->cam, tone->contrast, tone->saturation, tone->slope, tone->exposureBias,
+                            tone->blacks);
+        o8 = tone->out8;
+        of = tone->outf;
+    }
+    if (exact && tone)
+        hipLaunchKernelGGL((k_reassemble<true, true>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta);
+    else if (exact)
+        hipLaunchKernelGGL((k_reassemble<true, false>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta);
+    else if (tone)
+        hipLaunchKernelGGL((k_reassemble<false, true>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta);
+    else
+        hipLaunchKernelGGL((k_reassemble<false, false>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of,
+                           ta);
 }
 
 }  // namespace clim

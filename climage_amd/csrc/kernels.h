@@ -53,8 +53,15 @@ void launch_noise_stats(const Img& in, const Img& out, hipStream_t st);
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
                       hipStream_t st);
 void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st);
+// convertTosRGB applied to reassembleImage's result in the same kernel
+struct ToneLaunch {
+    Img out8, outf;  // outf.p may be null
+    const float* pre;  // YCbCr -> camera (with exposure), or null
+    const float* cam;
+    float contrast, saturation, slope, exposureBias, blacks;
+};
 void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& out, float sharpening,
-                       const ParamRef& nlf, hipStream_t st);
+                       const ParamRef& nlf, hipStream_t st, const ToneLaunch* tone = nullptr);
 void launch_tone(const Img& in, const Img& out8, const Img& outf, const float* pre, const float cam[9],
                  float contrast, float saturation, float slope, float exposureBias, float blacks, hipStream_t st);
 

@@ -8,8 +8,9 @@
 //    need a fitted value read it from device memory, so the frame has no
 //    host round trip (the reference maps six images per frame);
 //  - blendHighlightsImage and the camera->YCbCr transformImage run inside
-//    the interpolateRedBlue kernel, and the YCbCr->camera transformImage
-//    inside the convertTosRGB kernel (same per-pixel operations);
+//    the interpolateRedBlue kernel; the YCbCr->camera transformImage and
+//    convertTosRGB run inside the kernel that reassembles pyramid level 0
+//    (same per-pixel operations, without the level-0 image round trip);
 //  - work buffers follow the frame size (the reference allocates once and
 //    never checks, cpp/raw_converter.cpp:27).
 #include <hip/hip_runtime.h>
@@ -66,8 +67,11 @@ static Img sub_view(const Img& im, int w, int h) { return Img{im.p, w, h, im.pit
 // The pyramid of cpp/pyramidal_denoise.cpp:103-149 on device images.
 // level0: despeckled YCbCr; out: denoised level 0 (may not alias level0).
 // Fit slots 1..levels receive the per-level noise models.
+// With `tone`, level 0's reassembly hands its result to the tone map in the
+// same kernel (out is then left holding level 0's denoised-only texels).
 static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int levels, int algorithm,
-                           const clim_denoise_params* dp, const float (*nlf_in)[6]) {
+                           const clim_denoise_params* dp, const float (*nlf_in)[6],
+                           const clim::ToneLaunch* tone = nullptr) {
     Workspace& ws = ctx->ws;
     hipStream_t st = ctx->stream;
     Img lvl[CLIM_MAX_LEVELS], den[CLIM_MAX_LEVELS];
@@ -127,8 +131,10 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
             }
         }
         if (i < levels - 1) {
-            ProfScope pr(ctx, ST_REASSEMBLE, (long long)lvl[i].w * lvl[i].h);
-            clim::launch_reassemble(den[i], lvl[i + 1], den[i + 1], den[i], dp[i].sharpening, cal, st);
+            const bool fused = tone && i == 0;
+            ProfScope pr(ctx, fused ? ST_TONE : ST_REASSEMBLE, (long long)lvl[i].w * lvl[i].h);
+            clim::launch_reassemble(den[i], lvl[i + 1], den[i + 1], den[i], dp[i].sharpening, cal, st,
+                                    fused ? tone : nullptr);
         }
     }
     CLIM_TRY_HIP(ctx, hipGetLastError());
@@ -253,16 +259,14 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
     }
     CLIM_TRY_HIP(ctx, hipGetLastError());
 
-    CLIM_TRY(pyramid(ctx, ws.rgbB.img, ws.rgbA.img, p->levels, p->denoiseAlgorithm, p->denoiseParameters,
-                     p->noiseModel.pyramidNlf));
-
-    // transformImage(inverse(cam_to_ycbcr) * exposure) + convertTosRGB
+    // transformImage(inverse(cam_to_ycbcr) * exposure) + convertTosRGB, in the
+    // kernel that reassembles pyramid level 0
     const clim_rgb_conversion_params& rc = p->rgbConversionParameters;
-    {
-        ProfScope ps(ctx, ST_TONE, npx);
-        clim::launch_tone(ws.rgbA.img, view(out_rgba8), outf, y2c, cam, rc.contrast, rc.saturation, rc.toneCurveSlope,
-                          rc.exposureBias, rc.blacks, st);
-    }
+    const clim::ToneLaunch tone{view(out_rgba8), outf,          y2c,           cam,
+                                rc.contrast,      rc.saturation, rc.toneCurveSlope, rc.exposureBias,
+                                rc.blacks};
+    CLIM_TRY(pyramid(ctx, ws.rgbB.img, ws.rgbA.img, p->levels, p->denoiseAlgorithm, p->denoiseParameters,
+                     p->noiseModel.pyramidNlf, &tone));
     CLIM_TRY_HIP(ctx, hipGetLastError());
 
     if (nlf_out) {

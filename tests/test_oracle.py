@@ -511,3 +511,71 @@ def test_scene_noise_is_poisson_gaussian():
     for sel in (lam < 64, lam >= 64):
         zz = ((dn - mu) / sd)[ok & sel]
         assert abs(zz.var() - 1.0) < 0.05
+
+
+def _fixed_bilinear32(img, i0, j0, a, b):
+    """The sampler's weighted sum for given texel and weights, in float32 and
+    in the sampler's order (w00 t00 + w10 t10 + w01 t01 + w11 t11)."""
+    f = np.float32
+    a, b = f(a), f(b)
+    w00, w10, w01, w11 = (f(1) - a) * (f(1) - b), a * (f(1) - b), (f(1) - a) * b, a * b
+    t00, t10 = gather_clamped(img, i0, j0), gather_clamped(img, i0 + 1, j0)
+    t01, t11 = gather_clamped(img, i0, j0 + 1), gather_clamped(img, i0 + 1, j0 + 1)
+    return ((w00 * t00 + w10 * t10) + w01 * t01) + w11 * t11
+
+
+def test_sampler_exact_2x_ratios(o32):
+    """At exact 2x ratios the linear sampler's float32 coordinate arithmetic
+    puts every tap on a fixed texel with a fixed weight, for every output
+    size up to 6144 (downsample) and 16384 (upsample in reassemble); the HIP
+    kernels use those taps directly within these limits.  Checked
+    exhaustively, and through the FP32 oracle at sizes up to the limits."""
+    f = np.float32
+
+    def weight(u):
+        fu = np.floor(u)
+        return fu, np.floor((u - fu) * f(256) + f(0.5)) * f(0.00390625)
+
+    for wo in range(1, 6145):
+        x = np.arange(wo, dtype=np.float32)
+        nx = f(1) / f(wo)
+        px, sx = (x + f(0.5)) * nx, f(0.5) * nx
+        for tap, off in ((px - sx, -1), (px + sx, 1)):
+            fu, a = weight(tap * f(2 * wo) - f(0.5))
+            assert np.all(a == f(0.5)) and np.all(fu == 2 * x + off), wo
+    for w in range(2, 16385, 2):
+        x = np.arange(w)
+        fu, a = weight((x.astype(np.float32) + f(0.5)) * (f(1) / f(w)) * f(w // 2) - f(0.5))
+        assert np.all(a == np.where(x % 2 == 0, f(0.75), f(0.25))), w
+        assert np.all(fu == np.where(x % 2 == 0, x // 2 - 1, (x - 1) // 2)), w
+    # the limit is tight: 6145 is the first downsample size that breaks it
+    x = np.arange(6145, dtype=np.float32)
+    nx = f(1) / f(6145)
+    bad = [not np.all(weight(t * f(12290) - f(0.5))[1] == f(0.5)) or
+           not np.all(weight(t * f(12290) - f(0.5))[0] == 2 * x + o)
+           for t, o in (((x + f(0.5)) * nx - f(0.5) * nx, -1), ((x + f(0.5)) * nx + f(0.5) * nx, 1))]
+    assert any(bad)
+    # the oracle's downsample and reassemble at the largest sizes
+    rng = np.random.default_rng(21)
+    img = rng.uniform(0, 1, (4, 2 * 6144, 4)).astype(np.float32)
+    out = o32.downsample(img, 6144, 2)
+    yy, xx = np.mgrid[0:2, 0:6144]
+    acc = _fixed_bilinear32(img, 2 * xx - 1, 2 * yy - 1, 0.5, 0.5)
+    for i0, j0 in ((2 * xx + 1, 2 * yy - 1), (2 * xx - 1, 2 * yy + 1), (2 * xx + 1, 2 * yy + 1)):
+        acc = acc + _fixed_bilinear32(img, i0, j0, 0.5, 0.5)
+    np.testing.assert_array_equal(out, f(0.25) * acc)
+    d0 = rng.uniform(0, 1, (2, 16384, 4)).astype(np.float32)
+    n1 = rng.uniform(0, 1, (1, 8192, 4)).astype(np.float32)
+    d1 = rng.uniform(0, 1, (1, 8192, 4)).astype(np.float32)
+    out = o32.reassemble(d0, n1, d1, 1.0, [1e-4] * 6)
+    yy, xx = np.mgrid[0:2, 0:16384]
+    i0 = np.where(xx % 2 == 1, (xx - 1) // 2, xx // 2 - 1)
+    j0 = np.where(yy % 2 == 1, (yy - 1) // 2, yy // 2 - 1)
+    a = np.where(xx % 2 == 1, f(0.25), f(0.75))[..., None]
+    b = np.where(yy % 2 == 1, f(0.25), f(0.75))[..., None]
+    up_n = _fixed_bilinear32(n1, i0, j0, a, b)
+    up_d = _fixed_bilinear32(d1, i0, j0, a, b)
+    want = d0 - (up_n - up_d)
+    # d.x = mix(up(d1).x, d.x, sharpening) with sharpening = 1
+    want[..., 0] = up_d[..., 0] + (want[..., 0] - up_d[..., 0]) * f(1)
+    np.testing.assert_array_equal(out, want)
