@@ -35,15 +35,15 @@ def stage_bytes(stage, texels, levels_px=None):
     """Algorithmic bytes of one stage: each input read once, each output written
     once, FP32 texels (luma 4 B, RGBA 16 B), raw 2 B, RGBA8 4 B."""
     per = {
-        "scale_raw": 2 + 4,             # u16 in, f32 out (per pixel)
-        "raw_stats": 16 + 32,           # 4 f32 texels in, mean+var RGBA out (per quad)
-        "raw_fit": 2 * 32,              # two passes over mean+var (per quad)
-        "green": 4 + 4,
-        "red_blue_ycc": 4 + 4 + 16,
+        "scale_raw": 2 + 4,             # u16 in, f32 out (per pixel; stand-alone operator only)
+        "raw_stats": 8 + 32,            # 4 u16 texels in, mean+var RGBA out (per quad)
+        "raw_fit": 32,                  # the refit pass over mean+var (per quad)
+        "green": 2 + 4,                 # u16 mosaic in (scaled on the fly), f32 green out
+        "red_blue_ycc": 2 + 4 + 16,
         "despeckle": 16 + 16,
         "downsample": 4 * 16 + 16,      # per output texel: 4 input texels + 1 output
         "level_stats": 16 + 16,
-        "level_fit": 2 * 16,            # two passes over the statistics image
+        "level_fit": 16,                # the refit pass over the statistics image
         "denoise": 16 + 16,
         "reassemble": 16 + 16 + 16 / 4 + 16 / 4,
         "tone": 16 + 16 / 4 + 16 / 4 + 4,  # level-0 reassembly fused with the tone map: RGBA8 out

@@ -274,16 +274,38 @@ extern "C" clim_status clim_raw_noise_statistics(clim_ctx* ctx, const clim_image
     return CLIM_OK;
 }
 
-// runs the three regression passes of the jobs (all raw or all level fits)
-clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw) {
-    clim::launch_fit(jobs, raw, ctx->stream);
+// a scaled FP32 mosaic as the demosaic kernels' input
+static clim::MosaicArg scaled_mosaic(const clim_image* scaled, int bayer) {
+    clim::MosaicArg m{};
+    m.im = view(scaled);
+    m.b = make_bayer(bayer);
+    m.u16 = false;
+    return m;
+}
+
+// the statistics kernel of the jobs with the first regression pass
+// (raw fits: smul set reads the jobs' src as the uint16 mosaic scaled with
+// (smul, black), null as the scaled FP32 mosaic)
+clim_status run_stats_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw, const float* smul, float black) {
+    if (raw)
+        clim::launch_raw_stats_fit(jobs, smul, black, ctx->stream);
+    else
+        clim::launch_noise_stats_fit(jobs, ctx->stream);
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
 }
 
-clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& m, const Img& v, const float* nlf_in,
+// the rest of the regression: finalisation, refit, finalisation
+clim_status run_fit_rest(clim_ctx* ctx, clim::FitJobs& jobs, bool raw) {
+    clim::launch_fit_rest(jobs, raw, ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
+clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& src, const Img& m, const Img& v, const float* nlf_in,
                           const float* mult) {
     clim::FitJob jb{};
+    jb.src = src;
     jb.m = m;
     jb.v = v;
     jb.st = ctx->fit + slot;
@@ -313,11 +335,12 @@ extern "C" clim_status clim_compute_raw_noise_statistics(clim_ctx* ctx, const cl
     const int qw = scaled->width / 2, qh = scaled->height / 2;
     CLIM_TRY(owned_alloc(ctx, ctx->op_mean, qw, qh, CLIM_FMT_RGBA32F));
     CLIM_TRY(owned_alloc(ctx, ctx->op_var, qw, qh, CLIM_FMT_RGBA32F));
-    clim::launch_raw_stats(view(scaled), make_bayer(bayer), ctx->op_mean.img, ctx->op_var.img, ctx->stream);
     clim::FitJobs jobs{};
     jobs.njobs = 1;
-    jobs.j[0] = make_fit_job(ctx, 0, ctx->op_mean.img, ctx->op_var.img, nullptr, nullptr);
-    CLIM_TRY(run_fit(ctx, jobs, true));
+    jobs.j[0] = make_fit_job(ctx, 0, view(scaled), ctx->op_mean.img, ctx->op_var.img, nullptr, nullptr);
+    jobs.j[0].b = make_bayer(bayer);
+    CLIM_TRY(run_stats_fit(ctx, jobs, true, nullptr, 0.0f));
+    CLIM_TRY(run_fit_rest(ctx, jobs, true));
     CLIM_TRY(read_fit(ctx, 0, 1));
     if (ctx->fit_host[0].status == 1)
         return ctx_fail(ctx, CLIM_ENUMERIC, "computeRawNoiseStatistics: no sample in range");
@@ -331,7 +354,8 @@ extern "C" clim_status clim_interpolate_green(clim_ctx* ctx, const clim_image* s
     CLIM_TRY(check_img(ctx, scaled, CLIM_FMT_R32F, "scaled"));
     CLIM_TRY(check_img(ctx, green, CLIM_FMT_R32F, "green"));
     CLIM_TRY(check_same(ctx, scaled, green, "interpolateGreen"));
-    clim::launch_green(view(scaled), view(green), bayer, host_params(&luma_variance, 1), ctx->stream);
+    clim::launch_green(scaled_mosaic(scaled, bayer), view(green), bayer, host_params(&luma_variance, 1),
+                       ctx->stream);
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
 }
@@ -345,7 +369,7 @@ extern "C" clim_status clim_interpolate_red_blue(clim_ctx* ctx, const clim_image
     CLIM_TRY(check_img(ctx, rgb, CLIM_FMT_RGBA32F, "rgb"));
     CLIM_TRY(check_same(ctx, scaled, green, "interpolateRedBlue"));
     CLIM_TRY(check_same(ctx, scaled, rgb, "interpolateRedBlue"));
-    clim::launch_red_blue(view(scaled), view(green), view(rgb), bayer, make_bayer(bayer),
+    clim::launch_red_blue(scaled_mosaic(scaled, bayer), view(green), view(rgb), bayer, make_bayer(bayer),
                           host_params(&chroma_variance, 1), rotate_180 ? 1 : 0, false, 1.0f, nullptr, ctx->stream);
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
@@ -407,11 +431,11 @@ extern "C" clim_status clim_compute_noise_statistics(clim_ctx* ctx, const clim_i
     if (!ctx || !nlf_out) return ctx_fail(ctx, CLIM_EINVAL, "computeNoiseStatistics: null argument");
     CLIM_TRY(check_img(ctx, in, CLIM_FMT_RGBA32F, "in"));
     CLIM_TRY(owned_alloc(ctx, ctx->op_stats, in->width, in->height, CLIM_FMT_RGBA32F));
-    clim::launch_noise_stats(view(in), ctx->op_stats.img, ctx->stream);
     clim::FitJobs jobs{};
     jobs.njobs = 1;
-    jobs.j[0] = make_fit_job(ctx, 1, ctx->op_stats.img, ctx->op_stats.img, nullptr, nullptr);
-    CLIM_TRY(run_fit(ctx, jobs, false));
+    jobs.j[0] = make_fit_job(ctx, 1, view(in), ctx->op_stats.img, ctx->op_stats.img, nullptr, nullptr);
+    CLIM_TRY(run_stats_fit(ctx, jobs, false, nullptr, 0.0f));
+    CLIM_TRY(run_fit_rest(ctx, jobs, false));
     CLIM_TRY(read_fit(ctx, 1, 1));
     if (ctx->fit_host[1].status == 1)
         return ctx_fail(ctx, CLIM_ENUMERIC, "computeNoiseStatistics: no sample in range");

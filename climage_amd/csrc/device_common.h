@@ -118,9 +118,27 @@ __device__ __forceinline__ void stage(int tid, LoadF load, StoreF store) {
 // rows); the mosaic width is even.
 __device__ __forceinline__ float scale_texel(uint32_t v16, int site, float4 smul, float black) {
     const float sm = site == 0 ? smul.x : site == 1 ? smul.y : site == 2 ? smul.z : smul.w;
-    const float s = sm * ((float)v16 / 65535.0f - black);
+    const float s = sm * (clim_div65535(v16) - black);  // = v16 / 65535.0f, see clim_div65535
     return s > 0.0f ? s : 0.0f;
 }
+
+// The mosaic as FP32 texels with clamp-to-edge: the scaled FP32 mosaic
+// (U16 = false), or the uint16 mosaic scaled on the fly exactly as
+// scaleRawData does (U16 = true), which saves writing and re-reading the
+// scaled plane.
+template <bool U16>
+struct MosaicSrc {
+    Img im;
+    Bayer b;
+    float4 smul;
+    float black;
+    __device__ __forceinline__ float operator()(int x, int y) const {
+        if (!U16) return ld1(im, x, y);
+        const int xc = clampi(x, 0, im.w - 1), yc = clampi(y, 0, im.h - 1);
+        const uint32_t v16 = reinterpret_cast<const uint16_t*>(im.p + (size_t)yc * im.pitch)[xc];
+        return scale_texel(v16, b.site(xc, yc), smul, black);
+    }
+};
 
 struct RawQuadLoader {
     Img raw;

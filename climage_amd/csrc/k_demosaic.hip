@@ -6,6 +6,7 @@
 //
 // Reference: cl/demosaic.cl (cl/ = CLImage/app/src/main/cpp/OpenCL/); line
 // numbers are cited per kernel.  All loads clamp to the image.
+#include "fit_acc.h"
 #include "kernels.h"
 #include "tiles.h"
 
@@ -20,11 +21,7 @@ __global__ void __launch_bounds__(256) k_scale_raw(Img raw, Img out, Bayer b, fl
     const int y = blockIdx.y * 4 + threadIdx.y;
     if (x >= out.w || y >= out.h) return;
     const uint16_t v16 = reinterpret_cast<const uint16_t*>(raw.p + (size_t)y * raw.pitch)[x];
-    const int c = b.site(x, y);
-    const float sm = c == 0 ? smul.x : c == 1 ? smul.y : c == 2 ? smul.z : smul.w;
-    const float v = (float)v16 / 65535.0f;
-    const float s = sm * (v - black);
-    wrow_f(out, y)[x] = mmax(s, 0.0f);
+    wrow_f(out, y)[x] = scale_texel(v16, b.site(x, y), smul, black);
 }
 
 void launch_scale_raw(const Img& raw, const Img& out, const Bayer& b, const float smul[4], float black,
@@ -72,6 +69,66 @@ void launch_raw_stats(const Img& scaled, const Bayer& b, const Img& mean, const 
     hipLaunchKernelGGL(k_raw_stats, grd, blk, 0, st, scaled, b, mean, var);
 }
 
+// rawNoiseStatistics with the first regression pass of
+// computeRawNoiseStatistics accumulated on the way: each of the grid's blocks
+// takes a contiguous run of 16x16-quad tiles and keeps its regression sums
+// across them (one partial per block).
+template <bool U16>
+__global__ void __launch_bounds__(256) k_raw_stats_fit(FitJobs jobs, float4 smul, float black) {
+    constexpr int T = 16, R = 2, S = T + 2 * R;
+    using L = Layout<4, false, false>;
+    __shared__ QuadTile<T> sm;
+    __shared__ clim_fx128 tot[L::K];
+    static_assert(sizeof(QuadTile<T>) >= 1024 * sizeof(clim_fx128), "the tile storage serves as the reduction buffer");
+    const FitJob& jb = jobs.j[job_of_block(jobs, blockIdx.x)];
+    const int bi = blockIdx.x - jb.block_base;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const Bayer& b = jb.b;
+    const MosaicSrc<U16> src{jb.src, b, smul, black};
+    const int tiles_x = (jb.m.w + T - 1) / T;
+    const long long ntiles = (long long)tiles_x * ((jb.m.h + T - 1) / T);
+    const int t0 = (int)(ntiles * bi / jb.nblocks), t1 = (int)(ntiles * (bi + 1) / jb.nblocks);
+    Acc<4, false, false> acc;
+    acc.zero();
+    for (int t = t0; t < t1; t++) {
+        const int qx0 = (t % tiles_x) * T, qy0 = (t / tiles_x) * T;
+        stage<S * S, 256>(
+            tid,
+            [&](int i) {
+                const int qx = qx0 - R + i % S, qy = qy0 - R + i / S;
+                return make_float4(src(2 * qx + b.ox[0], 2 * qy + b.oy[0]), src(2 * qx + b.ox[1], 2 * qy + b.oy[1]),
+                                   src(2 * qx + b.ox[2], 2 * qy + b.oy[2]), src(2 * qx + b.ox[3], 2 * qy + b.oy[3]));
+            },
+            [&](int i, float4 v) { sm.q[i / S][i % S] = v; });
+        __syncthreads();
+        sm.hsums(tid);
+        __syncthreads();
+        const int qx = qx0 + tx, qy = qy0 + ty;
+        const bool inside = qx < jb.m.w && qy < jb.m.h;
+        float m[4], v[4];
+        sm.stats(tx, ty, m, v);
+        if (inside) {
+            wrow_f4(jb.m, qy)[qx] = make_float4(m[0], m[1], m[2], m[3]);
+            wrow_f4(jb.v, qy)[qx] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+        acc.add(m, v, inside);
+        __syncthreads();  // the next tile reuses the LDS
+    }
+    // the last tile is done (the loop ends with a barrier): its LDS is free
+    store_block_partials(acc, reinterpret_cast<clim_fx128*>(&sm), tot, jb.partials, bi, kFitBlocksPerJob);
+}
+
+void launch_raw_stats_fit(FitJobs& jobs, const float* smul, float black, hipStream_t st) {
+    long long tiles[5];
+    for (int j = 0; j < jobs.njobs; j++) tiles[j] = (long long)((jobs.j[j].m.w + 15) / 16) * ((jobs.j[j].m.h + 15) / 16);
+    const int grid = fit_share_blocks(jobs, tiles, 1024);
+    if (smul)
+        hipLaunchKernelGGL(k_raw_stats_fit<true>, dim3(grid), dim3(256), 0, st, jobs,
+                           make_float4(smul[0], smul[1], smul[2], smul[3]), black);
+    else
+        hipLaunchKernelGGL(k_raw_stats_fit<false>, dim3(grid), dim3(256), 0, st, jobs, make_float4(0, 0, 0, 0), 0.0f);
+}
+
 // --------------------------------------------------------- interpolateGreen
 // cl/demosaic.cl:96-129 and :160-227.  Tile of 32x16 output pixels; the raw
 // tile carries a 3-pixel halo, the |Sobel| responses a 2-pixel halo, so every
@@ -82,7 +139,8 @@ constexpr int GR_X = GT_X + 6, GR_Y = GT_Y + 6;   // raw tile
 constexpr int GS_X = GT_X + 4, GS_Y = GT_Y + 4;   // sobel tile
 }
 
-__global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int gyo, ParamRef lv) {
+template <bool U16>
+__global__ void __launch_bounds__(256) k_green(MosaicSrc<U16> raw, Img green, int gxo, int gyo, ParamRef lv) {
     __shared__ float r[GR_Y][GR_X + 1];
     __shared__ float sgx[GS_Y][GS_X + 1];
     __shared__ float sgy[GS_Y][GS_X + 1];
@@ -91,7 +149,7 @@ __global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int 
     const int x0 = tl.x * GT_X, y0 = tl.y * GT_Y;
     const int tid = threadIdx.y * 32 + threadIdx.x;
     stage<GR_X * GR_Y, 256>(
-        tid, [&](int i) { return ld1(raw, x0 - 3 + i % GR_X, y0 - 3 + i / GR_X); },
+        tid, [&](int i) { return raw(x0 - 3 + i % GR_X, y0 - 3 + i / GR_X); },
         [&](int i, float v) { r[i / GR_X][i % GR_X] = v; });
     __syncthreads();
     // sobel(), cl/demosaic.cl:96-112, value / sqrt(4.5) then abs()
@@ -189,11 +247,14 @@ __global__ void __launch_bounds__(256) k_green(Img raw, Img green, int gxo, int 
     }
 }
 
-void launch_green(const Img& raw, const Img& green, int bayer, const ParamRef& lv, hipStream_t st) {
+void launch_green(const MosaicArg& raw, const Img& green, int bayer, const ParamRef& lv, hipStream_t st) {
     // raw_green site of the OpenCL table
     const int gx[4] = {0, 0, 0, 0}, gy[4] = {0, 0, 1, 1};
     dim3 blk(32, 8), grd((green.w + GT_X - 1) / GT_X, (green.h + GT_Y - 1) / GT_Y);
-    hipLaunchKernelGGL(k_green, grd, blk, 0, st, raw, green, gx[bayer], gy[bayer], lv);
+    if (raw.u16)
+        hipLaunchKernelGGL(k_green<true>, grd, blk, 0, st, raw.src<true>(), green, gx[bayer], gy[bayer], lv);
+    else
+        hipLaunchKernelGGL(k_green<false>, grd, blk, 0, st, raw.src<false>(), green, gx[bayer], gy[bayer], lv);
 }
 
 // ------------------------------------------------------- interpolateRedBlue
@@ -229,9 +290,10 @@ __device__ __forceinline__ void blend_highlights_px(float& px, float& py, float&
 // plane around it are loaded once, and the quad's four sites are computed in
 // turn.  Which of them is R, G, B or G2 is the same for every quad, so every
 // lane of a wave runs the same code.
-template <bool kFuse>
-__global__ void __launch_bounds__(256) k_red_blue(Img raw, Img green, Img out, Bayer b, int red_is_c1_site,
-                                                  ParamRef cv, int rotate_180, float clip, Mat3 m) {
+template <bool kFuse, bool U16>
+__global__ void __launch_bounds__(256) k_red_blue(MosaicSrc<U16> raw, Img green, Img out, Bayer b,
+                                                  int red_is_c1_site, ParamRef cv, int rotate_180, float clip,
+                                                  Mat3 m) {
     const TileId tl = xcd_tile();
     const int qx = tl.x * 64 + threadIdx.x, qy = tl.y * 4 + threadIdx.y;
     const int x0 = 2 * qx, y0 = 2 * qy;
@@ -244,7 +306,7 @@ __global__ void __launch_bounds__(256) k_red_blue(Img raw, Img green, Img out, B
     for (int j = 0; j < 4; j++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            R[j][i] = ld1(raw, x0 - 1 + i, y0 - 1 + j);
+            R[j][i] = raw(x0 - 1 + i, y0 - 1 + j);
             G[j][i] = ld1(green, x0 - 1 + i, y0 - 1 + j);
         }
 #pragma unroll
@@ -297,7 +359,7 @@ __global__ void __launch_bounds__(256) k_red_blue(Img raw, Img green, Img out, B
     }
 }
 
-void launch_red_blue(const Img& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
+void launch_red_blue(const MosaicArg& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
                      const ParamRef& cv, int rotate_180, bool fuse, float clip, const float c2y[9],
                      hipStream_t st) {
     // cl/demosaic.cl:301: at green sites red is the horizontal difference on
@@ -306,11 +368,18 @@ void launch_red_blue(const Img& raw, const Img& green, const Img& out, int bayer
     Mat3 m;
     for (int i = 0; i < 9; i++) m.m[i] = c2y ? c2y[i] : (i % 4 == 0 ? 1.0f : 0.0f);
     dim3 blk(64, 4), grd((out.w + 127) / 128, (out.h + 7) / 8);  // one quad per thread
-    if (fuse)
-        hipLaunchKernelGGL(k_red_blue<true>, grd, blk, 0, st, raw, green, out, b, red_is_c1, cv, 0, clip, m);
+    if (fuse && raw.u16)
+        hipLaunchKernelGGL((k_red_blue<true, true>), grd, blk, 0, st, raw.src<true>(), green, out, b, red_is_c1, cv,
+                           0, clip, m);
+    else if (fuse)
+        hipLaunchKernelGGL((k_red_blue<true, false>), grd, blk, 0, st, raw.src<false>(), green, out, b, red_is_c1, cv,
+                           0, clip, m);
+    else if (raw.u16)
+        hipLaunchKernelGGL((k_red_blue<false, true>), grd, blk, 0, st, raw.src<true>(), green, out, b, red_is_c1, cv,
+                           rotate_180, clip, m);
     else
-        hipLaunchKernelGGL(k_red_blue<false>, grd, blk, 0, st, raw, green, out, b, red_is_c1, cv, rotate_180,
-                           clip, m);
+        hipLaunchKernelGGL((k_red_blue<false, false>), grd, blk, 0, st, raw.src<false>(), green, out, b, red_is_c1,
+                           cv, rotate_180, clip, m);
 }
 
 // ------------------------------------------- stand-alone pointwise operators

@@ -4,6 +4,7 @@
 // by the YCbCr->camera transformImage on the same pixel).
 //
 // Reference: cl/demosaic.cl (cl/ = CLImage/app/src/main/cpp/OpenCL/).
+#include "fit_acc.h"
 #include "kernels.h"
 #include "tiles.h"
 
@@ -120,6 +121,56 @@ void launch_noise_stats(const Img& in, const Img& out, hipStream_t st) {
     hipLaunchKernelGGL(k_noise_stats, grd, blk, 0, st, in, out);
 }
 
+// noiseStatistics of every pyramid level (one job per level) with the first
+// regression pass of computeNoiseStatistics accumulated on the way: each of
+// the grid's blocks takes a contiguous run of one level's 32x16 tiles and
+// keeps its regression sums across them (one partial per block).
+__global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs) {
+    using L = Layout<3, true, false>;
+    __shared__ StatsTile sm;
+    __shared__ clim_fx128 tot[L::K];
+    static_assert(sizeof(StatsTile) >= 1024 * sizeof(clim_fx128), "the tile storage serves as the reduction buffer");
+    const FitJob& jb = jobs.j[job_of_block(jobs, blockIdx.x)];
+    const int bi = blockIdx.x - jb.block_base;
+    const int tid = threadIdx.x, tx = tid & 31, ty0 = tid >> 5;
+    const Img& in = jb.src;
+    const Img& out = jb.m;
+    const int tiles_x = (out.w + TX - 1) / TX;
+    const long long ntiles = (long long)tiles_x * ((out.h + TY - 1) / TY);
+    const int t0 = (int)(ntiles * bi / jb.nblocks), t1 = (int)(ntiles * (bi + 1) / jb.nblocks);
+    Acc<3, true, false> acc;
+    acc.zero();
+    for (int t = t0; t < t1; t++) {
+        const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * TY;
+        sm.t.load(in, x0, y0, tid);
+        __syncthreads();
+        sm.r.compute(sm.t, tid);
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < 2; rr++) {
+            const int ty = ty0 + 8 * rr;
+            const int x = x0 + tx, y = y0 + ty;
+            const bool inside = x < out.w && y < out.h;
+            const float4 s4 = sm.r.stats(tx, ty);
+            if (inside) wrow_f4(out, y)[x] = s4;
+            const float m[1] = {s4.x};
+            const float v[3] = {s4.y, s4.z, s4.w};
+            acc.add(m, v, inside);
+        }
+        __syncthreads();  // the next tile reuses the LDS
+    }
+    // the last tile is done (the loop ends with a barrier): its LDS is free
+    store_block_partials(acc, reinterpret_cast<clim_fx128*>(&sm), tot, jb.partials, bi, kFitBlocksPerJob);
+}
+
+void launch_noise_stats_fit(FitJobs& jobs, hipStream_t st) {
+    long long tiles[5];
+    for (int j = 0; j < jobs.njobs; j++)
+        tiles[j] = (long long)((jobs.j[j].m.w + TX - 1) / TX) * ((jobs.j[j].m.h + TY - 1) / TY);
+    const int grid = fit_share_blocks(jobs, tiles, 1024);
+    hipLaunchKernelGGL(k_noise_stats_fit, dim3(grid), dim3(256), 0, st, jobs);
+}
+
 // --------------------------------------------------------------- denoiseImage
 // cl/demosaic.cl:735-765.  nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr.  The per-tap
 // division by sigma is a multiplication by 1/sigma and length(d) < edge is
@@ -156,7 +207,8 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
     e0 = e0 * e0;
     e1 = e1 * e1;
     const float i0 = 1.0f / sig0, i1 = 1.0f / sig1, i2 = 1.0f / sig2;
-    float f0 = 0, f1 = 0, f2 = 0, n0 = 0, n1 = 0;
+    float f0 = 0, f1 = 0, f2 = 0;
+    int c0 = 0, c1 = 0;  // weight sums (the weights are 0 or 1)
 #pragma unroll
     for (int j = 0; j < 5; j++) {
 #pragma unroll
@@ -164,19 +216,23 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
             const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
             const float d0 = (a - cY) * i0, d1 = (b - cCb) * i1, d2 = (c - cCr) * i2;
             const float l = d0 * d0 + d1 * d1 + d2 * d2;
-            const float w0 = 1.0f - (l < e0 ? 0.0f : 1.0f);
-            f0 += w0 * a; n0 += w0;
+            const bool in0 = l < e0;
+            const float w0 = in0 ? 1.0f : 0.0f;
+            f0 += w0 * a;
+            c0 += in0;
             if (SAME_EDGE) {
                 f1 += w0 * b;
                 f2 += w0 * c;
             } else {
-                const float w1 = 1.0f - (l < e1 ? 0.0f : 1.0f);
-                f1 += w1 * b; n1 += w1;
+                const bool in1 = l < e1;
+                const float w1 = in1 ? 1.0f : 0.0f;
+                f1 += w1 * b;
                 f2 += w1 * c;
+                c1 += in1;
             }
         }
     }
-    if (SAME_EDGE) n1 = n0;
+    const float n0 = (float)c0, n1 = SAME_EDGE ? n0 : (float)c1;
     // the Cr weight sum equals the Cb one (same threshold, same weights)
     return make_float4(f0 / n0, f1 / n1, f2 / n1, 0.0f);
 }
@@ -315,7 +371,7 @@ struct ToneArgs {
     float pre[9];
     float cam[9];
     float contrast, saturation, slope, ebmul, blacks;
-    float sig0, sig1;
+    float sig0, inv_range, inv_blacks;  // sigmoid(0), 1 / (sigmoid(1) - sigmoid(0)), 1 / (1 - blacks)
     int has_pre;
 };
 
@@ -348,9 +404,10 @@ __device__ __forceinline__ uint32_t tone_px(float4 p, const ToneArgs& a, float r
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const float t = sqrtf(0.95f * rgb[c]);
-        const float tc = (sigmoidf(t, a.slope) - a.sig0) / (a.sig1 - a.sig0);
+        // divisions by constants as multiplications by their reciprocals
+        const float tc = (sigmoidf(t, a.slope) - a.sig0) * a.inv_range;
         rgb[c] = clampf(tc, 0.0f, 1.0f);
-        if (a.blacks > 0.0f) rgb[c] = (rgb[c] - a.blacks) / (1.0f - a.blacks);
+        if (a.blacks > 0.0f) rgb[c] = (rgb[c] - a.blacks) * a.inv_blacks;
         const uint32_t q = (uint32_t)rintf(clampf(rgb[c], 0.0f, 1.0f) * 255.0f);
         packed |= q << (8 * c);
     }
@@ -387,7 +444,9 @@ static ToneArgs make_tone_args(const float* pre, const float cam[9], float contr
     a.ebmul = exposureBias != 0.0f ? clim_expf(exposureBias * 0.693147182f) : 1.0f;
     a.blacks = blacks;
     a.sig0 = 0.5f * (clim_tanhf(slope * 0.0f - 0.3f * slope) + 1.0f);
-    a.sig1 = 0.5f * (clim_tanhf(slope * 1.0f - 0.3f * slope) + 1.0f);
+    const float sig1 = 0.5f * (clim_tanhf(slope * 1.0f - 0.3f * slope) + 1.0f);
+    a.inv_range = 1.0f / (sig1 - a.sig0);
+    a.inv_blacks = 1.0f / (1.0f - blacks);
     return a;
 }
 

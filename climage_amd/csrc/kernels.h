@@ -6,7 +6,10 @@
 namespace clim {
 
 // ----------------------------------------------------- noise-model regression
+constexpr int kFitSums = 21;  // sums of the largest pass (raw first pass: 5 x 4 channels + count)
+
 struct FitState {
+    clim_fx128 tot[kFitSums];  // totals of the current pass
     double A[4], B[4], err2[4];
     unsigned long long n0;
     int status;            // 0 fitted, 1 no sample in range (fallback), 2 no inliers (first fit)
@@ -16,12 +19,16 @@ struct FitState {
 
 constexpr int kFitBlocksPerJob = 1024;
 
-// A fit streams over statistics images: raw fits over the mean and var images
-// of rawNoiseStatistics (float4 R, G, B, G2 each), level fits over the
-// noiseStatistics image (mean Y, var Y, var Cb, var Cr; v unused).
+// A fit's first pass runs inside the statistics kernel, which reads `src`
+// (the scaled mosaic with Bayer table `b`, or an RGBA level) and writes the
+// statistics images; the refit streams over those: raw fits over the mean and
+// var images of rawNoiseStatistics (float4 R, G, B, G2 each), level fits over
+// the noiseStatistics image (mean Y, var Y, var Cb, var Cr; v unused).
 struct FitJob {
+    Img src;
+    Bayer b;
     Img m, v;
-    int nblocks, block_base;  // set by launch_fit
+    int nblocks, block_base;  // set by the launchers
     FitState* st;
     clim_fx128* partials;     // nblocks x accumulator count
     float nlf_in[8];          // fallback model (written to out when no sample is in range)
@@ -33,14 +40,38 @@ struct FitJobs {
     int njobs;
 };
 
-void launch_fit(FitJobs& jobs, bool raw, hipStream_t st);
+// Statistics of every job plus the first regression pass (block partials;
+// sets the jobs' block shares for launch_fit_rest).
+// smul: read the jobs' src as the uint16 mosaic, scaled with (smul, black);
+// null: src is the scaled FP32 mosaic
+void launch_raw_stats_fit(FitJobs& jobs, const float* smul, float black, hipStream_t st);
+void launch_noise_stats_fit(FitJobs& jobs, hipStream_t st);
+// Gives each job a run of blocks in proportion to units[j] (at most one per
+// unit and kFitBlocksPerJob, at least one), about `grid` in all; returns the
+// total.
+int fit_share_blocks(FitJobs& jobs, const long long* units, int grid);
+// the passes after the statistics kernel: combine and solve, refit, combine and solve
+void launch_fit_rest(FitJobs& jobs, bool raw, hipStream_t st);
 
 // ------------------------------------------------------------ demosaic stages
 void launch_scale_raw(const Img& raw, const Img& out, const Bayer& b, const float smul[4], float black,
                       hipStream_t st);
 void launch_raw_stats(const Img& scaled, const Bayer& b, const Img& mean, const Img& var, hipStream_t st);
-void launch_green(const Img& raw, const Img& green, int bayer, const ParamRef& lv, hipStream_t st);
-void launch_red_blue(const Img& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
+// the mosaic a demosaic kernel reads: the scaled FP32 mosaic, or the uint16
+// mosaic with the scaleRawData parameters (scaled on the fly)
+struct MosaicArg {
+    Img im;
+    Bayer b;
+    float4 smul;
+    float black;
+    bool u16;
+    template <bool U16>
+    MosaicSrc<U16> src() const {
+        return MosaicSrc<U16>{im, b, smul, black};
+    }
+};
+void launch_green(const MosaicArg& raw, const Img& green, int bayer, const ParamRef& lv, hipStream_t st);
+void launch_red_blue(const MosaicArg& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
                      const ParamRef& cv, int rotate_180, bool fuse, float clip, const float c2y[9],
                      hipStream_t st);
 void launch_blend_highlights(const Img& in, const Img& out, float clip, hipStream_t st);

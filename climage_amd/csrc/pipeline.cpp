@@ -22,8 +22,9 @@
 using clim::Img;
 using clim::ParamRef;
 
-clim_status run_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw);
-clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& m, const Img& v, const float* nlf_in,
+clim_status run_stats_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw, const float* smul, float black);
+clim_status run_fit_rest(clim_ctx* ctx, clim::FitJobs& jobs, bool raw);
+clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& src, const Img& m, const Img& v, const float* nlf_in,
                           const float* mult);
 clim_status guided_scratch(clim_ctx* ctx, OwnedImg* imgs, int w, int h, clim::GuidedScratch* g);
 int inverse3x3_scaled(const float mf[9], double scale, float out[9]);
@@ -36,7 +37,6 @@ static ParamRef dev_params(const float* p) {
 
 static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int algorithm) {
     Workspace& ws = ctx->ws;
-    CLIM_TRY(owned_alloc(ctx, ws.scaled, w, h, CLIM_FMT_R32F));
     CLIM_TRY(owned_alloc(ctx, ws.green, w, h, CLIM_FMT_R32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbA, w, h, CLIM_FMT_RGBA32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbB, w, h, CLIM_FMT_RGBA32F));
@@ -93,19 +93,20 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         float in8[8] = {0};
         std::memcpy(in8, nlf_in[i], 6 * sizeof(float));
         const Img sv = sub_view(ws.stats[i].img, lvl[i].w, lvl[i].h);
-        jobs.j[i] = make_fit_job(ctx, 1 + i, sv, sv, in8, mult);
+        jobs.j[i] = make_fit_job(ctx, 1 + i, lvl[i], sv, sv, in8, mult);
     }
     jobs.njobs = levels;
     CLIM_TRY_HIP(ctx, hipGetLastError());
     long long nlev = 0;
     for (int i = 0; i < levels; i++) nlev += (long long)lvl[i].w * lvl[i].h;
-    for (int i = 0; i < levels; i++) {
-        ProfScope ps(ctx, ST_LEVEL_STATS, (long long)lvl[i].w * lvl[i].h);
-        clim::launch_noise_stats(lvl[i], jobs.j[i].m, st);
+    // noiseStatistics of every level with the first regression pass, then the refit
+    {
+        ProfScope ps(ctx, ST_LEVEL_STATS, nlev);
+        CLIM_TRY(run_stats_fit(ctx, jobs, false, nullptr, 0.0f));
     }
     {
         ProfScope ps(ctx, ST_LEVEL_FIT, nlev);
-        CLIM_TRY(run_fit(ctx, jobs, false));
+        CLIM_TRY(run_fit_rest(ctx, jobs, false));
     }
 
     // synthesis, coarse to fine
@@ -217,38 +218,44 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
     const int bayer = p->bayerPattern;
     const clim::Bayer b = make_bayer(bayer);
 
-    // scaleRawData; black level normalised as in cpp/raw_converter.cpp:105-106
+    // scaleRawData (black level normalised as in cpp/raw_converter.cpp:105-106)
+    // is applied on the fly by the kernels that read the mosaic
     const long long npx = (long long)w * h;
-    {
-        ProfScope ps(ctx, ST_SCALE, npx);
-        clim::launch_scale_raw(view(raw), ws.scaled.img, b, p->scale_mul, p->black_level / 65535.0f, st);
-    }
+    const float black = p->black_level / 65535.0f;
+    clim::MosaicArg mosaic{};
+    mosaic.im = view(raw);
+    mosaic.b = b;
+    mosaic.smul = make_float4(p->scale_mul[0], p->scale_mul[1], p->scale_mul[2], p->scale_mul[3]);
+    mosaic.black = black;
+    mosaic.u16 = true;
     // computeRawNoiseStatistics (cpp/raw_converter.cpp:108-109): fit slot 0,
     // rawNlf = B; the input rawNlf is the fallback when no quad is in range
-    CLIM_TRY_HIP(ctx, hipGetLastError());
+    clim::FitJobs rjobs{};
+    {
+        const float in8[8] = {0, 0, 0, 0, p->noiseModel.rawNlf[0], p->noiseModel.rawNlf[1], p->noiseModel.rawNlf[2],
+                              p->noiseModel.rawNlf[3]};
+        rjobs.njobs = 1;
+        rjobs.j[0] = make_fit_job(ctx, 0, view(raw), ws.raw_mean.img, ws.raw_var.img, in8, nullptr);
+        rjobs.j[0].b = b;
+    }
     {
         ProfScope ps(ctx, ST_RAW_STATS, npx / 4);
-        clim::launch_raw_stats(ws.scaled.img, b, ws.raw_mean.img, ws.raw_var.img, st);
+        CLIM_TRY(run_stats_fit(ctx, rjobs, true, p->scale_mul, black));
     }
     {
         ProfScope ps(ctx, ST_RAW_FIT, npx / 4);
-        clim::FitJobs jobs{};
-        float in8[8] = {0, 0, 0, 0, p->noiseModel.rawNlf[0], p->noiseModel.rawNlf[1], p->noiseModel.rawNlf[2],
-                        p->noiseModel.rawNlf[3]};
-        jobs.njobs = 1;
-        jobs.j[0] = make_fit_job(ctx, 0, ws.raw_mean.img, ws.raw_var.img, in8, nullptr);
-        CLIM_TRY(run_fit(ctx, jobs, true));
+        CLIM_TRY(run_fit_rest(ctx, rjobs, true));
     }
     const float* rawfit = ctx->fit[0].out;  // device: A[4], B[4]
     // interpolateGreen(rawNlf[1]) and interpolateRedBlue((rawNlf[0]+rawNlf[2])/2)
     // + blendHighlightsImage(clip = 1) + transformImage(cam_ycbcr(rgb_cam))
     {
         ProfScope ps(ctx, ST_GREEN, npx);
-        clim::launch_green(ws.scaled.img, ws.green.img, bayer, dev_params(rawfit + 5), st);
+        clim::launch_green(mosaic, ws.green.img, bayer, dev_params(rawfit + 5), st);
     }
     {
         ProfScope ps(ctx, ST_RED_BLUE, npx);
-        clim::launch_red_blue(ws.scaled.img, ws.green.img, ws.rgbA.img, bayer, b, dev_params(rawfit), 0, true, 1.0f,
+        clim::launch_red_blue(mosaic, ws.green.img, ws.rgbA.img, bayer, b, dev_params(rawfit), 0, true, 1.0f,
                               c2y, st);
     }
     // despeckleImage with the caller's pyramidNlf[0] (cpp/raw_converter.cpp:144-145)

@@ -199,6 +199,17 @@ CLIM_HD float clim_u01(uint32_t r) { return (float)(r >> 8) * 5.9604644775390625
 /* uniform in (0,1]: for logarithms */
 CLIM_HD float clim_u01_open0(uint32_t r) { return (float)((r >> 8) + 1u) * 5.9604644775390625e-08f; }
 
+/* --------------------------------------------------- v / 65535 for uint16 v
+ * The correctly rounded quotient from one reciprocal multiplication and one
+ * fused correction step: q = v * fl(1/65535), e = fma(-q, 65535, v) (exact),
+ * q + e * fl(1/65535) rounded once.  Equal to the IEEE division v / 65535.0f
+ * for every v in [0, 65535] (checked exhaustively). */
+CLIM_HD float clim_div65535(uint32_t v) {
+    const float x = (float)v, r = 1.0f / 65535.0f;
+    const float q = x * r;
+    return fmaf(fmaf(-q, 65535.0f, x), r, q);
+}
+
 /* ------------------------------------------ 128-bit fixed-point accumulator
  * Non-negative doubles below 2^37 are truncated to multiples of 2^-90 and
  * summed as unsigned 128-bit integers.  Integer addition is associative, so

@@ -990,6 +990,9 @@ void P(convert_to_srgb)(const real* in, int w, int h, const float transform[9], 
     const real ebmul = eb != K(0.0) ? clim_expf(eb * K(0.693147182)) : K(1.0);
 #endif
     const real sig0 = sigmoidr(K(0.0), s), sig1 = sigmoidr(K(1.0), s);
+    /* the two per-pixel divisions by constants as multiplications by their
+     * reciprocals (a relaxed-math form) */
+    const real inv_range = K(1.0) / (sig1 - sig0), inv_blacks = K(1.0) / (K(1.0) - blacks);
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < (long)w * h; i++) {
         real v[3] = {in[i * 4 + 0], in[i * 4 + 1], in[i * 4 + 2]};
@@ -1007,9 +1010,9 @@ void P(convert_to_srgb)(const real* in, int w, int h, const float transform[9], 
         for (int r = 0; r < 3; r++) rgb[r] = M[r * 3 + 0] * v[0] + M[r * 3 + 1] * v[1] + M[r * 3 + 2] * v[2];
         for (int c = 0; c < 3; c++) {
             real t = RSQRT(K(0.95) * rgb[c]);
-            real tc = (sigmoidr(t, s) - sig0) / (sig1 - sig0);
+            real tc = (sigmoidr(t, s) - sig0) * inv_range;
             rgb[c] = clampr(tc, K(0.0), K(1.0));
-            if (blacks > K(0.0)) rgb[c] = (rgb[c] - blacks) / (K(1.0) - blacks);
+            if (blacks > K(0.0)) rgb[c] = (rgb[c] - blacks) * inv_blacks;
         }
         for (int c = 0; c < 3; c++) {
             real q = clampr(rgb[c], K(0.0), K(1.0)) * K(255.0);

@@ -64,3 +64,10 @@ void om_fit_q(const uint64_t* sums, unsigned long long n, double* out) {
     clim_fit_solve_q(&q, n, &out[0], &out[1]);
     out[2] = clim_fit_rss_q(&q, n, out[0], out[1]);
 }
+/* out[v] = 1 when clim_div65535(v) and (float)v / 65535.0f have the same bits */
+void om_div65535_check(uint8_t* out) {
+    for (uint32_t v = 0; v <= 65535; v++) {
+        const float a = clim_div65535(v), b = (float)v / 65535.0f;
+        out[v] = clim_f2u(a) == clim_f2u(b);
+    }
+}

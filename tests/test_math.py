@@ -165,3 +165,14 @@ def test_fit_q_solve_and_moment_residuals(L):
                     for x, y in zip(mi, vi))
         scale = sum(Fraction(y, 2 ** 41) ** 2 for y in vi)
         assert abs(Fraction(rss) - exact) <= Fraction(1, 10 ** 12) * scale + Fraction(1, 10 ** 30), trial
+
+
+def test_div65535_equals_division_for_every_uint16(L):
+    """The kernels' scaleRawData quotient (reciprocal + one fused correction)
+    is bit-identical to the IEEE division for all 65536 inputs."""
+    out = np.zeros(65536, np.uint8)
+    L.om_div65535_check(out.ctypes.data_as(C.c_void_p))
+    assert out.all(), np.flatnonzero(out == 0)[:10]
+    # and against numpy's float32 division
+    v = np.arange(65536, dtype=np.float32)
+    assert np.array_equal(v / np.float32(65535), (v / np.float32(65535)).astype(np.float32))
