@@ -128,7 +128,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              f"(frames are independent, see DESIGN.md for how the path shards)", file=sys.stderr)
+              f"(frames are independent: a burst shards whole frames across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     w, h, bayer = args.width, args.height, ca.RGGB
