@@ -39,7 +39,7 @@ def stage_bytes(stage, texels, levels_px=None):
         "raw_stats": 8 + 32,            # 4 u16 texels in, mean+var RGBA out (per quad)
         "raw_fit": 32,                  # the refit pass over mean+var (per quad)
         "green": 2 + 4,                 # u16 mosaic in (scaled on the fly), f32 green out
-        "red_blue_ycc": 2 + 4 + 16,
+        "red_blue_ycc": 2 + 4 + 16,     # + highlights, YCbCr and despeckle in the same kernel
         "despeckle": 16 + 16,
         "downsample": 4 * 16 + 16,      # per output texel: 4 input texels + 1 output
         "level_stats": 16 + 16,
@@ -55,7 +55,7 @@ def stage_bytes(stage, texels, levels_px=None):
 # level), so that their event timing is that kernel's launch duration; the
 # names are the kernel instances this workload runs (as rocprofv3 lists them)
 SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green": "k_green",
-                 "red_blue_ycc": "k_red_blue<true>", "despeckle": "k_despeckle",
+                 "red_blue_ycc": "k_rb_despeckle<true>", "despeckle": "k_despeckle",
                  "downsample": "k_downsample<true>", "level_stats": "k_noise_stats",
                  "denoise": "k_bilateral<true>", "reassemble": "k_reassemble<true, false>",
                  "tone": "k_reassemble<true, true>"}

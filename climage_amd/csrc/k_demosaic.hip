@@ -286,20 +286,16 @@ __device__ __forceinline__ void blend_highlights_px(float& px, float& py, float&
     }
 }
 
-// One 2x2 quad per thread: the 4x4 windows of the mosaic and of the green
-// plane around it are loaded once, and the quad's four sites are computed in
-// turn.  Which of them is R, G, B or G2 is the same for every quad, so every
-// lane of a wave runs the same code.
+// interpolateRedBlue of one 2x2 quad at (x0, y0) (even): the 4x4 windows of
+// the mosaic and of the green plane around it are loaded once and the four
+// sites computed in turn; with kFuse, blendHighlights and the camera->YCbCr
+// transform follow on each pixel.  out[dy][dx] = (R, G, B) or (Y, Cb, Cr).
+// Which site is R, G, B or G2 is the same for every quad, so every lane of a
+// wave runs the same code.
 template <bool kFuse, bool U16>
-__global__ void __launch_bounds__(256) k_red_blue(MosaicSrc<U16> raw, Img green, Img out, Bayer b,
-                                                  int red_is_c1_site, ParamRef cv, int rotate_180, float clip,
-                                                  Mat3 m) {
-    const TileId tl = xcd_tile();
-    const int qx = tl.x * 64 + threadIdx.x, qy = tl.y * 4 + threadIdx.y;
-    const int x0 = 2 * qx, y0 = 2 * qy;
-    if (x0 >= out.w || y0 >= out.h) return;
-    float chromaVariance = cv.dev ? (cv.dev[4] + cv.dev[6]) / 2.0f : cv.host[0];
-    chromaVariance = mmax(chromaVariance, 1e-6f);
+__device__ __forceinline__ void red_blue_quad(const MosaicSrc<U16>& raw, const Img& green, int x0, int y0,
+                                              const Bayer& b, int red_is_c1_site, float chromaVariance, float clip,
+                                              const Mat3& m, float3 out[2][2]) {
     // R[j][i], G[j][i]: mosaic and green at (x0 - 1 + i, y0 - 1 + j), clamped
     float R[4][4], G[4][4];
 #pragma unroll
@@ -313,7 +309,6 @@ __global__ void __launch_bounds__(256) k_red_blue(MosaicSrc<U16> raw, Img green,
     for (int dy = 0; dy < 2; dy++) {
 #pragma unroll
         for (int dx = 0; dx < 2; dx++) {
-            const int x = x0 + dx, y = y0 + dy;
             const int color = b.site(dx, dy);  // x0, y0 are even
             const int X = 1 + dx, Y = 1 + dy;
             const float g = G[Y][X];
@@ -347,6 +342,32 @@ __global__ void __launch_bounds__(256) k_red_blue(MosaicSrc<U16> raw, Img green,
                 const float tz = m.m[6] * px + m.m[7] * py + m.m[8] * pz;
                 px = tx; py = ty; pz = tz;
             }
+            out[dy][dx] = make_float3(px, py, pz);
+        }
+    }
+}
+
+__device__ __forceinline__ float chroma_variance(const ParamRef& cv) {
+    const float v = cv.dev ? (cv.dev[4] + cv.dev[6]) / 2.0f : cv.host[0];
+    return mmax(v, 1e-6f);
+}
+
+// one quad per thread
+template <bool kFuse, bool U16>
+__global__ void __launch_bounds__(256) k_red_blue(MosaicSrc<U16> raw, Img green, Img out, Bayer b,
+                                                  int red_is_c1_site, ParamRef cv, int rotate_180, float clip,
+                                                  Mat3 m) {
+    const TileId tl = xcd_tile();
+    const int qx = tl.x * 64 + threadIdx.x, qy = tl.y * 4 + threadIdx.y;
+    const int x0 = 2 * qx, y0 = 2 * qy;
+    if (x0 >= out.w || y0 >= out.h) return;
+    float3 q[2][2];
+    red_blue_quad<kFuse, U16>(raw, green, x0, y0, b, red_is_c1_site, chroma_variance(cv), clip, m, q);
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+            const int x = x0 + dx, y = y0 + dy;
             if (x >= out.w || y >= out.h) continue;
             int ox = x, oy = y;
             if (rotate_180) {
@@ -354,7 +375,7 @@ __global__ void __launch_bounds__(256) k_red_blue(MosaicSrc<U16> raw, Img green,
                 oy = out.h - y;
                 if (ox >= out.w || oy >= out.h) continue;
             }
-            wrow_f4(out, oy)[ox] = make_float4(px, py, pz, 0.0f);
+            wrow_f4(out, oy)[ox] = make_float4(q[dy][dx].x, q[dy][dx].y, q[dy][dx].z, 0.0f);
         }
     }
 }
@@ -433,11 +454,10 @@ __device__ __forceinline__ float median9(float v0, float v1, float v2, float v3,
     return a1;
 }
 
-__global__ void __launch_bounds__(256) k_despeckle(Img in, Img out, float var_a, float var_b) {
-    const TileId tl = xcd_tile();
-    const int x = tl.x * 64 + threadIdx.x;
-    const int y = tl.y * 4 + threadIdx.y;
-    if (x >= out.w || y >= out.h) return;
+// One pixel; fetch(dx, dy) returns the YCbCr texel at offset (dx, dy),
+// clamped to the image.
+template <typename Fetch>
+__device__ __forceinline__ float4 despeckle_px(Fetch fetch, float var_a, float var_b) {
     // read order of fast_median3x3
     const int ox[9] = {0, 1, 0, 1, 0, 1, -1, -1, -1};
     const int oy[9] = {-1, -1, 0, 0, 1, 1, 1, 0, -1};
@@ -445,7 +465,7 @@ __global__ void __launch_bounds__(256) k_despeckle(Img in, Img out, float var_a,
     float cb[9], cr[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-        const float4 p = ld4(in, x + ox[k], y + oy[k]);
+        const float4 p = fetch(ox[k], oy[k]);
         const float v = p.x;
         secMax = v <= firstMax && v > secMax ? v : secMax;
         secMax = v > firstMax ? firstMax : secMax;
@@ -463,12 +483,82 @@ __global__ void __launch_bounds__(256) k_despeckle(Img in, Img out, float var_a,
     const float minVal = mixf(secMin, firstMin, smoothstep(sigma, 4.0f * sigma, secMin - firstMin));
     const float maxVal = mixf(secMax, firstMax, smoothstep(sigma, 4.0f * sigma, firstMax - secMax));
     sample = clampf(sample, minVal, maxVal);
-    wrow_f4(out, y)[x] = make_float4(sample, mcb, mcr, 0.0f);
+    return make_float4(sample, mcb, mcr, 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_despeckle(Img in, Img out, float var_a, float var_b) {
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x;
+    const int y = tl.y * 4 + threadIdx.y;
+    if (x >= out.w || y >= out.h) return;
+    wrow_f4(out, y)[x] = despeckle_px([&](int dx, int dy) { return ld4(in, x + dx, y + dy); }, var_a, var_b);
 }
 
 void launch_despeckle(const Img& in, const Img& out, float var_a, float var_b, hipStream_t st) {
     dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
     hipLaunchKernelGGL(k_despeckle, grd, blk, 0, st, in, out, var_a, var_b);
+}
+
+// ------------------------------ interpolateRedBlue + ... + despeckleImage
+// The pipeline's interpolateRedBlue (with blendHighlights and the
+// camera->YCbCr transform) and despeckleImage in one kernel: a 64x24 output
+// tile's YCbCr values plus a 2-pixel margin (34x14 quads) are computed into
+// LDS, and the despeckle reads its 3x3 neighbourhoods there, at the clamped
+// image coordinates the stand-alone kernels would read.  The level-0 YCbCr
+// image between the two is never written.
+namespace {
+constexpr int RD_X = 64, RD_Y = 24;                 // output tile
+constexpr int RD_RX = RD_X + 4, RD_RY = RD_Y + 4;   // YCbCr region (even origin)
+constexpr int RD_QX = RD_RX / 2, RD_QY = RD_RY / 2;  // its quads
+}
+
+template <bool U16>
+__global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img green, Img out, Bayer b,
+                                                      int red_is_c1_site, ParamRef cv, float clip, Mat3 m,
+                                                      float var_a, float var_b) {
+    __shared__ float4 t[RD_RY][RD_RX];
+    const TileId tl = xcd_tile();
+    const int x0 = tl.x * RD_X, y0 = tl.y * RD_Y;
+    const int rx0 = x0 - 2, ry0 = y0 - 2;
+    const int tid = threadIdx.x;
+    const float cvar = chroma_variance(cv);
+    for (int q = tid; q < RD_QX * RD_QY; q += 256) {
+        const int qx = q % RD_QX, qy = q / RD_QX;
+        float3 o[2][2];
+        red_blue_quad<true, U16>(raw, green, rx0 + 2 * qx, ry0 + 2 * qy, b, red_is_c1_site, cvar, clip, m, o);
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 2; dx++)
+                t[2 * qy + dy][2 * qx + dx] = make_float4(o[dy][dx].x, o[dy][dx].y, o[dy][dx].z, 0.0f);
+    }
+    __syncthreads();
+    for (int i = tid; i < RD_X * RD_Y; i += 256) {
+        const int x = x0 + i % RD_X, y = y0 + i / RD_X;
+        if (x >= out.w || y >= out.h) continue;
+        const float4 d = despeckle_px(
+            [&](int dx, int dy) {
+                const int gx = clampi(x + dx, 0, out.w - 1), gy = clampi(y + dy, 0, out.h - 1);
+                return t[gy - ry0][gx - rx0];
+            },
+            var_a, var_b);
+        wrow_f4(out, y)[x] = d;
+    }
+}
+
+void launch_rb_despeckle(const MosaicArg& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
+                         const ParamRef& cv, float clip, const float c2y[9], float var_a, float var_b,
+                         hipStream_t st) {
+    const int red_is_c1 = (bayer == 3 || bayer == 0) ? 1 : 3;  // as in launch_red_blue
+    Mat3 m;
+    for (int i = 0; i < 9; i++) m.m[i] = c2y[i];
+    dim3 blk(256), grd((out.w + RD_X - 1) / RD_X, (out.h + RD_Y - 1) / RD_Y);
+    if (raw.u16)
+        hipLaunchKernelGGL(k_rb_despeckle<true>, grd, blk, 0, st, raw.src<true>(), green, out, b, red_is_c1, cv,
+                           clip, m, var_a, var_b);
+    else
+        hipLaunchKernelGGL(k_rb_despeckle<false>, grd, blk, 0, st, raw.src<false>(), green, out, b, red_is_c1, cv,
+                           clip, m, var_a, var_b);
 }
 
 }  // namespace clim

@@ -77,6 +77,10 @@ void launch_red_blue(const MosaicArg& raw, const Img& green, const Img& out, int
 void launch_blend_highlights(const Img& in, const Img& out, float clip, hipStream_t st);
 void launch_transform(const Img& in, const Img& out, const float mat[9], hipStream_t st);
 void launch_despeckle(const Img& in, const Img& out, float var_a, float var_b, hipStream_t st);
+// interpolateRedBlue + blendHighlights + transformImage(c2y) + despeckleImage
+void launch_rb_despeckle(const MosaicArg& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
+                         const ParamRef& cv, float clip, const float c2y[9], float var_a, float var_b,
+                         hipStream_t st);
 
 // ------------------------------------------------------------- pyramid stages
 void launch_downsample(const Img& in, const Img& out, hipStream_t st);

@@ -253,16 +253,13 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
         ProfScope ps(ctx, ST_GREEN, npx);
         clim::launch_green(mosaic, ws.green.img, bayer, dev_params(rawfit + 5), st);
     }
-    {
-        ProfScope ps(ctx, ST_RED_BLUE, npx);
-        clim::launch_red_blue(mosaic, ws.green.img, ws.rgbA.img, bayer, b, dev_params(rawfit), 0, true, 1.0f,
-                              c2y, st);
-    }
-    // despeckleImage with the caller's pyramidNlf[0] (cpp/raw_converter.cpp:144-145)
+    // despeckleImage with the caller's pyramidNlf[0] (cpp/raw_converter.cpp:144-145),
+    // in the same kernel as interpolateRedBlue
     const float* np0 = p->noiseModel.pyramidNlf[0];
     {
-        ProfScope ps(ctx, ST_DESPECKLE, npx);
-        clim::launch_despeckle(ws.rgbA.img, ws.rgbB.img, np0[0], np0[3], st);
+        ProfScope ps(ctx, ST_RED_BLUE, npx);
+        clim::launch_rb_despeckle(mosaic, ws.green.img, ws.rgbB.img, bayer, b, dev_params(rawfit), 1.0f, c2y, np0[0],
+                                  np0[3], st);
     }
     CLIM_TRY_HIP(ctx, hipGetLastError());
 

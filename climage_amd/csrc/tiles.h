@@ -67,8 +67,9 @@ struct StatsRows {
             for (int j = 0; j < 5; j++) a += h[k][ly + j][lx];
             s[k] = a;
         }
-        return make_float4(s[0] / 25.0f, (s[3] - (s[0] * s[0]) / 25.0f) / 25.0f,
-                           (s[4] - (s[1] * s[1]) / 25.0f) / 25.0f, (s[5] - (s[2] * s[2]) / 25.0f) / 25.0f);
+        // x / 25.0f by clim_div25 (the same bits for every finite x)
+        return make_float4(clim_div25(s[0]), clim_div25(s[3] - clim_div25(s[0] * s[0])),
+                           clim_div25(s[4] - clim_div25(s[1] * s[1])), clim_div25(s[5] - clim_div25(s[2] * s[2])));
     }
 };
 
@@ -113,8 +114,8 @@ struct QuadTile {
         }
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            m[c] = s[c] / 25.0f;
-            v[c] = (s[4 + c] - (s[c] * s[c]) / 25.0f) / 25.0f;
+            m[c] = clim_div25(s[c]);  // = s[c] / 25.0f for every finite value
+            v[c] = clim_div25(s[4 + c] - clim_div25(s[c] * s[c]));
         }
     }
 };

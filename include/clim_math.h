@@ -210,6 +210,18 @@ CLIM_HD float clim_div65535(uint32_t v) {
     return fmaf(fmaf(-q, 65535.0f, x), r, q);
 }
 
+/* ---------------------------------------------------------------- x / 25
+ * One reciprocal multiplication and one fused correction (zeros passed
+ * through with their sign): equal to the IEEE division x / 25.0f for every
+ * finite float x (checked over all 2^32 bit patterns).  Not a general
+ * method: the same form differs from x / 1.41421356f for millions of x. */
+CLIM_HD float clim_div25(float x) {
+    const float r = 1.0f / 25.0f;
+    const float q = x * r;
+    const float c = fmaf(fmaf(-q, 25.0f, x), r, q);
+    return x == 0.0f ? q : c;
+}
+
 /* ------------------------------------------ 128-bit fixed-point accumulator
  * Non-negative doubles below 2^37 are truncated to multiples of 2^-90 and
  * summed as unsigned 128-bit integers.  Integer addition is associative, so

@@ -71,3 +71,15 @@ void om_div65535_check(uint8_t* out) {
         out[v] = clim_f2u(a) == clim_f2u(b);
     }
 }
+/* mismatches between clim_div25(x) and x / 25.0f over the bit patterns
+ * [lo, hi) (finite x only) */
+long om_div25_mismatches(uint32_t lo, uint32_t hi) {
+    long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = lo; i < (int64_t)hi; i++) {
+        const float x = clim_u2f((uint32_t)i);
+        if (!isfinite(x)) continue;
+        if (clim_f2u(clim_div25(x)) != clim_f2u(x / 25.0f)) bad++;
+    }
+    return bad;
+}

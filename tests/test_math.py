@@ -176,3 +176,17 @@ def test_div65535_equals_division_for_every_uint16(L):
     # and against numpy's float32 division
     v = np.arange(65536, dtype=np.float32)
     assert np.array_equal(v / np.float32(65535), (v / np.float32(65535)).astype(np.float32))
+
+
+def test_div25_equals_division(L):
+    """clim_div25 (reciprocal + fused correction) against x / 25.0f.  Division
+    by a constant commutes with scaling by 2^k unless the quotient is
+    subnormal (the correction term is then a multiple of a normal ulp, so
+    exact), so every float of one ordinary binade plus every float below
+    2^-121 (quotient subnormal) decide it, both signs; the build also checked
+    all 2^32 patterns once."""
+    L.om_div25_mismatches.restype = C.c_long
+    ranges = [(0x3F800000, 0x40000000), (0x00000000, 0x03000000)]
+    for lo, hi in ranges:
+        for sign in (0, 0x80000000):
+            assert L.om_div25_mismatches(C.c_uint32(lo | sign), C.c_uint32(hi | sign)) == 0, (hex(lo), sign)
