@@ -31,6 +31,15 @@ NOISE_A = [0.0005, 0.001, 0.002, 0.004, 0.008, 0.016]
 HBM_PEAK_GBS = 8000.0
 
 
+def path_bytes_per_px(levels):
+    """Algorithmic bytes per output pixel of the whole unfused path: every
+    reference stage reads each input once and writes each output once at its
+    own resolution, FP32 texels, each host fit one read of its statistics
+    image: 206 + 60 S(L-2) + 80 S(L-1), S(k) = sum of 4^-l for l <= k."""
+    S = lambda k: sum(4.0 ** -l for l in range(k + 1))
+    return 206 + 60 * S(levels - 2) + 80 * S(levels - 1)
+
+
 def stage_bytes(stage, texels, levels_px=None):
     """Algorithmic bytes of one stage: each input read once, each output written
     once, FP32 texels (luma 4 B, RGBA 16 B), raw 2 B, RGBA8 4 B."""
@@ -226,6 +235,9 @@ def main():
         "config": {"workload": f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, 5-level bilateral pyramid",
                    "frames_per_step": len(raws), "pixels_per_frame": w * h},
         "roofline": roofline,
+        "path_roofline": {"bytes_per_px": path_bytes_per_px(5),
+                          "achieved": round(path_bytes_per_px(5) * mpix * 1e6 / 1e9, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(path_bytes_per_px(5) * mpix * 1e6 / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "stages": stages,
     }

@@ -216,18 +216,20 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
             const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
             const float d0 = (a - cY) * i0, d1 = (b - cCb) * i1, d2 = (c - cCr) * i2;
             const float l = d0 * d0 + d1 * d1 + d2 * d2;
+            // f += w * s with w in {0, 1}: the product is exact, so the fused
+            // form rounds once on the same value (the same bits)
             const bool in0 = l < e0;
             const float w0 = in0 ? 1.0f : 0.0f;
-            f0 += w0 * a;
+            f0 = fmaf(w0, a, f0);
             c0 += in0;
             if (SAME_EDGE) {
-                f1 += w0 * b;
-                f2 += w0 * c;
+                f1 = fmaf(w0, b, f1);
+                f2 = fmaf(w0, c, f2);
             } else {
                 const bool in1 = l < e1;
                 const float w1 = in1 ? 1.0f : 0.0f;
-                f1 += w1 * b;
-                f2 += w1 * c;
+                f1 = fmaf(w1, b, f1);
+                f2 = fmaf(w1, c, f2);
                 c1 += in1;
             }
         }
