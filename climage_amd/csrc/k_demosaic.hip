@@ -152,21 +152,17 @@ __global__ void __launch_bounds__(256) k_green(MosaicSrc<U16> raw, Img green, in
         tid, [&](int i) { return raw(x0 - 3 + i % GR_X, y0 - 3 + i / GR_X); },
         [&](int i, float v) { r[i / GR_X][i % GR_X] = v; });
     __syncthreads();
-    // sobel(), cl/demosaic.cl:96-112, value / sqrt(4.5) then abs()
+    // sobel(), cl/demosaic.cl:96-112, value / sqrt(4.5) then abs().  The
+    // 9-tap sums in their order with the zero taps left out and the +-1, +-2
+    // taps as exact adds: every partial sum is the same value (up to the sign
+    // of a zero, which the abs() removes).
     for (int i = tid; i < GS_X * GS_Y; i += 256) {
         const int sy = i / GS_X, sx = i % GS_X;
-        float vx = 0.0f, vy = 0.0f;
-        const float kx[3][3] = {{1, 0, -1}, {2, 0, -2}, {1, 0, -1}};
-        const float ky[3][3] = {{1, 2, 1}, {0, 0, 0}, {-1, -2, -1}};
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const float s = r[sy + j][sx + k];
-                vx += kx[j][k] * s;
-                vy += ky[j][k] * s;
-            }
-        }
+        const float s00 = r[sy][sx], s01 = r[sy][sx + 1], s02 = r[sy][sx + 2];
+        const float s10 = r[sy + 1][sx], s12 = r[sy + 1][sx + 2];
+        const float s20 = r[sy + 2][sx], s21 = r[sy + 2][sx + 1], s22 = r[sy + 2][sx + 2];
+        const float vx = ((((s00 - s02) + 2.0f * s10) - 2.0f * s12) + s20) - s22;
+        const float vy = ((((s00 + 2.0f * s01) + s02) - s20) - 2.0f * s21) - s22;
         const float ax = fabsf(vx * 0.471404521f), ay = fabsf(vy * 0.471404521f);
         sgx[sy][sx] = ax;
         sgy[sy][sx] = ay;
@@ -199,19 +195,18 @@ __global__ void __launch_bounds__(256) k_green(MosaicSrc<U16> raw, Img green, in
         // bilateralSobel(), cl/demosaic.cl:114-129
         const int SX = tx + 2, SY = ty + 2;  // sobel tile coordinates
         const float cl = sgl[SY][SX];
-        // smoothstep(4 sigma, 16 sigma, d) with its division hoisted as a reciprocal
+        // 1 - smoothstep(4 sigma, 16 sigma, 0.05 |dl|) by clim_green_tap_weight
         const float e0 = 4.0f * lumaStdDev, inv = 1.0f / (16.0f * lumaStdDev - 4.0f * lumaStdDev);
+        const float c1 = 0.05f * inv, c0 = clim_green_tap_c0(e0, inv);
         float sx = 0.0f, sy = 0.0f, wsum = 0.0f;
 #pragma unroll
         for (int j = -2; j <= 2; j++) {
 #pragma unroll
             for (int i = -2; i <= 2; i++) {
                 const float gx = sgx[SY + j][SX + i], gy = sgy[SY + j][SX + i];
-                const float d = 0.05f * fabsf(sgl[SY + j][SX + i] - cl);
-                const float t = clampf((d - e0) * inv, 0.0f, 1.0f);
-                const float wt = 1.0f - t * t * (3.0f - 2.0f * t);
-                sx += wt * gx;
-                sy += wt * gy;
+                const float wt = clim_green_tap_weight(fabsf(sgl[SY + j][SX + i] - cl), c1, c0);
+                sx = fmaf(wt, gx, sx);
+                sy = fmaf(wt, gy, sy);
                 wsum += wt;  // the x and y weight sums are equal
             }
         }

@@ -172,9 +172,9 @@ void launch_noise_stats_fit(FitJobs& jobs, hipStream_t st) {
 }
 
 // --------------------------------------------------------------- denoiseImage
-// cl/demosaic.cl:735-765.  nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr.  The per-tap
-// division by sigma is a multiplication by 1/sigma and length(d) < edge is
-// dot(d, d) < edge^2 (the FP32 oracle evaluates the same expressions).
+// cl/demosaic.cl:735-765.  nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr.  The range
+// test length((s - c) / sigma) < edge is sum (s - c)^2 / sigma^2 < edge^2
+// (clim_bilateral_dist; the FP32 oracle evaluates the same expressions).
 // SAME_EDGE: gradientBoost <= 1 and the luma and chroma edges are equal, so
 // every tap's luma and chroma weights are equal (one test, one weight sum).
 template <bool SAME_EDGE>
@@ -183,9 +183,9 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
                                                float gradientBoost) {
     const int X = tx + HR, Y = ty + HR;
     const float cY = t.x[Y][X], cCb = t.y[Y][X], cCr = t.z[Y][X];
-    const float sig0 = sqrtf(va0 + vb0 * cY), sig1 = sqrtf(va1 + vb1 * cY), sig2 = sqrtf(va2 + vb2 * cY);
     float lumaBoost = 1.0f;
     if (!SAME_EDGE && gradientBoost > 1.0f) {
+        const float sig0 = sqrtf(va0 + vb0 * cY);
         // sobel() of the luma channel, cl/demosaic.cl:96-112
         const float kx[3][3] = {{1, 0, -1}, {2, 0, -2}, {1, 0, -1}};
         const float ky[3][3] = {{1, 2, 1}, {0, 0, 0}, {-1, -2, -1}};
@@ -206,7 +206,9 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
     float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
     e0 = e0 * e0;
     e1 = e1 * e1;
-    const float i0 = 1.0f / sig0, i1 = 1.0f / sig1, i2 = 1.0f / sig2;
+    // 1 / sigma^2 per channel; the squared distance by clim_bilateral_dist
+    const float k0 = clim_inv_var(va0 + vb0 * cY), k1 = clim_inv_var(va1 + vb1 * cY),
+                k2 = clim_inv_var(va2 + vb2 * cY);
     float f0 = 0, f1 = 0, f2 = 0;
     int c0 = 0, c1 = 0;  // weight sums (the weights are 0 or 1)
 #pragma unroll
@@ -214,8 +216,7 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
 #pragma unroll
         for (int i = 0; i < 5; i++) {
             const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
-            const float d0 = (a - cY) * i0, d1 = (b - cCb) * i1, d2 = (c - cCr) * i2;
-            const float l = d0 * d0 + d1 * d1 + d2 * d2;
+            const float l = clim_bilateral_dist(a - cY, b - cCb, c - cCr, k0, k1, k2);
             // f += w * s with w in {0, 1}: the product is exact, so the fused
             // form rounds once on the same value (the same bits)
             const bool in0 = l < e0;

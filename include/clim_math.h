@@ -199,6 +199,36 @@ CLIM_HD float clim_u01(uint32_t r) { return (float)(r >> 8) * 5.9604644775390625
 /* uniform in (0,1]: for logarithms */
 CLIM_HD float clim_u01_open0(uint32_t r) { return (float)((r >> 8) + 1u) * 5.9604644775390625e-08f; }
 
+/* ------------------------------------------- relaxed-math stencil forms
+ * Two per-tap expressions of the FP32 path in contracted form (the
+ * reference's -cl-fast-relaxed-math allows contraction and reassociation);
+ * the FP32 oracle and the kernels both evaluate these functions.
+ *
+ * bilateralSobel (cl/demosaic.cl:114-129) tap weight
+ * 1 - smoothstep(e0, e0 + span, 0.05 |dl|): with c1 = 0.05 / span and
+ * c0 = -e0 / span, t = clamp(fma(|dl|, c1, c0), 0, 1) and
+ * w = fma(-t*t, fma(-2, t, 3), 1).  For e0 = 0 (zero noise), c0 = 0: t is
+ * then 1 for |dl| > 0 and 0 for |dl| = 0, as (0.05 |dl| - 0) / 0 clamps. */
+CLIM_HD float clim_green_tap_c0(float e0, float inv) { return e0 != 0.0f ? -(e0 * inv) : 0.0f; }
+
+CLIM_HD float clim_green_tap_weight(float adl, float c1, float c0) {
+    const float t = clim_clampf(fmaf(adl, c1, c0), 0.0f, 1.0f);
+    return fmaf(-(t * t), fmaf(-2.0f, t, 3.0f), 1.0f);
+}
+
+/* the reciprocal variance k = 1 / sigma^2 of a noise model value s = a + b Y;
+ * NaN for s < 0, where the square-root form's sigma is NaN (so every weight
+ * of the pixel is zero, as there) */
+CLIM_HD float clim_inv_var(float s) { return s < 0.0f ? clim_u2f(0x7fc00000u) : 1.0f / s; }
+
+/* denoiseImage (cl/demosaic.cl:735-765) squared range distance
+ * |(s - c) / sigma|^2 as sum d_c^2 k_c, k_c = clim_inv_var(a_c + b_c Y):
+ * no square root, two fused steps. */
+CLIM_HD float clim_bilateral_dist(float d0, float d1, float d2, float k0, float k1, float k2) {
+    const float l = (d0 * k0) * d0;
+    return fmaf(d2 * k2, d2, fmaf(d1 * k1, d1, l));
+}
+
 /* --------------------------------------------------- v / 65535 for uint16 v
  * The correctly rounded quotient from one reciprocal multiplication and one
  * fused correction step: q = v * fl(1/65535), e = fma(-q, 65535, v) (exact),

@@ -391,16 +391,26 @@ static inline void bilateral_sobel(const real* img, int w, int h, int x, int y, 
     /* smoothstep(4 sigma, 16 sigma, d) with the division by the (per-pixel)
      * edge distance hoisted out of the 5x5 loop as a reciprocal */
     const real e0 = K(4.0) * sigma, inv = K(1.0) / (K(16.0) * sigma - K(4.0) * sigma);
+#if !ORACLE_DOUBLE
+    /* the tap weight by clim_green_tap_weight, the sums in fused form */
+    const real c1 = K(0.05) * inv, c0 = clim_green_tap_c0(e0, inv);
+#endif
     for (int j = -2; j <= 2; j++) {
         for (int i = -2; i <= 2; i++) {
             real gx, gy;
             sobel(img, w, h, x + i, y + j, &gx, &gy);
             gx = RABS(gx);
             gy = RABS(gy);
+#if ORACLE_DOUBLE
             real d = K(0.05) * RABS(len2(gx, gy) - len2(cx, cy));
             real wt = K(1.0) - smoothstep_inv(e0, inv, d);
             sx += wt * gx;
             sy += wt * gy;
+#else
+            real wt = clim_green_tap_weight(RABS(len2(gx, gy) - len2(cx, cy)), c1, c0);
+            sx = fmaf(wt, gx, sx);
+            sy = fmaf(wt, gy, sy);
+#endif
             wx += wt;
             wy += wt;
         }
@@ -710,8 +720,10 @@ void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], co
         for (int x = 0; x < w; x++) {
             const real* c0 = rd4(in, w, h, x, y);
             real cY = c0[0], cCb = c0[1], cCr = c0[2];
+#if ORACLE_DOUBLE
             real sigma[3];
             for (int c = 0; c < 3; c++) sigma[c] = RSQRT((real)var_a[c] + (real)var_b[c] * cY);
+#endif
             real lumaBoost = K(1.0);
             if ((real)gradient_boost > K(1.0)) {
                 /* sobel() of the x channel (cl/demosaic.cl:745) */
@@ -725,21 +737,35 @@ void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], co
                 vx = vx * INV_SQRT4_5;
                 vy = vy * INV_SQRT4_5;
                 real gb = (real)gradient_boost;
+#if ORACLE_DOUBLE
                 real is0 = K(1.0) / sigma[0];
+#else
+                real is0 = K(1.0) / RSQRT((real)var_a[0] + (real)var_b[0] * cY);
+#endif
                 lumaBoost = gb - (gb - K(1.0)) * smoothstepr(K(0.5), K(2.0), len2(vx * is0, vy * is0));
             }
             real edge[3] = {lumaBoost * K(1.41421356), (real)chroma_boost * K(1.41421356),
                             (real)chroma_boost * K(1.41421356)};
             for (int c = 0; c < 3; c++) edge[c] = edge[c] * edge[c];
+#if ORACLE_DOUBLE
             const real isig[3] = {K(1.0) / sigma[0], K(1.0) / sigma[1], K(1.0) / sigma[2]};
+#else
+            /* k_c = 1 / sigma_c^2, the distance by clim_bilateral_dist */
+            real kk[3];
+            for (int c = 0; c < 3; c++) kk[c] = clim_inv_var((real)var_a[c] + (real)var_b[c] * cY);
+#endif
             real fp[3] = {0, 0, 0}, kn[3] = {0, 0, 0};
             for (int j = -2; j <= 2; j++) {
                 for (int i = -2; i <= 2; i++) {
                     const real* s = rd4(in, w, h, x + i, y + j);
+#if ORACLE_DOUBLE
                     real d0 = (s[0] - cY) * isig[0];
                     real d1 = (s[1] - cCb) * isig[1];
                     real d2 = (s[2] - cCr) * isig[2];
                     real l = d0 * d0 + d1 * d1 + d2 * d2;
+#else
+                    real l = clim_bilateral_dist(s[0] - cY, s[1] - cCb, s[2] - cCr, kk[0], kk[1], kk[2]);
+#endif
                     for (int c = 0; c < 3; c++) {
                         real wt = K(1.0) - (l < edge[c] ? K(0.0) : K(1.0));
                         fp[c] += wt * s[c];

@@ -579,3 +579,25 @@ def test_sampler_exact_2x_ratios(o32):
     # d.x = mix(up(d1).x, d.x, sharpening) with sharpening = 1
     want[..., 0] = up_d[..., 0] + (want[..., 0] - up_d[..., 0]) * f(1)
     np.testing.assert_array_equal(out, want)
+
+
+def test_relaxed_forms_keep_edge_semantics(o32, o64):
+    """The FP32 path's contracted forms must decide like the square-root and
+    division forms at their edges: zero noise in the green interpolation (a
+    black patch: lumaStdDev = 0) and a negative noise variance in the
+    bilateral (sigma NaN: every weight zero, the output NaN)."""
+    rng = np.random.default_rng(31)
+    h, w = 40, 56
+    scaled = rng.uniform(0.05, 0.4, (h, w)).astype(np.float32)
+    scaled[10:30, 12:40] = 0.0
+    g32 = o32.interpolate_green(scaled, 2, 1e-3)
+    g64 = o64.interpolate_green(scaled.astype(np.float64), 2, 1e-3)
+    np.testing.assert_allclose(g32, g64, atol=2e-6)
+    img = rng.uniform(-0.2, 0.6, (24, 32, 4))
+    va, vb = [1e-4, 1e-8, 1e-8], [1e-3, 1e-3, 1.2e-3]
+    b32 = o32.denoise_bilateral(img.astype(np.float32), va, vb)
+    b64 = o64.denoise_bilateral(img, va, vb)
+    assert np.isnan(b64).any()
+    np.testing.assert_array_equal(np.isnan(b32), np.isnan(b64))
+    ok = ~np.isnan(b64)
+    np.testing.assert_allclose(b32[ok], b64[ok], atol=1e-5)
