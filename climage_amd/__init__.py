@@ -94,7 +94,7 @@ def exported_symbols():
                         "clim_transform_image", "clim_despeckle_image", "clim_downsample_image",
                         "clim_noise_statistics", "clim_compute_noise_statistics", "clim_denoise_image",
                         "clim_denoise_image_guided", "clim_guided_filter", "clim_reassemble_image",
-                        "clim_convert_to_srgb", "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
+                        "clim_convert_to_srgb", "clim_local_tone_mapping_mask", "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
                         "clim_pyramidal_denoise", "clim_raw_to_rgb", "clim_generate_bayer")]
 
 
@@ -268,10 +268,16 @@ class Context:
         self._check(lib().clim_reassemble_image(self.h, d0.ref(), n1.ref(), d1.ref(), C.c_float(sharpening),
                                                 _f(nlf), out.ref()), "reassembleImage")
 
-    def convertTosRGB(self, linear, out8, transform, params, out_f32=None):
-        self._check(lib().clim_convert_to_srgb(self.h, linear.ref(), None, out8.ref(),
+    def convertTosRGB(self, linear, out8, transform, params, out_f32=None, ltm_mask=None):
+        self._check(lib().clim_convert_to_srgb(self.h, linear.ref(),
+                                               ltm_mask.ref() if ltm_mask is not None else None, out8.ref(),
                                                out_f32.ref() if out_f32 is not None else None, _f(transform),
                                                C.byref(params)), "convertTosRGB")
+
+    def localToneMappingMask(self, inp, guide, ltm, transform, out):
+        """ltm: LTMParameters; transform: YCbCr -> camera RGB (3x3)."""
+        self._check(lib().clim_local_tone_mapping_mask(self.h, inp.ref(), guide.ref(), C.byref(ltm),
+                                                       _f(transform), out.ref()), "localToneMappingMask")
 
     def pyramidalDenoise(self, image, levels, algorithm, params, out, nlf=None):
         dp = (DenoiseParameters * 5)(*params)

@@ -519,19 +519,39 @@ extern "C" clim_status clim_convert_to_srgb(clim_ctx* ctx, const clim_image* lin
                                             clim_image* out_rgba8, clim_image* out_f32, const float transform[9],
                                             const clim_rgb_conversion_params* params) {
     if (!ctx || !transform || !params) return ctx_fail(ctx, CLIM_EINVAL, "convertTosRGB: null argument");
-    if (ltm_mask || params->localToneMapping)
-        return ctx_fail(ctx, CLIM_EINVAL, "convertTosRGB: local tone mapping is not supported");
     CLIM_TRY(check_img(ctx, linear, CLIM_FMT_RGBA32F, "linear"));
     CLIM_TRY(check_img(ctx, out_rgba8, CLIM_FMT_RGBA8, "out_rgba8"));
     CLIM_TRY(check_same(ctx, linear, out_rgba8, "convertTosRGB"));
-    Img of{nullptr, 0, 0, 0};
+    Img of{nullptr, 0, 0, 0}, mask{nullptr, 0, 0, 0};
     if (out_f32) {
         CLIM_TRY(check_img(ctx, out_f32, CLIM_FMT_RGBA32F, "out_f32"));
         CLIM_TRY(check_same(ctx, linear, out_f32, "convertTosRGB"));
         of = view(out_f32);
     }
-    clim::launch_tone(view(linear), view(out_rgba8), of, nullptr, transform, params->contrast, params->saturation,
-                      params->toneCurveSlope, params->exposureBias, params->blacks, ctx->stream);
+    // the mask is read only with localToneMapping set (cl/demosaic.cl:1456);
+    // otherwise it may be null or a placeholder of any size
+    if (params->localToneMapping) {
+        if (!ltm_mask) return ctx_fail(ctx, CLIM_EINVAL, "convertTosRGB: localToneMapping needs ltm_mask");
+        CLIM_TRY(check_img(ctx, ltm_mask, CLIM_FMT_R32F, "ltm_mask"));
+        CLIM_TRY(check_same(ctx, linear, ltm_mask, "convertTosRGB"));
+        mask = view(ltm_mask);
+    }
+    clim::launch_tone(view(linear), mask, view(out_rgba8), of, nullptr, transform, params->contrast,
+                      params->saturation, params->toneCurveSlope, params->exposureBias, params->blacks, ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_local_tone_mapping_mask(clim_ctx* ctx, const clim_image* input, const clim_image* guide,
+                                                    const clim_ltm_params* ltm, const float transform[9],
+                                                    clim_image* out) {
+    if (!ctx || !ltm || !transform) return ctx_fail(ctx, CLIM_EINVAL, "localToneMappingMask: null argument");
+    CLIM_TRY(check_img(ctx, input, CLIM_FMT_RGBA32F, "input"));
+    CLIM_TRY(check_img(ctx, guide, CLIM_FMT_RGBA32F, "guide"));
+    CLIM_TRY(check_img(ctx, out, CLIM_FMT_R32F, "out"));
+    CLIM_TRY(check_same(ctx, input, out, "localToneMappingMask"));
+    const float p[4] = {ltm->guidedFilterEps, ltm->shadows, ltm->highlights, ltm->detail};
+    clim::launch_ltm_mask(view(input), view(guide), p, transform, view(out), ctx->stream);
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
 }

@@ -380,15 +380,40 @@ struct ToneArgs {
 
 __device__ __forceinline__ float sigmoidf(float x, float s) { return 0.5f * (clim_tanhf(s * x - 0.3f * s) + 1.0f); }
 
-// one pixel: RGBA8 (packed) and the FP32 RGB before quantisation
-__device__ __forceinline__ uint32_t tone_px(float4 p, const ToneArgs& a, float rgb[3]) {
-    float v[3] = {p.x, p.y, p.z};
+// the optional transformImage in front of the tone map
+__device__ __forceinline__ void tone_pre(float4 p, const ToneArgs& a, float v[3]) {
+    v[0] = p.x; v[1] = p.y; v[2] = p.z;
     if (a.has_pre) {
         const float t0 = a.pre[0] * v[0] + a.pre[1] * v[1] + a.pre[2] * v[2];
         const float t1 = a.pre[3] * v[0] + a.pre[4] * v[1] + a.pre[5] * v[2];
         const float t2 = a.pre[6] * v[0] + a.pre[7] * v[1] + a.pre[8] * v[2];
         v[0] = t0; v[1] = t1; v[2] = t2;
     }
+}
+
+// The local tone mapping gain of cl/demosaic.cl:1456-1466, on the
+// tone-mapped pixel.  pow(luma, 0.5) is sqrt(luma) (luma >= 0 after the
+// clamp) and the division by 1 - luma is done once.
+__device__ __forceinline__ void ltm_boost(float rgb[3], float boost) {
+    if (boost > 1.0f) {
+        const float luma = 0.2126f * rgb[0] + 0.7152f * rgb[1] + 0.0722f * rgb[2];
+        const float t = sqrtf(luma);
+        const float f = luma < 1.0f ? (1.0f - boost * luma) / (1.0f - luma) : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float hi = luma < 1.0f ? 1.0f - (1.0f - rgb[c]) * f : rgb[c];
+            rgb[c] = mixf(rgb[c] * boost, hi, t);
+        }
+    } else if (boost < 1.0f) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) rgb[c] *= boost;
+    }
+}
+
+// one pixel from camera RGB v: RGBA8 (packed) and the FP32 RGB before
+// quantisation; LTMB applies the local tone mapping gain `boost`
+template <bool LTMB>
+__device__ __forceinline__ uint32_t tone_rest(float v[3], const ToneArgs& a, float rgb[3], float boost) {
 #pragma unroll
     for (int c = 0; c < 3; c++) v[c] *= a.ebmul;
     if (a.saturation != 1.0f) {
@@ -403,13 +428,17 @@ __device__ __forceinline__ uint32_t tone_px(float4 p, const ToneArgs& a, float r
     }
 #pragma unroll
     for (int r = 0; r < 3; r++) rgb[r] = a.cam[r * 3 + 0] * v[0] + a.cam[r * 3 + 1] * v[1] + a.cam[r * 3 + 2] * v[2];
-    uint32_t packed = 0;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const float t = sqrtf(0.95f * rgb[c]);
         // divisions by constants as multiplications by their reciprocals
         const float tc = (sigmoidf(t, a.slope) - a.sig0) * a.inv_range;
         rgb[c] = clampf(tc, 0.0f, 1.0f);
+    }
+    if (LTMB) ltm_boost(rgb, boost);
+    uint32_t packed = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
         if (a.blacks > 0.0f) rgb[c] = (rgb[c] - a.blacks) * a.inv_blacks;
         const uint32_t q = (uint32_t)rintf(clampf(rgb[c], 0.0f, 1.0f) * 255.0f);
         packed |= q << (8 * c);
@@ -417,19 +446,189 @@ __device__ __forceinline__ uint32_t tone_px(float4 p, const ToneArgs& a, float r
     return packed;
 }
 
-__device__ __forceinline__ void tone_store(const Img& out8, const Img& outf, int x, int y, float4 p,
-                                           const ToneArgs& a) {
+template <bool LTMB>
+__device__ __forceinline__ void tone_store_v(const Img& out8, const Img& outf, int x, int y, float v[3],
+                                             const ToneArgs& a, float boost) {
     float rgb[3];
-    const uint32_t packed = tone_px(p, a, rgb);
+    const uint32_t packed = tone_rest<LTMB>(v, a, rgb, boost);
     reinterpret_cast<uint32_t*>(out8.p + (size_t)y * out8.pitch)[x] = packed;
     if (outf.p) wrow_f4(outf, y)[x] = make_float4(rgb[0], rgb[1], rgb[2], 0.0f);
 }
 
-__global__ void __launch_bounds__(256) k_tone(Img in, Img out8, Img outf, ToneArgs a) {
+__device__ __forceinline__ void tone_store(const Img& out8, const Img& outf, int x, int y, float4 p,
+                                           const ToneArgs& a) {
+    float v[3];
+    tone_pre(p, a, v);
+    tone_store_v<false>(out8, outf, x, y, v, a, 1.0f);
+}
+
+// convertTosRGB; LTMB: with the localToneMapping mask (R32F, same size)
+template <bool LTMB>
+__global__ void __launch_bounds__(256) k_tone(Img in, Img mask, Img out8, Img outf, ToneArgs a) {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * 4 + threadIdx.y;
     if (x >= out8.w || y >= out8.h) return;
-    tone_store(out8, outf, x, y, row_f4(in, y)[x], a);
+    float v[3];
+    tone_pre(row_f4(in, y)[x], a, v);
+    tone_store_v<LTMB>(out8, outf, x, y, v, a, LTMB ? row_f(mask, y)[x] : 1.0f);
+}
+
+// ------------------------------------------------------ localToneMappingMask
+// cl/demosaic.cl:1024-1107.  The guide (an RGBA image read through .x;
+// pyramid level 3 in the pipeline) is sampled by the linear sampler at nine
+// taps per mask texel: offsets -1.5, 0.5, 2.0 (+ 0.5) texels OF THE MASK per
+// axis, box weights 2/5, 2/5, 1/5 per axis.  The "variance" is
+// sum((w s)^2) - mean^2 as in the reference; min(L, 1) follows the OpenCL
+// definition (1 if 1 < L, else L: NaN stays NaN).  The per-tap division by
+// var + eps is a multiplication by (luma - mean) / (var + eps), and
+// length(sqrt(2 rgb)) = sqrt(2r + 2g + 2b), NaN if a component is negative.
+struct LtmArgs {
+    Img guide;
+    float eps, shadows, highlights, detail;
+    float T[9];  // YCbCr -> camera RGB (the stand-alone operator)
+};
+
+// One axis of the taps: the sampler's texel and 8-bit weight per offset.
+// Non-decreasing in x and in the offset (float rounding is monotone), so a
+// block's taps lie between those of its first pixel's first offset and its
+// last pixel's last offset.
+__device__ __forceinline__ void ltm_axis(int x, float nx, int gsize, int i0[3], float a[3]) {
+    const float px = (float)x * nx;
+    const float off[3] = {-1.0f, 1.0f, 2.5f};  // offset + 0.5
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float u = (px + off[k] * nx) * (float)gsize - 0.5f;
+        const float fu = floorf(u);
+        a[k] = floorf((u - fu) * 256.0f + 0.5f) * 0.00390625f;
+        i0[k] = (int)fu;
+    }
+}
+
+constexpr int kLtmLW = 40, kLtmLH = 8;  // LDS guide footprint of a 64x4 block (ratios down to 1/2)
+
+struct LtmGuideLds {
+    const float* s;
+    int i0, j0;
+    __device__ __forceinline__ float at(int i, int j) const { return s[(j - j0) * kLtmLW + (i - i0)]; }
+};
+struct LtmGuideMem {
+    Img g;
+    __device__ __forceinline__ float at(int i, int j) const {
+        return reinterpret_cast<const float*>(row_f4(g, clampi(j, 0, g.h - 1)) + clampi(i, 0, g.w - 1))[0];
+    }
+};
+
+struct LtmFoot {
+    int i0, j0;
+    bool lds;
+};
+
+// The guide texels (clamped to the edge) under the taps of the pixels
+// [x0, x1] x [y0, y1], staged in LDS when they fit; block-uniform.
+__device__ __forceinline__ LtmFoot ltm_stage(float* sh, int x0, int x1, int y0, int y1, float nx, float ny,
+                                             const Img& g) {
+    int ia[3], ib[3], ja[3], jb[3];
+    float t[3];
+    ltm_axis(x0, nx, g.w, ia, t);
+    ltm_axis(x1, nx, g.w, ib, t);
+    ltm_axis(y0, ny, g.h, ja, t);
+    ltm_axis(y1, ny, g.h, jb, t);
+    LtmFoot f;
+    f.i0 = ia[0];
+    f.j0 = ja[0];
+    const int nc = ib[2] + 2 - ia[0], nr = jb[2] + 2 - ja[0];
+    f.lds = nc <= kLtmLW && nr <= kLtmLH;
+    if (f.lds) {
+        const LtmGuideMem m{g};
+        const int tid = threadIdx.x + threadIdx.y * blockDim.x, nt = blockDim.x * blockDim.y;
+        for (int k = tid; k < nc * nr; k += nt) {
+            const int r = k / nc, c = k - r * nc;
+            sh[r * kLtmLW + c] = m.at(f.i0 + c, f.j0 + r);
+        }
+        __syncthreads();
+    }
+    return f;
+}
+
+template <class G>
+__device__ __forceinline__ float ltm_mask_px(const G& g, int x, int y, float nx, float ny, int gw, int gh,
+                                             float luma, const float rgb[3], const LtmArgs& a) {
+    int ii[3], jj[3];
+    float fa[3], fb[3];
+    ltm_axis(x, nx, gw, ii, fa);
+    ltm_axis(y, ny, gh, jj, fb);
+    const float W[3][3] = {{0.16f, 0.16f, 0.08f}, {0.16f, 0.16f, 0.08f}, {0.08f, 0.08f, 0.04f}};  // [y][x]
+    float s[9], sum = 0.0f, sumSq = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float A = fa[i], B = fb[j];
+            const float w00 = (1.0f - A) * (1.0f - B), w10 = A * (1.0f - B);
+            const float w01 = (1.0f - A) * B, w11 = A * B;
+            const float v = w00 * g.at(ii[i], jj[j]) + w10 * g.at(ii[i] + 1, jj[j]) +
+                            w01 * g.at(ii[i], jj[j] + 1) + w11 * g.at(ii[i] + 1, jj[j] + 1);
+            s[j * 3 + i] = v;
+            const float ws = W[j][i] * v;
+            sum += ws;
+            sumSq += ws * ws;
+        }
+    }
+    const float mean = sum, var = sumSq - sum * sum;
+    const float q = (luma - mean) / (var + a.eps);
+    float filtered = 0.0f, norm = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const float sw = W[k / 3][k % 3] * (1.0f + (s[k] - mean) * q);
+        filtered += sw * s[k];
+        norm += sw;
+    }
+    const float illuminance = filtered / norm;
+    const float reflectance = luma / illuminance;
+    const float L = (rgb[0] >= 0.0f && rgb[1] >= 0.0f && rgb[2] >= 0.0f)
+                        ? sqrtf(2.0f * rgb[0] + 2.0f * rgb[1] + 2.0f * rgb[2])
+                        : __int_as_float(0x7fc00000);
+    const float hc = 1.0f < L ? 1.0f : L;
+    const float tc = mixf(a.shadows, a.highlights, hc);
+    return clim_powf(illuminance, 1.0f / tc) * clim_powf(reflectance, a.detail) / luma;
+}
+
+__device__ __forceinline__ float ltm_mask_at(const LtmFoot& f, const float* sh, int x, int y, float nx, float ny,
+                                             float luma, const float rgb[3], const LtmArgs& a) {
+    if (f.lds) return ltm_mask_px(LtmGuideLds{sh, f.i0, f.j0}, x, y, nx, ny, a.guide.w, a.guide.h, luma, rgb, a);
+    return ltm_mask_px(LtmGuideMem{a.guide}, x, y, nx, ny, a.guide.w, a.guide.h, luma, rgb, a);
+}
+
+__global__ void __launch_bounds__(256) k_ltm_mask(Img in, Img out, LtmArgs a) {
+    const TileId tl = xcd_tile();
+    const int x = tl.x * 64 + threadIdx.x;
+    const int y = tl.y * 4 + threadIdx.y;
+    const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
+    __shared__ float sh[kLtmLW * kLtmLH];
+    const LtmFoot f = ltm_stage(sh, tl.x * 64, min(tl.x * 64 + 63, out.w - 1), tl.y * 4,
+                                min(tl.y * 4 + 3, out.h - 1), nx, ny, a.guide);
+    if (x >= out.w || y >= out.h) return;
+    const float4 p = row_f4(in, y)[x];
+    const float rgb[3] = {a.T[0] * p.x + a.T[1] * p.y + a.T[2] * p.z, a.T[3] * p.x + a.T[4] * p.y + a.T[5] * p.z,
+                          a.T[6] * p.x + a.T[7] * p.y + a.T[8] * p.z};
+    wrow_f(out, y)[x] = ltm_mask_at(f, sh, x, y, nx, ny, p.x, rgb, a);
+}
+
+static LtmArgs make_ltm_args(const Img& guide, const float ltm[4], const float* T) {
+    LtmArgs a;
+    a.guide = guide;
+    a.eps = ltm[0];
+    a.shadows = ltm[1];
+    a.highlights = ltm[2];
+    a.detail = ltm[3];
+    for (int i = 0; i < 9; i++) a.T[i] = T ? T[i] : 0.0f;
+    return a;
+}
+
+void launch_ltm_mask(const Img& in, const Img& guide, const float ltm[4], const float T[9], const Img& out,
+                     hipStream_t st) {
+    dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
+    hipLaunchKernelGGL(k_ltm_mask, grd, blk, 0, st, in, out, make_ltm_args(guide, ltm, T));
 }
 
 static ToneArgs make_tone_args(const float* pre, const float cam[9], float contrast, float saturation, float slope,
@@ -453,26 +652,37 @@ static ToneArgs make_tone_args(const float* pre, const float cam[9], float contr
     return a;
 }
 
-void launch_tone(const Img& in, const Img& out8, const Img& outf, const float* pre, const float cam[9],
-                 float contrast, float saturation, float slope, float exposureBias, float blacks, hipStream_t st) {
+void launch_tone(const Img& in, const Img& mask, const Img& out8, const Img& outf, const float* pre,
+                 const float cam[9], float contrast, float saturation, float slope, float exposureBias, float blacks,
+                 hipStream_t st) {
     const ToneArgs a = make_tone_args(pre, cam, contrast, saturation, slope, exposureBias, blacks);
     dim3 blk(64, 4), grd((out8.w + 63) / 64, (out8.h + 3) / 4);
-    hipLaunchKernelGGL(k_tone, grd, blk, 0, st, in, out8, outf, a);
+    if (mask.p)
+        hipLaunchKernelGGL(k_tone<true>, grd, blk, 0, st, in, mask, out8, outf, a);
+    else
+        hipLaunchKernelGGL(k_tone<false>, grd, blk, 0, st, in, mask, out8, outf, a);
 }
 
 // ------------------------------------------------------------ reassembleImage
 // cl/demosaic.cl:867-896.  nlf: (A_Y, B_Y) at nlf.get(0), nlf.get(3).
 // out may alias d0 (each texel is read, then written, by one thread).
 // TONE: the level-0 pass of the pipeline, which hands its result straight to
-// convertTosRGB (out8 / outf) instead of writing it.
-template <bool EXACT2X, bool TONE>
+// convertTosRGB (out8 / outf) instead of writing it.  LTM (with TONE): the
+// localToneMapping mask of the result (guide: pyramid level 3) is computed
+// in the same thread and applied by the tone map (cpp/raw_converter.cpp:164-174).
+template <bool EXACT2X, bool TONE, bool LTM>
 __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img out, float sharpening0,
-                                                    ParamRef nlf, Img out8, Img outf, ToneArgs ta) {
+                                                    ParamRef nlf, Img out8, Img outf, ToneArgs ta, LtmArgs la) {
     const TileId tl = xcd_tile();
     const int x = tl.x * 64 + threadIdx.x;
     const int y = tl.y * 4 + threadIdx.y;
-    if (x >= out.w || y >= out.h) return;
     const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
+    __shared__ float gsh[LTM ? kLtmLW * kLtmLH : 1];
+    LtmFoot foot{0, 0, false};
+    if (LTM)
+        foot = ltm_stage(gsh, tl.x * 64, min(tl.x * 64 + 63, out.w - 1), tl.y * 4, min(tl.y * 4 + 3, out.h - 1), nx,
+                         ny, la.guide);
+    if (x >= out.w || y >= out.h) return;
     const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
     const float4 p0 = row_f4(d0, y)[x];
     float4 a1, b1;
@@ -500,10 +710,16 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
         sharpening = 1.0f + (sharpening - 1.0f) * detail;
     }
     d.x = mixf(b1.x, d.x, sharpening);
-    if (TONE)
+    if (TONE && LTM) {
+        float v[3];
+        tone_pre(d, ta, v);  // the camera RGB of the mask is the tone map's input
+        const float boost = ltm_mask_at(foot, gsh, x, y, nx, ny, d.x, v, la);
+        tone_store_v<true>(out8, outf, x, y, v, ta, boost);
+    } else if (TONE) {
         tone_store(out8, outf, x, y, d, ta);
-    else
+    } else {
         wrow_f4(out, y)[x] = d;
+    }
 }
 
 void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& out, float sharpening,
@@ -512,22 +728,31 @@ void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& o
     const bool exact = out.w == 2 * n1.w && out.h == 2 * n1.h && n1.w == d1.w && n1.h == d1.h &&
                        out.w <= kUp2xMax && out.h <= kUp2xMax;
     ToneArgs ta{};
+    LtmArgs la{};
     Img o8{nullptr, 0, 0, 0}, of{nullptr, 0, 0, 0};
+    const bool ltm = tone && tone->ltm_guide.p;
     if (tone) {
         ta = make_tone_args(tone->pre, tone->cam, tone->contrast, tone->saturation, tone->slope, tone->exposureBias,
                             tone->blacks);
         o8 = tone->out8;
         of = tone->outf;
+        if (ltm) la = make_ltm_args(tone->ltm_guide, tone->ltm, nullptr);
     }
-    if (exact && tone)
-        hipLaunchKernelGGL((k_reassemble<true, true>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta);
+#define CLIM_REASSEMBLE(E, T, L) \
+    hipLaunchKernelGGL((k_reassemble<E, T, L>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta, la)
+    if (exact && ltm)
+        CLIM_REASSEMBLE(true, true, true);
+    else if (exact && tone)
+        CLIM_REASSEMBLE(true, true, false);
     else if (exact)
-        hipLaunchKernelGGL((k_reassemble<true, false>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta);
+        CLIM_REASSEMBLE(true, false, false);
+    else if (ltm)
+        CLIM_REASSEMBLE(false, true, true);
     else if (tone)
-        hipLaunchKernelGGL((k_reassemble<false, true>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta);
+        CLIM_REASSEMBLE(false, true, false);
     else
-        hipLaunchKernelGGL((k_reassemble<false, false>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of,
-                           ta);
+        CLIM_REASSEMBLE(false, false, false);
+#undef CLIM_REASSEMBLE
 }
 
 }  // namespace clim

@@ -94,11 +94,21 @@ struct ToneLaunch {
     const float* pre;  // YCbCr -> camera (with exposure), or null
     const float* cam;
     float contrast, saturation, slope, exposureBias, blacks;
+    // localToneMapping: the guide (pyramid level 3; p null = off) and
+    // guidedFilterEps, shadows, highlights, detail; the mask's YCbCr -> camera
+    // transform is `pre`
+    Img ltm_guide;
+    float ltm[4];
 };
 void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& out, float sharpening,
                        const ParamRef& nlf, hipStream_t st, const ToneLaunch* tone = nullptr);
-void launch_tone(const Img& in, const Img& out8, const Img& outf, const float* pre, const float cam[9],
-                 float contrast, float saturation, float slope, float exposureBias, float blacks, hipStream_t st);
+// mask.p null: no local tone mapping
+void launch_tone(const Img& in, const Img& mask, const Img& out8, const Img& outf, const float* pre,
+                 const float cam[9], float contrast, float saturation, float slope, float exposureBias, float blacks,
+                 hipStream_t st);
+// localToneMappingMask: in the denoised YCbCr image, out R32F of its size
+void launch_ltm_mask(const Img& in, const Img& guide, const float ltm[4], const float T[9], const Img& out,
+                     hipStream_t st);
 
 // ----------------------------------------------- multi-pass guided filter
 // scratch: 7 images of w*h texels (float4) + 1 of float2, see k_guided.hip

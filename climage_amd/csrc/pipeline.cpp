@@ -148,8 +148,8 @@ static clim_status check_params(clim_ctx* ctx, const clim_demosaic_params* p) {
     if (p->denoiseAlgorithm < 0 || p->denoiseAlgorithm > 2) return ctx_fail(ctx, CLIM_EINVAL, "bad denoiseAlgorithm");
     if (p->noiseLevel > 0.6f)
         return ctx_fail(ctx, CLIM_EINVAL, "noiseLevel > 0.6 selects the high-noise raw branch, which is not built");
-    if (p->rgbConversionParameters.localToneMapping)
-        return ctx_fail(ctx, CLIM_EINVAL, "local tone mapping is not built");
+    if (p->rgbConversionParameters.localToneMapping && p->levels < 4)
+        return ctx_fail(ctx, CLIM_EINVAL, "local tone mapping reads pyramid level 3: levels must be 4 or 5");
     return CLIM_OK;
 }
 
@@ -264,11 +264,18 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
     CLIM_TRY_HIP(ctx, hipGetLastError());
 
     // transformImage(inverse(cam_to_ycbcr) * exposure) + convertTosRGB, in the
-    // kernel that reassembles pyramid level 0
+    // kernel that reassembles pyramid level 0, with localToneMappingMask of the
+    // denoised image (guide: denoisedImagePyramid[3], cpp/raw_converter.cpp:164-167)
     const clim_rgb_conversion_params& rc = p->rgbConversionParameters;
-    const clim::ToneLaunch tone{view(out_rgba8), outf,          y2c,           cam,
-                                rc.contrast,      rc.saturation, rc.toneCurveSlope, rc.exposureBias,
-                                rc.blacks};
+    clim::ToneLaunch tone{view(out_rgba8), outf,          y2c,           cam,
+                          rc.contrast,      rc.saturation, rc.toneCurveSlope, rc.exposureBias,
+                          rc.blacks,        Img{nullptr, 0, 0, 0}, {0, 0, 0, 0}};
+    if (rc.localToneMapping) {
+        tone.ltm_guide = ws.den[3].img;
+        const clim_ltm_params& lp = p->ltmParameters;
+        const float l4[4] = {lp.guidedFilterEps, lp.shadows, lp.highlights, lp.detail};
+        std::memcpy(tone.ltm, l4, sizeof(l4));
+    }
     CLIM_TRY(pyramid(ctx, ws.rgbB.img, ws.rgbA.img, p->levels, p->denoiseAlgorithm, p->denoiseParameters,
                      p->noiseModel.pyramidNlf, &tone));
     CLIM_TRY_HIP(ctx, hipGetLastError());

@@ -129,6 +129,27 @@ CLIM_HD float clim_logf(float x) {
     return (float)e * 6.931471825e-01f + lm;
 }
 
+/* ---------------------------------------------------------------- pow(x,y)
+ * |x|^y = exp(y * log|x|), with the special values of C99 Annex F (which
+ * OpenCL's pow follows): pow(x, +-0) = pow(1, y) = 1 even for NaN;
+ * pow(-1, +-inf) = 1; a finite negative x gives NaN unless y is an integer,
+ * whose parity gives the sign (also for -0 and -inf).  The rounding of
+ * y * log|x| dominates the error: relative error <= 4e-6 for
+ * |y * log|x|| <= 20, <= 1e-6 for |y * log|x|| <= 4. */
+CLIM_HD float clim_powf(float x, float y) {
+    if (y == 0.0f || x == 1.0f) return 1.0f;
+    if (x != x || y != y) return x + y;
+    const float ay = fabsf(y);
+    const int is_int = ay >= 16777216.0f || y == rintf(y);  /* +-inf counts as even */
+    const int odd = is_int && ay < 16777216.0f && ((int)ay & 1);
+    const int neg = (clim_f2u(x) >> 31) != 0;
+    const float ax = fabsf(x);
+    if (neg && !is_int && ax != 0.0f && ax != INFINITY) return clim_u2f(0x7fc00000u);
+    if (ax == 1.0f) return (neg && odd) ? -1.0f : 1.0f;
+    float r = clim_expf(y * clim_logf(ax));
+    return (neg && odd) ? -r : r;
+}
+
 /* --------------------------------------------------------------- tanh(x) */
 CLIM_HD float clim_tanhf(float x) {
     float ax = fabsf(x);

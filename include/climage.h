@@ -218,11 +218,19 @@ clim_status clim_reassemble_image(clim_ctx* ctx, const clim_image* denoised0, co
 
 /* convertTosRGB, cpp/demosaic_cl.cpp:236-265, cl/demosaic.cl:1380-1474.
  * out_rgba8 receives the UNORM8 result; out_f32 (nullable) the same values
- * before quantisation.  ltm_mask must be null: local tone mapping is not
- * part of this build (params->localToneMapping must be 0). */
+ * before quantisation.  ltm_mask (R32F, the size of linear) is read only when
+ * params->localToneMapping is set; otherwise it may be null. */
 clim_status clim_convert_to_srgb(clim_ctx* ctx, const clim_image* linear, const clim_image* ltm_mask,
                                  clim_image* out_rgba8, clim_image* out_f32, const float transform[9],
                                  const clim_rgb_conversion_params* params);
+
+/* localToneMappingMask, cpp/demosaic_cl.cpp:339-370, cl/demosaic.cl:1024-1107.
+ * input: the denoised YCbCr image (RGBA32F); guide: RGBA32F, read through its
+ * first channel (the pipeline passes pyramid level 3, cpp/raw_converter.cpp:165);
+ * transform: YCbCr -> camera RGB (row-major, inverse(cam_to_ycbcr) times
+ * the exposure multiplier); out: R32F, the size of input. */
+clim_status clim_local_tone_mapping_mask(clim_ctx* ctx, const clim_image* input, const clim_image* guide,
+                                         const clim_ltm_params* ltm, const float transform[9], clim_image* out);
 
 /* ------------------------------------------------------ host-side math */
 

@@ -50,6 +50,7 @@ typedef double real;
 #define ATAN2PI(y, x) (atan2(y, x) / 3.14159265358979323846)
 #define TANH(x) tanh(x)
 #define EXP(x) exp(x)
+#define POW(x, y) pow(x, y)
 #define INV_SQRT4_5 0.47140452079103168
 #else
 typedef float real;
@@ -63,6 +64,7 @@ typedef float real;
 #define ATAN2PI(y, x) clim_atan2pif(y, x)
 #define TANH(x) clim_tanhf(x)
 #define EXP(x) clim_expf(x)
+#define POW(x, y) clim_powf(x, y)
 #define INV_SQRT4_5 0.471404521f
 #endif
 
@@ -998,14 +1000,125 @@ void P(reassemble)(const real* d0, const real* n1, const real* d1, int w, int h,
     }
 }
 
+/* ------------------------------------------------------ localToneMappingMask
+ * cl/demosaic.cl:1024-1107 (localToneMappingMaskImage, localToneMappingMask),
+ * cpp/demosaic_cl.cpp:339-370.  in: the denoised YCbCr image (w x h RGBA);
+ * guide: an RGBA image (pyramid level 3 in the pipeline,
+ * cpp/raw_converter.cpp:164-167), read through its .x; mask: w x h.
+ * ltm: guidedFilterEps, shadows, highlights, detail.  transform: YCbCr ->
+ * camera RGB (inverse(cam_to_ycbcr) * exposure_multiplier).
+ *
+ * The box taps sit at (-1.5, 0.5, 2.0) + 0.5 texels of the MASK's size (the
+ * reference scales them by 1 / output size), each axis with weight 2/5, 2/5,
+ * 1/5 (a 5-texel box read by linear filtering: two pairs and a single); the
+ * guide is sampled with the linear sampler (bilinear4).  The input pixel is
+ * the texel itself (a linear read at its centre: the weights quantise to
+ * 0 / 1).  The reference's "variance" is sum((w s)^2) - (sum w s)^2, as
+ * written; min(L, 1) is the OpenCL definition "1 if 1 < L, else L" (NaN
+ * stays NaN).  FP32 relaxed forms: the per-tap division by var + eps is a
+ * multiplication by (luma - mean) / (var + eps), and
+ * length(sqrt(2 rgb)) = sqrt(2r + 2g + 2b), NaN when a component is
+ * negative (as the component's sqrt is). */
+static const real ltm_off[3] = {K(-1.5), K(0.5), K(2.0)};
+static const real ltm_w[3][3] = {{K(0.16), K(0.16), K(0.08)},   /* [y tap][x tap] */
+                                 {K(0.16), K(0.16), K(0.08)},
+                                 {K(0.08), K(0.08), K(0.04)}};
+
+static inline real ltm_hclip(real r, real g, real b) {
+#if ORACLE_DOUBLE
+    const real a = sqrt(2 * r), c = sqrt(2 * g), d = sqrt(2 * b);
+    const real L = sqrt(a * a + c * c + d * d);
+#else
+    const real L = (r >= K(0.0) && g >= K(0.0) && b >= K(0.0)) ? RSQRT(K(2.0) * r + K(2.0) * g + K(2.0) * b)
+                                                               : clim_u2f(0x7fc00000u);
+#endif
+    return K(1.0) < L ? K(1.0) : L;
+}
+
+void P(ltm_mask)(const real* in, int w, int h, const real* guide, int gw, int gh, const float ltm[4],
+                 const float transform[9], real* mask, real* var_eps) {
+    real M[9];
+    for (int k = 0; k < 9; k++) M[k] = (real)transform[k];
+    const real eps = ltm[0], shadows = ltm[1], highlights = ltm[2], detail = ltm[3];
+    const real nx = K(1.0) / (real)w, ny = K(1.0) / (real)h;
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++) {
+            const real px = (real)x * nx, py = (real)y * ny;
+            real s[9], sum = 0, sumSq = 0;
+            for (int j = 0; j < 3; j++) {
+                for (int i = 0; i < 3; i++) {
+                    real t[4];
+                    bilinear4(guide, gw, gh, px + (ltm_off[i] + K(0.5)) * nx, py + (ltm_off[j] + K(0.5)) * ny, t);
+                    s[j * 3 + i] = t[0];
+                    const real ws = ltm_w[j][i] * t[0];
+                    sum += ws;
+                    sumSq += ws * ws;
+                }
+            }
+            const real mean = sum, var = sumSq - sum * sum;
+            if (var_eps) var_eps[(size_t)y * w + x] = var + eps;  /* conditioning, for the tests */
+            const real* p = in + ((size_t)y * w + x) * 4;
+            const real luma = p[0];
+            real filtered = 0, norm = 0;
+#if !ORACLE_DOUBLE
+            const real q = (luma - mean) / (var + eps);
+#endif
+            for (int k = 0; k < 9; k++) {
+#if ORACLE_DOUBLE
+                const real sw = ltm_w[k / 3][k % 3] * (1 + (s[k] - mean) * (luma - mean) / (var + eps));
+#else
+                const real sw = ltm_w[k / 3][k % 3] * (K(1.0) + (s[k] - mean) * q);
+#endif
+                filtered += sw * s[k];
+                norm += sw;
+            }
+            const real illuminance = filtered / norm;
+            const real reflectance = luma / illuminance;
+            const real r = M[0] * p[0] + M[1] * p[1] + M[2] * p[2];
+            const real g = M[3] * p[0] + M[4] * p[1] + M[5] * p[2];
+            const real b = M[6] * p[0] + M[7] * p[1] + M[8] * p[2];
+            const real hc = ltm_hclip(r, g, b);
+            const real tc = mixr(shadows, highlights, hc);
+            mask[(size_t)y * w + x] = POW(illuminance, K(1.0) / tc) * POW(reflectance, detail) / luma;
+        }
+    }
+}
+
 /* --------------------------------------------------------------- convertTosRGB
- * cl/demosaic.cl:1391-1474 with localToneMapping = 0.  out8 gets UNORM8
- * (round to nearest even, saturating), outf (nullable) the float values. */
+ * cl/demosaic.cl:1391-1474.  out8 gets UNORM8 (round to nearest even,
+ * saturating), outf (nullable) the float values.  ltm (nullable): the
+ * localToneMapping mask (w x h), applied after the tone curve
+ * (cl/demosaic.cl:1456-1466); FP32 relaxed forms there: pow(luma, 0.5) is
+ * sqrt(luma) (luma >= 0 after the clamp) and the division by 1 - luma is
+ * done once per pixel. */
 static inline real sigmoidr(real x, real s) { return K(0.5) * (TANH(s * x - K(0.3) * s) + K(1.0)); }
+
+static inline void ltm_boost(real rgb[3], real boost) {
+    if (boost > K(1.0)) {
+        const real luma = K(0.2126) * rgb[0] + K(0.7152) * rgb[1] + K(0.0722) * rgb[2];
+#if ORACLE_DOUBLE
+        const real t = pow(luma, 0.5);
+        for (int c = 0; c < 3; c++) {
+            const real hi = luma < 1 ? 1 - (1 - rgb[c]) * (1 - boost * luma) / (1 - luma) : rgb[c];
+            rgb[c] = mixr(rgb[c] * boost, hi, t);
+        }
+#else
+        const real t = sqrtf(luma);
+        const real f = luma < K(1.0) ? (K(1.0) - boost * luma) / (K(1.0) - luma) : K(0.0);
+        for (int c = 0; c < 3; c++) {
+            const real hi = luma < K(1.0) ? K(1.0) - (K(1.0) - rgb[c]) * f : rgb[c];
+            rgb[c] = mixr(rgb[c] * boost, hi, t);
+        }
+#endif
+    } else if (boost < K(1.0)) {
+        for (int c = 0; c < 3; c++) rgb[c] *= boost;
+    }
+}
 
 void P(convert_to_srgb)(const real* in, int w, int h, const float transform[9], float contrast_f,
                         float saturation_f, float slope_f, float exposure_bias_f, float blacks_f,
-                        uint8_t* out8, real* outf) {
+                        const real* ltm, uint8_t* out8, real* outf) {
     real M[9];
     for (int k = 0; k < 9; k++) M[k] = (real)transform[k];
     const real contrast = contrast_f, saturation = saturation_f, s = slope_f;
@@ -1038,8 +1151,10 @@ void P(convert_to_srgb)(const real* in, int w, int h, const float transform[9], 
             real t = RSQRT(K(0.95) * rgb[c]);
             real tc = (sigmoidr(t, s) - sig0) * inv_range;
             rgb[c] = clampr(tc, K(0.0), K(1.0));
-            if (blacks > K(0.0)) rgb[c] = (rgb[c] - blacks) * inv_blacks;
         }
+        if (ltm) ltm_boost(rgb, ltm[i]);
+        for (int c = 0; c < 3; c++)
+            if (blacks > K(0.0)) rgb[c] = (rgb[c] - blacks) * inv_blacks;
         for (int c = 0; c < 3; c++) {
             real q = clampr(rgb[c], K(0.0), K(1.0)) * K(255.0);
 #if ORACLE_DOUBLE
@@ -1097,7 +1212,7 @@ typedef struct {
  * image: level 0 (w x h RGBA), out: w x h RGBA.  Returns a bit mask of the
  * levels whose fit fell back. */
 int P(pyramidal_denoise)(const real* image, int w, int h, int levels, int algorithm,
-                         const o_denoise_params* dp, float* nlf_inout, real* out) {
+                         const o_denoise_params* dp, float* nlf_inout, real* out, real* level3_out) {
     real* pyr[5] = {0};
     real* den[5] = {0};
     int lw[6], lh[6];
@@ -1142,6 +1257,8 @@ int P(pyramidal_denoise)(const real* image, int w, int h, int levels, int algori
                           dp[i].sharpening, nl, den[i]);
         }
     }
+    /* denoisedImagePyramid[3], the local tone mapping guide */
+    if (level3_out && levels > 3) memcpy(level3_out, den[3], (size_t)lw[3] * lh[3] * 4 * sizeof(real));
     for (int i = 1; i < levels; i++) { free(pyr[i]); free(den[i]); }
     return fallbacks;
 }
@@ -1156,6 +1273,8 @@ typedef struct {
     o_denoise_params dp[5];
     float contrast, saturation, toneCurveSlope, exposureBias, blacks;
     int levels, algorithm;
+    int localToneMapping;  /* needs levels >= 4 */
+    float ltm[4];          /* guidedFilterEps, shadows, highlights, detail */
 } o_pipeline_params;
 
 /* Full path.  nlf_out: rawNlf[4] then levels x 6.  Optional stage dumps
@@ -1198,13 +1317,24 @@ int P(raw_to_rgb)(const uint16_t* raw, int w, int h, const o_pipeline_params* pp
 
     float nlf[5 * 6];
     memcpy(nlf, pp->pyramidNlf, sizeof(nlf));
-    fallbacks |= P(pyramidal_denoise)(rgbB, w, h, pp->levels, pp->algorithm, pp->dp, nlf, rgbA);
+    const int ltm = pp->localToneMapping && pp->levels > 3;
+    const int gw = w >> 3, gh = h >> 3;  /* w/2/2/2 with truncation at every level */
+    real* guide = ltm ? (real*)malloc((size_t)gw * gh * 4 * sizeof(real)) : NULL;
+    fallbacks |= P(pyramidal_denoise)(rgbB, w, h, pp->levels, pp->algorithm, pp->dp, nlf, rgbA, guide);
     if (den_dump) memcpy(den_dump, rgbA, n * 4 * sizeof(real));
 
     if (P(inverse3x3)(c2y, (double)pp->exposure_multiplier, y2c)) fallbacks |= 1 << 9;
+    real* mask = NULL;
+    if (ltm) {
+        /* cpp/raw_converter.cpp:164-167: on the denoised YCbCr image */
+        mask = (real*)malloc(n * sizeof(real));
+        P(ltm_mask)(rgbA, w, h, guide, gw, gh, pp->ltm, y2c, mask, NULL);
+    }
     P(transform)(rgbA, w, h, y2c, rgbA);
     P(convert_to_srgb)(rgbA, w, h, pp->rgb_cam, pp->contrast, pp->saturation, pp->toneCurveSlope,
-                       pp->exposureBias, pp->blacks, out8, outf);
+                       pp->exposureBias, pp->blacks, mask, out8, outf);
+    free(guide);
+    free(mask);
     if (nlf_out) {
         memcpy(nlf_out, rawNlf, sizeof(rawNlf));
         memcpy(nlf_out + 4, nlf, (size_t)pp->levels * 6 * sizeof(float));

@@ -16,6 +16,9 @@ void om_exp(const float* x, float* out, long n) {
 void om_log(const float* x, float* out, long n) {
     for (long i = 0; i < n; i++) out[i] = clim_logf(x[i]);
 }
+void om_pow(const float* x, const float* y, float* out, long n) {
+    for (long i = 0; i < n; i++) out[i] = clim_powf(x[i], y[i]);
+}
 void om_tanh(const float* x, float* out, long n) {
     for (long i = 0; i < n; i++) out[i] = clim_tanhf(x[i]);
 }

@@ -43,7 +43,7 @@ class PipelineParams(C.Structure):
                 ("pyramidNlf", C.c_float * 30), ("dp", DenoiseParams * 5),
                 ("contrast", C.c_float), ("saturation", C.c_float), ("toneCurveSlope", C.c_float),
                 ("exposureBias", C.c_float), ("blacks", C.c_float), ("levels", C.c_int),
-                ("algorithm", C.c_int)]
+                ("algorithm", C.c_int), ("localToneMapping", C.c_int), ("ltm", C.c_float * 4)]
 
 
 # pyramidNlf row 0 of cpp/raw_pipeline.cpp:105-111 ("minimal noise model")
@@ -77,6 +77,10 @@ def default_params(bayer=2, black=64.0, white=4095.0, levels=5, algorithm=0):
     p.contrast, p.saturation, p.toneCurveSlope, p.exposureBias, p.blacks = 1.05, 1.0, 3.5, 0.0, 0.0
     p.levels = levels
     p.algorithm = algorithm
+    # LTMParameters defaults, cpp/demosaic.hpp:54-59
+    p.localToneMapping = 0
+    for k, v in enumerate((0.01, 1.25, 1.0, 1.1)):
+        p.ltm[k] = v
     return p
 
 
@@ -230,15 +234,30 @@ class Oracle:
         return out
 
     def convert_to_srgb(self, img, transform, contrast=1.05, saturation=1.0, slope=3.5, exposure_bias=0.0,
-                        blacks=0.0):
+                        blacks=0.0, ltm_mask=None):
         h, w, _ = img.shape
         img = self._chk(img)
+        mask = self._chk(ltm_mask, (h, w)) if ltm_mask is not None else None
         out8 = np.zeros((h, w, 4), np.uint8)
         outf = self._new(h, w, 4)
         self._fn("convert_to_srgb")(_ptr(img), w, h, _f(np.asarray(transform).ravel()), C.c_float(contrast),
                                     C.c_float(saturation), C.c_float(slope), C.c_float(exposure_bias),
-                                    C.c_float(blacks), _ptr(out8), _ptr(outf))
+                                    C.c_float(blacks), _ptr(mask) if mask is not None else None, _ptr(out8),
+                                    _ptr(outf))
         return out8, outf
+
+    def ltm_mask(self, img, guide, ltm, transform, var_eps=False):
+        """localToneMappingMask: img the denoised YCbCr image (H, W, 4), guide
+        (gh, gw, 4); ltm = (guidedFilterEps, shadows, highlights, detail).
+        With var_eps, also the per-pixel denominator var + eps."""
+        h, w, _ = img.shape
+        gh, gw, _ = guide.shape
+        img, guide = self._chk(img), self._chk(guide)
+        out = self._new(h, w)
+        ve = self._new(h, w) if var_eps else None
+        self._fn("ltm_mask")(_ptr(img), w, h, _ptr(guide), gw, gh, _f(ltm), _f(np.asarray(transform).ravel()),
+                             _ptr(out), _ptr(ve) if var_eps else None)
+        return (out, ve) if var_eps else out
 
     def cam_ycbcr(self, rgb_cam):
         out = (C.c_float * 9)()
@@ -251,14 +270,19 @@ class Oracle:
         rc = self._fn("inverse3x3")(_f(np.asarray(m).ravel()), C.c_double(scale), out)
         return rc, np.array(out[:], np.float32).reshape(3, 3)
 
-    def pyramidal_denoise(self, img, levels, algorithm, params, nlf_in):
+    def pyramidal_denoise(self, img, levels, algorithm, params, nlf_in, level3=False):
+        """Returns (fallback mask, denoised level 0, fitted NLF) and, with
+        level3, the reassembled level 3 (the local tone mapping guide)."""
         h, w, _ = img.shape
         img = self._chk(img)
         out = self._new(h, w, 4)
+        g = self._new(h >> 3, w >> 3, 4) if level3 and levels > 3 else None
         dp = (DenoiseParams * 5)(*params)
         nlf = _f(np.asarray(nlf_in, np.float32).ravel())
-        rc = self._fn("pyramidal_denoise")(_ptr(img), w, h, levels, algorithm, dp, nlf, _ptr(out))
-        return rc, out, np.array(nlf[:], np.float32).reshape(-1, 6)
+        rc = self._fn("pyramidal_denoise")(_ptr(img), w, h, levels, algorithm, dp, nlf, _ptr(out),
+                                           _ptr(g) if g is not None else None)
+        res = (rc, out, np.array(nlf[:], np.float32).reshape(-1, 6))
+        return res + (g,) if level3 else res
 
     def raw_to_rgb(self, raw, params, dumps=False):
         h, w = raw.shape

@@ -60,6 +60,46 @@ def test_exp_log_tanh_cos(L):
     assert _vec(L, "om_log", np.array([1.0], np.float32))[0] == 0.0
 
 
+def _vec2(L, name, x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    out = np.empty_like(x)
+    getattr(L, name)(x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+                     C.c_long(x.size))
+    return out
+
+
+def test_pow_accuracy_and_c99_special_values(L):
+    rng = np.random.default_rng(5)
+    x = (10.0 ** rng.uniform(-6, 3, 300000)).astype(np.float32)
+    y = rng.uniform(-3, 3, 300000).astype(np.float32)
+    ref = np.power(x.astype(np.float64), y.astype(np.float64))
+    keep = np.abs(y.astype(np.float64) * np.log(x.astype(np.float64))) <= 20
+    got = _vec2(L, "om_pow", x, y)
+    rel = np.abs(got[keep] / ref[keep] - 1)
+    assert rel.max() < 4e-6, rel.max()
+    small = np.abs(y.astype(np.float64) * np.log(x.astype(np.float64))) <= 4
+    assert np.abs(got[small] / ref[small] - 1).max() < 1e-6
+    # every pairing of the special operands against C's pow (numpy float64):
+    # NaN-ness, sign and value (exact for the special results)
+    sv = np.array([0.0, -0.0, 1.0, -1.0, 2.0, -2.0, 0.5, -0.5, np.inf, -np.inf, np.nan, 3.0, -3.0, 2.5, -2.5,
+                   1e-3, 16777216.0, -16777217.0], np.float32)
+    xs, ys = np.meshgrid(sv, sv, indexing="ij")
+    xs, ys = xs.ravel(), ys.ravel()
+    with np.errstate(all="ignore"):
+        ref = np.power(xs.astype(np.float64), ys.astype(np.float64)).astype(np.float32)
+    got = _vec2(L, "om_pow", xs, ys)
+    for a, b, g, r in zip(xs, ys, got, ref):
+        if np.isnan(r):
+            assert np.isnan(g), (a, b, g)
+            continue
+        assert not np.isnan(g) and np.signbit(g) == np.signbit(r), (a, b, g, r)
+        if np.isinf(r) or r == 0 or abs(np.log(abs(float(a))) * float(b)) < 1e-30:
+            assert g == r, (a, b, g, r)
+        else:
+            assert abs(g / r - 1) < 4e-6, (a, b, g, r)
+
+
 def test_smoothstep_opencl_semantics(L):
     x = np.array([-1, 0.0, 0.25, 0.5, 1.0, 2.0, np.nan], np.float32)
     out = np.empty_like(x)
