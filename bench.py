@@ -66,11 +66,11 @@ def stage_bytes(stage, texels, levels_px=None):
 SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green": "k_green",
                  "red_blue_ycc": "k_rb_despeckle<true>", "despeckle": "k_despeckle",
                  "downsample": "k_downsample<true>", "level_stats": "k_noise_stats",
-                 "denoise": "k_bilateral<true>", "reassemble": "k_reassemble<true, false>",
-                 "tone": "k_reassemble<true, true>"}
+                 "denoise": "k_bilateral<true>", "reassemble": "k_reassemble<true, false, false>",
+                 "tone": "k_reassemble<true, true, false>"}
 
 
-def cpu_baseline(w, h, bayer, seconds_min=10.0):
+def cpu_baseline(w, h, bayer, ltm=False, seconds_min=10.0):
     """The FP32 CPU restatement on the bench frames (same sizes and noise
     levels), as many as fit in about `seconds_min` seconds (at least one)."""
     from oracle import oracle as O
@@ -78,6 +78,7 @@ def cpu_baseline(w, h, bayer, seconds_min=10.0):
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
     o = O.Oracle(double=False)
     p = O.default_params(bayer, 64.0, 4095.0, levels=5)
+    p.localToneMapping = int(ltm)
     done, dt = 0, 0.0
     for i, a in enumerate(NOISE_A):
         raw = O.generate_bayer(w, h, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
@@ -130,6 +131,8 @@ def main():
     ap.add_argument("--width", type=int, default=4032)
     ap.add_argument("--height", type=int, default=3024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ltm", action="store_true",
+                    help="local tone mapping on (its mask is computed in the level-0 reassembly + tone kernel)")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device fill and device-to-device copy (reference kernels "
@@ -150,6 +153,10 @@ def main():
         raws.append(r)
         outs.append(ctx.image(w, h, ca.FMT_RGBA8))
     params = ca.default_params(bayer, 64.0, 4095.0, levels=5, algorithm=ca.BILATERAL)
+    params.rgbConversionParameters.localToneMapping = int(args.ltm)
+    single_kernel = dict(SINGLE_KERNEL)
+    if args.ltm:
+        single_kernel["tone"] = "k_reassemble<true, true, true>"
     ctx.sync()
 
     def step():
@@ -187,7 +194,7 @@ def main():
         stages = read_profile()
         for v in stages.values():
             v["us_per_frame"] = round(v["ms_total"] * 1e3 / (psteps * len(raws)), 2)
-        dom = max((k for k in stages if k in SINGLE_KERNEL), key=lambda k: stages[k]["ms_total"])
+        dom = max((k for k in stages if k in single_kernel), key=lambda k: stages[k]["ms_total"])
         # 2. the timed steps record events around the dominant kernel only
         lib.clim_profile_select(ctx.h, C.c_uint32(1 << names.index(dom)))
         read_profile()
@@ -212,12 +219,12 @@ def main():
         bytes_per_launch = stage_bytes(dom, s["texels"] / s["launches"])
         ach = bytes_per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(SINGLE_KERNEL[dom], w, h),
-                    "kernel": SINGLE_KERNEL[dom], "stage": dom, "avg_launch_us": round(avg_s * 1e6, 2),
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(single_kernel[dom], w, h),
+                    "kernel": single_kernel[dom], "stage": dom, "avg_launch_us": round(avg_s * 1e6, 2),
                     "launches_timed": s["launches"], "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(w, h, bayer)
+        cpu = cpu_baseline(w, h, bayer, ltm=args.ltm)
 
     res = {
         "metric": "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP",
@@ -232,7 +239,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scenes generated on the device)",
-        "config": {"workload": f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, 5-level bilateral pyramid",
+        "config": {"workload": f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, 5-level bilateral pyramid"
+                               + (", local tone mapping" if args.ltm else ""),
                    "frames_per_step": len(raws), "pixels_per_frame": w * h},
         "roofline": roofline,
         "path_roofline": {"bytes_per_px": path_bytes_per_px(5),

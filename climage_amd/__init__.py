@@ -95,7 +95,7 @@ def exported_symbols():
                         "clim_noise_statistics", "clim_compute_noise_statistics", "clim_denoise_image",
                         "clim_denoise_image_guided", "clim_guided_filter", "clim_reassemble_image",
                         "clim_convert_to_srgb", "clim_local_tone_mapping_mask", "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
-                        "clim_pyramidal_denoise", "clim_raw_to_rgb", "clim_generate_bayer")]
+                        "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_generate_bayer")]
 
 
 def _f(vals):
@@ -285,6 +285,18 @@ class Context:
                  + [0.0] * (30 - 6 * levels))
         self._check(lib().clim_pyramidal_denoise(self.h, image.ref(), levels, algorithm, dp, out.ref(), buf),
                     "pyramidalDenoise")
+        return np.array(buf[:6 * levels], np.float32).reshape(levels, 6)
+
+    def pyramidalDenoiseInto(self, image, levels, algorithm, params, image_pyramid, denoised_pyramid, nlf=None):
+        """clim_pyramidal_denoise_into: image_pyramid (levels - 1 images, level
+        i + 1 at index i) and denoised_pyramid (levels images) are filled."""
+        dp = (DenoiseParameters * 5)(*params)
+        buf = _f(np.zeros(30) if nlf is None else np.asarray(nlf, np.float32).ravel().tolist()
+                 + [0.0] * (30 - 6 * levels))
+        lv = (C.POINTER(Image) * len(image_pyramid))(*[C.pointer(im.im) for im in image_pyramid])
+        dn = (C.POINTER(Image) * len(denoised_pyramid))(*[C.pointer(im.im) for im in denoised_pyramid])
+        self._check(lib().clim_pyramidal_denoise_into(self.h, image.ref(), levels, algorithm, dp, lv, dn, buf),
+                    "pyramidalDenoiseInto")
         return np.array(buf[:6 * levels], np.float32).reshape(levels, 6)
 
     def raw_to_rgb(self, raw, out8, params, out_f32=None, writeback=True):

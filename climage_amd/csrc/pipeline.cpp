@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <string>
 
 #include "ctx.h"
 
@@ -69,17 +70,20 @@ static Img sub_view(const Img& im, int w, int h) { return Img{im.p, w, h, im.pit
 // Fit slots 1..levels receive the per-level noise models.
 // With `tone`, level 0's reassembly hands its result to the tone map in the
 // same kernel (out is then left holding level 0's denoised-only texels).
+// ext_lvl / ext_den (nullable): the caller's images for levels 1..levels-1
+// (the downsampled and the denoised pyramid) instead of the workspace's.
 static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int levels, int algorithm,
                            const clim_denoise_params* dp, const float (*nlf_in)[6],
-                           const clim::ToneLaunch* tone = nullptr) {
+                           const clim::ToneLaunch* tone = nullptr, const Img* ext_lvl = nullptr,
+                           const Img* ext_den = nullptr) {
     Workspace& ws = ctx->ws;
     hipStream_t st = ctx->stream;
     Img lvl[CLIM_MAX_LEVELS], den[CLIM_MAX_LEVELS];
     lvl[0] = level0;
     den[0] = out;
     for (int i = 1; i < levels; i++) {
-        lvl[i] = ws.pyr[i].img;
-        den[i] = ws.den[i].img;
+        lvl[i] = ext_lvl ? ext_lvl[i] : ws.pyr[i].img;
+        den[i] = ext_den ? ext_den[i] : ws.den[i].img;
     }
     // analysis: downsample and statistics of every level
     clim::FitJobs jobs{};
@@ -179,6 +183,50 @@ extern "C" clim_status clim_pyramidal_denoise(clim_ctx* ctx, const clim_image* i
     if (nlf_out) std::memcpy(nlf_in, nlf_out, sizeof(float) * 6 * levels);
     else std::memset(nlf_in, 0, sizeof(nlf_in));
     CLIM_TRY(pyramid(ctx, view(image), view(out), levels, algorithm, params, nlf_in));
+    if (nlf_out) {
+        CLIM_TRY_HIP(ctx, hipMemcpyAsync(ctx->fit_host + 1, ctx->fit + 1, sizeof(clim::FitState) * levels,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+        CLIM_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < levels; i++) std::memcpy(nlf_out + 6 * i, ctx->fit_host[1 + i].out, 6 * sizeof(float));
+    }
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_pyramidal_denoise_into(clim_ctx* ctx, const clim_image* image, int32_t levels,
+                                                   int32_t algorithm, const clim_denoise_params* params,
+                                                   clim_image* const* image_pyramid,
+                                                   clim_image* const* denoised_pyramid, float* nlf_out) {
+    if (!ctx || !params || !image_pyramid || !denoised_pyramid)
+        return ctx_fail(ctx, CLIM_EINVAL, "pyramidal denoise: null argument");
+    if (levels < 2 || levels > CLIM_MAX_LEVELS || algorithm < 0 || algorithm > 2)
+        return ctx_fail(ctx, CLIM_EINVAL, "pyramidal denoise: bad levels or algorithm");
+    CLIM_TRY(check_image(ctx, image, CLIM_FMT_RGBA32F, "image"));
+    const int w = image->width, h = image->height;
+    if ((w >> (levels - 1)) < 1 || (h >> (levels - 1)) < 1)
+        return ctx_fail(ctx, CLIM_EINVAL, "pyramidal denoise: image too small for the pyramid depth");
+    Img lvl[CLIM_MAX_LEVELS], den[CLIM_MAX_LEVELS];
+    for (int i = 0; i < levels; i++) {
+        // level i is width / 2^i x height / 2^i (cpp/pyramidal_denoise.cpp:71-76)
+        const clim_image* d = denoised_pyramid[i];
+        CLIM_TRY(check_image(ctx, d, CLIM_FMT_RGBA32F, "denoised_pyramid"));
+        if (d->width != (w >> i) || d->height != (h >> i) || d->data == image->data)
+            return ctx_fail(ctx, CLIM_EINVAL, "pyramidal denoise: denoised level " + std::to_string(i) +
+                                                  " must be a distinct image of width/2^i x height/2^i");
+        den[i] = view(d);
+        if (i > 0) {
+            const clim_image* l = image_pyramid[i - 1];
+            CLIM_TRY(check_image(ctx, l, CLIM_FMT_RGBA32F, "image_pyramid"));
+            if (l->width != (w >> i) || l->height != (h >> i) || l->data == d->data)
+                return ctx_fail(ctx, CLIM_EINVAL, "pyramidal denoise: image_pyramid[" + std::to_string(i - 1) +
+                                                      "] must be a distinct image of width/2^i x height/2^i");
+            lvl[i] = view(l);
+        }
+    }
+    CLIM_TRY(ensure_workspace(ctx, w, h, levels, algorithm));
+    float nlf_in[CLIM_MAX_LEVELS][6];
+    if (nlf_out) std::memcpy(nlf_in, nlf_out, sizeof(float) * 6 * levels);
+    else std::memset(nlf_in, 0, sizeof(nlf_in));
+    CLIM_TRY(pyramid(ctx, view(image), den[0], levels, algorithm, params, nlf_in, nullptr, lvl, den));
     if (nlf_out) {
         CLIM_TRY_HIP(ctx, hipMemcpyAsync(ctx->fit_host + 1, ctx->fit + 1, sizeof(clim::FitState) * levels,
                                          hipMemcpyDeviceToHost, ctx->stream));

@@ -2,8 +2,9 @@
 // RawPipeline does (cpp/raw_pipeline.cpp:216-256), on a seeded synthetic
 // mosaic instead of a DNG file.
 //
-//   raw_pipeline W H BAYER LEVELS OUT_PREFIX
+//   raw_pipeline W H BAYER LEVELS OUT_PREFIX [LTM]
 //
+// LTM = 1 turns on local tone mapping (rgbConversionParameters.localToneMapping).
 // Writes OUT_PREFIX.fused.rgb (RawConverter::demosaicImage + convertToRGBImage),
 // OUT_PREFIX.stages.rgb (the same path spelled out operator by operator, with a
 // caller-supplied ImageDenoiser so that PyramidalDenoise runs level by level
@@ -51,10 +52,11 @@ static void write_rgb(const std::string& path, const gls::image<gls::rgb_pixel>&
 }
 
 int main(int argc, char** argv) {
-    if (argc != 6) {
-        std::cerr << "usage: raw_pipeline W H BAYER LEVELS OUT_PREFIX\n";
+    if (argc != 6 && argc != 7) {
+        std::cerr << "usage: raw_pipeline W H BAYER LEVELS OUT_PREFIX [LTM]\n";
         return 2;
     }
+    const int ltm = argc == 7 ? std::atoi(argv[6]) : 0;
     const int w = std::atoi(argv[1]), h = std::atoi(argv[2]);
     const BayerPattern bayer = (BayerPattern)std::atoi(argv[3]);
     const int levels = std::atoi(argv[4]);
@@ -74,12 +76,14 @@ int main(int argc, char** argv) {
 
         // 1. the fused pipeline through RawConverter
         DemosaicParameters p1 = synthetic_params(bayer, black, white);
+        p1.rgbConversionParameters.localToneMapping = ltm;
         RawConverter rc(&ctx);
         auto rgb1 = RawConverter::convertToRGBImage(*rc.demosaicImage(*rawImage, &p1, nullptr, false));
         write_rgb(out + ".fused.rgb", *rgb1);
 
         // 2. operator by operator (cpp/raw_converter.cpp:101-174)
         DemosaicParameters p2 = synthetic_params(bayer, black, white);
+        p2.rgbConversionParameters.localToneMapping = ltm;
         gls::hip_image_2d<gls::luma_pixel_16> raw(&ctx, *rawImage);
         gls::hip_image_2d<gls::luma_pixel_float> scaled(&ctx, w, h), green(&ctx, w, h);
         gls::hip_image_2d<gls::rgba_pixel_float> A(&ctx, w, h), B(&ctx, w, h);
@@ -95,19 +99,33 @@ int main(int argc, char** argv) {
         transformImage(&ctx, A, &A, cam_to_ycbcr);
         const auto& np = p2.noiseModel.pyramidNlf[0];
         despeckleImage(&ctx, A, {np[0], np[1], np[2]}, {np[3], np[4], np[5]}, &B);
-        PyramidalDenoise<5> pyramid(&ctx, w, h);
-        for (int i = 0; i < 5; i++) pyramid.denoiser[i] = std::make_unique<MyBilateral>(&ctx, w >> i, h >> i);
-        auto* den = pyramid.denoise(&ctx, &p2.denoiseParameters, &B, p2.rgb_cam, nullptr, false,
-                                    &p2.noiseModel.pyramidNlf);
+        // the pyramid twice: with a caller plug-in (level by level, as the
+        // reference orchestrates it) and with the built-in filter (one fused
+        // call that fills the same level images)
         const Matrix3x3 inv = inverse(cam_to_ycbcr);  // exposure_multiplier = 1
-        transformImage(&ctx, *den, den, inv);
-        convertTosRGB(&ctx, *den, &srgb, p2);
-        auto rgb2 = RawConverter::convertToRGBImage(srgb);
-        write_rgb(out + ".stages.rgb", *rgb2);
+        DemosaicParameters p3 = p2;
+        for (int plugin = 1; plugin >= 0; plugin--) {
+            DemosaicParameters& pp = plugin ? p2 : p3;
+            PyramidalDenoise<5> pyramid(&ctx, w, h);
+            if (plugin)
+                for (int i = 0; i < 5; i++) pyramid.denoiser[i] = std::make_unique<MyBilateral>(&ctx, w >> i, h >> i);
+            // (the pyramid reads its input image, it does not write it)
+            auto* den = pyramid.denoise(&ctx, &pp.denoiseParameters, &B, pp.rgb_cam, nullptr, false,
+                                        &pp.noiseModel.pyramidNlf);
+            // local tone mapping mask on the denoised YCbCr image, guided by level 3
+            // (cpp/raw_converter.cpp:164-167); a 1x1 placeholder when off
+            gls::hip_image_2d<gls::luma_pixel_float> ltmMask(&ctx, ltm ? w : 1, ltm ? h : 1);
+            if (ltm)
+                localToneMappingMask(&ctx, *den, *pyramid.denoisedImagePyramid[3], pp.ltmParameters, inv, &ltmMask);
+            transformImage(&ctx, *den, den, inv);
+            convertTosRGB(&ctx, *den, ltmMask, &srgb, pp);
+            auto rgb2 = RawConverter::convertToRGBImage(srgb);
+            write_rgb(out + (plugin ? ".stages.rgb" : ".builtin.rgb"), *rgb2);
+        }
 
         std::ofstream f(out + ".nlf");
         f.precision(9);
-        for (const auto* p : {&p1, &p2}) {
+        for (const auto* p : {&p1, &p2, &p3}) {
             for (float v : p->noiseModel.rawNlf) f << v << ' ';
             for (const auto& row : p->noiseModel.pyramidNlf)
                 for (float v : row) f << v << ' ';

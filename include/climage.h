@@ -254,6 +254,18 @@ clim_status clim_pyramidal_denoise(clim_ctx* ctx, const clim_image* image, int32
                                    int32_t algorithm, const clim_denoise_params* params,
                                    clim_image* out, float* nlf_out);
 
+/* The same, in the caller's level images: PyramidalDenoise's imagePyramid
+ * (levels - 1 images, level i + 1 at index i) and denoisedImagePyramid
+ * (levels images, level 0 the result), cpp/pyramidal_denoise.hpp:37-51,
+ * each width / 2^i x height / 2^i RGBA32F.  They end up holding what the
+ * reference's members hold after denoise(): the downsampled levels and the
+ * reassembled denoised levels (level 3 is the local tone mapping guide,
+ * cpp/raw_converter.cpp:165). */
+clim_status clim_pyramidal_denoise_into(clim_ctx* ctx, const clim_image* image, int32_t levels,
+                                        int32_t algorithm, const clim_denoise_params* params,
+                                        clim_image* const* image_pyramid, clim_image* const* denoised_pyramid,
+                                        float* nlf_out);
+
 /* RawConverter::demosaicImage, cpp/raw_converter.cpp:73-186: raw mosaic to
  * RGBA8 (and optionally the FP32 values before quantisation).  Work
  * buffers are owned by the context and resized when the frame size changes.

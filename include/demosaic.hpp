@@ -185,14 +185,33 @@ inline void transformImage(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rg
     ctx->check(clim_transform_image(ctx->ctx(), linearImage.clim(), rgbImage->clim(), m.data()), "transformImage");
 }
 
+// cpp/demosaic_cl.hpp:69-73.  ltmMaskImage is read only with
+// rgbConversionParameters.localToneMapping set; otherwise a 1x1 placeholder
+// will do, as in cpp/raw_converter.cpp:36.
 inline void convertTosRGB(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgba_pixel_float>& linearImage,
+                          const gls::hip_image_2d<gls::luma_pixel_float>& ltmMaskImage,
                           gls::hip_image_2d<gls::rgba_pixel>* rgbImage, const DemosaicParameters& demosaicParameters) {
     auto m = flat(demosaicParameters.rgb_cam);
     const RGBConversionParameters& r = demosaicParameters.rgbConversionParameters;
     clim_rgb_conversion_params p{r.contrast, r.saturation, r.toneCurveSlope, r.exposureBias, r.blacks,
                                  r.localToneMapping};
-    ctx->check(clim_convert_to_srgb(ctx->ctx(), linearImage.clim(), nullptr, rgbImage->clim(), nullptr, m.data(), &p),
+    ctx->check(clim_convert_to_srgb(ctx->ctx(), linearImage.clim(), ltmMaskImage.clim(), rgbImage->clim(), nullptr,
+                                    m.data(), &p),
                "convertTosRGB");
+}
+
+// cpp/demosaic_cl.hpp:91-95.  inputImage: the denoised YCbCr image; guideImage:
+// pyramid level 3 in cpp/raw_converter.cpp:165; transform: YCbCr -> camera RGB.
+inline void localToneMappingMask(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgba_pixel_float>& inputImage,
+                                 const gls::hip_image_2d<gls::rgba_pixel_float>& guideImage,
+                                 const LTMParameters& ltmParameters, const Matrix3x3& transform,
+                                 gls::hip_image_2d<gls::luma_pixel_float>* outputImage) {
+    auto m = flat(transform);
+    const clim_ltm_params lp{ltmParameters.guidedFilterEps, ltmParameters.shadows, ltmParameters.highlights,
+                             ltmParameters.detail};
+    ctx->check(clim_local_tone_mapping_mask(ctx->ctx(), inputImage.clim(), guideImage.clim(), &lp, m.data(),
+                                            outputImage->clim()),
+               "localToneMappingMask");
 }
 
 inline void despeckleImage(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgba_pixel_float>& inputImage,

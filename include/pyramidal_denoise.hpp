@@ -111,8 +111,15 @@ struct PyramidalDenoise {
             float nlf[5 * 6];
             for (size_t i = 0; i < levels; i++)
                 for (int k = 0; k < 6; k++) nlf[i * 6 + k] = (*nlfParameters)[i][k];
-            ctx->check(clim_pyramidal_denoise(ctx->ctx(), image->clim(), (int)levels, (int)_algorithm, dp,
-                                              denoisedImagePyramid[0]->clim(), nlf),
+            // one device-resident call that leaves every level in this object's images
+            clim_image* lvl[levels - 1];
+            clim_image* den[levels];
+            for (size_t i = 0; i < levels; i++) {
+                den[i] = denoisedImagePyramid[i]->clim();
+                if (i + 1 < levels) lvl[i] = imagePyramid[i]->clim();
+            }
+            ctx->check(clim_pyramidal_denoise_into(ctx->ctx(), image->clim(), (int)levels, (int)_algorithm, dp, lvl,
+                                                   den, nlf),
                        "PyramidalDenoise::denoise");
             for (size_t i = 0; i < levels; i++)
                 for (int k = 0; k < 6; k++) (*nlfParameters)[i][k] = nlf[i * 6 + k];

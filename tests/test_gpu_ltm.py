@@ -127,3 +127,33 @@ def test_raw_to_rgb_ltm_needs_level3(ctx):
     r = ctx.upload(raw, ca.FMT_R16U)
     with pytest.raises(ca.ClimError):
         ctx.raw_to_rgb(r, ctx.image(64, 48, ca.FMT_RGBA8), p, writeback=False)
+
+
+@pytest.mark.parametrize("w,h,levels", [(256, 192, 5), (200, 136, 4)])
+def test_pyramidal_denoise_into_fills_levels(ctx, oracle32, w, h, levels):
+    """The fused pyramid run in the caller's level images: the downsampled
+    levels, the denoised level 0 and the reassembled level 3 (the local tone
+    mapping guide) equal the oracle's."""
+    raw = O.generate_bayer(w, h, ca.RGGB, 7, 8, 0.002, 1e-5, 64.0, 4095.0)
+    op = O.default_params(ca.RGGB, 64.0, 4095.0, levels=levels)
+    ycc = oracle32.raw_to_rgb(raw, op, dumps=True)["ycc"]
+    dp = [O.DenoiseParams(1.0, 1.0, 1.0, 1.0, 1.0)] * 5
+    rc, den0, nlf, g3 = oracle32.pyramidal_denoise(ycc, levels, 0, dp, O.DEFAULT_PYRAMID_NLF, level3=True)
+    img = ctx.upload(ycc, ca.FMT_RGBA32F)
+    lv = [ctx.image(w >> i, h >> i, ca.FMT_RGBA32F) for i in range(1, levels)]
+    dn = [ctx.image(w >> i, h >> i, ca.FMT_RGBA32F) for i in range(levels)]
+    cp = [ca.DenoiseParameters(1.0, 1.0, 1.0, 1.0, 1.0)] * 5
+    got_nlf = ctx.pyramidalDenoiseInto(img, levels, ca.BILATERAL, cp, lv, dn,
+                                       nlf=np.asarray(O.DEFAULT_PYRAMID_NLF[:levels]))
+    bitexact(dn[0].download(), den0, "denoised level 0")
+    bitexact(dn[3].download(), g3, "denoised level 3")
+    bitexact(got_nlf, nlf[:levels], "per-level noise model")
+    prev = ycc
+    for i in range(1, levels):
+        prev = oracle32.downsample(prev, w >> i, h >> i)
+        bitexact(lv[i - 1].download(), prev, f"downsampled level {i}")
+    # a level of the wrong size is rejected
+    bad = [ctx.image(w >> i, h >> i, ca.FMT_RGBA32F) for i in range(1, levels)]
+    bad[1] = ctx.image((w >> 2) + 1, h >> 2, ca.FMT_RGBA32F)
+    with pytest.raises(ca.ClimError):
+        ctx.pyramidalDenoiseInto(img, levels, ca.BILATERAL, cp, bad, dn)
