@@ -170,16 +170,17 @@ def main():
     import ctypes as C
 
     class Prof(C.Structure):
-        _fields_ = [("ms", C.c_double * 12), ("launches", C.c_int64 * 12), ("texels", C.c_int64 * 12)]
+        _fields_ = [("ms", C.c_double * ca.PROFILE_STAGES), ("launches", C.c_int64 * ca.PROFILE_STAGES),
+                    ("texels", C.c_int64 * ca.PROFILE_STAGES)]
 
     lib.clim_profile_stage_name.restype = C.c_char_p
-    names = [lib.clim_profile_stage_name(i).decode() for i in range(12)]
+    names = [lib.clim_profile_stage_name(i).decode() for i in range(ca.PROFILE_STAGES)]
 
     def read_profile():
         pr = Prof()
         lib.clim_profile_read(ctx.h, C.byref(pr))
         return {names[i]: {"ms_total": pr.ms[i], "launches": int(pr.launches[i]), "texels": int(pr.texels[i])}
-                for i in range(12) if pr.launches[i]}
+                for i in range(ca.PROFILE_STAGES) if pr.launches[i]}
 
     # 1. stage breakdown: a few untimed steps with every stage bracketed by events
     stages, dom = {}, None

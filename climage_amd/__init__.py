@@ -85,6 +85,9 @@ def lib():
     return _lib
 
 
+PROFILE_STAGES = 13  # CLIM_PROFILE_STAGES
+
+
 def exported_symbols():
     return [n for n in ("clim_create", "clim_destroy", "clim_last_error", "clim_status_string", "clim_sync",
                         "clim_stream", "clim_image_alloc", "clim_image_free", "clim_copy_from_host",
@@ -94,7 +97,9 @@ def exported_symbols():
                         "clim_transform_image", "clim_despeckle_image", "clim_downsample_image",
                         "clim_noise_statistics", "clim_compute_noise_statistics", "clim_denoise_image",
                         "clim_denoise_image_guided", "clim_guided_filter", "clim_reassemble_image",
-                        "clim_convert_to_srgb", "clim_local_tone_mapping_mask", "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
+                        "clim_convert_to_srgb", "clim_local_tone_mapping_mask", "clim_bayer_to_raw_rgba",
+                        "clim_raw_rgba_to_bayer", "clim_despeckle_raw_rgba", "clim_denoise_raw_rgba",
+                        "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
                         "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_generate_bayer")]
 
 
@@ -273,6 +278,19 @@ class Context:
                                                ltm_mask.ref() if ltm_mask is not None else None, out8.ref(),
                                                out_f32.ref() if out_f32 is not None else None, _f(transform),
                                                C.byref(params)), "convertTosRGB")
+
+    def bayerToRawRGBA(self, raw, rgba, bayer):
+        self._check(lib().clim_bayer_to_raw_rgba(self.h, raw.ref(), rgba.ref(), int(bayer)), "bayerToRawRGBA")
+
+    def rawRGBAToBayer(self, rgba, raw, bayer):
+        self._check(lib().clim_raw_rgba_to_bayer(self.h, rgba.ref(), raw.ref(), int(bayer)), "rawRGBAToBayer")
+
+    def despeckleRawRGBAImage(self, inp, out):
+        self._check(lib().clim_despeckle_raw_rgba(self.h, inp.ref(), out.ref()), "despeckleRawRGBAImage")
+
+    def denoiseRawRGBAImage(self, inp, raw_variance, out):
+        self._check(lib().clim_denoise_raw_rgba(self.h, inp.ref(), _f(raw_variance), out.ref()),
+                    "denoiseRawRGBAImage")
 
     def localToneMappingMask(self, inp, guide, ltm, transform, out):
         """ltm: LTMParameters; transform: YCbCr -> camera RGB (3x3)."""

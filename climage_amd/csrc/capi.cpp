@@ -408,6 +408,59 @@ extern "C" clim_status clim_despeckle_image(clim_ctx* ctx, const clim_image* in,
     return CLIM_OK;
 }
 
+// ------------------------------------------------ high-noise raw branch
+
+static clim_status check_quads(clim_ctx* ctx, const clim_image* raw, const clim_image* rgba, const char* what) {
+    if (raw->width != 2 * rgba->width || raw->height != 2 * rgba->height)
+        return ctx_fail(ctx, CLIM_EINVAL, std::string(what) + ": the raw image must be twice the RGBA image's size");
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_bayer_to_raw_rgba(clim_ctx* ctx, const clim_image* raw, clim_image* rgba,
+                                              int32_t bayer) {
+    if (!ctx || bayer < 0 || bayer > 3) return ctx_fail(ctx, CLIM_EINVAL, "bayerToRawRGBA: bad argument");
+    CLIM_TRY(check_img(ctx, raw, CLIM_FMT_R32F, "raw"));
+    CLIM_TRY(check_img(ctx, rgba, CLIM_FMT_RGBA32F, "rgba"));
+    CLIM_TRY(check_quads(ctx, raw, rgba, "bayerToRawRGBA"));
+    clim::launch_bayer_to_raw_rgba(view(raw), view(rgba), make_bayer(bayer), ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_raw_rgba_to_bayer(clim_ctx* ctx, const clim_image* rgba, clim_image* raw,
+                                              int32_t bayer) {
+    if (!ctx || bayer < 0 || bayer > 3) return ctx_fail(ctx, CLIM_EINVAL, "rawRGBAToBayer: bad argument");
+    CLIM_TRY(check_img(ctx, rgba, CLIM_FMT_RGBA32F, "rgba"));
+    CLIM_TRY(check_img(ctx, raw, CLIM_FMT_R32F, "raw"));
+    CLIM_TRY(check_quads(ctx, raw, rgba, "rawRGBAToBayer"));
+    clim::launch_raw_rgba_to_bayer(view(rgba), view(raw), make_bayer(bayer), ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_despeckle_raw_rgba(clim_ctx* ctx, const clim_image* in, clim_image* out) {
+    if (!ctx) return CLIM_EINVAL;
+    CLIM_TRY(check_img(ctx, in, CLIM_FMT_RGBA32F, "in"));
+    CLIM_TRY(check_img(ctx, out, CLIM_FMT_RGBA32F, "out"));
+    CLIM_TRY(check_same(ctx, in, out, "despeckleRawRGBAImage"));
+    if (in->data == out->data) return ctx_fail(ctx, CLIM_EINVAL, "despeckleRawRGBAImage: in and out must differ");
+    clim::launch_despeckle_raw_rgba(view(in), view(out), ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
+extern "C" clim_status clim_denoise_raw_rgba(clim_ctx* ctx, const clim_image* in, const float raw_variance[4],
+                                             clim_image* out) {
+    if (!ctx || !raw_variance) return ctx_fail(ctx, CLIM_EINVAL, "denoiseRawRGBAImage: null argument");
+    CLIM_TRY(check_img(ctx, in, CLIM_FMT_RGBA32F, "in"));
+    CLIM_TRY(check_img(ctx, out, CLIM_FMT_RGBA32F, "out"));
+    CLIM_TRY(check_same(ctx, in, out, "denoiseRawRGBAImage"));
+    if (in->data == out->data) return ctx_fail(ctx, CLIM_EINVAL, "denoiseRawRGBAImage: in and out must differ");
+    clim::launch_denoise_raw_rgba(view(in), host_params(raw_variance, 4), view(out), ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
 extern "C" clim_status clim_downsample_image(clim_ctx* ctx, const clim_image* in, clim_image* out) {
     if (!ctx) return CLIM_EINVAL;
     CLIM_TRY(check_img(ctx, in, CLIM_FMT_RGBA32F, "in"));
@@ -624,7 +677,7 @@ extern "C" void clim_default_params(clim_demosaic_params* p, int32_t bayer, floa
 
 static const char* kStageNames[CLIM_PROFILE_STAGES] = {
     "scale_raw", "raw_stats", "raw_fit", "green", "red_blue_ycc", "despeckle",
-    "downsample", "level_stats", "level_fit", "denoise", "reassemble", "tone"};
+    "downsample", "level_stats", "level_fit", "denoise", "reassemble", "tone", "raw_denoise"};
 
 static constexpr size_t kProfEvents = 4096;
 

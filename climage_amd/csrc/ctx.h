@@ -34,7 +34,7 @@ constexpr int kFitK = 21;                       // largest accumulator count (ra
 
 enum Stage {
     ST_SCALE = 0, ST_RAW_STATS, ST_RAW_FIT, ST_GREEN, ST_RED_BLUE, ST_DESPECKLE,
-    ST_DOWNSAMPLE, ST_LEVEL_STATS, ST_LEVEL_FIT, ST_DENOISE, ST_REASSEMBLE, ST_TONE
+    ST_DOWNSAMPLE, ST_LEVEL_STATS, ST_LEVEL_FIT, ST_DENOISE, ST_REASSEMBLE, ST_TONE, ST_RAW_DENOISE
 };
 
 struct ProfRec {

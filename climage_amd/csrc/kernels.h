@@ -110,6 +110,15 @@ void launch_tone(const Img& in, const Img& mask, const Img& out8, const Img& out
 void launch_ltm_mask(const Img& in, const Img& guide, const float ltm[4], const float T[9], const Img& out,
                      hipStream_t st);
 
+// -------------------------------------------- high-noise raw branch (RGBA raw)
+// One (R, G, B, G2) quad per texel, w/2 x h/2 (k_rawrgba.hip).
+void launch_bayer_to_raw_rgba(const Img& raw, const Img& rgba, const Bayer& b, hipStream_t st);
+void launch_raw_rgba_to_bayer(const Img& rgba, const Img& raw, const Bayer& b, hipStream_t st);
+void launch_despeckle_raw_rgba(const Img& in, const Img& out, hipStream_t st);      // out != in
+void launch_denoise_raw_rgba(const Img& in, const ParamRef& var, const Img& out, hipStream_t st);  // out != in
+// the four in one pass: mosaic -> despeckle -> denoise -> FP32 mosaic (out != m.im)
+void launch_raw_predenoise(const MosaicArg& m, const ParamRef& var, const Img& out, hipStream_t st);
+
 // ----------------------------------------------- multi-pass guided filter
 // scratch: 7 images of w*h texels (float4) + 1 of float2, see k_guided.hip
 struct GuidedScratch {

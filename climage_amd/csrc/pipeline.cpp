@@ -36,8 +36,10 @@ static ParamRef dev_params(const float* p) {
     return r;
 }
 
-static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int algorithm) {
+static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int algorithm,
+                                   bool scaled_plane = false) {
     Workspace& ws = ctx->ws;
+    if (scaled_plane) CLIM_TRY(owned_alloc(ctx, ws.scaled, w, h, CLIM_FMT_R32F));
     CLIM_TRY(owned_alloc(ctx, ws.green, w, h, CLIM_FMT_R32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbA, w, h, CLIM_FMT_RGBA32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbB, w, h, CLIM_FMT_RGBA32F));
@@ -150,8 +152,6 @@ static clim_status check_params(clim_ctx* ctx, const clim_demosaic_params* p) {
     if (p->bayerPattern < 0 || p->bayerPattern > 3) return ctx_fail(ctx, CLIM_EINVAL, "bad bayerPattern");
     if (p->levels < 2 || p->levels > CLIM_MAX_LEVELS) return ctx_fail(ctx, CLIM_EINVAL, "levels must be 2..5");
     if (p->denoiseAlgorithm < 0 || p->denoiseAlgorithm > 2) return ctx_fail(ctx, CLIM_EINVAL, "bad denoiseAlgorithm");
-    if (p->noiseLevel > 0.6f)
-        return ctx_fail(ctx, CLIM_EINVAL, "noiseLevel > 0.6 selects the high-noise raw branch, which is not built");
     if (p->rgbConversionParameters.localToneMapping && p->levels < 4)
         return ctx_fail(ctx, CLIM_EINVAL, "local tone mapping reads pyramid level 3: levels must be 4 or 5");
     return CLIM_OK;
@@ -260,7 +260,9 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
     clim_cam_ycbcr(cam, c2y);
     if (inverse3x3_scaled(c2y, (double)p->exposure_multiplier, y2c))
         return ctx_fail(ctx, CLIM_ENUMERIC, "raw_to_rgb: camera->YCbCr matrix is singular");
-    CLIM_TRY(ensure_workspace(ctx, w, h, p->levels, p->denoiseAlgorithm));
+    // cpp/raw_converter.cpp:82: the raw-domain pre-denoise of high-noise frames
+    const bool high_noise = p->noiseLevel > 0.6f;
+    CLIM_TRY(ensure_workspace(ctx, w, h, p->levels, p->denoiseAlgorithm, high_noise));
     Workspace& ws = ctx->ws;
     hipStream_t st = ctx->stream;
     const int bayer = p->bayerPattern;
@@ -295,6 +297,15 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
         CLIM_TRY(run_fit_rest(ctx, rjobs, true));
     }
     const float* rawfit = ctx->fit[0].out;  // device: A[4], B[4]
+    if (high_noise) {
+        // bayerToRawRGBA -> despeckleRawRGBAImage -> denoiseRawRGBAImage(rawNlf)
+        // -> rawRGBAToBayer (cpp/raw_converter.cpp:111-123) in one kernel; the
+        // demosaic then reads the denoised FP32 mosaic
+        ProfScope ps(ctx, ST_RAW_DENOISE, npx / 4);
+        clim::launch_raw_predenoise(mosaic, dev_params(rawfit + 4), ws.scaled.img, st);
+        mosaic.im = ws.scaled.img;
+        mosaic.u16 = false;
+    }
     // interpolateGreen(rawNlf[1]) and interpolateRedBlue((rawNlf[0]+rawNlf[2])/2)
     // + blendHighlightsImage(clip = 1) + transformImage(cam_ycbcr(rgb_cam))
     {

@@ -249,6 +249,11 @@ CLIM_HD float clim_bilateral_dist(float d0, float d1, float d2, float k0, float 
     const float l = (d0 * k0) * d0;
     return fmaf(d2 * k2, d2, fmaf(d1 * k1, d1, l));
 }
+/* The same over four channels (denoiseRawRGBA, cl/demosaic.cl:932-948:
+ * length((s - c) / sigma) < 1 with sigma^2 = rawVariance * c). */
+CLIM_HD float clim_raw_dist4(float d0, float d1, float d2, float d3, float k0, float k1, float k2, float k3) {
+    return fmaf(d3 * k3, d3, clim_bilateral_dist(d0, d1, d2, k0, k1, k2));
+}
 
 /* --------------------------------------------------- v / 65535 for uint16 v
  * The correctly rounded quotient from one reciprocal multiplication and one

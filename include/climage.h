@@ -118,7 +118,7 @@ typedef struct {
     float rgb_cam[3][3];
     clim_noise_model noiseModel;       /* pyramidNlf[0] is read by despeckle */
     clim_denoise_params denoiseParameters[CLIM_MAX_LEVELS];
-    float noiseLevel;                  /* must be <= 0.6 (high-noise branch not built) */
+    float noiseLevel;                  /* > 0.6: raw-domain pre-denoise (cpp/raw_converter.cpp:82) */
     clim_rgb_conversion_params rgbConversionParameters;
     clim_ltm_params ltmParameters;
     int32_t levels;                    /* 2..5 */
@@ -224,6 +224,26 @@ clim_status clim_convert_to_srgb(clim_ctx* ctx, const clim_image* linear, const 
                                  clim_image* out_rgba8, clim_image* out_f32, const float transform[9],
                                  const clim_rgb_conversion_params* params);
 
+/* ------------------------------------------------ high-noise raw branch
+ * cpp/raw_converter.cpp:111-123 (noiseLevel > 0.6).  An RGBA raw image holds
+ * one (R, G, B, G2) quad per texel and is half the mosaic's size. */
+
+/* bayerToRawRGBA, cpp/demosaic_cl.cpp:372-390, cl/demosaic.cl:898-912
+ * (raw: R32F 2w x 2h; rgba: RGBA32F w x h) */
+clim_status clim_bayer_to_raw_rgba(clim_ctx* ctx, const clim_image* raw, clim_image* rgba, int32_t bayer);
+
+/* rawRGBAToBayer, cpp/demosaic_cl.cpp:392-410, cl/demosaic.cl:914-929 */
+clim_status clim_raw_rgba_to_bayer(clim_ctx* ctx, const clim_image* rgba, clim_image* raw, int32_t bayer);
+
+/* despeckleRawRGBAImage, cpp/demosaic_cl.cpp:430-443, cl/demosaic.cl:530-553,1195-1201
+ * (out must differ from in) */
+clim_status clim_despeckle_raw_rgba(clim_ctx* ctx, const clim_image* in, clim_image* out);
+
+/* denoiseRawRGBAImage, cpp/demosaic_cl.cpp:412-428, cl/demosaic.cl:932-949,1187-1193.
+ * raw_variance: the raw noise slopes (rawNlf, R, G, B, G2); out must differ from in. */
+clim_status clim_denoise_raw_rgba(clim_ctx* ctx, const clim_image* in, const float raw_variance[4],
+                                  clim_image* out);
+
 /* localToneMappingMask, cpp/demosaic_cl.cpp:339-370, cl/demosaic.cl:1024-1107.
  * input: the denoised YCbCr image (RGBA32F); guide: RGBA32F, read through its
  * first channel (the pipeline passes pyramid level 3, cpp/raw_converter.cpp:165);
@@ -295,7 +315,7 @@ clim_status clim_generate_bayer(clim_ctx* ctx, clim_image* raw, clim_image* clea
  * launch of clim_raw_to_rgb is bracketed by two events; clim_profile_read
  * waits for the stream, adds the elapsed times per stage and clears the
  * record.  Stage names: clim_profile_stage_name(i), i < CLIM_PROFILE_STAGES. */
-#define CLIM_PROFILE_STAGES 12
+#define CLIM_PROFILE_STAGES 13
 typedef struct {
     double ms[CLIM_PROFILE_STAGES];         /* summed kernel time per stage */
     int64_t launches[CLIM_PROFILE_STAGES];  /* launches per stage */

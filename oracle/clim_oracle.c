@@ -1201,6 +1201,106 @@ int P(inverse3x3)(const float mf[9], double scale, float out[9]) {
     return 0;
 }
 
+/* ----------------------------------------------------- high-noise raw branch
+ * cpp/raw_converter.cpp:111-123 (noiseLevel > 0.6): bayerToRawRGBA,
+ * despeckleRawRGBAImage, denoiseRawRGBAImage, rawRGBAToBayer
+ * (cl/demosaic.cl:898-929, 530-553, 932-949 + 1187-1201).  The RGBA raw
+ * image holds one (R, G, B, G2) quad per texel, w/2 x h/2. */
+void P(bayer_to_raw_rgba)(const real* raw, int w, int h, int bayer, real* rgba) {
+    const int qw = w / 2, qh = h / 2;
+#pragma omp parallel for schedule(static)
+    for (int qy = 0; qy < qh; qy++)
+        for (int qx = 0; qx < qw; qx++)
+            for (int c = 0; c < 4; c++)
+                rgba[((size_t)qy * qw + qx) * 4 + c] =
+                    raw[(size_t)(2 * qy + kBayerOffsets[bayer][c][1]) * w + 2 * qx + kBayerOffsets[bayer][c][0]];
+}
+
+void P(raw_rgba_to_bayer)(const real* rgba, int qw, int qh, int bayer, real* raw) {
+    const int w = 2 * qw;
+#pragma omp parallel for schedule(static)
+    for (int qy = 0; qy < qh; qy++)
+        for (int qx = 0; qx < qw; qx++)
+            for (int c = 0; c < 4; c++)
+                raw[(size_t)(2 * qy + kBayerOffsets[bayer][c][1]) * w + 2 * qx + kBayerOffsets[bayer][c][0]] =
+                    rgba[((size_t)qy * qw + qx) * 4 + c];
+}
+
+/* despeckle_3x3x4: per channel, the centre clamped to the second smallest and
+ * second largest value of its 3x3 window (x outer, y inner), the trackers
+ * starting at 0 (maxima) and 65535 (minima) as in the reference, so the bounds
+ * are the second extremes of the window plus those two initial values; a
+ * NaN is never selected.  Comparisons only: FP32 and FP64 agree exactly. */
+void P(despeckle_raw_rgba)(const real* in, int qw, int qh, real* out) {
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < qh; y++) {
+        for (int x = 0; x < qw; x++) {
+            for (int c = 0; c < 4; c++) {
+                real fmx = 0, smx = 0, fmn = K(65535.0), smn = K(65535.0);
+                for (int dx = -1; dx <= 1; dx++) {
+                    for (int dy = -1; dy <= 1; dy++) {
+                        const real v = rd4(in, qw, qh, x + dx, y + dy)[c];
+                        smx = (v <= fmx && v > smx) ? v : smx;
+                        smx = v > fmx ? fmx : smx;
+                        fmx = v > fmx ? v : fmx;
+                        smn = (v >= fmn && v < smn) ? v : smn;
+                        smn = v < fmn ? fmn : smn;
+                        fmn = v < fmn ? v : fmn;
+                    }
+                }
+                out[((size_t)y * qw + x) * 4 + c] = clampr(in[((size_t)y * qw + x) * 4 + c], smn, smx);
+            }
+        }
+    }
+}
+
+/* denoiseRawRGBA: 5x5 average of the taps whose distance
+ * length((s - c) / sigma) < 1, sigma = sqrt(rawVariance * c) per channel.  A
+ * channel with c = 0 has sigma = 0: every distance is then NaN or infinite,
+ * no tap counts and the quad is 0/0 = NaN, as the reference's expression is
+ * under IEEE arithmetic.  FP32 relaxed form: sum (s - c)^2 / sigma^2 < 1
+ * (clim_raw_dist4, 1/sigma^2 NaN for a negative product as the reference's
+ * sqrt is), 0/1 weights accumulated by FMA (exact). */
+void P(denoise_raw_rgba)(const real* in, int qw, int qh, const float var_f[4], real* out) {
+    real var[4];
+    for (int c = 0; c < 4; c++) var[c] = var_f[c];
+#pragma omp parallel for schedule(static)
+    for (int y = 0; y < qh; y++) {
+        for (int x = 0; x < qw; x++) {
+            const real* p = in + ((size_t)y * qw + x) * 4;
+#if ORACLE_DOUBLE
+            real sigma[4];
+            for (int c = 0; c < 4; c++) sigma[c] = sqrt(var[c] * p[c]);
+#else
+            real k[4];
+            for (int c = 0; c < 4; c++) k[c] = clim_inv_var(var[c] * p[c]);
+#endif
+            real acc[4] = {0, 0, 0, 0}, norm = 0;
+            for (int dy = -2; dy <= 2; dy++) {
+                for (int dx = -2; dx <= 2; dx++) {
+                    const real* sv = rd4(in, qw, qh, x + dx, y + dy);
+#if ORACLE_DOUBLE
+                    real l = 0;
+                    for (int c = 0; c < 4; c++) {
+                        const real d = (sv[c] - p[c]) / sigma[c];
+                        l += d * d;
+                    }
+                    const real wgt = sqrt(l) < 1 ? 1 : 0;
+                    for (int c = 0; c < 4; c++) acc[c] += wgt * sv[c];
+#else
+                    const real l = clim_raw_dist4(sv[0] - p[0], sv[1] - p[1], sv[2] - p[2], sv[3] - p[3], k[0], k[1],
+                                                  k[2], k[3]);
+                    const real wgt = l < K(1.0) ? K(1.0) : K(0.0);
+                    for (int c = 0; c < 4; c++) acc[c] = fmaf(wgt, sv[c], acc[c]);
+#endif
+                    norm += wgt;
+                }
+            }
+            for (int c = 0; c < 4; c++) out[((size_t)y * qw + x) * 4 + c] = acc[c] / norm;
+        }
+    }
+}
+
 /* ------------------------------------------------------------- orchestration
  * PyramidalDenoise<levels>::denoise, cpp/pyramidal_denoise.cpp:103-149 and
  * RawConverter::demosaicImage, cpp/raw_converter.cpp:73-186. */
@@ -1275,6 +1375,7 @@ typedef struct {
     int levels, algorithm;
     int localToneMapping;  /* needs levels >= 4 */
     float ltm[4];          /* guidedFilterEps, shadows, highlights, detail */
+    float noiseLevel;      /* > 0.6: the high-noise raw branch */
 } o_pipeline_params;
 
 /* Full path.  nlf_out: rawNlf[4] then levels x 6.  Optional stage dumps
@@ -1304,6 +1405,19 @@ int P(raw_to_rgb)(const uint16_t* raw, int w, int h, const o_pipeline_params* pp
         else memcpy(rawNlf, fit + 4, sizeof(rawNlf));
         free(mean);
         free(var);
+    }
+
+    if (pp->noiseLevel > 0.6f) {
+        /* cpp/raw_converter.cpp:111-123, with the fitted rawNlf */
+        const int qw = w / 2, qh = h / 2;
+        real* q0 = (real*)malloc((size_t)qw * qh * 4 * sizeof(real));
+        real* q1 = (real*)malloc((size_t)qw * qh * 4 * sizeof(real));
+        P(bayer_to_raw_rgba)(scaled, w, h, pp->bayer, q0);
+        P(despeckle_raw_rgba)(q0, qw, qh, q1);
+        P(denoise_raw_rgba)(q1, qw, qh, rawNlf, q0);
+        P(raw_rgba_to_bayer)(q0, qw, qh, pp->bayer, scaled);
+        free(q0);
+        free(q1);
     }
 
     P(interpolate_green)(scaled, w, h, pp->bayer, rawNlf[1], green);

@@ -43,7 +43,8 @@ class PipelineParams(C.Structure):
                 ("pyramidNlf", C.c_float * 30), ("dp", DenoiseParams * 5),
                 ("contrast", C.c_float), ("saturation", C.c_float), ("toneCurveSlope", C.c_float),
                 ("exposureBias", C.c_float), ("blacks", C.c_float), ("levels", C.c_int),
-                ("algorithm", C.c_int), ("localToneMapping", C.c_int), ("ltm", C.c_float * 4)]
+                ("algorithm", C.c_int), ("localToneMapping", C.c_int), ("ltm", C.c_float * 4),
+                ("noiseLevel", C.c_float)]
 
 
 # pyramidNlf row 0 of cpp/raw_pipeline.cpp:105-111 ("minimal noise model")
@@ -245,6 +246,34 @@ class Oracle:
                                     C.c_float(blacks), _ptr(mask) if mask is not None else None, _ptr(out8),
                                     _ptr(outf))
         return out8, outf
+
+    def bayer_to_raw_rgba(self, raw, bayer):
+        h, w = raw.shape
+        raw = self._chk(raw)
+        out = self._new(h // 2, w // 2, 4)
+        self._fn("bayer_to_raw_rgba")(_ptr(raw), w, h, bayer, _ptr(out))
+        return out
+
+    def raw_rgba_to_bayer(self, rgba, bayer):
+        qh, qw, _ = rgba.shape
+        rgba = self._chk(rgba)
+        out = self._new(2 * qh, 2 * qw)
+        self._fn("raw_rgba_to_bayer")(_ptr(rgba), qw, qh, bayer, _ptr(out))
+        return out
+
+    def despeckle_raw_rgba(self, rgba):
+        qh, qw, _ = rgba.shape
+        rgba = self._chk(rgba)
+        out = self._new(qh, qw, 4)
+        self._fn("despeckle_raw_rgba")(_ptr(rgba), qw, qh, _ptr(out))
+        return out
+
+    def denoise_raw_rgba(self, rgba, raw_variance):
+        qh, qw, _ = rgba.shape
+        rgba = self._chk(rgba)
+        out = self._new(qh, qw, 4)
+        self._fn("denoise_raw_rgba")(_ptr(rgba), qw, qh, _f(raw_variance), _ptr(out))
+        return out
 
     def ltm_mask(self, img, guide, ltm, transform, var_eps=False):
         """localToneMappingMask: img the denoised YCbCr image (H, W, 4), guide
