@@ -56,6 +56,7 @@ def stage_bytes(stage, texels, levels_px=None):
         "denoise": 16 + 16,
         "reassemble": 16 + 16 + 16 / 4 + 16 / 4,
         "tone": 16 + 16 / 4 + 16 / 4 + 4,  # level-0 reassembly fused with the tone map: RGBA8 out
+        "raw_denoise": 8 + 16,          # 4 u16 texels in, 4 FP32 texels out (per quad)
     }
     return per[stage] * texels
 
@@ -67,10 +68,10 @@ SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green"
                  "red_blue_ycc": "k_rb_despeckle<true>", "despeckle": "k_despeckle",
                  "downsample": "k_downsample<true>", "level_stats": "k_noise_stats",
                  "denoise": "k_bilateral<true>", "reassemble": "k_reassemble<true, false, false>",
-                 "tone": "k_reassemble<true, true, false>"}
+                 "tone": "k_reassemble<true, true, false>", "raw_denoise": "k_raw_rgba_filter<2, true>"}
 
 
-def cpu_baseline(w, h, bayer, ltm=False, seconds_min=10.0):
+def cpu_baseline(w, h, bayer, ltm=False, noise_level=0.0, seconds_min=10.0):
     """The FP32 CPU restatement on the bench frames (same sizes and noise
     levels), as many as fit in about `seconds_min` seconds (at least one)."""
     from oracle import oracle as O
@@ -79,6 +80,7 @@ def cpu_baseline(w, h, bayer, ltm=False, seconds_min=10.0):
     o = O.Oracle(double=False)
     p = O.default_params(bayer, 64.0, 4095.0, levels=5)
     p.localToneMapping = int(ltm)
+    p.noiseLevel = noise_level
     done, dt = 0, 0.0
     for i, a in enumerate(NOISE_A):
         raw = O.generate_bayer(w, h, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
@@ -133,6 +135,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ltm", action="store_true",
                     help="local tone mapping on (its mask is computed in the level-0 reassembly + tone kernel)")
+    ap.add_argument("--noise-level", type=float, default=0.0,
+                    help="DemosaicParameters::noiseLevel; above 0.6 the raw-domain pre-denoise runs")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device fill and device-to-device copy (reference kernels "
@@ -154,6 +158,7 @@ def main():
         outs.append(ctx.image(w, h, ca.FMT_RGBA8))
     params = ca.default_params(bayer, 64.0, 4095.0, levels=5, algorithm=ca.BILATERAL)
     params.rgbConversionParameters.localToneMapping = int(args.ltm)
+    params.noiseLevel = args.noise_level
     single_kernel = dict(SINGLE_KERNEL)
     if args.ltm:
         single_kernel["tone"] = "k_reassemble<true, true, true>"
@@ -225,7 +230,7 @@ def main():
                     "launches_timed": s["launches"], "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(w, h, bayer, ltm=args.ltm)
+        cpu = cpu_baseline(w, h, bayer, ltm=args.ltm, noise_level=args.noise_level)
 
     res = {
         "metric": "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP",
@@ -241,7 +246,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scenes generated on the device)",
         "config": {"workload": f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, 5-level bilateral pyramid"
-                               + (", local tone mapping" if args.ltm else ""),
+                               + (", local tone mapping" if args.ltm else "")
+                               + (f", noiseLevel {args.noise_level} (raw pre-denoise)" if args.noise_level > 0.6 else ""),
                    "frames_per_step": len(raws), "pixels_per_frame": w * h},
         "roofline": roofline,
         "path_roofline": {"bytes_per_px": path_bytes_per_px(5),

@@ -2,9 +2,10 @@
 // RawPipeline does (cpp/raw_pipeline.cpp:216-256), on a seeded synthetic
 // mosaic instead of a DNG file.
 //
-//   raw_pipeline W H BAYER LEVELS OUT_PREFIX [LTM]
+//   raw_pipeline W H BAYER LEVELS OUT_PREFIX [LTM [NOISE_LEVEL]]
 //
-// LTM = 1 turns on local tone mapping (rgbConversionParameters.localToneMapping).
+// LTM = 1 turns on local tone mapping (rgbConversionParameters.localToneMapping);
+// NOISE_LEVEL > 0.6 the raw-domain pre-denoise of high-noise frames.
 // Writes OUT_PREFIX.fused.rgb (RawConverter::demosaicImage + convertToRGBImage),
 // OUT_PREFIX.stages.rgb (the same path spelled out operator by operator, with a
 // caller-supplied ImageDenoiser so that PyramidalDenoise runs level by level
@@ -52,11 +53,12 @@ static void write_rgb(const std::string& path, const gls::image<gls::rgb_pixel>&
 }
 
 int main(int argc, char** argv) {
-    if (argc != 6 && argc != 7) {
-        std::cerr << "usage: raw_pipeline W H BAYER LEVELS OUT_PREFIX [LTM]\n";
+    if (argc < 6 || argc > 8) {
+        std::cerr << "usage: raw_pipeline W H BAYER LEVELS OUT_PREFIX [LTM [NOISE_LEVEL]]\n";
         return 2;
     }
-    const int ltm = argc == 7 ? std::atoi(argv[6]) : 0;
+    const int ltm = argc >= 7 ? std::atoi(argv[6]) : 0;
+    const float noise_level = argc >= 8 ? (float)std::atof(argv[7]) : 0.0f;
     const int w = std::atoi(argv[1]), h = std::atoi(argv[2]);
     const BayerPattern bayer = (BayerPattern)std::atoi(argv[3]);
     const int levels = std::atoi(argv[4]);
@@ -77,6 +79,7 @@ int main(int argc, char** argv) {
         // 1. the fused pipeline through RawConverter
         DemosaicParameters p1 = synthetic_params(bayer, black, white);
         p1.rgbConversionParameters.localToneMapping = ltm;
+        p1.noiseLevel = noise_level;
         RawConverter rc(&ctx);
         auto rgb1 = RawConverter::convertToRGBImage(*rc.demosaicImage(*rawImage, &p1, nullptr, false));
         write_rgb(out + ".fused.rgb", *rgb1);
@@ -84,6 +87,7 @@ int main(int argc, char** argv) {
         // 2. operator by operator (cpp/raw_converter.cpp:101-174)
         DemosaicParameters p2 = synthetic_params(bayer, black, white);
         p2.rgbConversionParameters.localToneMapping = ltm;
+        p2.noiseLevel = noise_level;
         gls::hip_image_2d<gls::luma_pixel_16> raw(&ctx, *rawImage);
         gls::hip_image_2d<gls::luma_pixel_float> scaled(&ctx, w, h), green(&ctx, w, h);
         gls::hip_image_2d<gls::rgba_pixel_float> A(&ctx, w, h), B(&ctx, w, h);
@@ -91,6 +95,14 @@ int main(int argc, char** argv) {
         scaleRawData(&ctx, raw, &scaled, bayer, p2.scale_mul, p2.black_level / 0xffff);
         const auto rawNLF = computeRawNoiseStatistics(&ctx, scaled, bayer);
         p2.noiseModel.rawNlf = {rawNLF[4], rawNLF[5], rawNLF[6], rawNLF[7]};
+        if (p2.noiseLevel > 0.6f) {
+            // cpp/raw_converter.cpp:111-123
+            gls::hip_image_2d<gls::rgba_pixel_float> q0(&ctx, w / 2, h / 2), q1(&ctx, w / 2, h / 2);
+            bayerToRawRGBA(&ctx, scaled, &q0, bayer);
+            despeckleRawRGBAImage(&ctx, q0, &q1);
+            denoiseRawRGBAImage(&ctx, q1, p2.noiseModel.rawNlf, &q0);
+            rawRGBAToBayer(&ctx, q0, &scaled, bayer);
+        }
         interpolateGreen(&ctx, scaled, &green, bayer, p2.noiseModel.rawNlf[1]);
         interpolateRedBlue(&ctx, scaled, green, &A, bayer, (p2.noiseModel.rawNlf[0] + p2.noiseModel.rawNlf[2]) / 2,
                            false);

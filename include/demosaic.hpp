@@ -200,6 +200,30 @@ inline void convertTosRGB(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgb
                "convertTosRGB");
 }
 
+// The high-noise raw branch (cpp/demosaic_cl.hpp:102-119): an RGBA raw image
+// holds one (R, G, B, G2) quad per texel, half the mosaic's size.
+inline void bayerToRawRGBA(gls::HIPContext* ctx, const gls::hip_image_2d<gls::luma_pixel_float>& rawImage,
+                           gls::hip_image_2d<gls::rgba_pixel_float>* rgbaImage, BayerPattern bayerPattern) {
+    ctx->check(clim_bayer_to_raw_rgba(ctx->ctx(), rawImage.clim(), rgbaImage->clim(), bayerPattern), "bayerToRawRGBA");
+}
+
+inline void rawRGBAToBayer(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgba_pixel_float>& rgbaImage,
+                           gls::hip_image_2d<gls::luma_pixel_float>* rawImage, BayerPattern bayerPattern) {
+    ctx->check(clim_raw_rgba_to_bayer(ctx->ctx(), rgbaImage.clim(), rawImage->clim(), bayerPattern), "rawRGBAToBayer");
+}
+
+inline void denoiseRawRGBAImage(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgba_pixel_float>& inputImage,
+                                const std::array<float, 4> rawVariance,
+                                gls::hip_image_2d<gls::rgba_pixel_float>* outputImage) {
+    ctx->check(clim_denoise_raw_rgba(ctx->ctx(), inputImage.clim(), rawVariance.data(), outputImage->clim()),
+               "denoiseRawRGBAImage");
+}
+
+inline void despeckleRawRGBAImage(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgba_pixel_float>& inputImage,
+                                  gls::hip_image_2d<gls::rgba_pixel_float>* outputImage) {
+    ctx->check(clim_despeckle_raw_rgba(ctx->ctx(), inputImage.clim(), outputImage->clim()), "despeckleRawRGBAImage");
+}
+
 // cpp/demosaic_cl.hpp:91-95.  inputImage: the denoised YCbCr image; guideImage:
 // pyramid level 3 in cpp/raw_converter.cpp:165; transform: YCbCr -> camera RGB.
 inline void localToneMappingMask(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgba_pixel_float>& inputImage,

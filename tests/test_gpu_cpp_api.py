@@ -21,12 +21,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "climage_amd", "lib", "raw_pipeline")
 
 
-@pytest.mark.parametrize("bayer,ltm", [(0, 0), (2, 0), (2, 1)])
-def test_cpp_driver_fused_equals_stages_equals_oracle(tmp_path, bayer, ltm):
+@pytest.mark.parametrize("bayer,ltm,noise", [(0, 0, 0.0), (2, 0, 0.0), (2, 1, 0.0), (1, 0, 0.7)])
+def test_cpp_driver_fused_equals_stages_equals_oracle(tmp_path, bayer, ltm, noise):
     w, h = 512, 384
     prefix = str(tmp_path / "out")
-    r = subprocess.run([EXE, str(w), str(h), str(bayer), "5", prefix, str(ltm)], capture_output=True, text=True,
-                       timeout=120)
+    r = subprocess.run([EXE, str(w), str(h), str(bayer), "5", prefix, str(ltm), str(noise)], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     fused = np.fromfile(prefix + ".fused.rgb", np.uint8).reshape(h, w, 3)
     stages = np.fromfile(prefix + ".stages.rgb", np.uint8).reshape(h, w, 3)
@@ -39,6 +39,7 @@ def test_cpp_driver_fused_equals_stages_equals_oracle(tmp_path, bayer, ltm):
     raw = O.generate_bayer(w, h, bayer, 1000, 2000, 0.002, 1e-5, 64.0, 4095.0)
     op = O.default_params(bayer, 64.0, 4095.0, levels=5)
     op.localToneMapping = ltm
+    op.noiseLevel = noise
     ref = O.Oracle(False).raw_to_rgb(raw, op)
     np.testing.assert_array_equal(fused, ref["rgba8"][..., :3])
     np.testing.assert_array_equal(nlf[0][:4].astype(np.float32), ref["rawNlf"])
