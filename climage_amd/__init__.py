@@ -99,6 +99,7 @@ def exported_symbols():
                         "clim_denoise_image_guided", "clim_guided_filter", "clim_reassemble_image",
                         "clim_convert_to_srgb", "clim_local_tone_mapping_mask", "clim_bayer_to_raw_rgba",
                         "clim_raw_rgba_to_bayer", "clim_despeckle_raw_rgba", "clim_denoise_raw_rgba",
+                        "clim_fast_debayer", "clim_fast_demosaic",
                         "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
                         "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_generate_bayer")]
 
@@ -278,6 +279,15 @@ class Context:
                                                ltm_mask.ref() if ltm_mask is not None else None, out8.ref(),
                                                out_f32.ref() if out_f32 is not None else None, _f(transform),
                                                C.byref(params)), "convertTosRGB")
+
+    def fasteDebayer(self, raw, rgba, bayer):
+        self._check(lib().clim_fast_debayer(self.h, raw.ref(), rgba.ref(), int(bayer)), "fasteDebayer")
+
+    def fast_demosaic(self, raw, out8, params, out_f32=None):
+        """RawConverter::fastDemosaicImage: half-resolution RGBA8 (and float)."""
+        self._check(lib().clim_fast_demosaic(self.h, raw.ref(), out8.ref(),
+                                             out_f32.ref() if out_f32 is not None else None, C.byref(params)),
+                    "fastDemosaicImage")
 
     def bayerToRawRGBA(self, raw, rgba, bayer):
         self._check(lib().clim_bayer_to_raw_rgba(self.h, raw.ref(), rgba.ref(), int(bayer)), "bayerToRawRGBA")

@@ -408,6 +408,19 @@ extern "C" clim_status clim_despeckle_image(clim_ctx* ctx, const clim_image* in,
     return CLIM_OK;
 }
 
+// ------------------------------------------------ half-resolution preview
+
+extern "C" clim_status clim_fast_debayer(clim_ctx* ctx, const clim_image* raw, clim_image* rgba, int32_t bayer) {
+    if (!ctx || bayer < 0 || bayer > 3) return ctx_fail(ctx, CLIM_EINVAL, "fasteDebayer: bad argument");
+    CLIM_TRY(check_img(ctx, raw, CLIM_FMT_R32F, "raw"));
+    CLIM_TRY(check_img(ctx, rgba, CLIM_FMT_RGBA32F, "rgba"));
+    if (raw->width != 2 * rgba->width || raw->height != 2 * rgba->height)
+        return ctx_fail(ctx, CLIM_EINVAL, "fasteDebayer: the raw image must be twice the RGBA image's size");
+    clim::launch_fast_debayer(view(raw), view(rgba), make_bayer(bayer), ctx->stream);
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
 // ------------------------------------------------ high-noise raw branch
 
 static clim_status check_quads(clim_ctx* ctx, const clim_image* raw, const clim_image* rgba, const char* what) {

@@ -55,6 +55,30 @@ __device__ __forceinline__ float mmax(float a, float b) { return a > b ? a : b; 
 __device__ __forceinline__ float len2(float x, float y) { return sqrtf(x * x + y * y); }
 __device__ __forceinline__ float len3(float x, float y, float z) { return sqrtf(x * x + y * y + z * z); }
 
+// blendHighlightsImage on one pixel (cl/demosaic.cl:338-380): above the clip,
+// the opponent-space chroma is rescaled to that of the clipped pixel
+__device__ __forceinline__ void blend_highlights_px(float& px, float& py, float& pz, float clip) {
+    // cl/demosaic.cl:338-370
+    if (px > clip || py > clip || pz > clip) {
+        const float s3 = 1.7320508f;
+        const float c0[3] = {px, py, pz};
+        const float c1[3] = {mmin(px, clip), mmin(py, clip), mmin(pz, clip)};
+        const float l00 = 1.0f * c0[0] + 1.0f * c0[1] + 1.0f * c0[2];
+        const float l01 = s3 * c0[0] + -s3 * c0[1] + 0.0f * c0[2];
+        const float l02 = -1.0f * c0[0] + -1.0f * c0[1] + 2.0f * c0[2];
+        const float l11 = s3 * c1[0] + -s3 * c1[1] + 0.0f * c1[2];
+        const float l12 = -1.0f * c1[0] + -1.0f * c1[1] + 2.0f * c1[2];
+        const float sum0 = l01 * l01 + l02 * l02;
+        const float sum1 = l11 * l11 + l12 * l12;
+        const float chratio = sum0 > 0 ? sqrtf(sum1 / sum0) : 1.0f;
+        const float a = l00, bb = l01 * chratio, cc = l02 * chratio;
+        const float h = s3 / 2.0f;
+        px = (1.0f * a + h * bb + -0.5f * cc) / 3.0f;
+        py = (1.0f * a + -h * bb + -0.5f * cc) / 3.0f;
+        pz = (1.0f * a + 0.0f * bb + 1.0f * cc) / 3.0f;
+    }
+}
+
 // Bayer table of cl/demosaic.cl:29-34, resolved on the host for one pattern:
 // ox/oy[c] = (x, y) of raw_red, raw_green, raw_blue, raw_green2 in the quad;
 // site(x, y) = which of the four a pixel is (2 bits per parity class, packed

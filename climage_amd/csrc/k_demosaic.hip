@@ -259,28 +259,6 @@ void launch_green(const MosaicArg& raw, const Img& green, int bayer, const Param
 // model (cpp/raw_converter.cpp:127-128) when taken from the device.
 struct Mat3 { float m[9]; };
 
-__device__ __forceinline__ void blend_highlights_px(float& px, float& py, float& pz, float clip) {
-    // cl/demosaic.cl:338-370
-    if (px > clip || py > clip || pz > clip) {
-        const float s3 = 1.7320508f;
-        const float c0[3] = {px, py, pz};
-        const float c1[3] = {mmin(px, clip), mmin(py, clip), mmin(pz, clip)};
-        const float l00 = 1.0f * c0[0] + 1.0f * c0[1] + 1.0f * c0[2];
-        const float l01 = s3 * c0[0] + -s3 * c0[1] + 0.0f * c0[2];
-        const float l02 = -1.0f * c0[0] + -1.0f * c0[1] + 2.0f * c0[2];
-        const float l11 = s3 * c1[0] + -s3 * c1[1] + 0.0f * c1[2];
-        const float l12 = -1.0f * c1[0] + -1.0f * c1[1] + 2.0f * c1[2];
-        const float sum0 = l01 * l01 + l02 * l02;
-        const float sum1 = l11 * l11 + l12 * l12;
-        const float chratio = sum0 > 0 ? sqrtf(sum1 / sum0) : 1.0f;
-        const float a = l00, bb = l01 * chratio, cc = l02 * chratio;
-        const float h = s3 / 2.0f;
-        px = (1.0f * a + h * bb + -0.5f * cc) / 3.0f;
-        py = (1.0f * a + -h * bb + -0.5f * cc) / 3.0f;
-        pz = (1.0f * a + 0.0f * bb + 1.0f * cc) / 3.0f;
-    }
-}
-
 // interpolateRedBlue of one 2x2 quad at (x0, y0) (even): the 4x4 windows of
 // the mosaic and of the green plane around it are loaded once and the four
 // sites computed in turn; with kFuse, blendHighlights and the camera->YCbCr

@@ -119,6 +119,13 @@ void launch_denoise_raw_rgba(const Img& in, const ParamRef& var, const Img& out,
 // the four in one pass: mosaic -> despeckle -> denoise -> FP32 mosaic (out != m.im)
 void launch_raw_predenoise(const MosaicArg& m, const ParamRef& var, const Img& out, hipStream_t st);
 
+// -------------------------------------------------- half-resolution preview
+// fasteDebayer (FP32 mosaic -> half-size RGBA) and the whole
+// fastDemosaicImage (mosaic -> half-size RGBA8, k_preview.hip)
+void launch_fast_debayer(const Img& raw, const Img& rgba, const Bayer& b, hipStream_t st);
+void launch_fast_preview(const MosaicArg& m, const Img& out8, const Img& outf, const float cam[9], float contrast,
+                         float saturation, float slope, float exposureBias, float blacks, hipStream_t st);
+
 // ----------------------------------------------- multi-pass guided filter
 // scratch: 7 images of w*h texels (float4) + 1 of float2, see k_guided.hip
 struct GuidedScratch {

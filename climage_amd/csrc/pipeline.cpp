@@ -350,6 +350,44 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
     return CLIM_OK;
 }
 
+extern "C" clim_status clim_fast_demosaic(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8,
+                                          clim_image* out_f32, const clim_demosaic_params* p) {
+    if (!ctx || !p) return ctx_fail(ctx, CLIM_EINVAL, "fastDemosaicImage: null argument");
+    if (p->bayerPattern < 0 || p->bayerPattern > 3) return ctx_fail(ctx, CLIM_EINVAL, "bad bayerPattern");
+    CLIM_TRY(check_image(ctx, raw, CLIM_FMT_R16U, "raw"));
+    CLIM_TRY(check_image(ctx, out_rgba8, CLIM_FMT_RGBA8, "out_rgba8"));
+    const int w = raw->width, h = raw->height;
+    if ((w & 1) || (h & 1)) return ctx_fail(ctx, CLIM_EINVAL, "fastDemosaicImage: mosaic size must be even");
+    if (out_rgba8->width != w / 2 || out_rgba8->height != h / 2)
+        return ctx_fail(ctx, CLIM_EINVAL, "fastDemosaicImage: output must be half the mosaic's size");
+    Img outf{nullptr, 0, 0, 0};
+    if (out_f32) {
+        CLIM_TRY(check_image(ctx, out_f32, CLIM_FMT_RGBA32F, "out_f32"));
+        if (out_f32->width != w / 2 || out_f32->height != h / 2)
+            return ctx_fail(ctx, CLIM_EINVAL, "fastDemosaicImage: float output must be half the mosaic's size");
+        outf = view(out_f32);
+    }
+    // scaleRawData (black normalised as cpp/raw_converter.cpp:201-202) applied
+    // on the fly; localToneMapping is not applied (the reference hands this
+    // path a 1x1 placeholder mask, cpp/raw_converter.cpp:69,212)
+    clim::MosaicArg mosaic{};
+    mosaic.im = view(raw);
+    mosaic.b = make_bayer(p->bayerPattern);
+    mosaic.smul = make_float4(p->scale_mul[0], p->scale_mul[1], p->scale_mul[2], p->scale_mul[3]);
+    mosaic.black = p->black_level / 65535.0f;
+    mosaic.u16 = true;
+    float cam[9];
+    std::memcpy(cam, p->rgb_cam, sizeof(cam));
+    const clim_rgb_conversion_params& rc = p->rgbConversionParameters;
+    {
+        ProfScope ps(ctx, ST_TONE, (long long)(w / 2) * (h / 2));
+        clim::launch_fast_preview(mosaic, view(out_rgba8), outf, cam, rc.contrast, rc.saturation, rc.toneCurveSlope,
+                                  rc.exposureBias, rc.blacks, ctx->stream);
+    }
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
 extern "C" clim_status clim_generate_bayer(clim_ctx* ctx, clim_image* raw, clim_image* clean, int32_t bayer,
                                            uint32_t scene_seed, uint32_t noise_seed, float noise_a, float noise_b,
                                            float black_level, float white_level) {

@@ -135,6 +135,23 @@ int main(int argc, char** argv) {
             write_rgb(out + (plugin ? ".stages.rgb" : ".builtin.rgb"), *rgb2);
         }
 
+        // 3. the half-resolution preview (cpp/raw_converter.cpp:188-224), fused
+        //    and operator by operator
+        {
+            DemosaicParameters pf = synthetic_params(bayer, black, white);
+            auto prev = RawConverter::convertToRGBImage(*rc.fastDemosaicImage(*rawImage, pf));
+            write_rgb(out + ".preview.rgb", *prev);
+            gls::hip_image_2d<gls::luma_pixel_float> sc(&ctx, w, h);
+            gls::hip_image_2d<gls::rgba_pixel_float> lin(&ctx, w / 2, h / 2);
+            gls::hip_image_2d<gls::rgba_pixel> prgb(&ctx, w / 2, h / 2);
+            gls::hip_image_2d<gls::luma_pixel_float> placeholder(&ctx, 1, 1);
+            scaleRawData(&ctx, raw, &sc, bayer, pf.scale_mul, pf.black_level / 0xffff);
+            fasteDebayer(&ctx, sc, &lin, bayer);
+            blendHighlightsImage(&ctx, lin, 1.0, &lin);
+            convertTosRGB(&ctx, lin, placeholder, &prgb, pf);
+            write_rgb(out + ".preview_stages.rgb", *RawConverter::convertToRGBImage(prgb));
+        }
+
         std::ofstream f(out + ".nlf");
         f.precision(9);
         for (const auto* p : {&p1, &p2, &p3}) {

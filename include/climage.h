@@ -224,6 +224,20 @@ clim_status clim_convert_to_srgb(clim_ctx* ctx, const clim_image* linear, const 
                                  clim_image* out_rgba8, clim_image* out_f32, const float transform[9],
                                  const clim_rgb_conversion_params* params);
 
+/* ---------------------------------------------- half-resolution preview */
+
+/* fasteDebayer, cpp/demosaic_cl.cpp:87-105, cl/demosaic.cl:320-334
+ * (raw: R32F 2w x 2h scaled mosaic; rgba: RGBA32F w x h, (R, (G+G2)/2, B, 0)) */
+clim_status clim_fast_debayer(clim_ctx* ctx, const clim_image* raw, clim_image* rgba, int32_t bayer);
+
+/* RawConverter::fastDemosaicImage, cpp/raw_converter.cpp:188-224: scaleRawData,
+ * fasteDebayer, blendHighlightsImage(clip 1) and convertTosRGB(rgb_cam) at half
+ * resolution, one kernel.  raw: R16U w x h (even); out_rgba8: w/2 x h/2;
+ * out_f32 (nullable): the same values before quantisation.  localToneMapping
+ * is not applied (the reference passes this path a 1x1 placeholder mask). */
+clim_status clim_fast_demosaic(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8, clim_image* out_f32,
+                               const clim_demosaic_params* p);
+
 /* ------------------------------------------------ high-noise raw branch
  * cpp/raw_converter.cpp:111-123 (noiseLevel > 0.6).  An RGBA raw image holds
  * one (R, G, B, G2) quad per texel and is half the mosaic's size. */

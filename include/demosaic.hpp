@@ -200,6 +200,12 @@ inline void convertTosRGB(gls::HIPContext* ctx, const gls::hip_image_2d<gls::rgb
                "convertTosRGB");
 }
 
+// cpp/demosaic_cl.hpp:44-47: the half-resolution (R, (G+G2)/2, B) debayer
+inline void fasteDebayer(gls::HIPContext* ctx, const gls::hip_image_2d<gls::luma_pixel_float>& rawImage,
+                         gls::hip_image_2d<gls::rgba_pixel_float>* rgbImage, BayerPattern bayerPattern) {
+    ctx->check(clim_fast_debayer(ctx->ctx(), rawImage.clim(), rgbImage->clim(), bayerPattern), "fasteDebayer");
+}
+
 // The high-noise raw branch (cpp/demosaic_cl.hpp:102-119): an RGBA raw image
 // holds one (R, G, B, G2) quad per texel, half the mosaic's size.
 inline void bayerToRawRGBA(gls::HIPContext* ctx, const gls::hip_image_2d<gls::luma_pixel_float>& rawImage,

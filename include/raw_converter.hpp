@@ -39,6 +39,20 @@ class RawConverter {
         return _srgb.get();
     }
 
+    // cpp/raw_converter.cpp:188-224: the half-resolution preview (one kernel);
+    // the returned image is owned by the converter and valid until the next call
+    gls::hip_image_2d<gls::rgba_pixel>* fastDemosaicImage(const gls::image<gls::luma_pixel_16>& rawImage,
+                                                          const DemosaicParameters& demosaicParameters) {
+        if (!_raw || _raw->width != rawImage.width || _raw->height != rawImage.height)
+            _raw = std::make_unique<gls::hip_image_2d<gls::luma_pixel_16>>(_ctx, rawImage.width, rawImage.height);
+        if (!_fast || _fast->width != rawImage.width / 2 || _fast->height != rawImage.height / 2)
+            _fast = std::make_unique<gls::hip_image_2d<gls::rgba_pixel>>(_ctx, rawImage.width / 2, rawImage.height / 2);
+        _raw->copyPixelsFrom(rawImage);
+        clim_demosaic_params p = to_clim(demosaicParameters, _levels, _algorithm);
+        _ctx->check(clim_fast_demosaic(_ctx->ctx(), _raw->clim(), _fast->clim(), nullptr, &p), "fastDemosaicImage");
+        return _fast.get();
+    }
+
     // cpp/raw_converter.cpp:226-237
     static gls::image<gls::rgb_pixel>::unique_ptr convertToRGBImage(
         const gls::hip_image_2d<gls::rgba_pixel>& rgbaImage) {
@@ -54,10 +68,10 @@ class RawConverter {
 
    private:
     void allocateTextures(int width, int height) {
-        if (!_raw || _raw->width != width || _raw->height != height) {
+        if (!_raw || _raw->width != width || _raw->height != height)
             _raw = std::make_unique<gls::hip_image_2d<gls::luma_pixel_16>>(_ctx, width, height);
+        if (!_srgb || _srgb->width != width || _srgb->height != height)
             _srgb = std::make_unique<gls::hip_image_2d<gls::rgba_pixel>>(_ctx, width, height);
-        }
     }
 
     gls::HIPContext* _ctx;
@@ -65,4 +79,5 @@ class RawConverter {
     int _algorithm;
     std::unique_ptr<gls::hip_image_2d<gls::luma_pixel_16>> _raw;
     std::unique_ptr<gls::hip_image_2d<gls::rgba_pixel>> _srgb;
+    std::unique_ptr<gls::hip_image_2d<gls::rgba_pixel>> _fast;
 };

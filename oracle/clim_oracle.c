@@ -1201,6 +1201,25 @@ int P(inverse3x3)(const float mf[9], double scale, float out[9]) {
     return 0;
 }
 
+/* ---------------------------------------------------------- fasteDebayer
+ * cl/demosaic.cl:320-334: one (R, (G + G2) / 2, B, 0) texel per quad. */
+void P(fast_debayer)(const real* raw, int w, int h, int bayer, real* rgba) {
+    const int qw = w / 2, qh = h / 2;
+#pragma omp parallel for schedule(static)
+    for (int qy = 0; qy < qh; qy++) {
+        for (int qx = 0; qx < qw; qx++) {
+            real v[4];
+            for (int c = 0; c < 4; c++)
+                v[c] = raw[(size_t)(2 * qy + kBayerOffsets[bayer][c][1]) * w + 2 * qx + kBayerOffsets[bayer][c][0]];
+            real* o = rgba + ((size_t)qy * qw + qx) * 4;
+            o[0] = v[raw_red];
+            o[1] = (v[raw_green] + v[raw_green2]) / K(2.0);
+            o[2] = v[raw_blue];
+            o[3] = 0;
+        }
+    }
+}
+
 /* ----------------------------------------------------- high-noise raw branch
  * cpp/raw_converter.cpp:111-123 (noiseLevel > 0.6): bayerToRawRGBA,
  * despeckleRawRGBAImage, denoiseRawRGBAImage, rawRGBAToBayer
@@ -1381,6 +1400,21 @@ typedef struct {
 /* Full path.  nlf_out: rawNlf[4] then levels x 6.  Optional stage dumps
  * (nullable): ycc0 (despeckled level-0 YCbCr, RGBA), den0 (denoised level 0,
  * YCbCr).  Returns fallback mask (bit 8: raw fit). */
+/* RawConverter::fastDemosaicImage, cpp/raw_converter.cpp:188-224: the
+ * half-resolution preview (out8 / outf: w/2 x h/2). */
+void P(fast_demosaic)(const uint16_t* raw, int w, int h, const o_pipeline_params* pp, uint8_t* out8, real* outf) {
+    const int qw = w / 2, qh = h / 2;
+    real* scaled = (real*)malloc((size_t)w * h * sizeof(real));
+    real* rgb = (real*)malloc((size_t)qw * qh * 4 * sizeof(real));
+    P(scale_raw)(raw, w, h, pp->bayer, pp->scale_mul, pp->black_level / K(65535.0), scaled);
+    P(fast_debayer)(scaled, w, h, pp->bayer, rgb);
+    P(blend_highlights)(rgb, qw, qh, 1.0f, rgb);
+    P(convert_to_srgb)(rgb, qw, qh, pp->rgb_cam, pp->contrast, pp->saturation, pp->toneCurveSlope,
+                       pp->exposureBias, pp->blacks, NULL, out8, outf);
+    free(scaled);
+    free(rgb);
+}
+
 int P(raw_to_rgb)(const uint16_t* raw, int w, int h, const o_pipeline_params* pp, uint8_t* out8,
                   real* outf, float* nlf_out, real* ycc_dump, real* den_dump) {
     size_t n = (size_t)w * h;

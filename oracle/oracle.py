@@ -247,6 +247,22 @@ class Oracle:
                                     _ptr(outf))
         return out8, outf
 
+    def fast_debayer(self, raw, bayer):
+        h, w = raw.shape
+        raw = self._chk(raw)
+        out = self._new(h // 2, w // 2, 4)
+        self._fn("fast_debayer")(_ptr(raw), w, h, bayer, _ptr(out))
+        return out
+
+    def fast_demosaic(self, raw, params):
+        """RawConverter::fastDemosaicImage: (rgba8, float) at half resolution."""
+        h, w = raw.shape
+        raw = np.ascontiguousarray(raw, np.uint16)
+        out8 = np.zeros((h // 2, w // 2, 4), np.uint8)
+        outf = self._new(h // 2, w // 2, 4)
+        self._fn("fast_demosaic")(_ptr(raw), w, h, C.byref(params), _ptr(out8), _ptr(outf))
+        return out8, outf
+
     def bayer_to_raw_rgba(self, raw, bayer):
         h, w = raw.shape
         raw = self._chk(raw)

@@ -43,3 +43,9 @@ def test_cpp_driver_fused_equals_stages_equals_oracle(tmp_path, bayer, ltm, nois
     ref = O.Oracle(False).raw_to_rgb(raw, op)
     np.testing.assert_array_equal(fused, ref["rgba8"][..., :3])
     np.testing.assert_array_equal(nlf[0][:4].astype(np.float32), ref["rawNlf"])
+    # the half-resolution preview, fused and operator by operator
+    prev = np.fromfile(prefix + ".preview.rgb", np.uint8).reshape(h // 2, w // 2, 3)
+    prev_stages = np.fromfile(prefix + ".preview_stages.rgb", np.uint8).reshape(h // 2, w // 2, 3)
+    np.testing.assert_array_equal(prev, prev_stages)
+    e8, _ = O.Oracle(False).fast_demosaic(raw, O.default_params(bayer, 64.0, 4095.0, levels=5))
+    np.testing.assert_array_equal(prev, e8[..., :3])
