@@ -40,7 +40,7 @@ def path_bytes_per_px(levels):
     return 206 + 60 * S(levels - 2) + 80 * S(levels - 1)
 
 
-def stage_bytes(stage, texels, levels_px=None):
+def stage_bytes(stage, texels, levels_px=None, preview=False):
     """Algorithmic bytes of one stage: each input read once, each output written
     once, FP32 texels (luma 4 B, RGBA 16 B), raw 2 B, RGBA8 4 B."""
     per = {
@@ -58,6 +58,8 @@ def stage_bytes(stage, texels, levels_px=None):
         "tone": 16 + 16 / 4 + 16 / 4 + 4,  # level-0 reassembly fused with the tone map: RGBA8 out
         "raw_denoise": 8 + 16,          # 4 u16 texels in, 4 FP32 texels out (per quad)
     }
+    if preview:
+        per["tone"] = 8 + 4  # fastDemosaicImage: one u16 quad in, one RGBA8 texel out (per output texel)
     return per[stage] * texels
 
 
@@ -135,6 +137,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ltm", action="store_true",
                     help="local tone mapping on (its mask is computed in the level-0 reassembly + tone kernel)")
+    ap.add_argument("--preview", action="store_true",
+                    help="the half-resolution preview (RawConverter::fastDemosaicImage) instead of the full path")
     ap.add_argument("--noise-level", type=float, default=0.0,
                     help="DemosaicParameters::noiseLevel; above 0.6 the raw-domain pre-denoise runs")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
@@ -155,18 +159,23 @@ def main():
         r = ctx.image(w, h, ca.FMT_R16U)
         ctx.generate_bayer(r, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
         raws.append(r)
-        outs.append(ctx.image(w, h, ca.FMT_RGBA8))
+        outs.append(ctx.image(w // 2, h // 2, ca.FMT_RGBA8) if args.preview else ctx.image(w, h, ca.FMT_RGBA8))
     params = ca.default_params(bayer, 64.0, 4095.0, levels=5, algorithm=ca.BILATERAL)
     params.rgbConversionParameters.localToneMapping = int(args.ltm)
     params.noiseLevel = args.noise_level
     single_kernel = dict(SINGLE_KERNEL)
     if args.ltm:
         single_kernel["tone"] = "k_reassemble<true, true, true>"
+    if args.preview:
+        single_kernel["tone"] = "k_fast_preview<true>"
     ctx.sync()
 
     def step():
         for r, o in zip(raws, outs):
-            ctx.raw_to_rgb(r, o, params, writeback=False)
+            if args.preview:
+                ctx.fast_demosaic(r, o, params)
+            else:
+                ctx.raw_to_rgb(r, o, params, writeback=False)
 
     for _ in range(args.warmup):
         step()
@@ -222,18 +231,19 @@ def main():
         s = read_profile()[dom]
         lib.clim_profile_enable(ctx.h, 0)
         avg_s = s["ms_total"] / s["launches"] / 1e3
-        bytes_per_launch = stage_bytes(dom, s["texels"] / s["launches"])
+        bytes_per_launch = stage_bytes(dom, s["texels"] / s["launches"], preview=args.preview)
         ach = bytes_per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(single_kernel[dom], w, h),
                     "kernel": single_kernel[dom], "stage": dom, "avg_launch_us": round(avg_s * 1e6, 2),
                     "launches_timed": s["launches"], "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.preview:
         cpu = cpu_baseline(w, h, bayer, ltm=args.ltm, noise_level=args.noise_level)
 
     res = {
-        "metric": "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP",
+        "metric": ("Mpix/s RAW->RGB half-resolution preview (mosaic pixels) at 12 MP" if args.preview
+                   else "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP"),
         "value": round(mpix, 1),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -245,12 +255,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scenes generated on the device)",
-        "config": {"workload": f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, 5-level bilateral pyramid"
+        "config": {"workload": f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, "
+                               + ("half-resolution preview (fastDemosaicImage)" if args.preview
+                                  else "5-level bilateral pyramid")
                                + (", local tone mapping" if args.ltm else "")
                                + (f", noiseLevel {args.noise_level} (raw pre-denoise)" if args.noise_level > 0.6 else ""),
                    "frames_per_step": len(raws), "pixels_per_frame": w * h},
         "roofline": roofline,
-        "path_roofline": {"bytes_per_px": path_bytes_per_px(5),
+        "path_roofline": None if args.preview else {"bytes_per_px": path_bytes_per_px(5),
                           "achieved": round(path_bytes_per_px(5) * mpix * 1e6 / 1e9, 1), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": round(path_bytes_per_px(5) * mpix * 1e6 / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
