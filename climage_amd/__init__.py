@@ -67,6 +67,16 @@ class DemosaicParameters(C.Structure):
                 ("levels", C.c_int32), ("denoiseAlgorithm", C.c_int32)]
 
 
+class DngInfo(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("bayerPattern", C.c_int32),
+                ("orientation", C.c_int32), ("bits_per_sample", C.c_int32), ("compression", C.c_int32),
+                ("tiled", C.c_int32), ("black_level", C.c_float), ("white_level", C.c_float),
+                ("color_matrix1", C.c_float * 9), ("color_matrix2", C.c_float * 9),
+                ("has_color_matrix1", C.c_int32), ("has_color_matrix2", C.c_int32),
+                ("as_shot_neutral", C.c_float * 3), ("has_as_shot_neutral", C.c_int32),
+                ("baseline_exposure", C.c_float)]
+
+
 _lib = None
 
 
@@ -99,7 +109,8 @@ def exported_symbols():
                         "clim_denoise_image_guided", "clim_guided_filter", "clim_reassemble_image",
                         "clim_convert_to_srgb", "clim_local_tone_mapping_mask", "clim_bayer_to_raw_rgba",
                         "clim_raw_rgba_to_bayer", "clim_despeckle_raw_rgba", "clim_denoise_raw_rgba",
-                        "clim_fast_debayer", "clim_fast_demosaic",
+                        "clim_fast_debayer", "clim_fast_demosaic", "clim_dng_decode", "clim_unpack_dng_metadata",
+                        "clim_dng_last_error",
                         "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
                         "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_generate_bayer")]
 
@@ -107,6 +118,43 @@ def exported_symbols():
 def _f(vals):
     vals = [float(v) for v in np.asarray(vals, np.float32).ravel()]
     return (C.c_float * len(vals))(*vals)
+
+
+def dng_decode(data):
+    """read_dng_file from the file's bytes (host only, no GPU): (raw uint16 (H, W), DngInfo)."""
+    L = lib()
+    L.clim_dng_last_error.restype = C.c_char_p
+    buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+    info = DngInfo()
+    st = L.clim_dng_decode(buf, C.c_int64(len(data)), C.byref(info), None, C.c_int64(0))
+    if st != OK:
+        raise ClimError(st, "dng_decode: " + L.clim_dng_last_error().decode())
+    raw = np.zeros((info.height, info.width), np.uint16)
+    st = L.clim_dng_decode(buf, C.c_int64(len(data)), C.byref(info), raw.ctypes.data_as(C.c_void_p),
+                           C.c_int64(info.width))
+    if st != OK:
+        raise ClimError(st, "dng_decode: " + L.clim_dng_last_error().decode())
+    return raw, info
+
+
+def unpack_dng_metadata(info, raw=None, auto_white_balance=False, params=None):
+    """unpackDNGMetadata into DemosaicParameters (host only).  Returns
+    (params, exposure multiplier, highlights fraction, as-shot neutral used)."""
+    L = lib()
+    L.clim_dng_last_error.restype = C.c_char_p
+    p = params if params is not None else default_params(info.bayerPattern)
+    exp, hl = C.c_float(), C.c_float()
+    asn = (C.c_float * 3)()
+    rp = None
+    pitch = 0
+    if raw is not None:
+        raw = np.ascontiguousarray(raw, np.uint16)
+        rp, pitch = raw.ctypes.data_as(C.c_void_p), raw.shape[1]
+    st = L.clim_unpack_dng_metadata(C.byref(info), rp, C.c_int64(pitch), int(bool(auto_white_balance)), C.byref(p),
+                                    C.byref(exp), C.byref(hl), asn)
+    if st != OK:
+        raise ClimError(st, "unpackDNGMetadata: " + L.clim_dng_last_error().decode())
+    return p, exp.value, hl.value, np.array(asn[:], np.float32)
 
 
 def default_params(bayer=RGGB, black=64.0, white=4095.0, levels=5, algorithm=BILATERAL):

@@ -3,6 +3,7 @@
 // mosaic instead of a DNG file.
 //
 //   raw_pipeline W H BAYER LEVELS OUT_PREFIX [LTM [NOISE_LEVEL]]
+//   raw_pipeline --dng FILE OUT_PREFIX [AWB]        (or --dng-params: no GPU)
 //
 // LTM = 1 turns on local tone mapping (rgbConversionParameters.localToneMapping);
 // NOISE_LEVEL > 0.6 the raw-domain pre-denoise of high-noise frames.
@@ -52,7 +53,43 @@ static void write_rgb(const std::string& path, const gls::image<gls::rgb_pixel>&
     for (int y = 0; y < im.height; y++) f.write(reinterpret_cast<const char*>(im[y]), im.width * 3);
 }
 
+// A DNG file the way the reference's RawPipeline takes one
+// (cpp/raw_pipeline.cpp:94-104): read_dng_file, unpackDNGMetadata, then
+// RawConverter::demosaicImage.  Writes OUT_PREFIX.params (the parameters the
+// metadata gave) and, unless PARAMS_ONLY, OUT_PREFIX.dng.rgb.
+static int run_dng(const std::string& path, const std::string& out, bool awb, bool params_only) {
+    gls::tiff_metadata dng_metadata, exif_metadata;
+    const auto rawImage = gls::image<gls::luma_pixel_16>::read_dng_file(path, &dng_metadata, &exif_metadata);
+    DemosaicParameters p = synthetic_params(rggb, 0, 0xffff);
+    const float exposure = unpackDNGMetadata(*rawImage, &dng_metadata, &p, awb, nullptr, false);
+    {
+        std::ofstream f(out + ".params");
+        f.precision(9);
+        f << rawImage->width << ' ' << rawImage->height << ' ' << (int)p.bayerPattern << ' ' << p.black_level << ' '
+          << p.white_level << ' ' << p.exposure_multiplier << ' ' << exposure;
+        for (float v : p.scale_mul) f << ' ' << v;
+        for (const auto& r : p.rgb_cam)
+            for (float v : r) f << ' ' << v;
+        f << '\n';
+    }
+    if (params_only) return 0;
+    gls::HIPContext ctx(0);
+    RawConverter rc(&ctx);
+    auto rgb = RawConverter::convertToRGBImage(*rc.demosaicImage(*rawImage, &p, nullptr, false));
+    write_rgb(out + ".dng.rgb", *rgb);
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 4 && (std::string(argv[1]) == "--dng" || std::string(argv[1]) == "--dng-params")) {
+        try {
+            return run_dng(argv[2], argv[3], argc >= 5 && std::atoi(argv[4]) != 0,
+                           std::string(argv[1]) == "--dng-params");
+        } catch (const std::exception& e) {
+            std::cerr << "raw_pipeline: " << e.what() << std::endl;
+            return 1;
+        }
+    }
     if (argc < 6 || argc > 8) {
         std::cerr << "usage: raw_pipeline W H BAYER LEVELS OUT_PREFIX [LTM [NOISE_LEVEL]]\n";
         return 2;

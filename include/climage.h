@@ -224,6 +224,47 @@ clim_status clim_convert_to_srgb(clim_ctx* ctx, const clim_image* linear, const 
                                  clim_image* out_rgba8, clim_image* out_f32, const float transform[9],
                                  const clim_rgb_conversion_params* params);
 
+/* ------------------------------------------------------------ DNG ingest
+ * Host only: no context and no GPU needed.  Errors carry a message in
+ * clim_dng_last_error() (per thread). */
+
+/* What read_dng_file (src/gls_image_tiff.cpp:255-458) and unpackDNGMetadata
+ * (cpp/demosaic_utils.cpp:280-375) read from a DNG. */
+typedef struct {
+    int32_t width, height;           /* of the raw image after DefaultCrop / ActiveArea */
+    int32_t bayerPattern;            /* from CFAPattern (gbrg when absent, as the reference) */
+    int32_t orientation;
+    int32_t bits_per_sample, compression, tiled;
+    float black_level;               /* BlackLevel[0], 0 when absent */
+    float white_level;               /* WhiteLevel[0], 65535 when absent */
+    float color_matrix1[9], color_matrix2[9];
+    int32_t has_color_matrix1, has_color_matrix2;
+    float as_shot_neutral[3];
+    int32_t has_as_shot_neutral;
+    float baseline_exposure;         /* 0 when absent */
+} clim_dng_info;
+
+/* read_dng_file, src/gls_image_tiff.cpp:255-458, from the file's bytes:
+ * the raw IFD (IFD0, or the first full-resolution SubIFD), uncompressed
+ * (8..16-bit) or lossless JPEG (compression 7) strips or tiles, cropped by
+ * DefaultCropOrigin + ActiveArea and DefaultCropSize.  With pixels null only
+ * info is filled; else pixels receives info->height rows of pitch_elems
+ * uint16 samples. */
+clim_status clim_dng_decode(const uint8_t* data, int64_t size, clim_dng_info* info, uint16_t* pixels,
+                            int64_t pitch_elems);
+
+/* unpackDNGMetadata, cpp/demosaic_utils.cpp:280-375, without ColorChecker
+ * calibration: sets bayerPattern, black_level, white_level,
+ * exposure_multiplier, scale_mul and rgb_cam (cam_xyz_coeff, :25-52) of p.
+ * With auto_white_balance the raw image (host, pitch_elems per row) is
+ * analysed (autoWhiteBalance, :967-1026).  exposure_out (nullable): 2^BaselineExposure;
+ * highlights_out (nullable): the highlight pixel fraction of the white balance;
+ * as_shot_neutral_out (nullable): the white balance used, as AsShotNeutral. */
+clim_status clim_unpack_dng_metadata(const clim_dng_info* info, const uint16_t* raw, int64_t pitch_elems,
+                                     int32_t auto_white_balance, clim_demosaic_params* p, float* exposure_out,
+                                     float* highlights_out, float* as_shot_neutral_out);
+const char* clim_dng_last_error(void);
+
 /* ---------------------------------------------- half-resolution preview */
 
 /* fasteDebayer, cpp/demosaic_cl.cpp:87-105, cl/demosaic.cl:320-334

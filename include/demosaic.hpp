@@ -282,3 +282,6 @@ struct GuidedFilter {
                    "GuidedFilter::filter");
     }
 };
+
+// DNG ingest (read_dng_file, unpackDNGMetadata)
+#include "gls_dng.hpp"

@@ -9,8 +9,11 @@
 #include <array>
 #include <cassert>
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <span>
+#include <string>
+#include <variant>
 #include <vector>
 
 namespace gls {
@@ -78,6 +81,13 @@ typedef basic_luma_pixel<float_type> luma_pixel_float;
 typedef basic_rgba_pixel<float_type> rgba_pixel_float;
 typedef basic_rgba_pixel<float_type> pixel_float4;
 
+// TIFF / DNG metadata, tag -> values (src/gls_tiff_metadata.hpp); read with
+// getVector / getValue (gls_dng.hpp)
+typedef std::variant<std::vector<uint8_t>, std::vector<uint16_t>, std::vector<uint32_t>, std::vector<float>,
+                     std::vector<double>, std::string, float, uint16_t, uint32_t>
+    tiff_metadata_value;
+typedef std::map<uint16_t, tiff_metadata_value> tiff_metadata;
+
 // Row-major host image with a stride (in pixels); owns its storage or wraps
 // the caller's (src/gls_image.hpp:230-300).
 template <typename T>
@@ -94,6 +104,11 @@ class image {
     image(int w, int h, int s, std::span<T> data) : width(w), height(h), stride(s), _data(data) {
         assert((size_t)s * h <= data.size());
     }
+
+    // Image factory from a DNG file (src/gls_image.hpp:429-443); defined for
+    // luma_pixel_16 in gls_dng.hpp.  EXIF metadata is not read.
+    static unique_ptr read_dng_file(const std::string& filename, tiff_metadata* dng_metadata = nullptr,
+                                    tiff_metadata* exif_metadata = nullptr);
 
     T* operator[](int row) { return _data.data() + (size_t)row * stride; }
     const T* operator[](int row) const { return _data.data() + (size_t)row * stride; }

@@ -49,3 +49,28 @@ def test_cpp_driver_fused_equals_stages_equals_oracle(tmp_path, bayer, ltm, nois
     np.testing.assert_array_equal(prev, prev_stages)
     e8, _ = O.Oracle(False).fast_demosaic(raw, O.default_params(bayer, 64.0, 4095.0, levels=5))
     np.testing.assert_array_equal(prev, e8[..., :3])
+
+
+@pytest.mark.parametrize("name", ["lj92_tiles_be", "le14_preview_crop"])
+def test_cpp_dng_file_through_raw_converter(tmp_path, name):
+    """A DNG fixture the way RawPipeline takes a file (read_dng_file,
+    unpackDNGMetadata, RawConverter::demosaicImage) equals the FP32 oracle's
+    pipeline on the decoded mosaic with the parameters the metadata gave."""
+    gold = os.path.join(ROOT, "tests", "golden", "dng")
+    prefix = str(tmp_path / "d")
+    r = subprocess.run([EXE, "--dng", os.path.join(gold, name + ".dng"), prefix], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.load(os.path.join(gold, name + ".npy"), allow_pickle=False)
+    h, w = raw.shape
+    got = np.fromfile(prefix + ".dng.rgb", np.uint8).reshape(h, w, 3)
+    v = np.array(open(prefix + ".params").read().split(), np.float32)
+    op = O.default_params(int(v[2]), float(v[3]), float(v[4]), levels=5)
+    op.exposure_multiplier = float(v[5])
+    for c in range(4):
+        op.scale_mul[c] = float(v[7 + c])
+    for k in range(9):
+        op.rgb_cam[k] = float(v[11 + k])
+    ref = O.Oracle(False).raw_to_rgb(raw, op)
+    np.testing.assert_array_equal(got, ref["rgba8"][..., :3])
+
