@@ -116,6 +116,15 @@ __device__ __forceinline__ TileId xcd_tile() {
     return TileId{t % gx, t / gx};
 }
 
+// The same renumbering for a 1-D grid of persistent blocks that take
+// contiguous runs of tiles: XCD x runs logical blocks [x n/8, (x+1) n/8), so
+// vertically adjacent runs (whose halos overlap) share an L2.  A bijection.
+__device__ __forceinline__ int xcd_block() {
+    const int n = gridDim.x, lin = blockIdx.x;
+    const int q = n >> 3, r = n & 7, xcd = lin & 7;
+    return xcd * q + min(xcd, r) + (lin >> 3);
+}
+
 // Stages N elements into LDS with NT threads: every thread issues all of its
 // global loads before it waits for any (a loop with a run-time trip count
 // would wait for each load before issuing the next).

@@ -80,8 +80,9 @@ __global__ void __launch_bounds__(256) k_raw_stats_fit(FitJobs jobs, float4 smul
     __shared__ QuadTile<T> sm;
     __shared__ clim_fx128 tot[L::K];
     static_assert(sizeof(QuadTile<T>) >= 1024 * sizeof(clim_fx128), "the tile storage serves as the reduction buffer");
-    const FitJob& jb = jobs.j[job_of_block(jobs, blockIdx.x)];
-    const int bi = blockIdx.x - jb.block_base;
+    const int lb = xcd_block();
+    const FitJob& jb = jobs.j[job_of_block(jobs, lb)];
+    const int bi = lb - jb.block_base;
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     const Bayer& b = jb.b;
     const MosaicSrc<U16> src{jb.src, b, smul, black};

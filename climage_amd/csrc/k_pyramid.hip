@@ -131,8 +131,9 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs) {
     __shared__ StatsTile sm;
     __shared__ clim_fx128 tot[L::K];
     static_assert(sizeof(StatsTile) >= 1024 * sizeof(clim_fx128), "the tile storage serves as the reduction buffer");
-    const FitJob& jb = jobs.j[job_of_block(jobs, blockIdx.x)];
-    const int bi = blockIdx.x - jb.block_base;
+    const int lb = xcd_block();
+    const FitJob& jb = jobs.j[job_of_block(jobs, lb)];
+    const int bi = lb - jb.block_base;
     const int tid = threadIdx.x, tx = tid & 31, ty0 = tid >> 5;
     const Img& in = jb.src;
     const Img& out = jb.m;
