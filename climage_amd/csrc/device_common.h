@@ -145,6 +145,31 @@ __device__ __forceinline__ void stage(int tid, LoadF load, StoreF store) {
     }
 }
 
+// stage() in two steps: fetch() issues the global loads into registers and
+// put() stores them to LDS.  A persistent kernel fetches its next tile before
+// it computes the current one, so the loads are in flight during the compute.
+template <int N, int NT, typename T>
+struct Staged {
+    static constexpr int IT = (N + NT - 1) / NT;
+    T v[IT];
+    template <typename LoadF>
+    __device__ __forceinline__ void fetch(int tid, LoadF load) {
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int i = tid + k * NT;
+            if (i < N) v[k] = load(i);
+        }
+    }
+    template <typename StoreF>
+    __device__ __forceinline__ void put(int tid, StoreF store) const {
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int i = tid + k * NT;
+            if (i < N) store(i, v[k]);
+        }
+    }
+};
+
 // One raw quad (texels (2qx + ox[c], 2qy + oy[c]), c = R, G, B, G2) of the
 // uint16 mosaic, scaled as scaleRawData does (cl/demosaic.cl:69-78), with
 // texel coordinates clamped to the image.  Two 32-bit loads (the quad's two
@@ -195,6 +220,16 @@ struct RawQuadLoader {
     __device__ __forceinline__ float4 quad(int qx, int qy) const { return quad(words(qx, qy), qx, qy); }
     __device__ __forceinline__ float4 quad(uint2 wds, int qx, int qy) const {
         const uint32_t r0 = wds.x, r1 = wds.y;
+        if (qx >= 0 && qy >= 0 && qx < raw.w / 2 && qy < raw.h / 2) {
+            // no texel is clamped: texel c sits at a site of colour c
+            float v[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint32_t row = b.oy[c] ? r1 : r0;
+                v[c] = scale_texel(b.ox[c] ? (row >> 16) : (row & 0xffffu), c, smul, black);
+            }
+            return make_float4(v[0], v[1], v[2], v[3]);
+        }
         // site parity of the clamped rows: row 2qy clamps to 0 or H-1 only
         // at the borders, where both rows clamp to the same row
         const int y0 = clampi(2 * qy, 0, raw.h - 1), y1 = clampi(2 * qy + 1, 0, raw.h - 1);

@@ -69,6 +69,30 @@ void launch_raw_stats(const Img& scaled, const Bayer& b, const Img& mean, const 
     hipLaunchKernelGGL(k_raw_stats, grd, blk, 0, st, scaled, b, mean, var);
 }
 
+// The loads of one quad and their conversion to scaled (R, G, B, G2): the two
+// 32-bit row words of a uint16 mosaic, or the four texels of an FP32 one;
+// coordinates are clamped per texel (MosaicSrc).
+template <bool U16>
+struct QuadFetch;
+template <>
+struct QuadFetch<true> {
+    using Raw = uint2;
+    RawQuadLoader ld;
+    __device__ __forceinline__ Raw load(int qx, int qy) const { return ld.words(qx, qy); }
+    __device__ __forceinline__ float4 value(Raw w, int qx, int qy) const { return ld.quad(w, qx, qy); }
+};
+template <>
+struct QuadFetch<false> {
+    using Raw = float4;
+    MosaicSrc<false> src;
+    __device__ __forceinline__ Raw load(int qx, int qy) const {
+        const Bayer& b = src.b;
+        return make_float4(src(2 * qx + b.ox[0], 2 * qy + b.oy[0]), src(2 * qx + b.ox[1], 2 * qy + b.oy[1]),
+                           src(2 * qx + b.ox[2], 2 * qy + b.oy[2]), src(2 * qx + b.ox[3], 2 * qy + b.oy[3]));
+    }
+    __device__ __forceinline__ float4 value(Raw v, int, int) const { return v; }
+};
+
 // rawNoiseStatistics with the first regression pass of
 // computeRawNoiseStatistics accumulated on the way: each of the grid's blocks
 // takes a contiguous run of 16x16-quad tiles and keeps its regression sums
@@ -85,23 +109,26 @@ __global__ void __launch_bounds__(256) k_raw_stats_fit(FitJobs jobs, float4 smul
     const int bi = lb - jb.block_base;
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     const Bayer& b = jb.b;
-    const MosaicSrc<U16> src{jb.src, b, smul, black};
+    const QuadFetch<U16> src{{jb.src, b, smul, black}};
     const int tiles_x = (jb.m.w + T - 1) / T;
     const long long ntiles = (long long)tiles_x * ((jb.m.h + T - 1) / T);
     const int t0 = (int)(ntiles * bi / jb.nblocks), t1 = (int)(ntiles * (bi + 1) / jb.nblocks);
     Acc<4, false, false> acc;
     acc.zero();
+    // the next tile's loads are issued before the current tile's compute
+    Staged<S * S, 256, typename QuadFetch<U16>::Raw> pf;
+    auto fetch = [&](int t) {
+        const int qx0 = (t % tiles_x) * T - R, qy0 = (t / tiles_x) * T - R;
+        pf.fetch(tid, [&](int i) { return src.load(qx0 + i % S, qy0 + i / S); });
+    };
+    if (t0 < t1) fetch(t0);
     for (int t = t0; t < t1; t++) {
         const int qx0 = (t % tiles_x) * T, qy0 = (t / tiles_x) * T;
-        stage<S * S, 256>(
-            tid,
-            [&](int i) {
-                const int qx = qx0 - R + i % S, qy = qy0 - R + i / S;
-                return make_float4(src(2 * qx + b.ox[0], 2 * qy + b.oy[0]), src(2 * qx + b.ox[1], 2 * qy + b.oy[1]),
-                                   src(2 * qx + b.ox[2], 2 * qy + b.oy[2]), src(2 * qx + b.ox[3], 2 * qy + b.oy[3]));
-            },
-            [&](int i, float4 v) { sm.q[i / S][i % S] = v; });
+        pf.put(tid, [&](int i, typename QuadFetch<U16>::Raw r) {
+            sm.q[i / S][i % S] = src.value(r, qx0 - R + i % S, qy0 - R + i / S);
+        });
         __syncthreads();
+        if (t + 1 < t1) fetch(t + 1);
         sm.hsums(tid);
         __syncthreads();
         const int qx = qx0 + tx, qy = qy0 + ty;

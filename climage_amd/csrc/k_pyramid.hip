@@ -142,10 +142,14 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs) {
     const int t0 = (int)(ntiles * bi / jb.nblocks), t1 = (int)(ntiles * (bi + 1) / jb.nblocks);
     Acc<3, true, false> acc;
     acc.zero();
+    // the next tile's loads are issued before the current tile's compute
+    LevelTile<TX, TY>::Regs pf;
+    if (t0 < t1) LevelTile<TX, TY>::fetch(pf, in, (t0 % tiles_x) * TX, (t0 / tiles_x) * TY, tid);
     for (int t = t0; t < t1; t++) {
         const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * TY;
-        sm.t.load(in, x0, y0, tid);
+        sm.t.put(pf, tid);
         __syncthreads();
+        if (t + 1 < t1) LevelTile<TX, TY>::fetch(pf, in, ((t + 1) % tiles_x) * TX, ((t + 1) / tiles_x) * TY, tid);
         sm.r.compute(sm.t, tid);
         __syncthreads();
 #pragma unroll

@@ -19,14 +19,22 @@ struct LevelTile {
     static constexpr int SX = TX + 4, SY = TY + 4;
     float x[SY][SX + 1], y[SY][SX + 1], z[SY][SX + 1];
 
+    // the tile's texels in registers (fetch, then put)
+    using Regs = Staged<SX * SY, 256, float4>;
+    __device__ __forceinline__ static void fetch(Regs& r, const Img& in, int x0, int y0, int tid) {
+        r.fetch(tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); });
+    }
+    __device__ __forceinline__ void put(const Regs& r, int tid) {
+        r.put(tid, [&](int i, float4 p) {
+            x[i / SX][i % SX] = p.x;
+            y[i / SX][i % SX] = p.y;
+            z[i / SX][i % SX] = p.z;
+        });
+    }
     __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
-        stage<SX * SY, 256>(
-            tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
-            [&](int i, float4 p) {
-                x[i / SX][i % SX] = p.x;
-                y[i / SX][i % SX] = p.y;
-                z[i / SX][i % SX] = p.z;
-            });
+        Regs r;
+        fetch(r, in, x0, y0, tid);
+        put(r, tid);
     }
 };
 
