@@ -290,7 +290,7 @@ clim_status run_stats_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw, const fl
     if (raw)
         clim::launch_raw_stats_fit(jobs, smul, black, ctx->stream);
     else
-        clim::launch_noise_stats_fit(jobs, ctx->stream);
+        clim::launch_noise_stats_fit(jobs, 0, jobs.njobs, nullptr, ctx->stream);
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
 }

@@ -77,10 +77,13 @@ __global__ void __launch_bounds__(256) k_downsample(Img in, Img out) {
     wrow_f4(out, y)[x] = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.25f * acc.w);
 }
 
+bool downsample_exact2x(const Img& in, const Img& out) {
+    return in.w == 2 * out.w && in.h == 2 * out.h && out.w <= kDown2xMax && out.h <= kDown2xMax;
+}
+
 void launch_downsample(const Img& in, const Img& out, hipStream_t st) {
     dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
-    const bool exact = in.w == 2 * out.w && in.h == 2 * out.h && out.w <= kDown2xMax && out.h <= kDown2xMax;
-    if (exact)
+    if (downsample_exact2x(in, out))
         hipLaunchKernelGGL(k_downsample<true>, grd, blk, 0, st, in, out);
     else
         hipLaunchKernelGGL(k_downsample<false>, grd, blk, 0, st, in, out);
@@ -92,15 +95,16 @@ namespace {
 constexpr int TX = 32, TY = 16, HR = 2;
 }
 
+template <bool W = false>
 struct StatsTile {
-    LevelTile<TX, TY> t;
+    LevelTile<TX, TY, W> t;
     StatsRows<TX, TY> r;
 };
 
 // ------------------------------------------------------------ noiseStatistics
 // cl/demosaic.cl:1267-1286: (mean Y, var Y, var Cb, var Cr) over 5x5.
 __global__ void __launch_bounds__(256) k_noise_stats(Img in, Img out) {
-    __shared__ StatsTile sm;
+    __shared__ StatsTile<> sm;
     const TileId tl = xcd_tile();
     const int x0 = tl.x * TX, y0 = tl.y * TY;
     const int tid = threadIdx.y * 32 + threadIdx.x;
@@ -122,15 +126,48 @@ void launch_noise_stats(const Img& in, const Img& out, hipStream_t st) {
     hipLaunchKernelGGL(k_noise_stats, grd, blk, 0, st, in, out);
 }
 
-// noiseStatistics of every pyramid level (one job per level) with the first
+// downsampleImage (exact 2x: four taps of weight 1/2 at texels 2X-1 and
+// 2X+1, k_downsample<true>) of the half-size texels a 32x16 level tile
+// covers, from the tile in LDS.  Thread tid computes channels 2p, 2p+1
+// (p = tid / 128) of texel tid % 128 (16x8, row-major) with the same
+// arithmetic as bilinear4_at and k_downsample.
+__device__ __forceinline__ void downsample_from_tile(const LevelTile<TX, TY, true>& t, const Img& out, int x0,
+                                                     int y0, int tid) {
+    const int p = tid >> 7, o = tid & 127, lx = o & 15, ly = o >> 4;
+    const int X = (x0 >> 1) + lx, Y = (y0 >> 1) + ly;
+    if (X >= out.w || Y >= out.h) return;
+    // texel (2X - 1, 2Y - 1) sits at tile column 2 lx + 1, row 2 ly + 1
+    const int c = 2 * lx + 1, r = 2 * ly + 1;
+    float res[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float(*a)[TX + 5] = p == 0 ? (k == 0 ? t.x : t.y) : (k == 0 ? t.z : t.w);
+        auto tap = [&](int i, int j) {
+            return 0.25f * a[j][i] + 0.25f * a[j][i + 1] + 0.25f * a[j + 1][i] + 0.25f * a[j + 1][i + 1];
+        };
+        float acc = tap(c, r);
+        acc += tap(c + 2, r);
+        acc += tap(c, r + 2);
+        acc += tap(c + 2, r + 2);
+        res[k] = 0.25f * acc;
+    }
+    reinterpret_cast<float2*>(wrow_f4(out, Y) + X)[p] = make_float2(res[0], res[1]);
+}
+
+// noiseStatistics of pyramid levels (one job per level) with the first
 // regression pass of computeNoiseStatistics accumulated on the way: each of
 // the grid's blocks takes a contiguous run of one level's 32x16 tiles and
 // keeps its regression sums across them (one partial per block).
-__global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs) {
+// DOWN (one job): each tile also writes the next level's texels it covers
+// (downsampleImage), so that level is made without reading this one again.
+template <bool DOWN>
+__global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down) {
     using L = Layout<3, true, false>;
-    __shared__ StatsTile sm;
+    using Tile = LevelTile<TX, TY, DOWN>;
+    __shared__ StatsTile<DOWN> sm;
     __shared__ clim_fx128 tot[L::K];
-    static_assert(sizeof(StatsTile) >= 1024 * sizeof(clim_fx128), "the tile storage serves as the reduction buffer");
+    static_assert(sizeof(StatsTile<DOWN>) >= 1024 * sizeof(clim_fx128),
+                  "the tile storage serves as the reduction buffer");
     const int lb = xcd_block();
     const FitJob& jb = jobs.j[job_of_block(jobs, lb)];
     const int bi = lb - jb.block_base;
@@ -143,14 +180,15 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs) {
     Acc<3, true, false> acc;
     acc.zero();
     // the next tile's loads are issued before the current tile's compute
-    LevelTile<TX, TY>::Regs pf;
-    if (t0 < t1) LevelTile<TX, TY>::fetch(pf, in, (t0 % tiles_x) * TX, (t0 / tiles_x) * TY, tid);
+    typename Tile::Regs pf;
+    if (t0 < t1) Tile::fetch(pf, in, (t0 % tiles_x) * TX, (t0 / tiles_x) * TY, tid);
     for (int t = t0; t < t1; t++) {
         const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * TY;
         sm.t.put(pf, tid);
         __syncthreads();
-        if (t + 1 < t1) LevelTile<TX, TY>::fetch(pf, in, ((t + 1) % tiles_x) * TX, ((t + 1) / tiles_x) * TY, tid);
+        if (t + 1 < t1) Tile::fetch(pf, in, ((t + 1) % tiles_x) * TX, ((t + 1) / tiles_x) * TY, tid);
         sm.r.compute(sm.t, tid);
+        if constexpr (DOWN) downsample_from_tile(sm.t, down, x0, y0, tid);
         __syncthreads();
 #pragma unroll
         for (int rr = 0; rr < 2; rr++) {
@@ -169,12 +207,20 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs) {
     store_block_partials(acc, reinterpret_cast<clim_fx128*>(&sm), tot, jb.partials, bi, kFitBlocksPerJob);
 }
 
-void launch_noise_stats_fit(FitJobs& jobs, hipStream_t st) {
+void launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipStream_t st) {
+    FitJobs sub{};
+    sub.njobs = j1 - j0;
     long long tiles[5];
-    for (int j = 0; j < jobs.njobs; j++)
-        tiles[j] = (long long)((jobs.j[j].m.w + TX - 1) / TX) * ((jobs.j[j].m.h + TY - 1) / TY);
-    const int grid = fit_share_blocks(jobs, tiles, 1024);
-    hipLaunchKernelGGL(k_noise_stats_fit, dim3(grid), dim3(256), 0, st, jobs);
+    for (int j = j0; j < j1; j++) {
+        sub.j[j - j0] = jobs.j[j];
+        tiles[j - j0] = (long long)((jobs.j[j].m.w + TX - 1) / TX) * ((jobs.j[j].m.h + TY - 1) / TY);
+    }
+    const int grid = fit_share_blocks(sub, tiles, 1024);
+    for (int j = j0; j < j1; j++) jobs.j[j] = sub.j[j - j0];
+    if (down)
+        hipLaunchKernelGGL(k_noise_stats_fit<true>, dim3(grid), dim3(256), 0, st, sub, *down);
+    else
+        hipLaunchKernelGGL(k_noise_stats_fit<false>, dim3(grid), dim3(256), 0, st, sub, Img{});
 }
 
 // --------------------------------------------------------------- denoiseImage

@@ -45,7 +45,14 @@ struct FitJobs {
 // smul: read the jobs' src as the uint16 mosaic, scaled with (smul, black);
 // null: src is the scaled FP32 mosaic
 void launch_raw_stats_fit(FitJobs& jobs, const float* smul, float black, hipStream_t st);
-void launch_noise_stats_fit(FitJobs& jobs, hipStream_t st);
+// jobs [j0, j1) of `jobs` in one launch (their block shares are set in
+// `jobs`); down (one job, downsample_exact2x(src, *down)): the launch also
+// writes downsampleImage of the job's level into *down
+void launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipStream_t st);
+// downsampleImage from `in` to `out` is an exact 2x reduction (every tap at
+// an odd texel offset with weight 1/2); launch_noise_stats_fit can then do it
+// on the way
+bool downsample_exact2x(const Img& in, const Img& out);
 // Gives each job a run of blocks in proportion to units[j] (at most one per
 // unit and kFitBlocksPerJob, at least one), about `grid` in all; returns the
 // total.

@@ -90,10 +90,6 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
     // analysis: downsample and statistics of every level
     clim::FitJobs jobs{};
     for (int i = 0; i < levels; i++) {
-        if (i < levels - 1) {
-            ProfScope ps(ctx, ST_DOWNSAMPLE, (long long)lvl[i + 1].w * lvl[i + 1].h);
-            clim::launch_downsample(lvl[i], lvl[i + 1], st);
-        }
         const float lm = dp[i].luma * dp[i].luma, cm = dp[i].chroma * dp[i].chroma;
         const float mult[8] = {lm, cm, cm, lm, cm, cm, 1.0f, 1.0f};
         float in8[8] = {0};
@@ -102,14 +98,26 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         jobs.j[i] = make_fit_job(ctx, 1 + i, lvl[i], sv, sv, in8, mult);
     }
     jobs.njobs = levels;
-    CLIM_TRY_HIP(ctx, hipGetLastError());
     long long nlev = 0;
     for (int i = 0; i < levels; i++) nlev += (long long)lvl[i].w * lvl[i].h;
-    // noiseStatistics of every level with the first regression pass, then the refit
+    // noiseStatistics of every level with the first regression pass, then the
+    // refit.  Level 0's statistics kernel writes level 1 from its tiles when
+    // the reduction is exactly 2x (level 0 is read once for both); the other
+    // levels follow their downsampling in one launch.
+    const bool fused = clim::downsample_exact2x(lvl[0], lvl[1]);
     {
-        ProfScope ps(ctx, ST_LEVEL_STATS, nlev);
-        CLIM_TRY(run_stats_fit(ctx, jobs, false, nullptr, 0.0f));
+        ProfScope ps(ctx, ST_LEVEL_STATS, (long long)lvl[0].w * lvl[0].h);
+        clim::launch_noise_stats_fit(jobs, 0, 1, fused ? &lvl[1] : nullptr, st);
     }
+    for (int i = fused ? 1 : 0; i < levels - 1; i++) {
+        ProfScope ps(ctx, ST_DOWNSAMPLE, (long long)lvl[i + 1].w * lvl[i + 1].h);
+        clim::launch_downsample(lvl[i], lvl[i + 1], st);
+    }
+    {
+        ProfScope ps(ctx, ST_LEVEL_STATS, nlev - (long long)lvl[0].w * lvl[0].h);
+        clim::launch_noise_stats_fit(jobs, 1, levels, nullptr, st);
+    }
+    CLIM_TRY_HIP(ctx, hipGetLastError());
     {
         ProfScope ps(ctx, ST_LEVEL_FIT, nlev);
         CLIM_TRY(run_fit_rest(ctx, jobs, false));

@@ -13,11 +13,13 @@
 namespace clim {
 
 // ------------------------------------------------------------ RGBA levels
-// TXxTY outputs, 256 threads, a 2-texel halo; only x, y, z enter the filters.
-template <int TX, int TY>
+// TXxTY outputs, 256 threads, a 2-texel halo; only x, y, z enter the filters
+// (W: the fourth channel is staged too).
+template <int TX, int TY, bool W = false>
 struct LevelTile {
     static constexpr int SX = TX + 4, SY = TY + 4;
     float x[SY][SX + 1], y[SY][SX + 1], z[SY][SX + 1];
+    float w[W ? SY : 1][SX + 1];
 
     // the tile's texels in registers (fetch, then put)
     using Regs = Staged<SX * SY, 256, float4>;
@@ -29,6 +31,7 @@ struct LevelTile {
             x[i / SX][i % SX] = p.x;
             y[i / SX][i % SX] = p.y;
             z[i / SX][i % SX] = p.z;
+            if (W) w[i / SX][i % SX] = p.w;
         });
     }
     __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
@@ -45,7 +48,8 @@ struct StatsRows {
     static constexpr int SY = TY + 4;
     float h[6][SY][TX + 1];
 
-    __device__ __forceinline__ void compute(const LevelTile<TX, TY>& t, int tid) {
+    template <bool W>
+    __device__ __forceinline__ void compute(const LevelTile<TX, TY, W>& t, int tid) {
         constexpr int N = SY * TX, IT = (N + 255) / 256;
 #pragma unroll
         for (int k = 0; k < IT; k++) {

@@ -146,6 +146,32 @@ def test_downsample(ctx, chain, oracle32, size):
     bitexact(b.download(), oracle32.downsample(src, ow, oh), f"downsampleImage {size}")
 
 
+@pytest.mark.parametrize("w,h,levels", [(320, 200, 5), (200, 136, 4), (203, 141, 4)])
+def test_pyramid_levels_of_random_rgba(ctx, oracle32, w, h, levels):
+    """The image pyramid pyramidalDenoiseInto builds from a random RGBA image
+    (all four channels vary): level 1 is written by level 0's statistics
+    kernel when the reduction is exactly 2x (320x200, 200x136 with partial
+    tiles) and by the downsample kernel otherwise (203x141); every level, the
+    per-level noise model and the denoised level 0 equal the oracle's."""
+    rng = np.random.default_rng(w * h)
+    yy, xx = np.mgrid[0:h, 0:w]
+    smooth = np.stack([0.2 + 0.15 * np.sin(xx / 17.0 + k) * np.cos(yy / 23.0 - k) for k in range(4)], -1)
+    img = (smooth + rng.normal(0.0, 0.004, (h, w, 4))).astype(np.float32)
+    dp = [O.DenoiseParams(1.0, 1.0, 1.0, 1.0, 1.0)] * 5
+    rc, den0, nlf = oracle32.pyramidal_denoise(img, levels, 0, dp, O.DEFAULT_PYRAMID_NLF)
+    lv = [ctx.image(w >> i, h >> i, ca.FMT_RGBA32F) for i in range(1, levels)]
+    dn = [ctx.image(w >> i, h >> i, ca.FMT_RGBA32F) for i in range(levels)]
+    cp = [ca.DenoiseParameters(1.0, 1.0, 1.0, 1.0, 1.0)] * 5
+    got_nlf = ctx.pyramidalDenoiseInto(up(ctx, img, ca.FMT_RGBA32F), levels, ca.BILATERAL, cp, lv, dn,
+                                       nlf=np.asarray(O.DEFAULT_PYRAMID_NLF[:levels]))
+    prev = img
+    for i in range(1, levels):
+        prev = oracle32.downsample(prev, w >> i, h >> i)
+        bitexact(lv[i - 1].download(), prev, f"level {i} of {w}x{h}")
+    bitexact(got_nlf, nlf[:levels], "per-level noise model")
+    bitexact(dn[0].download(), den0, "denoised level 0")
+
+
 def test_noise_statistics_and_fit(ctx, chain, oracle32):
     src = chain[ca.RGGB]["desp"]
     a = up(ctx, src, ca.FMT_RGBA32F)
