@@ -293,19 +293,12 @@ struct Mat3 { float m[9]; };
 // transform follow on each pixel.  out[dy][dx] = (R, G, B) or (Y, Cb, Cr).
 // Which site is R, G, B or G2 is the same for every quad, so every lane of a
 // wave runs the same code.
-template <bool kFuse, bool U16>
-__device__ __forceinline__ void red_blue_quad(const MosaicSrc<U16>& raw, const Img& green, int x0, int y0,
-                                              const Bayer& b, int red_is_c1_site, float chromaVariance, float clip,
-                                              const Mat3& m, float3 out[2][2]) {
-    // R[j][i], G[j][i]: mosaic and green at (x0 - 1 + i, y0 - 1 + j), clamped
-    float R[4][4], G[4][4];
-#pragma unroll
-    for (int j = 0; j < 4; j++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            R[j][i] = raw(x0 - 1 + i, y0 - 1 + j);
-            G[j][i] = ld1(green, x0 - 1 + i, y0 - 1 + j);
-        }
+// The quad's 4x4 windows: R[j][i], G[j][i] are the mosaic and green at
+// (x0 - 1 + i, y0 - 1 + j), clamped to the image.
+template <bool kFuse>
+__device__ __forceinline__ void red_blue_from(const float (&R)[4][4], const float (&G)[4][4], const Bayer& b,
+                                              int red_is_c1_site, float chromaVariance, float clip, const Mat3& m,
+                                              float3 out[2][2]) {
 #pragma unroll
     for (int dy = 0; dy < 2; dy++) {
 #pragma unroll
@@ -346,6 +339,21 @@ __device__ __forceinline__ void red_blue_quad(const MosaicSrc<U16>& raw, const I
             out[dy][dx] = make_float3(px, py, pz);
         }
     }
+}
+
+template <bool kFuse, bool U16>
+__device__ __forceinline__ void red_blue_quad(const MosaicSrc<U16>& raw, const Img& green, int x0, int y0,
+                                              const Bayer& b, int red_is_c1_site, float chromaVariance, float clip,
+                                              const Mat3& m, float3 out[2][2]) {
+    float R[4][4], G[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            R[j][i] = raw(x0 - 1 + i, y0 - 1 + j);
+            G[j][i] = ld1(green, x0 - 1 + i, y0 - 1 + j);
+        }
+    red_blue_from<kFuse>(R, G, b, red_is_c1_site, chromaVariance, clip, m, out);
 }
 
 __device__ __forceinline__ float chroma_variance(const ParamRef& cv) {
@@ -455,9 +463,32 @@ __device__ __forceinline__ float median9(float v0, float v1, float v2, float v3,
     return a1;
 }
 
+// The median of nine values without NaN, as the selection network above
+// returns it: sort three rows of three, then the median of (the largest row
+// minimum, the median of the row medians, the smallest row maximum).  Equal
+// to median9 for every NaN-free input (checked exhaustively over {0..3}^9);
+// v_min3 / v_med3 / v_max3 do each sort of three.
+__device__ __forceinline__ float med3f(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
+__device__ __forceinline__ float median9_nonan(const float* v) {
+    float lo[3], md[3], hi[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const float a = v[3 * r], b = v[3 * r + 1], c = v[3 * r + 2];
+        lo[r] = fminf(fminf(a, b), c);
+        md[r] = med3f(a, b, c);
+        hi[r] = fmaxf(fmaxf(a, b), c);
+    }
+    return med3f(fmaxf(fmaxf(lo[0], lo[1]), lo[2]), med3f(md[0], md[1], md[2]), fminf(fminf(hi[0], hi[1]), hi[2]));
+}
+
 // One pixel; fetch(dx, dy) returns the YCbCr texel at offset (dx, dy),
-// clamped to the image.
-template <typename Fetch>
+// clamped to the image.  NONAN (the caller knows the neighbourhood holds no
+// NaN): the luma trackers keep the two largest of {0, 0, Y...} and the two
+// smallest of {65535, 65535, Y...} with max / min and a median of three (a
+// running clamp), and the chroma medians come from median9_nonan: the same
+// values as the reference's comparisons for every NaN-free input (a zero's
+// sign aside), in about a third of the instructions.
+template <bool NONAN = false, typename Fetch>
 __device__ __forceinline__ float4 despeckle_px(Fetch fetch, float var_a, float var_b) {
     // read order of fast_median3x3
     const int ox[9] = {0, 1, 0, 1, 0, 1, -1, -1, -1};
@@ -468,18 +499,25 @@ __device__ __forceinline__ float4 despeckle_px(Fetch fetch, float var_a, float v
     for (int k = 0; k < 9; k++) {
         const float4 p = fetch(ox[k], oy[k]);
         const float v = p.x;
-        secMax = v <= firstMax && v > secMax ? v : secMax;
-        secMax = v > firstMax ? firstMax : secMax;
-        firstMax = v > firstMax ? v : firstMax;
-        secMin = v >= firstMin && v < secMin ? v : secMin;
-        secMin = v < firstMin ? firstMin : secMin;
-        firstMin = v < firstMin ? v : firstMin;
+        if (NONAN) {
+            secMax = med3f(secMax, v, firstMax);
+            firstMax = fmaxf(firstMax, v);
+            secMin = med3f(firstMin, v, secMin);
+            firstMin = fminf(firstMin, v);
+        } else {
+            secMax = v <= firstMax && v > secMax ? v : secMax;
+            secMax = v > firstMax ? firstMax : secMax;
+            firstMax = v > firstMax ? v : firstMax;
+            secMin = v >= firstMin && v < secMin ? v : secMin;
+            secMin = v < firstMin ? firstMin : secMin;
+            firstMin = v < firstMin ? v : firstMin;
+        }
         if (k == 2) sample = v;
         cb[k] = p.y;
         cr[k] = p.z;
     }
-    const float mcb = median9(cb[0], cb[1], cb[2], cb[3], cb[4], cb[5], cb[6], cb[7], cb[8]);
-    const float mcr = median9(cr[0], cr[1], cr[2], cr[3], cr[4], cr[5], cr[6], cr[7], cr[8]);
+    const float mcb = NONAN ? median9_nonan(cb) : median9(cb[0], cb[1], cb[2], cb[3], cb[4], cb[5], cb[6], cb[7], cb[8]);
+    const float mcr = NONAN ? median9_nonan(cr) : median9(cr[0], cr[1], cr[2], cr[3], cr[4], cr[5], cr[6], cr[7], cr[8]);
     const float sigma = sqrtf(var_a + var_b * sample);
     const float minVal = mixf(secMin, firstMin, smoothstep(sigma, 4.0f * sigma, secMin - firstMin));
     const float maxVal = mixf(secMax, firstMax, smoothstep(sigma, 4.0f * sigma, firstMax - secMax));
@@ -506,44 +544,101 @@ void launch_despeckle(const Img& in, const Img& out, float var_a, float var_b, h
 // tile's YCbCr values plus a 2-pixel margin (34x14 quads) are computed into
 // LDS, and the despeckle reads its 3x3 neighbourhoods there, at the clamped
 // image coordinates the stand-alone kernels would read.  The level-0 YCbCr
-// image between the two is never written.
+// image between the two is never written.  The quads' 4x4 windows come from
+// the scaled mosaic and the green plane of the region (70x30 texels, clamped
+// coordinates) staged in LDS first, each texel loaded and scaled once.
 namespace {
 constexpr int RD_X = 64, RD_Y = 24;                 // output tile
 constexpr int RD_RX = RD_X + 4, RD_RY = RD_Y + 4;   // YCbCr region (even origin)
 constexpr int RD_QX = RD_RX / 2, RD_QY = RD_RY / 2;  // its quads
+constexpr int RD_WX = RD_RX + 2, RD_WY = RD_RY + 2;  // mosaic / green window of the region
+constexpr int RD_QPT = (RD_QX * RD_QY + 255) / 256;  // quads per thread
 }
+
+union RbDespeckleLds {
+    struct {
+        float r[RD_WY][RD_WX], g[RD_WY][RD_WX];
+    } win;
+    float4 t[RD_RY][RD_RX];
+};
 
 template <bool U16>
 __global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img green, Img out, Bayer b,
                                                       int red_is_c1_site, ParamRef cv, float clip, Mat3 m,
                                                       float var_a, float var_b) {
-    __shared__ float4 t[RD_RY][RD_RX];
+    __shared__ RbDespeckleLds sm;
     const TileId tl = xcd_tile();
     const int x0 = tl.x * RD_X, y0 = tl.y * RD_Y;
     const int rx0 = x0 - 2, ry0 = y0 - 2;
     const int tid = threadIdx.x;
     const float cvar = chroma_variance(cv);
-    for (int q = tid; q < RD_QX * RD_QY; q += 256) {
+    // the region's mosaic (scaled) and green, texel (rx0 - 1 + i, ry0 - 1 + j)
+    stage<RD_WX * RD_WY, 256>(
+        tid,
+        [&](int k) {
+            const int i = k % RD_WX, j = k / RD_WX;
+            return make_float2(raw(rx0 - 1 + i, ry0 - 1 + j), ld1(green, rx0 - 1 + i, ry0 - 1 + j));
+        },
+        [&](int k, float2 v) {
+            sm.win.r[k / RD_WX][k % RD_WX] = v.x;
+            sm.win.g[k / RD_WX][k % RD_WX] = v.y;
+        });
+    __syncthreads();
+    float3 o[RD_QPT][2][2];
+    int has_nan = 0;
+#pragma unroll
+    for (int n = 0; n < RD_QPT; n++) {
+        const int q = tid + 256 * n;
+        if (q >= RD_QX * RD_QY) break;
         const int qx = q % RD_QX, qy = q / RD_QX;
-        float3 o[2][2];
-        red_blue_quad<true, U16>(raw, green, rx0 + 2 * qx, ry0 + 2 * qy, b, red_is_c1_site, cvar, clip, m, o);
+        float R[4][4], G[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                R[j][i] = sm.win.r[2 * qy + j][2 * qx + i];
+                G[j][i] = sm.win.g[2 * qy + j][2 * qx + i];
+            }
+        red_blue_from<true>(R, G, b, red_is_c1_site, cvar, clip, m, o[n]);
 #pragma unroll
         for (int dy = 0; dy < 2; dy++)
 #pragma unroll
             for (int dx = 0; dx < 2; dx++)
-                t[2 * qy + dy][2 * qx + dx] = make_float4(o[dy][dx].x, o[dy][dx].y, o[dy][dx].z, 0.0f);
+                has_nan |= isnan(o[n][dy][dx].x) | isnan(o[n][dy][dx].y) | isnan(o[n][dy][dx].z);
+    }
+    // the windows are read (the YCbCr region takes their place); a region
+    // without NaN takes the despeckle's NaN-free form
+    const bool nonan = !__syncthreads_or(has_nan);
+#pragma unroll
+    for (int n = 0; n < RD_QPT; n++) {
+        const int q = tid + 256 * n;
+        if (q >= RD_QX * RD_QY) break;
+        const int qx = q % RD_QX, qy = q / RD_QX;
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+            for (int dx = 0; dx < 2; dx++)
+                sm.t[2 * qy + dy][2 * qx + dx] = make_float4(o[n][dy][dx].x, o[n][dy][dx].y, o[n][dy][dx].z, 0.0f);
     }
     __syncthreads();
+    // every 3x3 neighbourhood of an interior tile lies inside the image: no clamps
+    const bool interior = x0 >= 1 && y0 >= 1 && x0 + RD_X + 1 <= out.w && y0 + RD_Y + 1 <= out.h;
+    if (interior && nonan) {
+        for (int i = tid; i < RD_X * RD_Y; i += 256) {
+            const int lx = i % RD_X, ly = i / RD_X;
+            wrow_f4(out, y0 + ly)[x0 + lx] =
+                despeckle_px<true>([&](int dx, int dy) { return sm.t[ly + 2 + dy][lx + 2 + dx]; }, var_a, var_b);
+        }
+        return;
+    }
     for (int i = tid; i < RD_X * RD_Y; i += 256) {
         const int x = x0 + i % RD_X, y = y0 + i / RD_X;
         if (x >= out.w || y >= out.h) continue;
-        const float4 d = despeckle_px(
-            [&](int dx, int dy) {
-                const int gx = clampi(x + dx, 0, out.w - 1), gy = clampi(y + dy, 0, out.h - 1);
-                return t[gy - ry0][gx - rx0];
-            },
-            var_a, var_b);
-        wrow_f4(out, y)[x] = d;
+        auto fetch = [&](int dx, int dy) {
+            const int gx = clampi(x + dx, 0, out.w - 1), gy = clampi(y + dy, 0, out.h - 1);
+            return sm.t[gy - ry0][gx - rx0];
+        };
+        wrow_f4(out, y)[x] = nonan ? despeckle_px<true>(fetch, var_a, var_b) : despeckle_px<false>(fetch, var_a, var_b);
     }
 }
 

@@ -230,3 +230,73 @@ def test_div25_equals_division(L):
     for lo, hi in ranges:
         for sign in (0, 0x80000000):
             assert L.om_div25_mismatches(C.c_uint32(lo | sign), C.c_uint32(hi | sign)) == 0, (hex(lo), sign)
+
+
+def _median9_network(q):
+    """The forgetful-selection network of fast_median3x3 (cl/demosaic.cl:384-413)
+    with the comparison-select min/max of the reference's macros."""
+    def s(a, b):
+        return (a if a < b else b), (a if a > b else b)
+    a0, a1, a2, a3, a4, a5 = q[:6]
+    a0, a1 = s(a0, a1); a2, a3 = s(a2, a3); a4, a5 = s(a4, a5); a0, a2 = s(a0, a2)
+    a1, a3 = s(a1, a3); a0, a4 = s(a0, a4); a3, a5 = s(a3, a5)
+    a0 = q[6]
+    a0, a1 = s(a0, a1); a2, a3 = s(a2, a3); a0, a2 = s(a0, a2); a1, a3 = s(a1, a3); a0, a4 = s(a0, a4)
+    a3, a4 = s(a3, a4)
+    a0 = q[7]
+    a0, a1 = s(a0, a1); a2, a3 = s(a2, a3); a0, a2 = s(a0, a2); a1, a3 = s(a1, a3)
+    a0 = q[8]
+    a0, a1 = s(a0, a1); a0, a2 = s(a0, a2); a1, a2 = s(a1, a2)
+    return a1
+
+
+def _med3(a, b, c):
+    return sorted((a, b, c))[1]
+
+
+def test_despeckle_nan_free_forms():
+    """The despeckle's NaN-free form (the fused red/blue + despeckle kernel takes
+    it for tiles without NaN): the median of nine as med3(max of row minima,
+    median of row medians, min of row maxima) over rows of three, and the
+    luma trackers as a running max/min plus a median of three, give the same
+    values as the reference's selection network and comparison trackers.
+    Exhaustive over {0..3}^9 (every order and tie pattern of up to four
+    levels) and over {-1, 0, 0.5, 1, 70000}^7 for the trackers (values on
+    both sides of the 0 and 65535 starting points), plus random reals."""
+    import itertools
+
+    def fast_median(q):
+        rows = [q[0:3], q[3:6], q[6:9]]
+        return _med3(max(min(r) for r in rows), _med3(*[_med3(*r) for r in rows]), min(max(r) for r in rows))
+
+    for q in itertools.product(range(4), repeat=9):
+        m = sorted(q)[4]
+        assert _median9_network(q) == m and fast_median(q) == m, q
+
+    def trackers_ref(vs):
+        fM = sM = 0.0
+        fm = sm = 65535.0
+        for v in vs:
+            sM = v if (v <= fM and v > sM) else sM
+            sM = fM if v > fM else sM
+            fM = v if v > fM else fM
+            sm = v if (v >= fm and v < sm) else sm
+            sm = fm if v < fm else sm
+            fm = v if v < fm else fm
+        return fM, sM, fm, sm
+
+    def trackers_fast(vs):
+        fM = sM = 0.0
+        fm = sm = 65535.0
+        for v in vs:
+            sM, fM = _med3(sM, v, fM), max(fM, v)
+            sm, fm = _med3(fm, v, sm), min(fm, v)
+        return fM, sM, fm, sm
+
+    for q in itertools.product([-1.0, 0.0, 0.5, 1.0, 70000.0], repeat=7):
+        assert trackers_ref(q) == trackers_fast(q), q
+    rng = np.random.default_rng(9)
+    for _ in range(20000):
+        q = list(rng.normal(size=9) * 3)
+        assert _median9_network(q) == fast_median(q) == sorted(q)[4]
+        assert trackers_ref(q) == trackers_fast(q)
