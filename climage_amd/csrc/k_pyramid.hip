@@ -225,16 +225,18 @@ void launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipS
 
 // --------------------------------------------------------------- denoiseImage
 // cl/demosaic.cl:735-765.  nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr.  The range
-// test length((s - c) / sigma) < edge is sum (s - c)^2 / sigma^2 < edge^2
-// (clim_bilateral_dist; the FP32 oracle evaluates the same expressions).
+// test length((s - c) / sigma) < edge is sum u_c^2 < edge^2 with
+// u_c = fma(s_c, r_c, -(c_c r_c)), r_c = 1 / sigma_c (clim_range_len2; the FP32
+// oracle evaluates the same expressions).
 // SAME_EDGE: gradientBoost <= 1 and the luma and chroma edges are equal, so
 // every tap's luma and chroma weights are equal (one test, one weight sum).
 template <bool SAME_EDGE>
-__device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int tx, int ty, float va0, float va1,
+__device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int tx, int ty, float va0, float va1,
                                                float va2, float vb0, float vb1, float vb2, float chromaBoost,
                                                float gradientBoost) {
     const int X = tx + HR, Y = ty + HR;
-    const float cY = t.x[Y][X], cCb = t.y[Y][X], cCr = t.z[Y][X];
+    const float4 cp = t.p[Y][X];
+    const float cY = cp.x, cCb = cp.y, cCr = cp.z;
     float lumaBoost = 1.0f;
     if (!SAME_EDGE && gradientBoost > 1.0f) {
         const float sig0 = sqrtf(va0 + vb0 * cY);
@@ -246,7 +248,7 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
         for (int j = 0; j < 3; j++)
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-                const float s = t.x[Y - 1 + j][X - 1 + i];
+                const float s = t.p[Y - 1 + j][X - 1 + i].x;
                 vx += kx[j][i] * s;
                 vy += ky[j][i] * s;
             }
@@ -258,17 +260,19 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
     float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
     e0 = e0 * e0;
     e1 = e1 * e1;
-    // 1 / sigma^2 per channel; the squared distance by clim_bilateral_dist
-    const float k0 = clim_inv_var(va0 + vb0 * cY), k1 = clim_inv_var(va1 + vb1 * cY),
-                k2 = clim_inv_var(va2 + vb2 * cY);
+    // 1 / sigma per channel; the squared distance by clim_range_len2
+    const float r0 = clim_inv_sigma(va0 + vb0 * cY), r1 = clim_inv_sigma(va1 + vb1 * cY),
+                r2 = clim_inv_sigma(va2 + vb2 * cY);
+    const float cr0 = cY * r0, cr1 = cCb * r1, cr2 = cCr * r2;
     float f0 = 0, f1 = 0, f2 = 0;
     int c0 = 0, c1 = 0;  // weight sums (the weights are 0 or 1)
 #pragma unroll
     for (int j = 0; j < 5; j++) {
 #pragma unroll
         for (int i = 0; i < 5; i++) {
-            const float a = t.x[ty + j][tx + i], b = t.y[ty + j][tx + i], c = t.z[ty + j][tx + i];
-            const float l = clim_bilateral_dist(a - cY, b - cCb, c - cCr, k0, k1, k2);
+            const float4 v = t.p[ty + j][tx + i];
+            const float a = v.x, b = v.y, c = v.z;
+            const float l = clim_range_len2(a, b, c, r0, r1, r2, cr0, cr1, cr2);
             // f += w * s with w in {0, 1}: the product is exact, so the fused
             // form rounds once on the same value (the same bits)
             const bool in0 = l < e0;
@@ -295,7 +299,7 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile<TX, TY>& t, int t
 template <bool SAME_EDGE>
 __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
                                                    float gradientBoost) {
-    __shared__ LevelTile<TX, TY> t;
+    __shared__ LevelTile4<TX, TY> t;
     const TileId tl = xcd_tile();
     const int x0 = tl.x * TX, y0 = tl.y * TY;
     t.load(in, x0, y0, threadIdx.y * 32 + threadIdx.x);

@@ -41,6 +41,19 @@ struct LevelTile {
     }
 };
 
+// The same tile with the texels interleaved: one 16-byte LDS read per tap
+// and constant offsets from one address for a whole 5x5 window.
+template <int TX, int TY>
+struct LevelTile4 {
+    static constexpr int SX = TX + 4, SY = TY + 4;
+    float4 p[SY][SX];
+    __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
+        stage<SX * SY, 256>(
+            tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
+            [&](int i, float4 v) { p[i / SX][i % SX] = v; });
+    }
+};
+
 // Horizontal 5-tap sums of the level tile for all SY rows: h[0..2] the
 // channel sums, h[3..5] the sums of squares (noiseStatistics, cl/demosaic.cl:1267-1286).
 template <int TX, int TY>

@@ -242,7 +242,38 @@ CLIM_HD float clim_green_tap_weight(float adl, float c1, float c0) {
  * of the pixel is zero, as there) */
 CLIM_HD float clim_inv_var(float s) { return s < 0.0f ? clim_u2f(0x7fc00000u) : 1.0f / s; }
 
-/* denoiseImage (cl/demosaic.cl:735-765) squared range distance
+/* 1 / sqrt(s) from integer operations, multiplications and fused
+ * multiply-adds only, so every IEEE machine computes the same bits: the
+ * bit-level estimate 0x5f375a86 - (bits(s) >> 1) (Lomont's constant for the
+ * classic estimate) refined by three Newton steps y <- y (3/2 - (s/2) y^2).
+ * Within 2.5 ulp of 1/sqrt(s) for s in [2^-124, 2^124] (exhaustive over
+ * [1, 4); multiplying s by 4 halves every intermediate exactly, so that
+ * binade pair decides the others while no intermediate leaves the normal
+ * range), within 4 ulp for every normal s; subnormal s get a coarser
+ * estimate.  NaN for s < 0 and NaN, +inf for s = 0, 0 for s = +inf. */
+CLIM_HD float clim_rsqrt(float s) {
+    float y = clim_u2f(0x5f375a86u - (clim_f2u(s) >> 1));
+    const float h = 0.5f * s;
+    for (int i = 0; i < 3; i++) y = y * fmaf(-h, y * y, 1.5f);
+    if (!(s > 0.0f)) return s == 0.0f ? clim_u2f(0x7f800000u) : clim_u2f(0x7fc00000u);
+    return s == clim_u2f(0x7f800000u) ? 0.0f : y;
+}
+
+/* denoiseImage (cl/demosaic.cl:735-765) range distance |(s - c) / sigma|^2
+ * in the kernels' form: u_c = (s_c - c_c) / sigma_c as one fused step
+ * fma(s_c, r_c, -cr_c), with r_c = 1 / sigma_c = clim_inv_sigma(a_c + b_c Y)
+ * (clim_rsqrt) and cr_c = c_c r_c per centre pixel, then the squared length
+ * in two fused steps.  A zero variance gives r = inf and a negative one NaN:
+ * every tap's distance is then NaN or inf (as the literal form's), so no tap
+ * has weight. */
+CLIM_HD float clim_inv_sigma(float s) { return clim_rsqrt(s); }
+CLIM_HD float clim_range_len2(float s0, float s1, float s2, float r0, float r1, float r2, float cr0, float cr1,
+                              float cr2) {
+    const float u0 = fmaf(s0, r0, -cr0), u1 = fmaf(s1, r1, -cr1), u2 = fmaf(s2, r2, -cr2);
+    return fmaf(u2, u2, fmaf(u1, u1, u0 * u0));
+}
+
+/* The squared range distance
  * |(s - c) / sigma|^2 as sum d_c^2 k_c, k_c = clim_inv_var(a_c + b_c Y):
  * no square root, two fused steps. */
 CLIM_HD float clim_bilateral_dist(float d0, float d1, float d2, float k0, float k1, float k2) {

@@ -752,9 +752,13 @@ void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], co
 #if ORACLE_DOUBLE
             const real isig[3] = {K(1.0) / sigma[0], K(1.0) / sigma[1], K(1.0) / sigma[2]};
 #else
-            /* k_c = 1 / sigma_c^2, the distance by clim_bilateral_dist */
-            real kk[3];
-            for (int c = 0; c < 3; c++) kk[c] = clim_inv_var((real)var_a[c] + (real)var_b[c] * cY);
+            /* r_c = 1 / sigma_c, the distance by clim_range_len2 */
+            real rr[3], cr[3];
+            const real cc[3] = {cY, cCb, cCr};
+            for (int c = 0; c < 3; c++) {
+                rr[c] = clim_inv_sigma((real)var_a[c] + (real)var_b[c] * cY);
+                cr[c] = cc[c] * rr[c];
+            }
 #endif
             real fp[3] = {0, 0, 0}, kn[3] = {0, 0, 0};
             for (int j = -2; j <= 2; j++) {
@@ -766,7 +770,7 @@ void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], co
                     real d2 = (s[2] - cCr) * isig[2];
                     real l = d0 * d0 + d1 * d1 + d2 * d2;
 #else
-                    real l = clim_bilateral_dist(s[0] - cY, s[1] - cCb, s[2] - cCr, kk[0], kk[1], kk[2]);
+                    real l = clim_range_len2(s[0], s[1], s[2], rr[0], rr[1], rr[2], cr[0], cr[1], cr[2]);
 #endif
                     for (int c = 0; c < 3; c++) {
                         real wt = K(1.0) - (l < edge[c] ? K(0.0) : K(1.0));

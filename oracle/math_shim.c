@@ -86,3 +86,19 @@ long om_div25_mismatches(uint32_t lo, uint32_t hi) {
     }
     return bad;
 }
+
+/* the largest |clim_rsqrt(s) - 1/sqrt(s)| / (1/sqrt(s)) over the bit patterns
+ * [lo, hi) (positive finite floats), the exact value in double precision */
+double om_rsqrt_max_rel_err(uint32_t lo, uint32_t hi) {
+    double worst = 0.0;
+#pragma omp parallel for reduction(max : worst) schedule(static)
+    for (int64_t i = lo; i < (int64_t)hi; i++) {
+        const float s = clim_u2f((uint32_t)i);
+        const double ex = 1.0 / sqrt((double)s);
+        const double e = fabs((double)clim_rsqrt(s) - ex) / ex;
+        if (e > worst) worst = e;
+    }
+    return worst;
+}
+
+float om_rsqrt(float s) { return clim_rsqrt(s); }

@@ -300,3 +300,26 @@ def test_despeckle_nan_free_forms():
         q = list(rng.normal(size=9) * 3)
         assert _median9_network(q) == fast_median(q) == sorted(q)[4]
         assert trackers_ref(q) == trackers_fast(q)
+
+
+def test_rsqrt_accuracy_and_special_values(L):
+    """clim_rsqrt (the denoise's 1/sigma): within 2.5 ulp of 1/sqrt(s) over the
+    binade pair [1, 4), which decides every s in [2^-124, 2^124] (s -> 4s
+    halves every intermediate exactly), within 4 ulp over the binades at both
+    ends of the normal range; NaN for s < 0 and NaN, +inf at zero, 0 at +inf."""
+    L.om_rsqrt_max_rel_err.restype = C.c_double
+    L.om_rsqrt_max_rel_err.argtypes = [C.c_uint32, C.c_uint32]
+    L.om_rsqrt.restype = C.c_float
+    L.om_rsqrt.argtypes = [C.c_float]
+    ulp = 2.0 ** -24
+    assert L.om_rsqrt_max_rel_err(0x3F800000, 0x40800000) <= 2.5 * ulp
+    for lo in (0x01800000, 0x20000000, 0x5F000000, 0x7D800000):
+        assert L.om_rsqrt_max_rel_err(lo, lo + (1 << 20)) <= 2.5 * ulp, hex(lo)
+    assert L.om_rsqrt_max_rel_err(0x00800000, 0x01800000) <= 4.0 * ulp
+    assert L.om_rsqrt_max_rel_err(0x7E000000, 0x7F800000) <= 4.0 * ulp
+    # the period: s and 4 s give results exactly a factor 2 apart
+    for s in (1.0, 1.37, 2.9, 3.999):
+        a, b = L.om_rsqrt(s), L.om_rsqrt(4.0 * s)
+        assert np.float32(a) == np.float32(2.0) * np.float32(b)
+    assert np.isnan(L.om_rsqrt(-1.0)) and np.isnan(L.om_rsqrt(float("nan")))
+    assert L.om_rsqrt(0.0) == float("inf") and L.om_rsqrt(float("inf")) == 0.0
