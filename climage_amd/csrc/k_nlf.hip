@@ -17,9 +17,11 @@
 // keeps them in 64- and 96-bit accumulators; blocks combine 128-bit sums, so
 // the totals do not depend on the reduction order and equal the CPU oracle's.
 // Each pass ends with a reduce kernel (one block per job and sum) that adds
-// the blocks' partial sums, and a solve kernel (one thread per job) in FP64.  The kernel boundary makes the
-// partials visible; a device-scope fence per block would write back and
-// invalidate the L2 of its XCD every time.  One launch serves several jobs
+// the blocks' partial sums; its last block solves every job in FP64 (one
+// thread per job).  The kernel boundary makes the statistics kernels'
+// partials visible (a device-scope fence per statistics block would write
+// back the L2 of its XCD, full of the statistics images, every time); the
+// reduce blocks' fences cover a few totals only.  One launch serves several jobs
 // (all pyramid levels); each block takes an even share of one job.
 #include "fit_acc.h"
 #include "kernels.h"
@@ -91,46 +93,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     store_block_partials(acc, buf, tot, jb.partials, bi, kFitBlocksPerJob);
 }
 
-// One block per (job, sum): adds the job's block partials of that sum (four
-// per thread, plain loads of clamped indexes, masked, all in flight at once)
-// into the job's FitState::tot.
+// The least-squares solve of one job from its totals.
 template <bool RAW, bool REFIT>
-__global__ void __launch_bounds__(256) k_fit_reduce(FitJobs jobs) {
-    constexpr int K = Layout<RAW ? 4 : 3, !RAW, REFIT>::K;
-    static_assert(kFitBlocksPerJob <= 4 * 256, "four partials per thread");
-    static_assert(K <= kFitSums, "FitState::tot holds every sum");
-    __shared__ clim_fx128 wsum[4];
-    const int j = blockIdx.x / K, k = blockIdx.x % K;
-    const FitJob& jb = jobs.j[j];
-    FitState* st = jb.st;
-    if (REFIT && st->status == 1) return;  // no sample in range: fallback already written
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const clim_fx128* pk = jb.partials + (size_t)k * kFitBlocksPerJob;
-    clim_fx128 v[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) v[i] = pk[min(tid + 256 * i, jb.nblocks - 1)];
-    clim_fx128 s = {0, 0};
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const bool mine = tid + 256 * i < jb.nblocks;
-        v[i].hi = mine ? v[i].hi : 0;
-        v[i].lo = mine ? v[i].lo : 0;
-        s = clim_fx_add(s, v[i]);
-    }
-    s = wave_sum_dpp(s);
-    if (lane == 63) wsum[wave] = s;
-    __syncthreads();
-    if (tid == 0) st->tot[k] = clim_fx_add(clim_fx_add(wsum[0], wsum[1]), clim_fx_add(wsum[2], wsum[3]));
-}
-
-// One thread per job: the least-squares solve from the job's totals.
-template <bool RAW, bool REFIT>
-__global__ void __launch_bounds__(64) k_fit_solve(FitJobs jobs) {
+__device__ void fit_solve_job(const FitJob& jb) {
     constexpr int NCH = RAW ? 4 : 3;
     constexpr bool SHARED_M = !RAW;
     using L = Layout<NCH, SHARED_M, REFIT>;
-    if ((int)threadIdx.x >= jobs.njobs) return;
-    const FitJob& jb = jobs.j[threadIdx.x];
     FitState* st = jb.st;
     if (REFIT && st->status == 1) return;
     clim_fx128 tot[L::K];
@@ -180,12 +148,59 @@ __global__ void __launch_bounds__(64) k_fit_solve(FitJobs jobs) {
     }
 }
 
+// One block per (job, sum): adds the job's block partials of that sum (four
+// per thread, plain loads of clamped indexes, masked, all in flight at once)
+// into the job's FitState::tot.  The last block of the launch to finish
+// (a device-scope counter in the first job's FitState, released and acquired
+// around the atomic increment, reset by that block) then solves every job,
+// one thread per job.
+template <bool RAW, bool REFIT>
+__global__ void __launch_bounds__(256) k_fit_reduce(FitJobs jobs) {
+    constexpr int K = Layout<RAW ? 4 : 3, !RAW, REFIT>::K;
+    static_assert(kFitBlocksPerJob <= 4 * 256, "four partials per thread");
+    static_assert(K <= kFitSums, "FitState::tot holds every sum");
+    __shared__ clim_fx128 wsum[4];
+    __shared__ int last;
+    const int j = blockIdx.x / K, k = blockIdx.x % K;
+    const FitJob& jb = jobs.j[j];
+    FitState* st = jb.st;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // no sample in range: the fallback is already written, nothing to add
+    if (!(REFIT && st->status == 1)) {
+        const clim_fx128* pk = jb.partials + (size_t)k * kFitBlocksPerJob;
+        clim_fx128 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) v[i] = pk[min(tid + 256 * i, jb.nblocks - 1)];
+        clim_fx128 s = {0, 0};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const bool mine = tid + 256 * i < jb.nblocks;
+            v[i].hi = mine ? v[i].hi : 0;
+            v[i].lo = mine ? v[i].lo : 0;
+            s = clim_fx_add(s, v[i]);
+        }
+        s = wave_sum_dpp(s);
+        if (lane == 63) wsum[wave] = s;
+        __syncthreads();
+        if (tid == 0) st->tot[k] = clim_fx_add(clim_fx_add(wsum[0], wsum[1]), clim_fx_add(wsum[2], wsum[3]));
+    }
+    if (tid == 0) {
+        __threadfence();  // this block's total before the count
+        unsigned int* done = &jobs.j[0].st->reduced;
+        last = atomicAdd(done, 1u) == (unsigned int)(jobs.njobs * K) - 1u;
+        if (last) *done = 0u;  // every other block has counted
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // every block's total after the count
+    if (tid < jobs.njobs) fit_solve_job<RAW, REFIT>(jobs.j[tid]);
+}
+
 // the combination of a pass's block partials and the solve
 template <bool RAW, bool REFIT>
 static void launch_finalize(const FitJobs& jobs, hipStream_t st) {
     constexpr int K = Layout<RAW ? 4 : 3, !RAW, REFIT>::K;
     hipLaunchKernelGGL((k_fit_reduce<RAW, REFIT>), dim3(jobs.njobs * K), dim3(256), 0, st, jobs);
-    hipLaunchKernelGGL((k_fit_solve<RAW, REFIT>), dim3(1), dim3(64), 0, st, jobs);
 }
 
 int fit_share_blocks(FitJobs& jobs, const long long* units, int grid) {

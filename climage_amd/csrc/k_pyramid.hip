@@ -627,9 +627,12 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
     const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
     const float4 p0 = row_f4(d0, y)[x];
     float4 a1, b1;
+    // exact 2x: the taps of the samples at px, py (i0, j0; weights a, b); those
+    // at px +- nx, py +- ny (the sharpening gradient) sit at texels x/2 and
+    // x/2 - 1 (y/2, y/2 - 1), rounded down, with weight 1 - a (1 - b)
+    const int i0 = (x & 1) ? (x - 1) >> 1 : (x >> 1) - 1, j0 = (y & 1) ? (y - 1) >> 1 : (y >> 1) - 1;
+    const float a = (x & 1) ? 0.25f : 0.75f, b = (y & 1) ? 0.25f : 0.75f;
     if (EXACT2X) {
-        const int i0 = (x & 1) ? (x - 1) >> 1 : (x >> 1) - 1, j0 = (y & 1) ? (y - 1) >> 1 : (y >> 1) - 1;
-        const float a = (x & 1) ? 0.25f : 0.75f, b = (y & 1) ? 0.25f : 0.75f;
         a1 = bilinear4_at(n1, i0, j0, a, b);
         b1 = bilinear4_at(d1, i0, j0, a, b);
     } else {
@@ -643,8 +646,16 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
     d.w = p0.w - (a1.w - b1.w);
     float sharpening = sharpening0;
     if (sharpening > 1.0f) {
-        const float dx = (bilinear4(d1, px + nx, py).x - bilinear4(d1, px - nx, py).x) / 2.0f;
-        const float dy = (bilinear4(d1, px, py + ny).x - bilinear4(d1, px, py - ny).x) / 2.0f;
+        float dx, dy;
+        if (EXACT2X) {
+            const float ax = 1.0f - a, by = 1.0f - b;
+            const int xh = x >> 1, yh = y >> 1;
+            dx = (bilinear4_at(d1, xh, j0, ax, b).x - bilinear4_at(d1, xh - 1, j0, ax, b).x) / 2.0f;
+            dy = (bilinear4_at(d1, i0, yh, a, by).x - bilinear4_at(d1, i0, yh - 1, a, by).x) / 2.0f;
+        } else {
+            dx = (bilinear4(d1, px + nx, py).x - bilinear4(d1, px - nx, py).x) / 2.0f;
+            dy = (bilinear4(d1, px, py + ny).x - bilinear4(d1, px, py - ny).x) / 2.0f;
+        }
         const float threshold = 0.25f * sqrtf(nlf.get(0) + nlf.get(3) * b1.x);
         const float detail = smoothstep(0.25f * threshold, threshold, len2(dx, dy)) *
                              (1.0f - smoothstep(0.95f, 1.0f, d.x)) * (0.6f + 0.4f * smoothstep(0.0f, 0.1f, d.x));

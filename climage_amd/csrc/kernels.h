@@ -15,6 +15,7 @@ struct FitState {
     int status;            // 0 fitted, 1 no sample in range (fallback), 2 no inliers (first fit)
     float out[8];          // effective model: A[0..nch), B[0..nch)
     float cal[8];          // out * mult (calibrated per-level model)
+    unsigned int reduced;  // blocks of the running combination that are done (0 between launches)
 };
 
 constexpr int kFitBlocksPerJob = 1024;
