@@ -527,7 +527,8 @@ def _fixed_bilinear32(img, i0, j0, a, b):
 def test_sampler_exact_2x_ratios(o32):
     """At exact 2x ratios the linear sampler's float32 coordinate arithmetic
     puts every tap on a fixed texel with a fixed weight, for every output
-    size up to 6144 (downsample) and 16384 (upsample in reassemble); the HIP
+    size up to 6144 (downsample) and 16384 (upsample in reassemble, the
+    sharpening gradient's four samples included); the HIP
     kernels use those taps directly within these limits.  Checked
     exhaustively, and through the FP32 oracle at sizes up to the limits."""
     f = np.float32
@@ -548,6 +549,16 @@ def test_sampler_exact_2x_ratios(o32):
         fu, a = weight((x.astype(np.float32) + f(0.5)) * (f(1) / f(w)) * f(w // 2) - f(0.5))
         assert np.all(a == np.where(x % 2 == 0, f(0.75), f(0.25))), w
         assert np.all(fu == np.where(x % 2 == 0, x // 2 - 1, (x - 1) // 2)), w
+    # the sharpening gradient's samples at px +- 1/w (and py +- 1/h): texels
+    # floor(x/2) and floor(x/2) - 1 with weight 1 - a
+    for w in range(2, 16385, 2):
+        x = np.arange(w)
+        nx = f(1) / f(w)
+        px = (x.astype(np.float32) + f(0.5)) * nx
+        a_side = np.where(x % 2 == 0, f(0.25), f(0.75))
+        for t, i in ((px + nx, x // 2), (px - nx, x // 2 - 1)):
+            fu, a = weight(t * f(w // 2) - f(0.5))
+            assert np.all(a == a_side) and np.all(fu == i), w
     # the limit is tight: 6145 is the first downsample size that breaks it
     x = np.arange(6145, dtype=np.float32)
     nx = f(1) / f(6145)
