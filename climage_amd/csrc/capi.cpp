@@ -2,6 +2,7 @@
 // (include/climage.h).  The pipeline entry points are in pipeline.cpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -147,6 +148,7 @@ extern "C" clim_status clim_create(clim_ctx** out, int device) {
     if (e == hipSuccess) e = hipMalloc(&ctx->fit, sizeof(clim::FitState) * kFitJobs);
     if (e == hipSuccess) e = hipMemset(ctx->fit, 0, sizeof(clim::FitState) * kFitJobs);
     if (e == hipSuccess) e = hipMalloc(&ctx->partials, sizeof(clim_fx128) * kFitJobs * clim::kFitBlocksPerJob * kFitK);
+    if (e == hipSuccess) e = hipMalloc(&ctx->fit_counts, sizeof(unsigned int) * kFitJobs * clim::kFitBlocksPerJob);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->fit_host, sizeof(clim::FitState) * kFitJobs, 0);
     if (e == hipSuccess) e = hipMalloc(&ctx->scene, sizeof(clim::SceneParams));
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->scene_host, sizeof(clim::SceneParams), 0);
@@ -181,6 +183,7 @@ extern "C" void clim_destroy(clim_ctx* ctx) {
     owned_free(ctx->op_var);
     owned_free(ctx->op_stats);
     if (ctx->fit) (void)hipFree(ctx->fit);
+    if (ctx->fit_counts) (void)hipFree(ctx->fit_counts);
     if (ctx->partials) (void)hipFree(ctx->partials);
     if (ctx->fit_host) (void)hipHostFree(ctx->fit_host);
     if (ctx->scene) (void)hipFree(ctx->scene);
@@ -287,12 +290,19 @@ static clim::MosaicArg scaled_mosaic(const clim_image* scaled, int bayer) {
 // (raw fits: smul set reads the jobs' src as the uint16 mosaic scaled with
 // (smul, black), null as the scaled FP32 mosaic)
 clim_status run_stats_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw, const float* smul, float black) {
-    if (raw)
-        clim::launch_raw_stats_fit(jobs, smul, black, ctx->stream);
-    else
-        clim::launch_noise_stats_fit(jobs, 0, jobs.njobs, nullptr, ctx->stream);
+    const bool ok = raw ? clim::launch_raw_stats_fit(jobs, smul, black, ctx->stream)
+                        : clim::launch_noise_stats_fit(jobs, 0, jobs.njobs, nullptr, ctx->stream);
+    if (!ok) return ctx_fail(ctx, CLIM_EINVAL, "noise statistics: sample list smaller than the image");
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
+}
+
+// a sample-list buffer (an RGBA32F allocation) for the statistics of a w x h
+// level (raw: of a w x h mosaic)
+clim_status fit_list_alloc(clim_ctx* ctx, OwnedImg& o, bool raw, int w, int h) {
+    int aw = 0, ah = 0;
+    clim::fit_list_dims(raw, w, h, &aw, &ah);
+    return owned_alloc(ctx, o, aw, ah, CLIM_FMT_RGBA32F);
 }
 
 // the rest of the regression: finalisation, refit, finalisation
@@ -310,6 +320,8 @@ clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& src, const Img& m,
     jb.v = v;
     jb.st = ctx->fit + slot;
     jb.partials = ctx->partials + (size_t)slot * clim::kFitBlocksPerJob * kFitK;
+    jb.counts = ctx->fit_counts + (size_t)slot * clim::kFitBlocksPerJob;
+    jb.cap = std::min((long long)m.pitch * m.h, (long long)v.pitch * v.h) / 16;
     for (int k = 0; k < 8; k++) {
         jb.nlf_in[k] = nlf_in ? nlf_in[k] : 0.0f;
         jb.mult[k] = mult ? mult[k] : 1.0f;
@@ -331,10 +343,9 @@ extern "C" clim_status clim_compute_raw_noise_statistics(clim_ctx* ctx, const cl
     CLIM_TRY(check_img(ctx, scaled, CLIM_FMT_R32F, "scaled"));
     if ((scaled->width & 1) || (scaled->height & 1))
         return ctx_fail(ctx, CLIM_EINVAL, "computeRawNoiseStatistics: odd mosaic size");
-    // rawNoiseStatistics into the context's mean/var images, then the fit
-    const int qw = scaled->width / 2, qh = scaled->height / 2;
-    CLIM_TRY(owned_alloc(ctx, ctx->op_mean, qw, qh, CLIM_FMT_RGBA32F));
-    CLIM_TRY(owned_alloc(ctx, ctx->op_var, qw, qh, CLIM_FMT_RGBA32F));
+    // rawNoiseStatistics into the context's mean/var sample lists, then the fit
+    CLIM_TRY(fit_list_alloc(ctx, ctx->op_mean, true, scaled->width, scaled->height));
+    CLIM_TRY(fit_list_alloc(ctx, ctx->op_var, true, scaled->width, scaled->height));
     clim::FitJobs jobs{};
     jobs.njobs = 1;
     jobs.j[0] = make_fit_job(ctx, 0, view(scaled), ctx->op_mean.img, ctx->op_var.img, nullptr, nullptr);
@@ -496,7 +507,7 @@ extern "C" clim_status clim_noise_statistics(clim_ctx* ctx, const clim_image* in
 extern "C" clim_status clim_compute_noise_statistics(clim_ctx* ctx, const clim_image* in, float nlf_out[6]) {
     if (!ctx || !nlf_out) return ctx_fail(ctx, CLIM_EINVAL, "computeNoiseStatistics: null argument");
     CLIM_TRY(check_img(ctx, in, CLIM_FMT_RGBA32F, "in"));
-    CLIM_TRY(owned_alloc(ctx, ctx->op_stats, in->width, in->height, CLIM_FMT_RGBA32F));
+    CLIM_TRY(fit_list_alloc(ctx, ctx->op_stats, false, in->width, in->height));
     clim::FitJobs jobs{};
     jobs.njobs = 1;
     jobs.j[0] = make_fit_job(ctx, 1, view(in), ctx->op_stats.img, ctx->op_stats.img, nullptr, nullptr);

@@ -50,6 +50,7 @@ struct clim_ctx {
     Workspace ws;
     clim::FitState* fit = nullptr;            // device [kFitJobs]
     clim_fx128* partials = nullptr;           // device [kFitJobs][kFitBlocksPerJob][kFitK]
+    unsigned int* fit_counts = nullptr;       // device [kFitJobs][kFitBlocksPerJob] samples listed per block
     clim::FitState* fit_host = nullptr;       // pinned mirror for read-back
     clim::SceneParams* scene = nullptr;       // device
     clim::SceneParams* scene_host = nullptr;  // pinned staging

@@ -120,14 +120,21 @@ struct Acc {
     // <= 0.001, which is the same test for float inputs.  Terms of samples
     // that fail it (or, when refitting, are outliers) are zero, so every lane
     // of a wave runs the same instructions.
-    __device__ __forceinline__ void add(const float* m, const float* v, bool inside) {
+    __device__ __forceinline__ static bool in_range(const float* m, const float* v) {
         constexpr float kMeanMin = 0x1.0624dep-10f, kVarMax = 0x1.0624dcp-10f;
-        bool ok = inside;
+        bool ok = true;
 #pragma unroll
         for (int c = 0; c < NCH; c++) {
             const float mm = m[SHARED_M ? 0 : c], vv = v[c];
             ok = ok && mm >= kMeanMin && mm <= 0.5f && vv >= 0.0f && vv <= kVarMax;
         }
+        return ok;
+    }
+    __device__ __forceinline__ void add(const float* m, const float* v, bool inside) {
+        add_tested(m, v, inside && in_range(m, v));
+    }
+    // a sample whose range test (and `inside`) gave `ok`
+    __device__ __forceinline__ void add_tested(const float* m, const float* v, bool ok) {
         if (REFIT) {
 #pragma unroll
             for (int c = 0; c < NCH; c++) {
@@ -182,6 +189,19 @@ __device__ __forceinline__ void store_block_partials(const Acc<NCH, SHARED_M, RE
     acc.store(sums);
     block_sum<K>(sums, buf, tot);
     if ((int)threadIdx.x < K) partials[(size_t)threadIdx.x * stride + bi] = tot[threadIdx.x];
+}
+
+// The slot of this lane's sample in the block's run of a sample list: the
+// wave's lanes with `ok` take consecutive slots after the block's LDS counter
+// (every lane of the wave calls this; the slots of different waves do not
+// follow a fixed order, which the exact sums do not depend on).
+__device__ __forceinline__ int list_slot(bool ok, int* count) {
+    const unsigned long long mask = __ballot(ok);
+    const int lane = __lane_id();
+    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+    int base = 0;
+    if (lane == 0 && mask) base = atomicAdd(count, __popcll(mask));
+    return __shfl(base, 0) + rank;
 }
 
 // the job of block b: jobs own consecutive runs of blocks starting at block_base

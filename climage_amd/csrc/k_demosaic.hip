@@ -96,13 +96,15 @@ struct QuadFetch<false> {
 // rawNoiseStatistics with the first regression pass of
 // computeRawNoiseStatistics accumulated on the way: each of the grid's blocks
 // takes a contiguous run of 16x16-quad tiles and keeps its regression sums
-// across them (one partial per block).
+// across them (one partial per block); the quads in range are listed for the
+// refit.
 template <bool U16>
 __global__ void __launch_bounds__(256) k_raw_stats_fit(FitJobs jobs, float4 smul, float black) {
     constexpr int T = 16, R = 2, S = T + 2 * R;
     using L = Layout<4, false, false>;
     __shared__ QuadTile<T> sm;
     __shared__ clim_fx128 tot[L::K];
+    __shared__ int nlisted;
     static_assert(sizeof(QuadTile<T>) >= 1024 * sizeof(clim_fx128), "the tile storage serves as the reduction buffer");
     const int lb = xcd_block();
     const FitJob& jb = jobs.j[job_of_block(jobs, lb)];
@@ -110,9 +112,15 @@ __global__ void __launch_bounds__(256) k_raw_stats_fit(FitJobs jobs, float4 smul
     const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
     const Bayer& b = jb.b;
     const QuadFetch<U16> src{{jb.src, b, smul, black}};
-    const int tiles_x = (jb.m.w + T - 1) / T;
-    const long long ntiles = (long long)tiles_x * ((jb.m.h + T - 1) / T);
+    const int qw = jb.src.w / 2, qh = jb.src.h / 2;  // quads
+    const int tiles_x = (qw + T - 1) / T;
+    const long long ntiles = jb.ntiles;
     const int t0 = (int)(ntiles * bi / jb.nblocks), t1 = (int)(ntiles * (bi + 1) / jb.nblocks);
+    // this block's runs of the sample lists (the first barrier of the loop
+    // orders the counter's reset before its first use)
+    float4* lm = reinterpret_cast<float4*>(jb.m.p) + (size_t)t0 * (T * T);
+    float4* lv = reinterpret_cast<float4*>(jb.v.p) + (size_t)t0 * (T * T);
+    if (tid == 0) nlisted = 0;
     Acc<4, false, false> acc;
     acc.zero();
     // the next tile's loads are issued before the current tile's compute
@@ -132,29 +140,38 @@ __global__ void __launch_bounds__(256) k_raw_stats_fit(FitJobs jobs, float4 smul
         sm.hsums(tid);
         __syncthreads();
         const int qx = qx0 + tx, qy = qy0 + ty;
-        const bool inside = qx < jb.m.w && qy < jb.m.h;
         float m[4], v[4];
         sm.stats(tx, ty, m, v);
-        if (inside) {
-            wrow_f4(jb.m, qy)[qx] = make_float4(m[0], m[1], m[2], m[3]);
-            wrow_f4(jb.v, qy)[qx] = make_float4(v[0], v[1], v[2], v[3]);
+        const bool ok = qx < qw && qy < qh && Acc<4, false, false>::in_range(m, v);
+        const int slot = list_slot(ok, &nlisted);
+        if (ok) {
+            lm[slot] = make_float4(m[0], m[1], m[2], m[3]);
+            lv[slot] = make_float4(v[0], v[1], v[2], v[3]);
         }
-        acc.add(m, v, inside);
+        acc.add_tested(m, v, ok);
         __syncthreads();  // the next tile reuses the LDS
     }
     // the last tile is done (the loop ends with a barrier): its LDS is free
+    if (tid == 0) jb.counts[bi] = (unsigned int)nlisted;
     store_block_partials(acc, reinterpret_cast<clim_fx128*>(&sm), tot, jb.partials, bi, kFitBlocksPerJob);
 }
 
-void launch_raw_stats_fit(FitJobs& jobs, const float* smul, float black, hipStream_t st) {
+bool launch_raw_stats_fit(FitJobs& jobs, const float* smul, float black, hipStream_t st) {
     long long tiles[5];
-    for (int j = 0; j < jobs.njobs; j++) tiles[j] = (long long)((jobs.j[j].m.w + 15) / 16) * ((jobs.j[j].m.h + 15) / 16);
+    for (int j = 0; j < jobs.njobs; j++) {
+        FitJob& jb = jobs.j[j];
+        jb.ntiles = ((jb.src.w / 2 + 15) / 16) * ((jb.src.h / 2 + 15) / 16);
+        jb.seg = 16 * 16;
+        if ((long long)jb.ntiles * jb.seg > jb.cap) return false;
+        tiles[j] = jb.ntiles;
+    }
     const int grid = fit_share_blocks(jobs, tiles, 1024);
     if (smul)
         hipLaunchKernelGGL(k_raw_stats_fit<true>, dim3(grid), dim3(256), 0, st, jobs,
                            make_float4(smul[0], smul[1], smul[2], smul[3]), black);
     else
         hipLaunchKernelGGL(k_raw_stats_fit<false>, dim3(grid), dim3(256), 0, st, jobs, make_float4(0, 0, 0, 0), 0.0f);
+    return true;
 }
 
 // --------------------------------------------------------- interpolateGreen

@@ -12,7 +12,8 @@
 //           accumulate these sums as they compute the statistics
 //           (k_raw_stats_fit, k_noise_stats_fit).
 //   refit   the same sums over inliers (residual^2 <= err2 in every
-//           channel) -> A, B, streaming over the statistics images (k_fit).
+//           channel) -> A, B, streaming over the samples the statistics
+//           kernels listed (those in range; k_refit).
 // The grid terms are 32-bit integers, their products 64-bit, and each thread
 // keeps them in 64- and 96-bit accumulators; blocks combine 128-bit sums, so
 // the totals do not depend on the reduction order and equal the CPU oracle's.
@@ -28,66 +29,56 @@
 
 namespace clim {
 
-// RAW: mean and var images of (R, G, B, G2) quads; level: one statistics
-// image (mean Y, var Y, var Cb, var Cr)
-template <bool RAW, bool REFIT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fit(FitJobs jobs) {
+// The refit: the regression sums over the listed samples (those in range)
+// that are inliers of the first fit.  Block bi of a job takes the run the
+// statistics kernel's block bi listed (the same block shares), U samples per
+// thread in flight (plain loads of clamped slots, masked).  RAW: the mean and
+// var lists of (R, G, B, G2) quads; level: one list of (mean Y, var Y,
+// var Cb, var Cr).
+template <bool RAW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_refit(FitJobs jobs) {
     constexpr int NCH = RAW ? 4 : 3;
     constexpr bool SHARED_M = !RAW;
-    using L = Layout<NCH, SHARED_M, REFIT>;
-    using AccT = Acc<NCH, SHARED_M, REFIT>;
+    using L = Layout<NCH, SHARED_M, true>;
+    using AccT = Acc<NCH, SHARED_M, true>;
     constexpr int K = L::K;
-    constexpr int U = RAW ? 2 : 4;  // samples in flight per thread (chunks in launch_fit_rest)
+    constexpr int U = RAW ? 2 : 4;
     __shared__ clim_fx128 buf[1024], tot[K];
 
     const FitJob& jb = jobs.j[job_of_block(jobs, blockIdx.x)];
     FitState* st = jb.st;
-    if (REFIT && st->status == 1) return;  // no sample in range: fallback already written
+    if (st->status == 1) return;  // no sample in range: fallback already written
     const int bi = blockIdx.x - jb.block_base;
     const int tid = threadIdx.x;
 
     AccT acc;
     acc.zero();
-    if (REFIT) {
 #pragma unroll
-        for (int c = 0; c < NCH; c++) { acc.A[c] = st->A[c]; acc.B[c] = st->B[c]; acc.e2[c] = st->err2[c]; }
-    }
-    // This block's share of the job: chunks [c0, c1) of 256*U consecutive
-    // samples of a row (cpr chunks per row), U samples per thread per chunk.
-    constexpr int CH = 256 * U;
-    const int w = jb.m.w, cpr = (w + CH - 1) / CH;
-    const long long nchunks = (long long)jb.m.h * cpr;
-    const int c0 = (int)(nchunks * bi / jb.nblocks), c1 = (int)(nchunks * (bi + 1) / jb.nblocks);
-    int y = c0 / cpr, xb = (c0 - y * cpr) * CH;
-    for (int c = c0; c < c1; c++) {
-        const float4* mrow = row_f4(jb.m, y);
-        const float4* vrow = row_f4(jb.v, y);
-        // unconditional loads of clamped columns (no branch, so all U are in
-        // flight together); columns past the row end are masked
+    for (int c = 0; c < NCH; c++) { acc.A[c] = st->A[c]; acc.B[c] = st->B[c]; acc.e2[c] = st->err2[c]; }
+    const long long t0 = (long long)jb.ntiles * bi / jb.nblocks;
+    const float4* lm = reinterpret_cast<const float4*>(jb.m.p) + (size_t)t0 * jb.seg;
+    const float4* lv = reinterpret_cast<const float4*>(jb.v.p) + (size_t)t0 * jb.seg;
+    const int n = (int)jb.counts[bi];
+    for (int i0 = 0; i0 < n; i0 += 256 * U) {
         float4 pm[U], pv[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const int x = min(xb + tid + 256 * u, w - 1);
-            pm[u] = mrow[x];
-            if (RAW) pv[u] = vrow[x];
+            const int i = min(i0 + tid + 256 * u, n - 1);
+            pm[u] = lm[i];
+            if (RAW) pv[u] = lv[i];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const bool inside = xb + tid + 256 * u < w;
+            const bool mine = i0 + tid + 256 * u < n;
             if (RAW) {
                 const float m[4] = {pm[u].x, pm[u].y, pm[u].z, pm[u].w};
                 const float v[4] = {pv[u].x, pv[u].y, pv[u].z, pv[u].w};
-                acc.add(m, v, inside);
+                acc.add(m, v, mine);
             } else {
                 const float m[1] = {pm[u].x};
                 const float v[3] = {pm[u].y, pm[u].z, pm[u].w};
-                acc.add(m, v, inside);
+                acc.add(m, v, mine);
             }
-        }
-        xb += CH;
-        if (xb >= w) {
-            xb = 0;
-            y++;
         }
     }
     store_block_partials(acc, buf, tot, jb.partials, bi, kFitBlocksPerJob);
@@ -219,26 +210,26 @@ int fit_share_blocks(FitJobs& jobs, const long long* units, int grid) {
     return base;
 }
 
-// The finalisation of the first pass (whose partials the statistics kernel
-// wrote with the jobs' current block shares), then the refit streaming over
-// the statistics images: kFitGrid blocks in all (every block resident at
-// once), one per chunk of samples at most, and its finalisation.
+// The finalisation of the first pass (whose partials the statistics kernels
+// wrote with the jobs' block shares), the refit over the sample lists with
+// the same shares (block_base renumbered for one launch over every job), and
+// its finalisation.
 void launch_fit_rest(FitJobs& jobs, bool raw, hipStream_t st) {
-    constexpr int kFitGrid = 1024;
     const dim3 blk(256);
     if (raw)
         launch_finalize<true, false>(jobs, st);
     else
         launch_finalize<false, false>(jobs, st);
-    const int ch = 256 * (raw ? 2 : 4);
-    long long chunks[5];
-    for (int j = 0; j < jobs.njobs; j++) chunks[j] = (long long)jobs.j[j].m.h * ((jobs.j[j].m.w + ch - 1) / ch);
-    const int base = fit_share_blocks(jobs, chunks, kFitGrid);
+    int base = 0;
+    for (int j = 0; j < jobs.njobs; j++) {
+        jobs.j[j].block_base = base;
+        base += jobs.j[j].nblocks;
+    }
     if (raw) {
-        hipLaunchKernelGGL((k_fit<true, true>), dim3(base), blk, 0, st, jobs);
+        hipLaunchKernelGGL(k_refit<true>, dim3(base), blk, 0, st, jobs);
         launch_finalize<true, true>(jobs, st);
     } else {
-        hipLaunchKernelGGL((k_fit<false, true>), dim3(base), blk, 0, st, jobs);
+        hipLaunchKernelGGL(k_refit<false>, dim3(base), blk, 0, st, jobs);
         launch_finalize<false, true>(jobs, st);
     }
 }

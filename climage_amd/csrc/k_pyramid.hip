@@ -157,7 +157,8 @@ __device__ __forceinline__ void downsample_from_tile(const LevelTile<TX, TY, tru
 // noiseStatistics of pyramid levels (one job per level) with the first
 // regression pass of computeNoiseStatistics accumulated on the way: each of
 // the grid's blocks takes a contiguous run of one level's 32x16 tiles and
-// keeps its regression sums across them (one partial per block).
+// keeps its regression sums across them (one partial per block); the
+// statistics in range are listed for the refit.
 // DOWN (one job): each tile also writes the next level's texels it covers
 // (downsampleImage), so that level is made without reading this one again.
 template <bool DOWN>
@@ -166,6 +167,7 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down)
     using Tile = LevelTile<TX, TY, DOWN>;
     __shared__ StatsTile<DOWN> sm;
     __shared__ clim_fx128 tot[L::K];
+    __shared__ int nlisted;
     static_assert(sizeof(StatsTile<DOWN>) >= 1024 * sizeof(clim_fx128),
                   "the tile storage serves as the reduction buffer");
     const int lb = xcd_block();
@@ -173,10 +175,13 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down)
     const int bi = lb - jb.block_base;
     const int tid = threadIdx.x, tx = tid & 31, ty0 = tid >> 5;
     const Img& in = jb.src;
-    const Img& out = jb.m;
-    const int tiles_x = (out.w + TX - 1) / TX;
-    const long long ntiles = (long long)tiles_x * ((out.h + TY - 1) / TY);
+    const int tiles_x = (in.w + TX - 1) / TX;
+    const long long ntiles = jb.ntiles;
     const int t0 = (int)(ntiles * bi / jb.nblocks), t1 = (int)(ntiles * (bi + 1) / jb.nblocks);
+    // this block's run of the sample list (the first barrier of the loop
+    // orders the counter's reset before its first use)
+    float4* list = reinterpret_cast<float4*>(jb.m.p) + (size_t)t0 * (TX * TY);
+    if (tid == 0) nlisted = 0;
     Acc<3, true, false> acc;
     acc.zero();
     // the next tile's loads are issued before the current tile's compute
@@ -194,26 +199,43 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down)
         for (int rr = 0; rr < 2; rr++) {
             const int ty = ty0 + 8 * rr;
             const int x = x0 + tx, y = y0 + ty;
-            const bool inside = x < out.w && y < out.h;
             const float4 s4 = sm.r.stats(tx, ty);
-            if (inside) wrow_f4(out, y)[x] = s4;
             const float m[1] = {s4.x};
             const float v[3] = {s4.y, s4.z, s4.w};
-            acc.add(m, v, inside);
+            const bool ok = x < in.w && y < in.h && Acc<3, true, false>::in_range(m, v);
+            const int slot = list_slot(ok, &nlisted);
+            if (ok) list[slot] = s4;
+            acc.add_tested(m, v, ok);
         }
         __syncthreads();  // the next tile reuses the LDS
     }
     // the last tile is done (the loop ends with a barrier): its LDS is free
+    if (tid == 0) jb.counts[bi] = (unsigned int)nlisted;
     store_block_partials(acc, reinterpret_cast<clim_fx128*>(&sm), tot, jb.partials, bi, kFitBlocksPerJob);
 }
 
-void launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipStream_t st) {
+void fit_list_dims(bool raw, int w, int h, int* aw, int* ah) {
+    // raw: 16x16-quad tiles of the (w/2) x (h/2) quads; levels: 32x16 tiles
+    if (raw) {
+        *aw = (w / 2 + 15) / 16 * 16;
+        *ah = (h / 2 + 15) / 16 * 16;
+    } else {
+        *aw = (w + TX - 1) / TX * TX;
+        *ah = (h + TY - 1) / TY * TY;
+    }
+}
+
+bool launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipStream_t st) {
     FitJobs sub{};
     sub.njobs = j1 - j0;
     long long tiles[5];
     for (int j = j0; j < j1; j++) {
-        sub.j[j - j0] = jobs.j[j];
-        tiles[j - j0] = (long long)((jobs.j[j].m.w + TX - 1) / TX) * ((jobs.j[j].m.h + TY - 1) / TY);
+        FitJob& jb = jobs.j[j];
+        jb.ntiles = ((jb.src.w + TX - 1) / TX) * ((jb.src.h + TY - 1) / TY);
+        jb.seg = TX * TY;
+        if ((long long)jb.ntiles * jb.seg > jb.cap) return false;
+        sub.j[j - j0] = jb;
+        tiles[j - j0] = jb.ntiles;
     }
     const int grid = fit_share_blocks(sub, tiles, 1024);
     for (int j = j0; j < j1; j++) jobs.j[j] = sub.j[j - j0];
@@ -221,6 +243,7 @@ void launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipS
         hipLaunchKernelGGL(k_noise_stats_fit<true>, dim3(grid), dim3(256), 0, st, sub, *down);
     else
         hipLaunchKernelGGL(k_noise_stats_fit<false>, dim3(grid), dim3(256), 0, st, sub, Img{});
+    return true;
 }
 
 // --------------------------------------------------------------- denoiseImage

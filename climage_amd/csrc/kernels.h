@@ -21,15 +21,21 @@ struct FitState {
 constexpr int kFitBlocksPerJob = 1024;
 
 // A fit's first pass runs inside the statistics kernel, which reads `src`
-// (the scaled mosaic with Bayer table `b`, or an RGBA level) and writes the
-// statistics images; the refit streams over those: raw fits over the mean and
-// var images of rawNoiseStatistics (float4 R, G, B, G2 each), level fits over
-// the noiseStatistics image (mean Y, var Y, var Cb, var Cr; v unused).
+// (the scaled mosaic with Bayer table `b`, or an RGBA level) and appends the
+// samples that pass the range test to sample lists; the refit streams over
+// those.  Raw fits list the rawNoiseStatistics mean and var (float4 R, G, B,
+// G2) in m and v, level fits the noiseStatistics values (mean Y, var Y,
+// var Cb, var Cr) in m.  The lists are split in runs, one per statistics
+// block: block bi of a job owns the slots from t0(bi) * seg (t0: its first
+// tile, seg: statistics per tile) and writes its sample count to counts[bi].
 struct FitJob {
     Img src;
     Bayer b;
-    Img m, v;
+    Img m, v;                 // sample lists (float4 slots from m.p, v.p)
+    long long cap;            // slots in each list
     int nblocks, block_base;  // set by the launchers
+    int ntiles, seg;          // set by the statistics launchers
+    unsigned int* counts;     // [kFitBlocksPerJob] samples listed per block
     FitState* st;
     clim_fx128* partials;     // nblocks x accumulator count
     float nlf_in[8];          // fallback model (written to out when no sample is in range)
@@ -41,15 +47,20 @@ struct FitJobs {
     int njobs;
 };
 
-// Statistics of every job plus the first regression pass (block partials;
-// sets the jobs' block shares for launch_fit_rest).
+// Statistics of every job plus the first regression pass (block partials,
+// sample lists; sets the jobs' block shares and tiling for launch_fit_rest).
 // smul: read the jobs' src as the uint16 mosaic, scaled with (smul, black);
-// null: src is the scaled FP32 mosaic
-void launch_raw_stats_fit(FitJobs& jobs, const float* smul, float black, hipStream_t st);
+// null: src is the scaled FP32 mosaic.  False (nothing launched) when a
+// job's lists have fewer than ntiles * seg slots.
+bool launch_raw_stats_fit(FitJobs& jobs, const float* smul, float black, hipStream_t st);
 // jobs [j0, j1) of `jobs` in one launch (their block shares are set in
 // `jobs`); down (one job, downsample_exact2x(src, *down)): the launch also
-// writes downsampleImage of the job's level into *down
-void launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipStream_t st);
+// writes downsampleImage of the job's level into *down.  False as above.
+bool launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipStream_t st);
+// list slots a statistics launch needs for a w x h level (texels) or a w x h
+// mosaic (raw: quads), as allocation sizes: the width and height to allocate
+// an RGBA32F image with
+void fit_list_dims(bool raw, int w, int h, int* aw, int* ah);
 // downsampleImage from `in` to `out` is an exact 2x reduction (every tap at
 // an odd texel offset with weight 1/2); launch_noise_stats_fit can then do it
 // on the way
@@ -58,7 +69,8 @@ bool downsample_exact2x(const Img& in, const Img& out);
 // unit and kFitBlocksPerJob, at least one), about `grid` in all; returns the
 // total.
 int fit_share_blocks(FitJobs& jobs, const long long* units, int grid);
-// the passes after the statistics kernel: combine and solve, refit, combine and solve
+// the passes after the statistics kernel: combine and solve, refit over the
+// sample lists (the statistics launches' block shares), combine and solve
 void launch_fit_rest(FitJobs& jobs, bool raw, hipStream_t st);
 
 // ------------------------------------------------------------ demosaic stages

@@ -25,6 +25,7 @@ using clim::ParamRef;
 
 clim_status run_stats_fit(clim_ctx* ctx, clim::FitJobs& jobs, bool raw, const float* smul, float black);
 clim_status run_fit_rest(clim_ctx* ctx, clim::FitJobs& jobs, bool raw);
+clim_status fit_list_alloc(clim_ctx* ctx, OwnedImg& o, bool raw, int w, int h);
 clim::FitJob make_fit_job(clim_ctx* ctx, int slot, const Img& src, const Img& m, const Img& v, const float* nlf_in,
                           const float* mult);
 clim_status guided_scratch(clim_ctx* ctx, OwnedImg* imgs, int w, int h, clim::GuidedScratch* g);
@@ -43,11 +44,11 @@ static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int
     CLIM_TRY(owned_alloc(ctx, ws.green, w, h, CLIM_FMT_R32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbA, w, h, CLIM_FMT_RGBA32F));
     CLIM_TRY(owned_alloc(ctx, ws.rgbB, w, h, CLIM_FMT_RGBA32F));
-    CLIM_TRY(owned_alloc(ctx, ws.raw_mean, w / 2, h / 2, CLIM_FMT_RGBA32F));
-    CLIM_TRY(owned_alloc(ctx, ws.raw_var, w / 2, h / 2, CLIM_FMT_RGBA32F));
+    CLIM_TRY(fit_list_alloc(ctx, ws.raw_mean, true, w, h));
+    CLIM_TRY(fit_list_alloc(ctx, ws.raw_var, true, w, h));
     for (int i = 0, scale = 1; i < levels; i++, scale *= 2) {
         const int lw = w / scale, lh = h / scale;
-        CLIM_TRY(owned_alloc(ctx, ws.stats[i], lw, lh, CLIM_FMT_RGBA32F));
+        CLIM_TRY(fit_list_alloc(ctx, ws.stats[i], false, lw, lh));
         if (i > 0) {
             CLIM_TRY(owned_alloc(ctx, ws.pyr[i], lw, lh, CLIM_FMT_RGBA32F));
             CLIM_TRY(owned_alloc(ctx, ws.den[i], lw, lh, CLIM_FMT_RGBA32F));
@@ -94,8 +95,7 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         const float mult[8] = {lm, cm, cm, lm, cm, cm, 1.0f, 1.0f};
         float in8[8] = {0};
         std::memcpy(in8, nlf_in[i], 6 * sizeof(float));
-        const Img sv = sub_view(ws.stats[i].img, lvl[i].w, lvl[i].h);
-        jobs.j[i] = make_fit_job(ctx, 1 + i, lvl[i], sv, sv, in8, mult);
+        jobs.j[i] = make_fit_job(ctx, 1 + i, lvl[i], ws.stats[i].img, ws.stats[i].img, in8, mult);
     }
     jobs.njobs = levels;
     long long nlev = 0;
@@ -105,9 +105,10 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
     // the reduction is exactly 2x (level 0 is read once for both); the other
     // levels follow their downsampling in one launch.
     const bool fused = clim::downsample_exact2x(lvl[0], lvl[1]);
+    bool listed;
     {
         ProfScope ps(ctx, ST_LEVEL_STATS, (long long)lvl[0].w * lvl[0].h);
-        clim::launch_noise_stats_fit(jobs, 0, 1, fused ? &lvl[1] : nullptr, st);
+        listed = clim::launch_noise_stats_fit(jobs, 0, 1, fused ? &lvl[1] : nullptr, st);
     }
     for (int i = fused ? 1 : 0; i < levels - 1; i++) {
         ProfScope ps(ctx, ST_DOWNSAMPLE, (long long)lvl[i + 1].w * lvl[i + 1].h);
@@ -115,8 +116,9 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
     }
     {
         ProfScope ps(ctx, ST_LEVEL_STATS, nlev - (long long)lvl[0].w * lvl[0].h);
-        clim::launch_noise_stats_fit(jobs, 1, levels, nullptr, st);
+        listed = listed && clim::launch_noise_stats_fit(jobs, 1, levels, nullptr, st);
     }
+    if (!listed) return ctx_fail(ctx, CLIM_EINVAL, "pyramid: sample list smaller than a level");
     CLIM_TRY_HIP(ctx, hipGetLastError());
     {
         ProfScope ps(ctx, ST_LEVEL_FIT, nlev);
