@@ -193,7 +193,7 @@ __device__ __forceinline__ void store_block_partials(const Acc<NCH, SHARED_M, RE
 
 // The slot of this lane's sample in the block's run of a sample list: the
 // wave's lanes with `ok` take consecutive slots after the block's LDS counter
-// (every lane of the wave calls this; the slots of different waves do not
+// (every lane of a full wave calls this; the slots of different waves do not
 // follow a fixed order, which the exact sums do not depend on).
 __device__ __forceinline__ int list_slot(bool ok, int* count) {
     const unsigned long long mask = __ballot(ok);
@@ -201,7 +201,18 @@ __device__ __forceinline__ int list_slot(bool ok, int* count) {
     const int rank = __popcll(mask & ((1ull << lane) - 1ull));
     int base = 0;
     if (lane == 0 && mask) base = atomicAdd(count, __popcll(mask));
-    return __shfl(base, 0) + rank;
+    return __builtin_amdgcn_readfirstlane(base) + rank;
+}
+// the same for two samples per lane, with one counter update per wave
+__device__ __forceinline__ void list_slots2(bool ok0, bool ok1, int* count, int& s0, int& s1) {
+    const unsigned long long m0 = __ballot(ok0), m1 = __ballot(ok1);
+    const unsigned long long below = (1ull << __lane_id()) - 1ull;
+    const int c0 = __popcll(m0);
+    int base = 0;
+    if (__lane_id() == 0 && (m0 | m1)) base = atomicAdd(count, c0 + __popcll(m1));
+    base = __builtin_amdgcn_readfirstlane(base);
+    s0 = base + __popcll(m0 & below);
+    s1 = base + c0 + __popcll(m1 & below);
 }
 
 // the job of block b: jobs own consecutive runs of blocks starting at block_base

@@ -195,17 +195,24 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down)
         sm.r.compute(sm.t, tid);
         if constexpr (DOWN) downsample_from_tile(sm.t, down, x0, y0, tid);
         __syncthreads();
+        float4 s4[2];
+        bool ok[2];
+        int slot[2];
 #pragma unroll
         for (int rr = 0; rr < 2; rr++) {
-            const int ty = ty0 + 8 * rr;
-            const int x = x0 + tx, y = y0 + ty;
-            const float4 s4 = sm.r.stats(tx, ty);
-            const float m[1] = {s4.x};
-            const float v[3] = {s4.y, s4.z, s4.w};
-            const bool ok = x < in.w && y < in.h && Acc<3, true, false>::in_range(m, v);
-            const int slot = list_slot(ok, &nlisted);
-            if (ok) list[slot] = s4;
-            acc.add_tested(m, v, ok);
+            const int x = x0 + tx, y = y0 + ty0 + 8 * rr;
+            s4[rr] = sm.r.stats(tx, ty0 + 8 * rr);
+            const float m[1] = {s4[rr].x};
+            const float v[3] = {s4[rr].y, s4[rr].z, s4[rr].w};
+            ok[rr] = x < in.w && y < in.h && Acc<3, true, false>::in_range(m, v);
+        }
+        list_slots2(ok[0], ok[1], &nlisted, slot[0], slot[1]);
+#pragma unroll
+        for (int rr = 0; rr < 2; rr++) {
+            if (ok[rr]) list[slot[rr]] = s4[rr];
+            const float m[1] = {s4[rr].x};
+            const float v[3] = {s4[rr].y, s4[rr].z, s4[rr].w};
+            acc.add_tested(m, v, ok[rr]);
         }
         __syncthreads();  // the next tile reuses the LDS
     }
