@@ -69,7 +69,7 @@ def stage_bytes(stage, texels, levels_px=None, preview=False):
 SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green": "k_green",
                  "red_blue_ycc": "k_rb_despeckle<true>", "despeckle": "k_despeckle",
                  "downsample": "k_downsample<true>",
-                 "denoise": "k_bilateral<true>", "reassemble": "k_reassemble<true, false, false>",
+                 "denoise": "k_bilateral_levels<true>", "reassemble": "k_reassemble<true, false, false>",
                  "tone": "k_reassemble<true, true, false>", "raw_denoise": "k_raw_rgba_filter<2, true>"}
 
 

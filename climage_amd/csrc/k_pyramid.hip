@@ -326,12 +326,10 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int 
     return make_float4(f0 / n0, f1 / n1, f2 / n1, 0.0f);
 }
 
+// one 32x16 tile (x0, y0) of denoiseImage
 template <bool SAME_EDGE>
-__global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
-                                                   float gradientBoost) {
-    __shared__ LevelTile4<TX, TY> t;
-    const TileId tl = xcd_tile();
-    const int x0 = tl.x * TX, y0 = tl.y * TY;
+__device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, const Img& in, const Img& out, int x0, int y0,
+                                               const ParamRef& nlf, float chromaBoost, float gradientBoost) {
     t.load(in, x0, y0, threadIdx.y * 32 + threadIdx.x);
     __syncthreads();
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
@@ -346,6 +344,28 @@ __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf
     }
 }
 
+template <bool SAME_EDGE>
+__global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
+                                                   float gradientBoost) {
+    __shared__ LevelTile4<TX, TY> t;
+    const TileId tl = xcd_tile();
+    bilateral_tile<SAME_EDGE>(t, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost);
+}
+
+// denoiseImage of every pyramid level in one launch (the levels' denoising
+// reads only the noisy levels): a 1-D grid of all levels' tiles, level 0
+// first, in XCD-contiguous runs.
+template <bool SAME_EDGE>
+__global__ void __launch_bounds__(256) k_bilateral_levels(BilateralLevels lv) {
+    __shared__ LevelTile4<TX, TY> t;
+    const int b = xcd_block();
+    int l = 0;
+    while (l + 1 < lv.n && b >= lv.base[l + 1]) l++;
+    const int i = b - lv.base[l], tiles_x = (lv.out[l].w + TX - 1) / TX;
+    bilateral_tile<SAME_EDGE>(t, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY, lv.nlf[l],
+                              lv.chromaBoost[l], lv.gradientBoost[l]);
+}
+
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
                       hipStream_t st) {
     dim3 blk(32, 8), grd((out.w + TX - 1) / TX, (out.h + TY - 1) / TY);
@@ -357,6 +377,24 @@ void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float 
         hipLaunchKernelGGL(k_bilateral<true>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost);
     else
         hipLaunchKernelGGL(k_bilateral<false>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost);
+}
+
+// the one-test form when launch_bilateral would take it for every level
+void launch_bilateral_levels(BilateralLevels& lv, hipStream_t st) {
+    bool same = true;
+    int base = 0;
+    for (int l = 0; l < lv.n; l++) {
+        float e0 = 1.0f * 1.41421356f, e1 = lv.chromaBoost[l] * 1.41421356f;
+        e0 = e0 * e0;
+        e1 = e1 * e1;
+        same = same && !(lv.gradientBoost[l] > 1.0f) && e0 == e1;
+        lv.base[l] = base;
+        base += ((lv.out[l].w + TX - 1) / TX) * ((lv.out[l].h + TY - 1) / TY);
+    }
+    if (same)
+        hipLaunchKernelGGL(k_bilateral_levels<true>, dim3(base), dim3(32, 8), 0, st, lv);
+    else
+        hipLaunchKernelGGL(k_bilateral_levels<false>, dim3(base), dim3(32, 8), 0, st, lv);
 }
 
 // --------------------------------------------------------- denoiseImageGuided

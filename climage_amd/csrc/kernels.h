@@ -107,6 +107,15 @@ void launch_downsample(const Img& in, const Img& out, hipStream_t st);
 void launch_noise_stats(const Img& in, const Img& out, hipStream_t st);
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
                       hipStream_t st);
+// denoiseImage of n pyramid levels in one launch (base: set by the launcher)
+struct BilateralLevels {
+    Img in[5], out[5];
+    ParamRef nlf[5];
+    float chromaBoost[5], gradientBoost[5];
+    int base[5];
+    int n;
+};
+void launch_bilateral_levels(BilateralLevels& lv, hipStream_t st);
 void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st);
 // convertTosRGB applied to reassembleImage's result in the same kernel
 struct ToneLaunch {

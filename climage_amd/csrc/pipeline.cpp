@@ -125,12 +125,26 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         CLIM_TRY(run_fit_rest(ctx, jobs, false));
     }
 
-    // synthesis, coarse to fine
+    // synthesis, coarse to fine.  The bilateral denoiser reads only the noisy
+    // levels, so every level is denoised in one launch ahead of the
+    // reassembly chain.
+    if (algorithm == CLIM_DENOISE_BILATERAL) {
+        ProfScope ps(ctx, ST_DENOISE, nlev);
+        clim::BilateralLevels bl{};
+        bl.n = levels;
+        for (int i = 0; i < levels; i++) {
+            bl.in[i] = lvl[i];
+            bl.out[i] = den[i];
+            bl.nlf[i] = dev_params(ctx->fit[1 + i].cal);
+            bl.chromaBoost[i] = dp[i].chromaBoost;
+            bl.gradientBoost[i] = dp[i].gradientBoost;
+        }
+        clim::launch_bilateral_levels(bl, st);
+    }
     for (int i = levels - 1; i >= 0; i--) {
         const ParamRef cal = dev_params(ctx->fit[1 + i].cal);
-        const bool bilateral = algorithm == CLIM_DENOISE_BILATERAL ||
-                               (algorithm == CLIM_DENOISE_GUIDED_FAST && i == levels - 1);
-        {
+        const bool bilateral = algorithm == CLIM_DENOISE_GUIDED_FAST && i == levels - 1;
+        if (algorithm != CLIM_DENOISE_BILATERAL) {
             ProfScope ps(ctx, ST_DENOISE, (long long)lvl[i].w * lvl[i].h);
             if (algorithm == CLIM_DENOISE_GUIDED_PRECISE) {
                 clim::GuidedScratch g{ws.g_meanI.img, ws.g_s1.img, ws.g_s2.img, ws.g_ipr.img,
