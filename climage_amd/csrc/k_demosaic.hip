@@ -98,8 +98,11 @@ struct QuadFetch<false> {
 // takes a contiguous run of 16x16-quad tiles and keeps its regression sums
 // across them (one partial per block); the quads in range are listed for the
 // refit.
+// Five waves per SIMD for the uint16 mosaic; the FP32 mosaic's wider loads
+// need the registers of four.
 template <bool U16>
-__global__ void __launch_bounds__(256) k_raw_stats_fit(FitJobs jobs, float4 smul, float black) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U16 ? 5 : 4)))
+k_raw_stats_fit(FitJobs jobs, float4 smul, float black) {
     constexpr int T = 16, R = 2, S = T + 2 * R;
     using L = Layout<4, false, false>;
     __shared__ QuadTile<T> sm;
@@ -165,7 +168,7 @@ bool launch_raw_stats_fit(FitJobs& jobs, const float* smul, float black, hipStre
         if ((long long)jb.ntiles * jb.seg > jb.cap) return false;
         tiles[j] = jb.ntiles;
     }
-    const int grid = fit_share_blocks(jobs, tiles, 1024);
+    const int grid = fit_share_blocks(jobs, tiles, kRawStatsGrid);
     if (smul)
         hipLaunchKernelGGL(k_raw_stats_fit<true>, dim3(grid), dim3(256), 0, st, jobs,
                            make_float4(smul[0], smul[1], smul[2], smul[3]), black);

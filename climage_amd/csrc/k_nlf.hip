@@ -139,7 +139,7 @@ __device__ void fit_solve_job(const FitJob& jb) {
     }
 }
 
-// One block per (job, sum): adds the job's block partials of that sum (four
+// One block per (job, sum): adds the job's block partials of that sum (eight
 // per thread, plain loads of clamped indexes, masked, all in flight at once)
 // into the job's FitState::tot.  The last block of the launch to finish
 // (a device-scope counter in the first job's FitState, released and acquired
@@ -148,7 +148,8 @@ __device__ void fit_solve_job(const FitJob& jb) {
 template <bool RAW, bool REFIT>
 __global__ void __launch_bounds__(256) k_fit_reduce(FitJobs jobs) {
     constexpr int K = Layout<RAW ? 4 : 3, !RAW, REFIT>::K;
-    static_assert(kFitBlocksPerJob <= 4 * 256, "four partials per thread");
+    constexpr int PT = kFitBlocksPerJob / 256;  // partials per thread
+    static_assert(kFitBlocksPerJob == PT * 256, "whole partials per thread");
     static_assert(K <= kFitSums, "FitState::tot holds every sum");
     __shared__ clim_fx128 wsum[4];
     __shared__ int last;
@@ -159,12 +160,12 @@ __global__ void __launch_bounds__(256) k_fit_reduce(FitJobs jobs) {
     // no sample in range: the fallback is already written, nothing to add
     if (!(REFIT && st->status == 1)) {
         const clim_fx128* pk = jb.partials + (size_t)k * kFitBlocksPerJob;
-        clim_fx128 v[4];
+        clim_fx128 v[PT];
 #pragma unroll
-        for (int i = 0; i < 4; i++) v[i] = pk[min(tid + 256 * i, jb.nblocks - 1)];
+        for (int i = 0; i < PT; i++) v[i] = pk[min(tid + 256 * i, jb.nblocks - 1)];
         clim_fx128 s = {0, 0};
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < PT; i++) {
             const bool mine = tid + 256 * i < jb.nblocks;
             v[i].hi = mine ? v[i].hi : 0;
             v[i].lo = mine ? v[i].lo : 0;

@@ -244,7 +244,7 @@ bool launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipS
         sub.j[j - j0] = jb;
         tiles[j - j0] = jb.ntiles;
     }
-    const int grid = fit_share_blocks(sub, tiles, 1024);
+    const int grid = fit_share_blocks(sub, tiles, kLevelStatsGrid);
     for (int j = j0; j < j1; j++) jobs.j[j] = sub.j[j - j0];
     if (down)
         hipLaunchKernelGGL(k_noise_stats_fit<true>, dim3(grid), dim3(256), 0, st, sub, *down);

@@ -18,7 +18,11 @@ struct FitState {
     unsigned int reduced;  // blocks of the running combination that are done (0 between launches)
 };
 
-constexpr int kFitBlocksPerJob = 1024;
+constexpr int kFitBlocksPerJob = 2048;
+// blocks of a statistics launch: as many 256-thread blocks per CU as the
+// kernel's registers allow (raw: five waves per SIMD; levels: four, which
+// measured faster than five with fewer registers)
+constexpr int kRawStatsGrid = 5 * 256, kLevelStatsGrid = 4 * 256;
 
 // A fit's first pass runs inside the statistics kernel, which reads `src`
 // (the scaled mosaic with Bayer table `b`, or an RGBA level) and appends the
