@@ -624,7 +624,7 @@ __global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img gr
         for (int dy = 0; dy < 2; dy++)
 #pragma unroll
             for (int dx = 0; dx < 2; dx++)
-                has_nan |= isnan(o[n][dy][dx].x) | isnan(o[n][dy][dx].y) | isnan(o[n][dy][dx].z);
+                has_nan |= (int)(isnan(o[n][dy][dx].x) || isnan(o[n][dy][dx].y) || isnan(o[n][dy][dx].z));
     }
     // the windows are read (the YCbCr region takes their place); a region
     // without NaN takes the despeckle's NaN-free form

@@ -101,22 +101,27 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
     long long nlev = 0;
     for (int i = 0; i < levels; i++) nlev += (long long)lvl[i].w * lvl[i].h;
     // noiseStatistics of every level with the first regression pass, then the
-    // refit.  Level 0's statistics kernel writes level 1 from its tiles when
-    // the reduction is exactly 2x (level 0 is read once for both); the other
-    // levels follow their downsampling in one launch.
-    const bool fused = clim::downsample_exact2x(lvl[0], lvl[1]);
-    bool listed;
-    {
-        ProfScope ps(ctx, ST_LEVEL_STATS, (long long)lvl[0].w * lvl[0].h);
-        listed = clim::launch_noise_stats_fit(jobs, 0, 1, fused ? &lvl[1] : nullptr, st);
+    // refit.  The statistics kernels of levels 0 and 1 write the next level
+    // from their tiles when the reduction is exactly 2x (a level is then read
+    // once for both); the remaining levels follow their downsampling in one
+    // launch.
+    constexpr int kFusedLevels = 2;
+    bool listed = true;
+    int next = 0;  // the first level whose statistics are not launched yet
+    while (next < kFusedLevels && next + 1 < levels && clim::downsample_exact2x(lvl[next], lvl[next + 1])) {
+        ProfScope ps(ctx, ST_LEVEL_STATS, (long long)lvl[next].w * lvl[next].h);
+        listed = listed && clim::launch_noise_stats_fit(jobs, next, next + 1, &lvl[next + 1], st);
+        next++;
     }
-    for (int i = fused ? 1 : 0; i < levels - 1; i++) {
+    long long rest = 0;
+    for (int i = next; i < levels; i++) rest += (long long)lvl[i].w * lvl[i].h;
+    for (int i = next; i < levels - 1; i++) {
         ProfScope ps(ctx, ST_DOWNSAMPLE, (long long)lvl[i + 1].w * lvl[i + 1].h);
         clim::launch_downsample(lvl[i], lvl[i + 1], st);
     }
-    {
-        ProfScope ps(ctx, ST_LEVEL_STATS, nlev - (long long)lvl[0].w * lvl[0].h);
-        listed = listed && clim::launch_noise_stats_fit(jobs, 1, levels, nullptr, st);
+    if (next < levels) {
+        ProfScope ps(ctx, ST_LEVEL_STATS, rest);
+        listed = listed && clim::launch_noise_stats_fit(jobs, next, levels, nullptr, st);
     }
     if (!listed) return ctx_fail(ctx, CLIM_EINVAL, "pyramid: sample list smaller than a level");
     CLIM_TRY_HIP(ctx, hipGetLastError());
