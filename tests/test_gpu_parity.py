@@ -93,6 +93,30 @@ def test_scale_and_raw_stats(ctx, chain, bayer):
     bitexact(fit, c["fit"], "computeRawNoiseStatistics")
 
 
+@pytest.mark.parametrize("w,h", [(202, 142), (66, 34), (32, 32)])
+def test_fits_on_partial_tiles(ctx, oracle32, w, h):
+    """computeRawNoiseStatistics and computeNoiseStatistics at sizes whose
+    statistics tiles (16x16 quads, 32x16 texels) are cut by the image edge:
+    the samples the statistics kernels list and the refit over those lists
+    give the oracle's fits (a 32x32 mosaic leaves a single raw tile)."""
+    rng = np.random.default_rng(w + h)
+    yy, xx = np.mgrid[0:h, 0:w]
+    field = 64.0 + 4031.0 * (0.2 + 0.1 * np.sin(xx / 11.0) * np.cos(yy / 7.0))
+    raw = np.clip(np.rint(field + rng.normal(0.0, 20.0, (h, w))), 0, 4095).astype(np.uint16)
+    sm = [65535.0 / (4095.0 - 64.0)] * 4
+    black = np.float32(64.0) / np.float32(65535.0)
+    scaled = oracle32.scale_raw(raw, ca.GBRG, sm, black)
+    mean, var = oracle32.raw_noise_statistics(scaled, ca.GBRG)
+    rc, fit = oracle32.nlf_fit_raw(mean, var)
+    assert rc in (0, 2)
+    bitexact(ctx.computeRawNoiseStatistics(up(ctx, scaled, ca.FMT_R32F), ca.GBRG), fit, f"raw fit {w}x{h}")
+    img = np.stack([0.2 + 0.1 * np.sin(xx / 9.0 + k) for k in range(4)], -1)
+    img = (img + rng.normal(0.0, 0.003, (h, w, 4))).astype(np.float32)
+    rc, lfit = oracle32.nlf_fit_level(oracle32.noise_statistics(img))
+    assert rc in (0, 2)
+    bitexact(ctx.computeNoiseStatistics(up(ctx, img, ca.FMT_RGBA32F)), lfit, f"level fit {w}x{h}")
+
+
 @pytest.mark.parametrize("bayer", BAYERS)
 def test_interpolate_green(ctx, chain, bayer):
     c = chain[bayer]
