@@ -41,13 +41,21 @@ static size_t pitch_for(int w, int format) {
 }
 
 clim_status owned_alloc(clim_ctx* ctx, OwnedImg& o, int w, int h, int format) {
-    if (o.img.p && o.img.w == w && o.img.h == h && o.format == format) return CLIM_OK;
+    const bool same = o.img.p && o.format == format;
+    if (same && w <= o.cap_w && h <= o.cap_h) {
+        o.img.w = w;
+        o.img.h = h;
+        return CLIM_OK;
+    }
+    const int cw = same ? std::max(w, o.cap_w) : w, ch = same ? std::max(h, o.cap_h) : h;
     owned_free(o);
-    const size_t pitch = pitch_for(w, format);
+    const size_t pitch = pitch_for(cw, format);
     void* p = nullptr;
-    CLIM_TRY_HIP(ctx, hipMalloc(&p, pitch * (size_t)h));
+    CLIM_TRY_HIP(ctx, hipMalloc(&p, pitch * (size_t)ch));
     o.img = Img{(char*)p, w, h, (int)pitch};
     o.format = format;
+    o.cap_w = cw;
+    o.cap_h = ch;
     return CLIM_OK;
 }
 
@@ -55,6 +63,7 @@ void owned_free(OwnedImg& o) {
     if (o.img.p) (void)hipFree(o.img.p);
     o.img = Img{nullptr, 0, 0, 0};
     o.format = 0;
+    o.cap_w = o.cap_h = 0;
 }
 
 // OpenCL Bayer table, cl/demosaic.cl:29-34

@@ -13,9 +13,13 @@ struct DevBuf {
 };
 
 // device image owned by the context
+// A device image the context owns.  The allocation only grows: img is a view
+// of the requested size on an allocation of cap_w x cap_h texels (the pitch
+// of cap_w), so frames of different sizes reuse it.
 struct OwnedImg {
     clim::Img img{nullptr, 0, 0, 0};
     int format = 0;
+    int cap_w = 0, cap_h = 0;
 };
 
 // work buffers of one frame geometry (RawConverter::allocateTextures,
