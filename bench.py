@@ -83,19 +83,20 @@ def cpu_baseline(w, h, bayer, ltm=False, noise_level=0.0, seconds_min=10.0):
     p = O.default_params(bayer, 64.0, 4095.0, levels=5)
     p.localToneMapping = int(ltm)
     p.noiseLevel = noise_level
+    raws = [O.generate_bayer(w, h, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
+            for i, a in enumerate(NOISE_A)]
+    # the six frames in turn until seconds_min of CPU time (at most 8 passes)
     done, dt = 0, 0.0
-    for i, a in enumerate(NOISE_A):
-        raw = O.generate_bayer(w, h, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
+    while dt < seconds_min and done < 8 * len(raws):
+        raw = raws[done % len(raws)]
         t0 = time.perf_counter()
         o.raw_to_rgb(raw, p)
         dt += time.perf_counter() - t0
         done += 1
-        if dt >= seconds_min:
-            break
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(done * w * h / dt / 1e6, 2), "unit": "Mpix/s", "cores": cores, "kind": "port",
-            "sample": f"{done} of the {len(NOISE_A)} bench frames ({w}x{h} RGGB) through the FP32 CPU "
-                      f"restatement in {dt:.1f} s, OpenMP row bands"}
+            "sample": f"{done} frames ({w}x{h} RGGB, the {len(NOISE_A)} bench noise levels in turn) through the "
+                      f"FP32 CPU restatement in {dt:.1f} s, OpenMP row bands"}
 
 
 def measured_traffic(kernel, w, h):
@@ -141,6 +142,9 @@ def main():
                     help="the half-resolution preview (RawConverter::fastDemosaicImage) instead of the full path")
     ap.add_argument("--noise-level", type=float, default=0.0,
                     help="DemosaicParameters::noiseLevel; above 0.6 the raw-domain pre-denoise runs")
+    ap.add_argument("--burst", action="store_true",
+                    help="the uneven burst: 16 frames, 4 each of 4032x3024, 4000x3000, 3264x2448 and 1920x1080, "
+                         "RGGB and BGGR alternating, noise levels drawn from the headline set, one context")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device fill and device-to-device copy (reference kernels "
@@ -154,15 +158,28 @@ def main():
     w, h, bayer = args.width, args.height, ca.RGGB
     lib = ca.lib()
     ctx = ca.Context(0)
-    raws, outs = [], []
-    for i, a in enumerate(NOISE_A):
-        r = ctx.image(w, h, ca.FMT_R16U)
-        ctx.generate_bayer(r, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
+    # frames: (width, height, CFA, noise a, frame id)
+    if args.burst:
+        if args.preview:
+            print("bench.py: --burst runs the full path", file=sys.stderr)
+            sys.exit(2)
+        sizes = [(4032, 3024), (4000, 3000), (3264, 2448), (1920, 1080)]
+        draws = np.random.default_rng(3000).integers(0, len(NOISE_A), 16)
+        frames = [(*sizes[k // 4], ca.RGGB if k % 2 == 0 else ca.BGGR, NOISE_A[draws[k]], k) for k in range(16)]
+    else:
+        frames = [(w, h, bayer, a, i) for i, a in enumerate(NOISE_A)]
+    raws, outs, fparams = [], [], []
+    for fw, fh, fb, a, i in frames:
+        r = ctx.image(fw, fh, ca.FMT_R16U)
+        ctx.generate_bayer(r, fb, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
         raws.append(r)
-        outs.append(ctx.image(w // 2, h // 2, ca.FMT_RGBA8) if args.preview else ctx.image(w, h, ca.FMT_RGBA8))
-    params = ca.default_params(bayer, 64.0, 4095.0, levels=5, algorithm=ca.BILATERAL)
-    params.rgbConversionParameters.localToneMapping = int(args.ltm)
-    params.noiseLevel = args.noise_level
+        outs.append(ctx.image(fw // 2, fh // 2, ca.FMT_RGBA8) if args.preview else ctx.image(fw, fh, ca.FMT_RGBA8))
+        fp = ca.default_params(fb, 64.0, 4095.0, levels=5, algorithm=ca.BILATERAL)
+        fp.rgbConversionParameters.localToneMapping = int(args.ltm)
+        fp.noiseLevel = args.noise_level
+        fparams.append(fp)
+    params = fparams[0]
+    px_per_step = sum(fw * fh for fw, fh, _, _, _ in frames)
     single_kernel = dict(SINGLE_KERNEL)
     if args.ltm:
         single_kernel["tone"] = "k_reassemble<true, true, true>"
@@ -171,11 +188,11 @@ def main():
     ctx.sync()
 
     def step():
-        for r, o in zip(raws, outs):
+        for r, o, fp in zip(raws, outs, fparams):
             if args.preview:
-                ctx.fast_demosaic(r, o, params)
+                ctx.fast_demosaic(r, o, fp)
             else:
-                ctx.raw_to_rgb(r, o, params, writeback=False)
+                ctx.raw_to_rgb(r, o, fp, writeback=False)
 
     for _ in range(args.warmup):
         step()
@@ -224,8 +241,19 @@ def main():
     if args.copy_calibration:
         copy_calibration(256 << 20)
 
-    frames = args.steps * len(raws)
-    mpix = frames * w * h / dt / 1e6
+    mpix = args.steps * px_per_step / dt / 1e6
+    latency = None
+    if args.burst:
+        # per-frame latency: each frame launched on an idle device and waited for
+        lat = []
+        for _ in range(3):
+            for r, o, fp in zip(raws, outs, fparams):
+                t1 = time.perf_counter()
+                ctx.raw_to_rgb(r, o, fp, writeback=False)
+                ctx.sync()
+                lat.append((time.perf_counter() - t1) * 1e3)
+        latency = {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                   "frames": len(lat)}
     roofline = None
     if prof_on:
         s = read_profile()[dom]
@@ -234,16 +262,28 @@ def main():
         bytes_per_launch = stage_bytes(dom, s["texels"] / s["launches"], preview=args.preview)
         ach = bytes_per_launch / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(single_kernel[dom], w, h),
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None if args.burst else measured_traffic(single_kernel[dom], w, h),
                     "kernel": single_kernel[dom], "stage": dom, "avg_launch_us": round(avg_s * 1e6, 2),
                     "launches_timed": s["launches"], "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     cpu = None
     if not args.no_cpu_baseline and not args.preview:
-        cpu = cpu_baseline(w, h, bayer, ltm=args.ltm, noise_level=args.noise_level)
+        # the burst's CPU sample: its smallest frame size
+        cw, ch = (1920, 1080) if args.burst else (w, h)
+        cpu = cpu_baseline(cw, ch, bayer, ltm=args.ltm, noise_level=args.noise_level)
 
+    if args.burst:
+        metric = "Mpix/s RAW->RGB (demosaic+pyramid denoise), burst of 16 uneven frames"
+        workload = ("burst: 4 each of 4032x3024, 4000x3000, 3264x2448, 1920x1080 RGGB/BGGR uint16 12-bit, "
+                    "per-frame noise level drawn from the headline set, 5-level bilateral pyramid, one context")
+    else:
+        metric = ("Mpix/s RAW->RGB half-resolution preview (mosaic pixels) at 12 MP" if args.preview
+                  else "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP")
+        workload = (f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, "
+                    + ("half-resolution preview (fastDemosaicImage)" if args.preview else "5-level bilateral pyramid")
+                    + (", local tone mapping" if args.ltm else "")
+                    + (f", noiseLevel {args.noise_level} (raw pre-denoise)" if args.noise_level > 0.6 else ""))
     res = {
-        "metric": ("Mpix/s RAW->RGB half-resolution preview (mosaic pixels) at 12 MP" if args.preview
-                   else "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP"),
+        "metric": metric,
         "value": round(mpix, 1),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -255,12 +295,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scenes generated on the device)",
-        "config": {"workload": f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, "
-                               + ("half-resolution preview (fastDemosaicImage)" if args.preview
-                                  else "5-level bilateral pyramid")
-                               + (", local tone mapping" if args.ltm else "")
-                               + (f", noiseLevel {args.noise_level} (raw pre-denoise)" if args.noise_level > 0.6 else ""),
-                   "frames_per_step": len(raws), "pixels_per_frame": w * h},
+        "config": {"workload": workload, "frames_per_step": len(raws), "pixels_per_step": px_per_step},
         "roofline": roofline,
         "path_roofline": None if args.preview else {"bytes_per_px": path_bytes_per_px(5),
                           "achieved": round(path_bytes_per_px(5) * mpix * 1e6 / 1e9, 1), "peak": HBM_PEAK_GBS,
@@ -268,6 +303,8 @@ def main():
         "cpu_baseline": cpu,
         "stages": stages,
     }
+    if latency:
+        res["latency"] = latency
     print(json.dumps(res))
     ctx.close()
 
