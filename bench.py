@@ -5,14 +5,18 @@ white 4095, six noise levels a in {0.0005, 0.001, 0.002, 0.004, 0.008, 0.016},
 b = a/200, seeded synthetic scenes generated on the device.  One step runs the
 full path (scale, raw noise fit, demosaic, highlights, YCbCr, despeckle,
 5-level bilateral pyramid with per-level noise fits, tone map to RGBA8) on
-all six frames; inputs are resident in HBM before timing starts.
+all six frames with one clim_raw_to_rgb_batch call (the frames alternate
+between two streams; --sequential: one clim_raw_to_rgb call per frame);
+inputs are resident in HBM before timing starts.
 
 Prints one JSON line.  `stages` comes from a few untimed steps with every
 stage bracketed by HIP events; `roofline` is measured for the dominant
 single-kernel stage with HIP events on the library's stream during the timed
 steps (only that stage records events there, so the headline rate is not
-perturbed); `cpu_baseline` times the FP32 CPU restatement (the oracle) on the
-same frames.
+perturbed; batched, those launches share the device with the other stream's
+frame, and `roofline.isolated` gives the same kernel timed alone in the
+profiled steps); `cpu_baseline` times the FP32 CPU restatement (the oracle) on
+the same frames.
 """
 import argparse
 import json
@@ -145,6 +149,8 @@ def main():
     ap.add_argument("--burst", action="store_true",
                     help="the uneven burst: 16 frames, 4 each of 4032x3024, 4000x3000, 3264x2448 and 1920x1080, "
                          "RGGB and BGGR alternating, noise levels drawn from the headline set, one context")
+    ap.add_argument("--sequential", action="store_true",
+                    help="one clim_raw_to_rgb call per frame instead of clim_raw_to_rgb_batch over the step's frames")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device fill and device-to-device copy (reference kernels "
@@ -187,7 +193,10 @@ def main():
         single_kernel["tone"] = "k_fast_preview<true>"
     ctx.sync()
 
-    def step():
+    def step(batched=not args.sequential):
+        if batched and not args.preview:
+            ctx.raw_to_rgb_batch(raws, outs, fparams)
+            return
         for r, o, fp in zip(raws, outs, fparams):
             if args.preview:
                 ctx.fast_demosaic(r, o, fp)
@@ -222,7 +231,7 @@ def main():
         lib.clim_profile_enable(ctx.h, 1)
         read_profile()
         for _ in range(psteps):
-            step()
+            step(batched=False)  # every frame on the profiled stream
         stages = read_profile()
         for v in stages.values():
             v["us_per_frame"] = round(v["ms_total"] * 1e3 / (psteps * len(raws)), 2)
@@ -242,6 +251,7 @@ def main():
         copy_calibration(256 << 20)
 
     mpix = args.steps * px_per_step / dt / 1e6
+    batched = not args.sequential and not args.preview
     latency = None
     if args.burst:
         # per-frame latency: each frame launched on an idle device and waited for
@@ -265,6 +275,15 @@ def main():
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None if args.burst else measured_traffic(single_kernel[dom], w, h),
                     "kernel": single_kernel[dom], "stage": dom, "avg_launch_us": round(avg_s * 1e6, 2),
                     "launches_timed": s["launches"], "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+        if batched:
+            # the timed launches share the device with the second lane's frame;
+            # the same kernel alone (the profiled steps, one frame at a time):
+            iso = stages[dom]
+            iso_s = iso["ms_total"] / iso["launches"] / 1e3
+            iso_ach = stage_bytes(dom, iso["texels"] / iso["launches"], preview=args.preview) / iso_s / 1e9
+            roofline["lanes"] = 2
+            roofline["isolated"] = {"avg_launch_us": round(iso_s * 1e6, 2), "achieved": round(iso_ach, 1),
+                                    "frac": round(iso_ach / HBM_PEAK_GBS, 4)}
     cpu = None
     if not args.no_cpu_baseline and not args.preview:
         # the burst's CPU sample: its smallest frame size

@@ -112,7 +112,8 @@ def exported_symbols():
                         "clim_fast_debayer", "clim_fast_demosaic", "clim_dng_decode", "clim_unpack_dng_metadata",
                         "clim_dng_last_error",
                         "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
-                        "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_generate_bayer")]
+                        "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_raw_to_rgb_batch",
+                        "clim_generate_bayer")]
 
 
 def _f(vals):
@@ -386,6 +387,27 @@ class Context:
             return None
         return (np.array(nm.rawNlf[:], np.float32),
                 np.array([nm.pyramidNlf[i][:] for i in range(params.levels)], np.float32))
+
+    def raw_to_rgb_batch(self, raws, outs8, params, outs_f32=None, writeback=False):
+        """Several frames (clim_raw_to_rgb_batch): raws, outs8, params (and
+        outs_f32) are lists of equal length.  With writeback, blocks and
+        returns each frame's measured noise model as (rawNlf, pyramidNlf)."""
+        n = len(raws)
+        if len(outs8) != n or len(params) != n or (outs_f32 is not None and len(outs_f32) != n):
+            raise ValueError("raw_to_rgb_batch: lists of different lengths")
+        R = (C.POINTER(Image) * n)(*[C.pointer(r.im) for r in raws])
+        O8 = (C.POINTER(Image) * n)(*[C.pointer(o.im) for o in outs8])
+        OF = None
+        if outs_f32 is not None:
+            OF = (C.POINTER(Image) * n)(*[C.pointer(o.im) if o is not None else None for o in outs_f32])
+        P = (DemosaicParameters * n)(*params)
+        NM = (NoiseModel * n)() if writeback else None
+        self._check(lib().clim_raw_to_rgb_batch(self.h, n, R, O8, OF, P, NM), "raw_to_rgb_batch")
+        if not writeback:
+            return None
+        return [(np.array(NM[i].rawNlf[:], np.float32),
+                 np.array([NM[i].pyramidNlf[l][:] for l in range(params[i].levels)], np.float32))
+                for i in range(n)]
 
     def generate_bayer(self, raw, bayer, scene_seed, noise_seed, a, b, black, white, clean=None):
         self._check(lib().clim_generate_bayer(self.h, raw.ref() if raw is not None else None,

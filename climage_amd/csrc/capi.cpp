@@ -191,6 +191,10 @@ extern "C" void clim_destroy(clim_ctx* ctx) {
     owned_free(ctx->op_mean);
     owned_free(ctx->op_var);
     owned_free(ctx->op_stats);
+    if (ctx->lane) clim_destroy(ctx->lane);
+    for (hipEvent_t e : ctx->lane_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->fit_hist) (void)hipFree(ctx->fit_hist);
     if (ctx->fit) (void)hipFree(ctx->fit);
     if (ctx->fit_counts) (void)hipFree(ctx->fit_counts);
     if (ctx->partials) (void)hipFree(ctx->partials);

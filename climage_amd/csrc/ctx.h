@@ -63,6 +63,13 @@ struct clim_ctx {
     OwnedImg op_guided_imgs[8];
     // statistics images of the stand-alone regression operators
     OwnedImg op_mean, op_var, op_stats;
+    // clim_raw_to_rgb_batch: a second lane (a context of its own on the same
+    // device: stream, work buffers, fit slots), the events that order the two
+    // streams, and device copies of the frames' fit states
+    clim_ctx* lane = nullptr;
+    hipEvent_t lane_ev[2] = {nullptr, nullptr};
+    clim::FitState* fit_hist = nullptr;
+    size_t fit_hist_n = 0;
     // stage profiling
     bool prof = false;
     uint32_t prof_mask = 0xffffffffu;  // stages that record events

@@ -15,8 +15,10 @@
 //    never checks, cpp/raw_converter.cpp:27).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "ctx.h"
 
@@ -445,3 +447,69 @@ extern "C" clim_status clim_generate_bayer(clim_ctx* ctx, clim_image* raw, clim_
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
 }
+
+// Several frames, each as clim_raw_to_rgb, alternating between the context's
+// stream and a second lane's, so that one frame's small launches and launch
+// tails overlap the other's kernels.  The second lane starts after the work
+// already queued on the context's stream, and the context's stream waits for
+// the second lane at the end: later work on the context sees every output.
+extern "C" clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const clim_image* const* raws,
+                                             clim_image* const* out_rgba8, clim_image* const* out_f32,
+                                             const clim_demosaic_params* params, clim_noise_model* nlf_out) {
+    if (!ctx || n < 0 || (n > 0 && (!raws || !out_rgba8 || !params)))
+        return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb_batch: null argument");
+    if (n == 0) return CLIM_OK;
+    if (!ctx->lane && n > 1) {
+        clim_ctx* lane = nullptr;
+        const clim_status s = clim_create(&lane, ctx->device);
+        if (s != CLIM_OK) return ctx_fail(ctx, s, "raw_to_rgb_batch: second lane");
+        ctx->lane = lane;
+        for (hipEvent_t& e : ctx->lane_ev) CLIM_TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    int max_levels = 0;
+    for (int i = 0; i < n; i++) max_levels = std::max(max_levels, (int)params[i].levels);
+    const size_t per = 1 + (size_t)std::min(std::max(max_levels, 1), CLIM_MAX_LEVELS);
+    if (nlf_out && ctx->fit_hist_n < per * n) {
+        if (ctx->fit_hist) (void)hipFree(ctx->fit_hist);
+        ctx->fit_hist = nullptr;
+        ctx->fit_hist_n = 0;
+        CLIM_TRY_HIP(ctx, hipMalloc(&ctx->fit_hist, sizeof(clim::FitState) * per * n));
+        ctx->fit_hist_n = per * n;
+    }
+    if (n > 1) {
+        CLIM_TRY_HIP(ctx, hipEventRecord(ctx->lane_ev[0], ctx->stream));
+        CLIM_TRY_HIP(ctx, hipStreamWaitEvent(ctx->lane->stream, ctx->lane_ev[0], 0));
+    }
+    clim_status st = CLIM_OK;
+    for (int i = 0; i < n && st == CLIM_OK; i++) {
+        clim_ctx* c = (i & 1) ? ctx->lane : ctx;
+        st = clim_raw_to_rgb(c, raws[i], out_rgba8[i], out_f32 ? out_f32[i] : nullptr, &params[i], nullptr);
+        if (st != CLIM_OK) {
+            const std::string msg = c->last_error;
+            ctx_fail(ctx, st, "raw_to_rgb_batch frame " + std::to_string(i) + ": " + msg);
+            break;
+        }
+        if (nlf_out) {
+            const hipError_t e = hipMemcpyAsync(ctx->fit_hist + per * i, c->fit,
+                                                sizeof(clim::FitState) * (1 + params[i].levels),
+                                                hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) st = ctx_hip(ctx, e, "raw_to_rgb_batch: fit copy");
+        }
+    }
+    if (n > 1) {
+        CLIM_TRY_HIP(ctx, hipEventRecord(ctx->lane_ev[1], ctx->lane->stream));
+        CLIM_TRY_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->lane_ev[1], 0));
+    }
+    if (st != CLIM_OK || !nlf_out) return st;
+    CLIM_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<clim::FitState> fits(per * n);
+    CLIM_TRY_HIP(ctx, hipMemcpy(fits.data(), ctx->fit_hist, sizeof(clim::FitState) * per * n,
+                                hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+        std::memcpy(nlf_out[i].rawNlf, fits[per * i].out + 4, 4 * sizeof(float));
+        for (int l = 0; l < params[i].levels; l++)
+            std::memcpy(nlf_out[i].pyramidNlf[l], fits[per * i + 1 + l].out, 6 * sizeof(float));
+    }
+    return CLIM_OK;
+}
+

@@ -351,6 +351,22 @@ clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* ou
                             clim_image* out_f32, const clim_demosaic_params* params,
                             clim_noise_model* nlf_out);
 
+/* n frames, each as clim_raw_to_rgb(raws[i], out_rgba8[i], out_f32 ? out_f32[i]
+ * : null, &params[i]) - RawConverter::demosaicImage called once per frame of a
+ * burst (cpp/raw_converter.cpp:73-186), the frames of any sizes and CFA
+ * patterns.  No reference counterpart as one call: for callers with several
+ * frames at hand.  The frames alternate between the context's stream and a
+ * second stream with its own work buffers (created on first use), so that
+ * one frame's small launches overlap the other's; the results are those of
+ * the single-frame calls.  Work queued on the context before the call is
+ * complete before any frame starts, and work queued after it sees every
+ * output.  out_f32 may be null (or hold nulls).  nlf_out (nullable, n
+ * entries): the call blocks at the end and writes each frame's measured
+ * noise model. */
+clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const clim_image* const* raws,
+                                  clim_image* const* out_rgba8, clim_image* const* out_f32,
+                                  const clim_demosaic_params* params, clim_noise_model* nlf_out);
+
 /* ------------------------------------------------- synthetic scenes
  * Seeded synthetic RAW frames (smooth background, 200 Gaussian blobs, 50
  * hard-edged rectangles; Poisson-Gaussian noise y = a*Poisson(x/a) + N(0,b);
