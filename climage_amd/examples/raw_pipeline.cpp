@@ -189,9 +189,23 @@ int main(int argc, char** argv) {
             write_rgb(out + ".preview_stages.rgb", *RawConverter::convertToRGBImage(prgb));
         }
 
+        // 4. two frames in one RawConverter::demosaicImages call (the second
+        //    on the context's second stream): each equals the fused result
+        DemosaicParameters p4 = synthetic_params(bayer, black, white), p5 = p4;
+        for (DemosaicParameters* pp : {&p4, &p5}) {
+            pp->rgbConversionParameters.localToneMapping = ltm;
+            pp->noiseLevel = noise_level;
+        }
+        {
+            gls::hip_image_2d<gls::rgba_pixel> b0(&ctx, w, h), b1(&ctx, w, h);
+            rc.demosaicImages({&rawDev, &rawDev}, {&b0, &b1}, {&p4, &p5});
+            write_rgb(out + ".batch0.rgb", *RawConverter::convertToRGBImage(b0));
+            write_rgb(out + ".batch1.rgb", *RawConverter::convertToRGBImage(b1));
+        }
+
         std::ofstream f(out + ".nlf");
         f.precision(9);
-        for (const auto* p : {&p1, &p2, &p3}) {
+        for (const auto* p : {&p1, &p2, &p3, &p4, &p5}) {
             for (float v : p->noiseModel.rawNlf) f << v << ' ';
             for (const auto& row : p->noiseModel.pyramidNlf)
                 for (float v : row) f << v << ' ';

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "pyramidal_denoise.hpp"
 
@@ -37,6 +38,35 @@ class RawConverter {
         for (int l = 0; l < _levels; l++)
             for (int k = 0; k < 6; k++) demosaicParameters->noiseModel.pyramidNlf[l][k] = nm.pyramidNlf[l][k];
         return _srgb.get();
+    }
+
+    // Several device-resident frames in one call (clim_raw_to_rgb_batch):
+    // frame i from raws[i] into outs[i] (the caller's images, of the mosaic's
+    // size); demosaicParameters[i]->noiseModel receives frame i's measured
+    // noise model.  The frames alternate between two streams of the context.
+    void demosaicImages(const std::vector<const gls::hip_image_2d<gls::luma_pixel_16>*>& raws,
+                        const std::vector<gls::hip_image_2d<gls::rgba_pixel>*>& outs,
+                        const std::vector<DemosaicParameters*>& demosaicParameters) {
+        const size_t n = raws.size();
+        if (outs.size() != n || demosaicParameters.size() != n)
+            throw gls::clim_error(CLIM_EINVAL, "demosaicImages: lists of different lengths");
+        std::vector<const clim_image*> r(n);
+        std::vector<clim_image*> o(n);
+        std::vector<clim_demosaic_params> p(n);
+        std::vector<clim_noise_model> nm(n);
+        for (size_t i = 0; i < n; i++) {
+            r[i] = raws[i]->clim();
+            o[i] = outs[i]->clim();
+            p[i] = to_clim(*demosaicParameters[i], _levels, _algorithm);
+        }
+        _ctx->check(clim_raw_to_rgb_batch(_ctx->ctx(), (int32_t)n, r.data(), o.data(), nullptr, p.data(), nm.data()),
+                    "demosaicImages");
+        for (size_t i = 0; i < n; i++) {
+            auto& model = demosaicParameters[i]->noiseModel;
+            for (int c = 0; c < 4; c++) model.rawNlf[c] = nm[i].rawNlf[c];
+            for (int l = 0; l < _levels; l++)
+                for (int k = 0; k < 6; k++) model.pyramidNlf[l][k] = nm[i].pyramidNlf[l][k];
+        }
     }
 
     // cpp/raw_converter.cpp:188-224: the half-resolution preview (one kernel);

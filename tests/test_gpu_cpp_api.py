@@ -1,7 +1,8 @@
 """The C++ operator API (include/*.hpp) on the GPU.
 
-climage_amd/examples/raw_pipeline.cpp runs the path three times: through
-RawConverter::demosaicImage (fused, device-resident), and operator by
+climage_amd/examples/raw_pipeline.cpp runs the path through
+RawConverter::demosaicImage (fused, device-resident), through
+RawConverter::demosaicImages (two frames, one per stream), and operator by
 operator twice, once with a caller-supplied ImageDenoiser, which makes
 PyramidalDenoise orchestrate level by level as the reference does, and once
 with the built-in filter (one call that fills PyramidalDenoise's level
@@ -33,9 +34,11 @@ def test_cpp_driver_fused_equals_stages_equals_oracle(tmp_path, bayer, ltm, nois
     builtin = np.fromfile(prefix + ".builtin.rgb", np.uint8).reshape(h, w, 3)
     np.testing.assert_array_equal(fused, stages)
     np.testing.assert_array_equal(fused, builtin)
+    for k in (0, 1):  # RawConverter::demosaicImages, one frame on each stream
+        np.testing.assert_array_equal(fused, np.fromfile(prefix + f".batch{k}.rgb", np.uint8).reshape(h, w, 3))
     nlf = np.loadtxt(prefix + ".nlf")
-    np.testing.assert_array_equal(nlf[0], nlf[1])
-    np.testing.assert_array_equal(nlf[0], nlf[2])
+    for k in range(1, 5):
+        np.testing.assert_array_equal(nlf[0], nlf[k])
     raw = O.generate_bayer(w, h, bayer, 1000, 2000, 0.002, 1e-5, 64.0, 4095.0)
     op = O.default_params(bayer, 64.0, 4095.0, levels=5)
     op.localToneMapping = ltm
