@@ -61,7 +61,7 @@ struct clim_ctx {
     // scratch of the stand-alone guided filter
     clim::GuidedScratch op_guided{};
     OwnedImg op_guided_imgs[8];
-    // statistics images of the stand-alone regression operators
+    // sample lists of the stand-alone regression operators
     OwnedImg op_mean, op_var, op_stats;
     // clim_raw_to_rgb_batch: a second lane (a context of its own on the same
     // device: stream, work buffers, fit slots), the events that order the two

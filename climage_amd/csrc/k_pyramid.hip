@@ -553,10 +553,18 @@ struct LtmFoot {
     bool lds;
 };
 
-// The guide texels (clamped to the edge) under the taps of the pixels
-// [x0, x1] x [y0, y1], staged in LDS when they fit; block-uniform.
-__device__ __forceinline__ LtmFoot ltm_stage(float* sh, int x0, int x1, int y0, int y1, float nx, float ny,
-                                             const Img& g) {
+// The taps (ltm_axis) of a 64x4 block's columns and rows, computed once per
+// column and row instead of once per pixel.
+struct LtmAxes {
+    int i[64][3], j[4][3];
+    float a[64][3], b[4][3];
+};
+
+// For a 64x4 block at (x0, y0): the guide texels (clamped to the edge) under
+// the taps of the pixels [x0, x1] x [y0, y1], staged in LDS when they fit
+// (block-uniform), and the block's column and row taps.
+__device__ __forceinline__ LtmFoot ltm_stage(float* sh, LtmAxes* ax, int x0, int x1, int y0, int y1, float nx,
+                                             float ny, const Img& g) {
     int ia[3], ib[3], ja[3], jb[3];
     float t[3];
     ltm_axis(x0, nx, g.w, ia, t);
@@ -568,25 +576,35 @@ __device__ __forceinline__ LtmFoot ltm_stage(float* sh, int x0, int x1, int y0, 
     f.j0 = ja[0];
     const int nc = ib[2] + 2 - ia[0], nr = jb[2] + 2 - ja[0];
     f.lds = nc <= kLtmLW && nr <= kLtmLH;
+    const int tid = threadIdx.x + threadIdx.y * blockDim.x, nt = blockDim.x * blockDim.y;
+    if (tid < 64)
+        ltm_axis(x0 + tid, nx, g.w, ax->i[tid], ax->a[tid]);
+    else if (tid < 68)
+        ltm_axis(y0 + tid - 64, ny, g.h, ax->j[tid - 64], ax->b[tid - 64]);
     if (f.lds) {
         const LtmGuideMem m{g};
-        const int tid = threadIdx.x + threadIdx.y * blockDim.x, nt = blockDim.x * blockDim.y;
         for (int k = tid; k < nc * nr; k += nt) {
             const int r = k / nc, c = k - r * nc;
             sh[r * kLtmLW + c] = m.at(f.i0 + c, f.j0 + r);
         }
-        __syncthreads();
     }
+    __syncthreads();
     return f;
 }
 
+// pixel (lx, ly) of the block whose taps ax holds
 template <class G>
-__device__ __forceinline__ float ltm_mask_px(const G& g, int x, int y, float nx, float ny, int gw, int gh,
-                                             float luma, const float rgb[3], const LtmArgs& a) {
+__device__ __forceinline__ float ltm_mask_px(const G& g, const LtmAxes& ax, int lx, int ly, float luma,
+                                             const float rgb[3], const LtmArgs& a) {
     int ii[3], jj[3];
     float fa[3], fb[3];
-    ltm_axis(x, nx, gw, ii, fa);
-    ltm_axis(y, ny, gh, jj, fb);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        ii[k] = ax.i[lx][k];
+        fa[k] = ax.a[lx][k];
+        jj[k] = ax.j[ly][k];
+        fb[k] = ax.b[ly][k];
+    }
     const float W[3][3] = {{0.16f, 0.16f, 0.08f}, {0.16f, 0.16f, 0.08f}, {0.08f, 0.08f, 0.04f}};  // [y][x]
     float s[9], sum = 0.0f, sumSq = 0.0f;
 #pragma unroll
@@ -623,10 +641,10 @@ __device__ __forceinline__ float ltm_mask_px(const G& g, int x, int y, float nx,
     return clim_powf(illuminance, 1.0f / tc) * clim_powf(reflectance, a.detail) / luma;
 }
 
-__device__ __forceinline__ float ltm_mask_at(const LtmFoot& f, const float* sh, int x, int y, float nx, float ny,
+__device__ __forceinline__ float ltm_mask_at(const LtmFoot& f, const float* sh, const LtmAxes& ax, int lx, int ly,
                                              float luma, const float rgb[3], const LtmArgs& a) {
-    if (f.lds) return ltm_mask_px(LtmGuideLds{sh, f.i0, f.j0}, x, y, nx, ny, a.guide.w, a.guide.h, luma, rgb, a);
-    return ltm_mask_px(LtmGuideMem{a.guide}, x, y, nx, ny, a.guide.w, a.guide.h, luma, rgb, a);
+    if (f.lds) return ltm_mask_px(LtmGuideLds{sh, f.i0, f.j0}, ax, lx, ly, luma, rgb, a);
+    return ltm_mask_px(LtmGuideMem{a.guide}, ax, lx, ly, luma, rgb, a);
 }
 
 __global__ void __launch_bounds__(256) k_ltm_mask(Img in, Img out, LtmArgs a) {
@@ -635,13 +653,14 @@ __global__ void __launch_bounds__(256) k_ltm_mask(Img in, Img out, LtmArgs a) {
     const int y = tl.y * 4 + threadIdx.y;
     const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
     __shared__ float sh[kLtmLW * kLtmLH];
-    const LtmFoot f = ltm_stage(sh, tl.x * 64, min(tl.x * 64 + 63, out.w - 1), tl.y * 4,
+    __shared__ LtmAxes ax;
+    const LtmFoot f = ltm_stage(sh, &ax, tl.x * 64, min(tl.x * 64 + 63, out.w - 1), tl.y * 4,
                                 min(tl.y * 4 + 3, out.h - 1), nx, ny, a.guide);
     if (x >= out.w || y >= out.h) return;
     const float4 p = row_f4(in, y)[x];
     const float rgb[3] = {a.T[0] * p.x + a.T[1] * p.y + a.T[2] * p.z, a.T[3] * p.x + a.T[4] * p.y + a.T[5] * p.z,
                           a.T[6] * p.x + a.T[7] * p.y + a.T[8] * p.z};
-    wrow_f(out, y)[x] = ltm_mask_at(f, sh, x, y, nx, ny, p.x, rgb, a);
+    wrow_f(out, y)[x] = ltm_mask_at(f, sh, ax, threadIdx.x, threadIdx.y, p.x, rgb, a);
 }
 
 static LtmArgs make_ltm_args(const Img& guide, const float ltm[4], const float* T) {
@@ -687,10 +706,11 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
     const int y = tl.y * 4 + threadIdx.y;
     const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
     __shared__ float gsh[LTM ? kLtmLW * kLtmLH : 1];
+    __shared__ LtmAxes gax[1];  // (dropped by the compiler without LTM)
     LtmFoot foot{0, 0, false};
     if (LTM)
-        foot = ltm_stage(gsh, tl.x * 64, min(tl.x * 64 + 63, out.w - 1), tl.y * 4, min(tl.y * 4 + 3, out.h - 1), nx,
-                         ny, la.guide);
+        foot = ltm_stage(gsh, gax, tl.x * 64, min(tl.x * 64 + 63, out.w - 1), tl.y * 4, min(tl.y * 4 + 3, out.h - 1),
+                         nx, ny, la.guide);
     if (x >= out.w || y >= out.h) return;
     const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
     const float4 p0 = row_f4(d0, y)[x];
@@ -733,7 +753,7 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
     if (TONE && LTM) {
         float v[3];
         tone_pre(d, ta, v);  // the camera RGB of the mask is the tone map's input
-        const float boost = ltm_mask_at(foot, gsh, x, y, nx, ny, d.x, v, la);
+        const float boost = ltm_mask_at(foot, gsh, gax[0], threadIdx.x, threadIdx.y, d.x, v, la);
         tone_store_v<true>(out8, outf, x, y, v, ta, boost);
     } else if (TONE) {
         tone_store(out8, outf, x, y, d, ta);
