@@ -59,7 +59,7 @@ struct LevelTile4 {
 template <int TX, int TY>
 struct StatsRows {
     static constexpr int SY = TY + 4;
-    float h[6][SY][TX + 1];
+    float h[6][SY][TX];  // unpadded rows of 32: the two rows a wave touches meet fewer LDS bank conflicts
 
     template <bool W>
     __device__ __forceinline__ void compute(const LevelTile<TX, TY, W>& t, int tid) {
@@ -104,7 +104,7 @@ template <int RT>
 struct QuadTile {
     static constexpr int S = RT + 4;
     float4 q[S][S];
-    float h[8][S][RT + 1];  // horizontal sums: 4 channel sums, 4 sums of squares
+    float h[8][S][RT];  // horizontal sums: 4 channel sums, 4 sums of squares (unpadded rows, fewer bank conflicts)
 
     __device__ __forceinline__ void hsums(int tid) {
         constexpr int N = S * RT, IT = (N + 255) / 256;
