@@ -14,9 +14,10 @@ namespace clim {
 // CL_FILTER_LINEAR | CLK_ADDRESS_CLAMP_TO_EDGE with normalised coordinates:
 // u = x*W - 0.5, i0 = floor(u), weight frac(u) rounded to 8 fractional bits.
 // The sampling half, given the texel (i0, j0) and the rounded weights (a, b):
-__device__ __forceinline__ float4 bilinear4_at(const Img& im, int i0, int j0, float a, float b) {
-    const float4 t00 = ld4(im, i0, j0), t10 = ld4(im, i0 + 1, j0);
-    const float4 t01 = ld4(im, i0, j0 + 1), t11 = ld4(im, i0 + 1, j0 + 1);
+template <typename Fetch>
+__device__ __forceinline__ float4 bilinear4_at(Fetch ld, int i0, int j0, float a, float b) {
+    const float4 t00 = ld(i0, j0), t10 = ld(i0 + 1, j0);
+    const float4 t01 = ld(i0, j0 + 1), t11 = ld(i0 + 1, j0 + 1);
     const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b);
     const float w01 = (1.0f - a) * b, w11 = a * b;
     float4 r;
@@ -25,6 +26,9 @@ __device__ __forceinline__ float4 bilinear4_at(const Img& im, int i0, int j0, fl
     r.z = w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z;
     r.w = w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w;
     return r;
+}
+__device__ __forceinline__ float4 bilinear4_at(const Img& im, int i0, int j0, float a, float b) {
+    return bilinear4_at([&](int i, int j) { return ld4(im, i, j); }, i0, j0, a, b);
 }
 
 __device__ __forceinline__ float4 bilinear4(const Img& im, float un, float vn) {
@@ -711,6 +715,25 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
     if (LTM)
         foot = ltm_stage(gsh, gax, tl.x * 64, min(tl.x * 64 + 63, out.w - 1), tl.y * 4, min(tl.y * 4 + 3, out.h - 1),
                          nx, ny, la.guide);
+    // exact 2x, without the tone map: the half-size texels the block's 64x4
+    // pixels read (columns x0/2 - 1 .. x0/2 + 32, rows y0/2 - 1 .. y0/2 + 2,
+    // clamped to the image) of both half-size images, staged once.  (The
+    // level-0 pass with the tone map measured slower staged: its own VALU
+    // work hides the loads, and the barrier costs more than the clamps.)
+    constexpr bool STAGE = EXACT2X && !TONE;
+    constexpr int RW = 34, RH = 4;
+    __shared__ float4 hs[STAGE ? 2 : 1][STAGE ? RH : 1][STAGE ? RW : 1];
+    const int hx0 = tl.x * 32 - 1, hy0 = tl.y * 2 - 1;
+    if (STAGE) {
+        const int tid = threadIdx.y * 64 + threadIdx.x;
+        for (int k = tid; k < 2 * RH * RW; k += 256) {
+            const int m = k / (RH * RW), r = (k / RW) % RH, c = k % RW;
+            hs[m][r][c] = ld4(m ? d1 : n1, hx0 + c, hy0 + r);
+        }
+        __syncthreads();
+    }
+    auto half_n1 = [&](int i, int j) { return STAGE ? hs[0][j - hy0][i - hx0] : ld4(n1, i, j); };
+    auto half_d1 = [&](int i, int j) { return STAGE ? hs[STAGE ? 1 : 0][j - hy0][i - hx0] : ld4(d1, i, j); };
     if (x >= out.w || y >= out.h) return;
     const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
     const float4 p0 = row_f4(d0, y)[x];
@@ -721,8 +744,8 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
     const int i0 = (x & 1) ? (x - 1) >> 1 : (x >> 1) - 1, j0 = (y & 1) ? (y - 1) >> 1 : (y >> 1) - 1;
     const float a = (x & 1) ? 0.25f : 0.75f, b = (y & 1) ? 0.25f : 0.75f;
     if (EXACT2X) {
-        a1 = bilinear4_at(n1, i0, j0, a, b);
-        b1 = bilinear4_at(d1, i0, j0, a, b);
+        a1 = bilinear4_at(half_n1, i0, j0, a, b);
+        b1 = bilinear4_at(half_d1, i0, j0, a, b);
     } else {
         a1 = bilinear4(n1, px, py);
         b1 = bilinear4(d1, px, py);
@@ -738,8 +761,8 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
         if (EXACT2X) {
             const float ax = 1.0f - a, by = 1.0f - b;
             const int xh = x >> 1, yh = y >> 1;
-            dx = (bilinear4_at(d1, xh, j0, ax, b).x - bilinear4_at(d1, xh - 1, j0, ax, b).x) / 2.0f;
-            dy = (bilinear4_at(d1, i0, yh, a, by).x - bilinear4_at(d1, i0, yh - 1, a, by).x) / 2.0f;
+            dx = (bilinear4_at(half_d1, xh, j0, ax, b).x - bilinear4_at(half_d1, xh - 1, j0, ax, b).x) / 2.0f;
+            dy = (bilinear4_at(half_d1, i0, yh, a, by).x - bilinear4_at(half_d1, i0, yh - 1, a, by).x) / 2.0f;
         } else {
             dx = (bilinear4(d1, px + nx, py).x - bilinear4(d1, px - nx, py).x) / 2.0f;
             dy = (bilinear4(d1, px, py + ny).x - bilinear4(d1, px, py - ny).x) / 2.0f;
