@@ -99,16 +99,15 @@ namespace {
 constexpr int TX = 32, TY = 16, HR = 2;
 }
 
-template <bool W = false>
 struct StatsTile {
-    LevelTile<TX, TY, W> t;
+    LevelTile<TX, TY> t;
     StatsRows<TX, TY> r;
 };
 
 // ------------------------------------------------------------ noiseStatistics
 // cl/demosaic.cl:1267-1286: (mean Y, var Y, var Cb, var Cr) over 5x5.
 __global__ void __launch_bounds__(256) k_noise_stats(Img in, Img out) {
-    __shared__ StatsTile<> sm;
+    __shared__ StatsTile sm;
     const TileId tl = xcd_tile();
     const int x0 = tl.x * TX, y0 = tl.y * TY;
     const int tid = threadIdx.y * 32 + threadIdx.x;
@@ -130,32 +129,50 @@ void launch_noise_stats(const Img& in, const Img& out, hipStream_t st) {
     hipLaunchKernelGGL(k_noise_stats, grd, blk, 0, st, in, out);
 }
 
-// downsampleImage (exact 2x: four taps of weight 1/2 at texels 2X-1 and
-// 2X+1, k_downsample<true>) of the half-size texels a 32x16 level tile
-// covers, from the tile in LDS.  Thread tid computes channels 2p, 2p+1
-// (p = tid / 128) of texel tid % 128 (16x8, row-major) with the same
-// arithmetic as bilinear4_at and k_downsample.
-__device__ __forceinline__ void downsample_from_tile(const LevelTile<TX, TY, true>& t, const Img& out, int x0,
-                                                     int y0, int tid) {
+// downsampleImage (exact 2x, k_downsample<true>) of the 16x8 half-size
+// texels a 32x16 level tile covers, in two steps.  Half-size texel (X, Y)
+// sums the bilinear taps (weights 1/4) at texels (2X -/+ 1, 2Y -/+ 1); tap
+// (2X - 1, 2Y - 1) sits at tile column 2 lx + 1, row 2 ly + 1 (lx, ly: the
+// texel's place in the tile's 16x8 block) and neighbouring texels share
+// taps, so the tile's 17x9 distinct taps are computed once (down_taps: all
+// four channels, the arithmetic of bilinear4_at) and each texel adds its
+// four in k_downsample's order (down_write).
+constexpr int kTapW = TX / 2 + 1, kTapH = TY / 2 + 1;
+
+// the statistics kernel's LDS: the interleaved level tile, its horizontal
+// sums and (DOWN) the next level's taps, channels (x, y) and (z, w)
+template <bool DOWN>
+struct StatsFitTile {
+    LevelTile4<TX, TY> t;
+    StatsRows<TX, TY> r;
+    float2 taps[DOWN ? 2 : 1][DOWN ? kTapH : 1][DOWN ? kTapW : 1];
+};
+
+__device__ __forceinline__ void down_taps(const LevelTile4<TX, TY>& t, float2 (*tp)[kTapH][kTapW], int tid) {
+    // taps go to the threads with two rows of horizontal sums (tid >= 128) first
+    const int k = 255 - tid;
+    if (k >= kTapW * kTapH) return;
+    const int i = k % kTapW, j = k / kTapW, c = 2 * i + 1, r = 2 * j + 1;
+    const float4 a = t.p[r][c], b = t.p[r][c + 1], d = t.p[r + 1][c], e = t.p[r + 1][c + 1];
+    tp[0][j][i] = make_float2(0.25f * a.x + 0.25f * b.x + 0.25f * d.x + 0.25f * e.x,
+                              0.25f * a.y + 0.25f * b.y + 0.25f * d.y + 0.25f * e.y);
+    tp[1][j][i] = make_float2(0.25f * a.z + 0.25f * b.z + 0.25f * d.z + 0.25f * e.z,
+                              0.25f * a.w + 0.25f * b.w + 0.25f * d.w + 0.25f * e.w);
+}
+
+// thread tid writes channels 2p, 2p + 1 (p = tid / 128) of texel tid % 128
+// (16x8, row-major)
+__device__ __forceinline__ void down_write(const float2 (*tp)[kTapH][kTapW], const Img& out, int x0, int y0,
+                                           int tid) {
     const int p = tid >> 7, o = tid & 127, lx = o & 15, ly = o >> 4;
     const int X = (x0 >> 1) + lx, Y = (y0 >> 1) + ly;
     if (X >= out.w || Y >= out.h) return;
-    // texel (2X - 1, 2Y - 1) sits at tile column 2 lx + 1, row 2 ly + 1
-    const int c = 2 * lx + 1, r = 2 * ly + 1;
-    float res[2];
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const float(*a)[TX + 5] = p == 0 ? (k == 0 ? t.x : t.y) : (k == 0 ? t.z : t.w);
-        auto tap = [&](int i, int j) {
-            return 0.25f * a[j][i] + 0.25f * a[j][i + 1] + 0.25f * a[j + 1][i] + 0.25f * a[j + 1][i + 1];
-        };
-        float acc = tap(c, r);
-        acc += tap(c + 2, r);
-        acc += tap(c, r + 2);
-        acc += tap(c + 2, r + 2);
-        res[k] = 0.25f * acc;
-    }
-    reinterpret_cast<float2*>(wrow_f4(out, Y) + X)[p] = make_float2(res[0], res[1]);
+    const float2 t0 = tp[p][ly][lx], t1 = tp[p][ly][lx + 1], t2 = tp[p][ly + 1][lx], t3 = tp[p][ly + 1][lx + 1];
+    float ax = t0.x, ay = t0.y;
+    ax += t1.x; ay += t1.y;
+    ax += t2.x; ay += t2.y;
+    ax += t3.x; ay += t3.y;
+    reinterpret_cast<float2*>(wrow_f4(out, Y) + X)[p] = make_float2(0.25f * ax, 0.25f * ay);
 }
 
 // noiseStatistics of pyramid levels (one job per level) with the first
@@ -168,11 +185,11 @@ __device__ __forceinline__ void downsample_from_tile(const LevelTile<TX, TY, tru
 template <bool DOWN>
 __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down) {
     using L = Layout<3, true, false>;
-    using Tile = LevelTile<TX, TY, DOWN>;
-    __shared__ StatsTile<DOWN> sm;
+    using Tile = LevelTile4<TX, TY>;
+    __shared__ StatsFitTile<DOWN> sm;
     __shared__ clim_fx128 tot[L::K];
     __shared__ int nlisted;
-    static_assert(sizeof(StatsTile<DOWN>) >= 1024 * sizeof(clim_fx128),
+    static_assert(sizeof(StatsFitTile<DOWN>) >= 1024 * sizeof(clim_fx128),
                   "the tile storage serves as the reduction buffer");
     const int lb = xcd_block();
     const FitJob& jb = jobs.j[job_of_block(jobs, lb)];
@@ -197,8 +214,9 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down)
         __syncthreads();
         if (t + 1 < t1) Tile::fetch(pf, in, ((t + 1) % tiles_x) * TX, ((t + 1) / tiles_x) * TY, tid);
         sm.r.compute(sm.t, tid);
-        if constexpr (DOWN) downsample_from_tile(sm.t, down, x0, y0, tid);
+        if constexpr (DOWN) down_taps(sm.t, sm.taps, tid);
         __syncthreads();
+        if constexpr (DOWN) down_write(sm.taps, down, x0, y0, tid);
         float4 s4[2];
         bool ok[2];
         int slot[2];

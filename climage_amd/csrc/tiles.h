@@ -13,13 +13,11 @@
 namespace clim {
 
 // ------------------------------------------------------------ RGBA levels
-// TXxTY outputs, 256 threads, a 2-texel halo; only x, y, z enter the filters
-// (W: the fourth channel is staged too).
-template <int TX, int TY, bool W = false>
+// TXxTY outputs, 256 threads, a 2-texel halo; only x, y, z enter the filters.
+template <int TX, int TY>
 struct LevelTile {
     static constexpr int SX = TX + 4, SY = TY + 4;
     float x[SY][SX + 1], y[SY][SX + 1], z[SY][SX + 1];
-    float w[W ? SY : 1][SX + 1];
 
     // the tile's texels in registers (fetch, then put)
     using Regs = Staged<SX * SY, 256, float4>;
@@ -31,7 +29,6 @@ struct LevelTile {
             x[i / SX][i % SX] = p.x;
             y[i / SX][i % SX] = p.y;
             z[i / SX][i % SX] = p.z;
-            if (W) w[i / SX][i % SX] = p.w;
         });
     }
     __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
@@ -52,6 +49,14 @@ struct LevelTile4 {
             tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
             [&](int i, float4 v) { p[i / SX][i % SX] = v; });
     }
+    // the tile's texels in registers (fetch, then put)
+    using Regs = Staged<SX * SY, 256, float4>;
+    __device__ __forceinline__ static void fetch(Regs& r, const Img& in, int x0, int y0, int tid) {
+        r.fetch(tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); });
+    }
+    __device__ __forceinline__ void put(const Regs& r, int tid) {
+        r.put(tid, [&](int i, float4 v) { p[i / SX][i % SX] = v; });
+    }
 };
 
 // Horizontal 5-tap sums of the level tile for all SY rows: h[0..2] the
@@ -61,8 +66,7 @@ struct StatsRows {
     static constexpr int SY = TY + 4;
     float h[6][SY][TX];  // unpadded rows of 32: the two rows a wave touches meet fewer LDS bank conflicts
 
-    template <bool W>
-    __device__ __forceinline__ void compute(const LevelTile<TX, TY, W>& t, int tid) {
+    __device__ __forceinline__ void compute(const LevelTile<TX, TY>& t, int tid) {
         constexpr int N = SY * TX, IT = (N + 255) / 256;
 #pragma unroll
         for (int k = 0; k < IT; k++) {
@@ -76,6 +80,27 @@ struct StatsRows {
                 s0 += a; q0 += a * a;
                 s1 += b; q1 += b * b;
                 s2 += e; q2 += e * e;
+            }
+            h[0][r][c] = s0; h[1][r][c] = s1; h[2][r][c] = s2;
+            h[3][r][c] = q0; h[4][r][c] = q1; h[5][r][c] = q2;
+        }
+    }
+
+    // the same sums from the interleaved tile (one LDS read per texel)
+    __device__ __forceinline__ void compute(const LevelTile4<TX, TY>& t, int tid) {
+        constexpr int N = SY * TX, IT = (N + 255) / 256;
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int i = tid + k * 256;
+            if (i >= N) break;
+            const int r = i / TX, c = i % TX;
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < 5; d++) {
+                const float4 v = t.p[r][c + d];
+                s0 += v.x; q0 += v.x * v.x;
+                s1 += v.y; q1 += v.y * v.y;
+                s2 += v.z; q2 += v.z * v.z;
             }
             h[0][r][c] = s0; h[1][r][c] = s1; h[2][r][c] = s2;
             h[3][r][c] = q0; h[4][r][c] = q1; h[5][r][c] = q2;
