@@ -117,9 +117,10 @@ struct StatsRows {
             for (int j = 0; j < 5; j++) a += h[k][ly + j][lx];
             s[k] = a;
         }
-        // x / 25.0f by clim_div25 (the same bits for every finite x)
-        return make_float4(clim_div25(s[0]), clim_div25(s[3] - clim_div25(s[0] * s[0])),
-                           clim_div25(s[4] - clim_div25(s[1] * s[1])), clim_div25(s[5] - clim_div25(s[2] * s[2])));
+        // x / 25.0f by clim_div25_nz: the sums start from +0, so no operand is -0
+        return make_float4(clim_div25_nz(s[0]), clim_div25_nz(s[3] - clim_div25_nz(s[0] * s[0])),
+                           clim_div25_nz(s[4] - clim_div25_nz(s[1] * s[1])),
+                           clim_div25_nz(s[5] - clim_div25_nz(s[2] * s[2])));
     }
 };
 
@@ -164,8 +165,9 @@ struct QuadTile {
         }
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            m[c] = clim_div25(s[c]);  // = s[c] / 25.0f for every finite value
-            v[c] = clim_div25(s[4 + c] - clim_div25(s[c] * s[c]));
+            // = x / 25.0f: the sums start from +0, so no operand is -0 (clim_div25_nz)
+            m[c] = clim_div25_nz(s[c]);
+            v[c] = clim_div25_nz(s[4 + c] - clim_div25_nz(s[c] * s[c]));
         }
     }
 };

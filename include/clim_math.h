@@ -309,6 +309,16 @@ CLIM_HD float clim_div25(float x) {
     return x == 0.0f ? q : c;
 }
 
+/* The same form without the zero test: equal to x / 25.0f for every finite
+ * x except -0, for which it gives +0.  Under round-to-nearest a sum that
+ * starts from +0 is never -0, nor is a square or the difference of two such
+ * values, so the 5x5 window statistics (whose sums start from +0) use it. */
+CLIM_HD float clim_div25_nz(float x) {
+    const float r = 1.0f / 25.0f;
+    const float q = x * r;
+    return fmaf(fmaf(-q, 25.0f, x), r, q);
+}
+
 /* ------------------------------------------ 128-bit fixed-point accumulator
  * Non-negative doubles below 2^37 are truncated to multiples of 2^-90 and
  * summed as unsigned 128-bit integers.  Integer addition is associative, so

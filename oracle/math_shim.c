@@ -87,6 +87,21 @@ long om_div25_mismatches(uint32_t lo, uint32_t hi) {
     return bad;
 }
 
+/* mismatches between clim_div25_nz(x) and x / 25.0f over the bit patterns
+ * [lo, hi) (finite x other than -0) */
+long om_div25nz_mismatches(uint32_t lo, uint32_t hi) {
+    long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = lo; i < (int64_t)hi; i++) {
+        const float x = clim_u2f((uint32_t)i);
+        if (!isfinite(x) || (uint32_t)i == 0x80000000u) continue;
+        if (clim_f2u(clim_div25_nz(x)) != clim_f2u(x / 25.0f)) bad++;
+    }
+    return bad;
+}
+/* clim_div25_nz at one value (its -0 case) */
+float om_div25_nz(float x) { return clim_div25_nz(x); }
+
 /* the largest |clim_rsqrt(s) - 1/sqrt(s)| / (1/sqrt(s)) over the bit patterns
  * [lo, hi) (positive finite floats), the exact value in double precision */
 double om_rsqrt_max_rel_err(uint32_t lo, uint32_t hi) {

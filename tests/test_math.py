@@ -226,10 +226,38 @@ def test_div25_equals_division(L):
     2^-121 (quotient subnormal) decide it, both signs; the build also checked
     all 2^32 patterns once."""
     L.om_div25_mismatches.restype = C.c_long
+    L.om_div25nz_mismatches.restype = C.c_long
     ranges = [(0x3F800000, 0x40000000), (0x00000000, 0x03000000)]
     for lo, hi in ranges:
         for sign in (0, 0x80000000):
             assert L.om_div25_mismatches(C.c_uint32(lo | sign), C.c_uint32(hi | sign)) == 0, (hex(lo), sign)
+            # the form without the zero test (window statistics, no -0 operand)
+            assert L.om_div25nz_mismatches(C.c_uint32(lo | sign), C.c_uint32(hi | sign)) == 0, (hex(lo), sign)
+    # ... which differs at -0 alone: it gives +0 there
+    L.om_div25_nz.restype = C.c_float
+    L.om_div25_nz.argtypes = [C.c_float]
+    z = L.om_div25_nz(-0.0)
+    assert z == 0.0 and np.signbit(np.float32(z)) == False
+
+
+def test_window_sums_are_never_negative_zero():
+    """The premise of clim_div25_nz in the window statistics: under
+    round-to-nearest, +0 + (-0) = +0 and x + (-x) = +0, so a float sum
+    that starts from +0 is never -0, nor is a square or the difference of
+    two such values; checked on sums of signed zeros and cancelling terms."""
+    f = np.float32
+    vals = [f(0.0), f(-0.0), f(1.5), f(-1.5), f(2.0 ** -149), f(-(2.0 ** -149))]
+    rng = np.random.default_rng(5)
+    for _ in range(2000):
+        terms = rng.choice(len(vals), 5)
+        s = f(0.0)
+        for t in terms:
+            s = f(s + vals[t])
+        assert not (s == 0 and np.signbit(s)), terms
+        sq = f(s * s)
+        assert not np.signbit(sq)
+        d = f(sq - sq)
+        assert not (d == 0 and np.signbit(d))
 
 
 def _median9_network(q):
