@@ -84,58 +84,56 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     store_block_partials(acc, buf, tot, jb.partials, bi, kFitBlocksPerJob);
 }
 
-// The least-squares solve of one job from its totals.
+// The least-squares solve of channel c of one job from the job's totals T
+// (staged in LDS) and, for the refit, the first pass's status.  Each channel
+// is solved by its own thread; channel 0's thread also writes the job's
+// status and sample count.
 template <bool RAW, bool REFIT>
-__device__ void fit_solve_job(const FitJob& jb) {
+__device__ void fit_solve_channel(const FitJob& jb, const clim_fx128* T, int status, int c) {
     constexpr int NCH = RAW ? 4 : 3;
     constexpr bool SHARED_M = !RAW;
     using L = Layout<NCH, SHARED_M, REFIT>;
     FitState* st = jb.st;
-    if (REFIT && st->status == 1) return;
-    clim_fx128 tot[L::K];
-#pragma unroll
-    for (int k = 0; k < L::K; k++) tot[k] = st->tot[k];
-    const unsigned long long cnt = tot[L::n].lo;
-    auto sums_of = [&](int c) {
-        clim_qsums q;
-        q.sx = tot[L::sx(c)];
-        q.sxx = tot[L::sxx(c)];
-        q.sy = tot[L::sy(c)];
-        q.sxy = tot[L::sxy(c)];
-        q.svv = REFIT ? clim_fx_u64(0) : tot[L::svv(c)];
-        return q;
-    };
+    if (REFIT && status == 1) return;
+    const unsigned long long cnt = T[L::n].lo;
+    clim_qsums q;
+    q.sx = T[L::sx(c)];
+    q.sxx = T[L::sxx(c)];
+    q.sy = T[L::sy(c)];
+    q.sxy = T[L::sxy(c)];
+    q.svv = REFIT ? clim_fx_u64(0) : T[L::svv(c)];
     float* out = st->out;
     if (!REFIT) {
-        st->n0 = cnt;
+        if (c == 0) {
+            st->n0 = cnt;
+            st->status = cnt == 0 ? 1 : 0;
+        }
         if (cnt == 0) {
-            st->status = 1;
-            for (int k = 0; k < 2 * NCH; k++) out[k] = jb.nlf_in[k];
+            // no sample in range: the model passed in, calibrated
+            out[c] = jb.nlf_in[c];
+            out[NCH + c] = jb.nlf_in[NCH + c];
+            st->cal[c] = out[c] * jb.mult[c];
+            st->cal[NCH + c] = out[NCH + c] * jb.mult[NCH + c];
         } else {
-            st->status = 0;
-            for (int c = 0; c < NCH; c++) {
-                const clim_qsums q = sums_of(c);
-                clim_fit_solve_q(&q, cnt, &st->A[c], &st->B[c]);
-                st->err2[c] = sqrt(clim_fit_rss_q(&q, cnt, st->A[c], st->B[c]) / (double)cnt);
-            }
+            double A, B;
+            clim_fit_solve_q(&q, cnt, &A, &B);
+            st->A[c] = A;
+            st->B[c] = B;
+            st->err2[c] = sqrt(clim_fit_rss_q(&q, cnt, A, B) / (double)cnt);
         }
     } else {
+        // no inliers: the first pass's model stays
+        double A = st->A[c], B = st->B[c];
         if (cnt > 0) {
-            for (int c = 0; c < NCH; c++) {
-                const clim_qsums q = sums_of(c);
-                clim_fit_solve_q(&q, cnt, &st->A[c], &st->B[c]);
-            }
-            st->status = 0;
-        } else {
-            st->status = 2;
+            clim_fit_solve_q(&q, cnt, &A, &B);
+            st->A[c] = A;
+            st->B[c] = B;
         }
-        for (int c = 0; c < NCH; c++) {
-            out[c] = (float)st->A[c];
-            out[NCH + c] = (float)st->B[c];
-        }
-    }
-    if (REFIT || st->status == 1) {
-        for (int k = 0; k < 2 * NCH; k++) st->cal[k] = out[k] * jb.mult[k];
+        if (c == 0) st->status = cnt > 0 ? 0 : 2;
+        out[c] = (float)A;
+        out[NCH + c] = (float)B;
+        st->cal[c] = out[c] * jb.mult[c];
+        st->cal[NCH + c] = out[NCH + c] * jb.mult[NCH + c];
     }
 }
 
@@ -185,7 +183,16 @@ __global__ void __launch_bounds__(256) k_fit_reduce(FitJobs jobs) {
     __syncthreads();
     if (!last) return;
     __threadfence();  // every block's total after the count
-    if (tid < jobs.njobs) fit_solve_job<RAW, REFIT>(jobs.j[tid]);
+    // every job's totals (and first-pass status) in LDS with one round of
+    // loads, then one thread per (job, channel)
+    constexpr int NCH = RAW ? 4 : 3;
+    __shared__ clim_fx128 T[5][K];
+    __shared__ int status[5];
+    for (int i = tid; i < jobs.njobs * K; i += 256) T[i / K][i % K] = jobs.j[i / K].st->tot[i % K];
+    if (REFIT && tid < jobs.njobs) status[tid] = jobs.j[tid].st->status;
+    __syncthreads();
+    if (tid < jobs.njobs * NCH)
+        fit_solve_channel<RAW, REFIT>(jobs.j[tid / NCH], T[tid / NCH], REFIT ? status[tid / NCH] : 0, tid % NCH);
 }
 
 // the combination of a pass's block partials and the solve
