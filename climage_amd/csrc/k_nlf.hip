@@ -59,25 +59,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const float4* lm = reinterpret_cast<const float4*>(jb.m.p) + (size_t)t0 * jb.seg;
     const float4* lv = reinterpret_cast<const float4*>(jb.v.p) + (size_t)t0 * jb.seg;
     const int n = (int)jb.counts[bi];
-    for (int i0 = 0; i0 < n; i0 += 256 * U) {
-        float4 pm[U], pv[U];
+    // the next U samples' loads are issued before the current ones are added
+    float4 pm[U], pv[U];
+    auto fetch = [&](int i0) {
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int i = min(i0 + tid + 256 * u, n - 1);
             pm[u] = lm[i];
             if (RAW) pv[u] = lv[i];
         }
+    };
+    if (n > 0) fetch(0);
+    for (int i0 = 0; i0 < n; i0 += 256 * U) {
+        float4 cm[U], cv[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
+            cm[u] = pm[u];
+            cv[u] = pv[u];
+        }
+        if (i0 + 256 * U < n) fetch(i0 + 256 * U);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // every listed sample passed the range test when it was listed
             const bool mine = i0 + tid + 256 * u < n;
             if (RAW) {
-                const float m[4] = {pm[u].x, pm[u].y, pm[u].z, pm[u].w};
-                const float v[4] = {pv[u].x, pv[u].y, pv[u].z, pv[u].w};
-                acc.add(m, v, mine);
+                const float m[4] = {cm[u].x, cm[u].y, cm[u].z, cm[u].w};
+                const float v[4] = {cv[u].x, cv[u].y, cv[u].z, cv[u].w};
+                acc.add_tested(m, v, mine);
             } else {
-                const float m[1] = {pm[u].x};
-                const float v[3] = {pm[u].y, pm[u].z, pm[u].w};
-                acc.add(m, v, mine);
+                const float m[1] = {cm[u].x};
+                const float v[3] = {cm[u].y, cm[u].z, cm[u].w};
+                acc.add_tested(m, v, mine);
             }
         }
     }
