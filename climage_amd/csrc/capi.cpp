@@ -191,7 +191,8 @@ extern "C" void clim_destroy(clim_ctx* ctx) {
     owned_free(ctx->op_mean);
     owned_free(ctx->op_var);
     owned_free(ctx->op_stats);
-    if (ctx->lane) clim_destroy(ctx->lane);
+    for (clim_ctx* l : ctx->lanes)
+        if (l) clim_destroy(l);
     for (hipEvent_t e : ctx->lane_ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->fit_hist) (void)hipFree(ctx->fit_hist);

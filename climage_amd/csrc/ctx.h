@@ -7,6 +7,12 @@
 #include "climage.h"
 #include "kernels.h"
 
+// streams a clim_raw_to_rgb_batch call spreads its frames over (two fill
+// the device: three and four measured no faster on 12 MP frames)
+#ifndef CLIM_BATCH_LANES
+#define CLIM_BATCH_LANES 2
+#endif
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -63,11 +69,14 @@ struct clim_ctx {
     OwnedImg op_guided_imgs[8];
     // sample lists of the stand-alone regression operators
     OwnedImg op_mean, op_var, op_stats;
-    // clim_raw_to_rgb_batch: a second lane (a context of its own on the same
-    // device: stream, work buffers, fit slots), the events that order the two
-    // streams, and device copies of the frames' fit states
-    clim_ctx* lane = nullptr;
-    hipEvent_t lane_ev[2] = {nullptr, nullptr};
+    // clim_raw_to_rgb_batch: the further lanes (contexts of their own on the
+    // same device: stream, work buffers, fit slots; frame i runs on lane
+    // i % kBatchLanes, lane 0 being this context), the events that order the
+    // lanes' streams (ev[0]: the batch's start on this stream, ev[k]: lane k's
+    // end), and device copies of the frames' fit states
+    static constexpr int kBatchLanes = CLIM_BATCH_LANES;
+    clim_ctx* lanes[kBatchLanes - 1] = {};
+    hipEvent_t lane_ev[kBatchLanes] = {};
     clim::FitState* fit_hist = nullptr;
     size_t fit_hist_n = 0;
     // stage profiling

@@ -459,13 +459,17 @@ extern "C" clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const cli
     if (!ctx || n < 0 || (n > 0 && (!raws || !out_rgba8 || !params)))
         return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb_batch: null argument");
     if (n == 0) return CLIM_OK;
-    if (!ctx->lane && n > 1) {
+    // lanes in use: one per frame up to kBatchLanes, created on first use
+    const int nl = std::min(n, (int)clim_ctx::kBatchLanes);
+    for (int k = 1; k < nl; k++) {
+        if (ctx->lanes[k - 1]) continue;
         clim_ctx* lane = nullptr;
         const clim_status s = clim_create(&lane, ctx->device);
-        if (s != CLIM_OK) return ctx_fail(ctx, s, "raw_to_rgb_batch: second lane");
-        ctx->lane = lane;
-        for (hipEvent_t& e : ctx->lane_ev) CLIM_TRY_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (s != CLIM_OK) return ctx_fail(ctx, s, "raw_to_rgb_batch: lane " + std::to_string(k));
+        ctx->lanes[k - 1] = lane;
     }
+    for (int k = 0; k < nl && nl > 1; k++)
+        if (!ctx->lane_ev[k]) CLIM_TRY_HIP(ctx, hipEventCreateWithFlags(&ctx->lane_ev[k], hipEventDisableTiming));
     int max_levels = 0;
     for (int i = 0; i < n; i++) max_levels = std::max(max_levels, (int)params[i].levels);
     const size_t per = 1 + (size_t)std::min(std::max(max_levels, 1), CLIM_MAX_LEVELS);
@@ -476,13 +480,15 @@ extern "C" clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const cli
         CLIM_TRY_HIP(ctx, hipMalloc(&ctx->fit_hist, sizeof(clim::FitState) * per * n));
         ctx->fit_hist_n = per * n;
     }
-    if (n > 1) {
+    if (nl > 1) {
         CLIM_TRY_HIP(ctx, hipEventRecord(ctx->lane_ev[0], ctx->stream));
-        CLIM_TRY_HIP(ctx, hipStreamWaitEvent(ctx->lane->stream, ctx->lane_ev[0], 0));
+        for (int k = 1; k < nl; k++)
+            CLIM_TRY_HIP(ctx, hipStreamWaitEvent(ctx->lanes[k - 1]->stream, ctx->lane_ev[0], 0));
     }
     clim_status st = CLIM_OK;
     for (int i = 0; i < n && st == CLIM_OK; i++) {
-        clim_ctx* c = (i & 1) ? ctx->lane : ctx;
+        const int k = i % nl;
+        clim_ctx* c = k ? ctx->lanes[k - 1] : ctx;
         st = clim_raw_to_rgb(c, raws[i], out_rgba8[i], out_f32 ? out_f32[i] : nullptr, &params[i], nullptr);
         if (st != CLIM_OK) {
             const std::string msg = c->last_error;
@@ -496,9 +502,9 @@ extern "C" clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const cli
             if (e != hipSuccess) st = ctx_hip(ctx, e, "raw_to_rgb_batch: fit copy");
         }
     }
-    if (n > 1) {
-        CLIM_TRY_HIP(ctx, hipEventRecord(ctx->lane_ev[1], ctx->lane->stream));
-        CLIM_TRY_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->lane_ev[1], 0));
+    for (int k = 1; k < nl; k++) {
+        CLIM_TRY_HIP(ctx, hipEventRecord(ctx->lane_ev[k], ctx->lanes[k - 1]->stream));
+        CLIM_TRY_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->lane_ev[k], 0));
     }
     if (st != CLIM_OK || !nlf_out) return st;
     CLIM_TRY_HIP(ctx, hipStreamSynchronize(ctx->stream));
