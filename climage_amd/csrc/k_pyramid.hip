@@ -144,7 +144,7 @@ constexpr int kTapW = TX / 2 + 1, kTapH = TY / 2 + 1;
 template <bool DOWN>
 struct StatsFitTile {
     LevelTile4<TX, TY> t;
-    StatsRows<TX, TY> r;
+    StatsRows<TX, TY, true> r;
     float2 taps[DOWN ? 2 : 1][DOWN ? kTapH : 1][DOWN ? kTapW : 1];
 };
 
@@ -217,13 +217,14 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down)
         if constexpr (DOWN) down_taps(sm.t, sm.taps, tid);
         __syncthreads();
         if constexpr (DOWN) down_write(sm.taps, down, x0, y0, tid);
+        // this thread's outputs: rows 2 (tid / 32) and 2 (tid / 32) + 1 of column tid % 32
         float4 s4[2];
         bool ok[2];
         int slot[2];
+        sm.r.stats2(tx, 2 * ty0, s4[0], s4[1]);
 #pragma unroll
         for (int rr = 0; rr < 2; rr++) {
-            const int x = x0 + tx, y = y0 + ty0 + 8 * rr;
-            s4[rr] = sm.r.stats(tx, ty0 + 8 * rr);
+            const int x = x0 + tx, y = y0 + 2 * ty0 + rr;
             const float m[1] = {s4[rr].x};
             const float v[3] = {s4[rr].y, s4[rr].z, s4[rr].w};
             ok[rr] = x < in.w && y < in.h && Acc<3, true, false>::in_range(m, v);

@@ -61,10 +61,24 @@ struct LevelTile4 {
 
 // Horizontal 5-tap sums of the level tile for all SY rows: h[0..2] the
 // channel sums, h[3..5] the sums of squares (noiseStatistics, cl/demosaic.cl:1267-1286).
-template <int TX, int TY>
+// PAIRS (TX = 32): each thread finishes two vertically adjacent outputs
+// (stats2), which share four of their five rows of sums.  A wave then reads
+// rows two apart in its two halves, so the rows are 48 floats long (two
+// rows = 96 words, half the 64 banks apart), and the horizontal pass writes
+// rows two apart per wave too (row_of); without PAIRS the rows are unpadded
+// (one output row per half-wave, rows one apart = 32 words).
+template <int TX, int TY, bool PAIRS = false>
 struct StatsRows {
     static constexpr int SY = TY + 4;
-    float h[6][SY][TX];  // unpadded rows of 32: the two rows a wave touches meet fewer LDS bank conflicts
+    static constexpr int RS = PAIRS ? TX + TX / 2 : TX;
+    static_assert(!PAIRS || (TX == 32 && SY % 4 == 0), "paired rows: 32-wide tiles, rows in fours");
+    float h[6][SY][RS];
+
+    // the tile row of work item i (one row per half-wave)
+    __device__ __forceinline__ static int row_of(int i) {
+        const int hw = i / TX;
+        return PAIRS ? (hw & ~3) | ((hw & 1) << 1) | ((hw >> 1) & 1) : hw;
+    }
 
     __device__ __forceinline__ void compute(const LevelTile<TX, TY>& t, int tid) {
         constexpr int N = SY * TX, IT = (N + 255) / 256;
@@ -72,7 +86,7 @@ struct StatsRows {
         for (int k = 0; k < IT; k++) {
             const int i = tid + k * 256;
             if (i >= N) break;
-            const int r = i / TX, c = i % TX;
+            const int r = row_of(i), c = i % TX;
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
 #pragma unroll
             for (int d = 0; d < 5; d++) {
@@ -93,7 +107,7 @@ struct StatsRows {
         for (int k = 0; k < IT; k++) {
             const int i = tid + k * 256;
             if (i >= N) break;
-            const int r = i / TX, c = i % TX;
+            const int r = row_of(i), c = i % TX;
             float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, q0 = 0.0f, q1 = 0.0f, q2 = 0.0f;
 #pragma unroll
             for (int d = 0; d < 5; d++) {
@@ -107,6 +121,14 @@ struct StatsRows {
         }
     }
 
+    // (mean Y, var Y, var Cb, var Cr) from the six window sums
+    __device__ __forceinline__ static float4 finish(const float (&s)[6]) {
+        // x / 25.0f by clim_div25_nz: the sums start from +0, so no operand is -0
+        return make_float4(clim_div25_nz(s[0]), clim_div25_nz(s[3] - clim_div25_nz(s[0] * s[0])),
+                           clim_div25_nz(s[4] - clim_div25_nz(s[1] * s[1])),
+                           clim_div25_nz(s[5] - clim_div25_nz(s[2] * s[2])));
+    }
+
     // (mean Y, var Y, var Cb, var Cr) of output (lx, ly)
     __device__ __forceinline__ float4 stats(int lx, int ly) const {
         float s[6];
@@ -117,10 +139,29 @@ struct StatsRows {
             for (int j = 0; j < 5; j++) a += h[k][ly + j][lx];
             s[k] = a;
         }
-        // x / 25.0f by clim_div25_nz: the sums start from +0, so no operand is -0
-        return make_float4(clim_div25_nz(s[0]), clim_div25_nz(s[3] - clim_div25_nz(s[0] * s[0])),
-                           clim_div25_nz(s[4] - clim_div25_nz(s[1] * s[1])),
-                           clim_div25_nz(s[5] - clim_div25_nz(s[2] * s[2])));
+        return finish(s);
+    }
+
+    // outputs (lx, ly) and (lx, ly + 1) from six rows of sums, each output's
+    // vertical sum in its own order (0 + rows ly .. ly + 4, 0 + rows ly + 1 .. ly + 5)
+    __device__ __forceinline__ void stats2(int lx, int ly, float4& o0, float4& o1) const {
+        float s0[6], s1[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            float r[6];
+#pragma unroll
+            for (int j = 0; j < 6; j++) r[j] = h[k][ly + j][lx];
+            float a = 0.0f, b = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+                a += r[j];
+                b += r[j + 1];
+            }
+            s0[k] = a;
+            s1[k] = b;
+        }
+        o0 = finish(s0);
+        o1 = finish(s1);
     }
 };
 
