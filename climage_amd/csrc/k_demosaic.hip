@@ -185,14 +185,21 @@ namespace {
 constexpr int GT_X = 32, GT_Y = 16;
 constexpr int GR_X = GT_X + 6, GR_Y = GT_Y + 6;   // raw tile
 constexpr int GS_X = GT_X + 4, GS_Y = GT_Y + 4;   // sobel tile
+// Row length of the LDS tiles: a wave reads four tile rows, each at every
+// second column (one non-green pixel per pair) with the column parity
+// alternating by row.  48-float rows put rows two apart 96 words (half the
+// 64 banks) apart and neighbouring rows on banks of the other parity, so the
+// 64 reads of a tap meet 64 distinct banks.
+constexpr int G_RS = 48;
+static_assert(G_RS >= GR_X && G_RS >= GS_X, "tile rows fit");
 }
 
 template <bool U16>
 __global__ void __launch_bounds__(256) k_green(MosaicSrc<U16> raw, Img green, int gxo, int gyo, ParamRef lv) {
-    __shared__ float r[GR_Y][GR_X + 1];
-    __shared__ float sgx[GS_Y][GS_X + 1];
-    __shared__ float sgy[GS_Y][GS_X + 1];
-    __shared__ float sgl[GS_Y][GS_X + 1];
+    __shared__ float r[GR_Y][G_RS];
+    __shared__ float sgx[GS_Y][G_RS];
+    __shared__ float sgy[GS_Y][G_RS];
+    __shared__ float sgl[GS_Y][G_RS];
     const TileId tl = xcd_tile();
     const int x0 = tl.x * GT_X, y0 = tl.y * GT_Y;
     const int tid = threadIdx.y * 32 + threadIdx.x;
