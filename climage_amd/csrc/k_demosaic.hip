@@ -580,11 +580,20 @@ constexpr int RD_RX = RD_X + 4, RD_RY = RD_Y + 4;   // YCbCr region (even origin
 constexpr int RD_QX = RD_RX / 2, RD_QY = RD_RY / 2;  // its quads
 constexpr int RD_WX = RD_RX + 2, RD_WY = RD_RY + 2;  // mosaic / green window of the region
 constexpr int RD_QPT = (RD_QX * RD_QY + 255) / 256;  // quads per thread
+// A window row in LDS: its even and its odd columns as two planes of
+// RD_PS floats, RD_WS floats per row.  Quad (qx, qy) reads column 2 qx + i
+// of row 2 qy + j, i.e. word (2 qy + j) RD_WS + (i & 1) RD_PS + qx + i / 2;
+// two rows are 162 words, 34 mod 64, so the bank is q mod 64 plus a
+// constant (q = 34 qy + qx) and a wave's 64 consecutive quads meet 64
+// distinct banks.
+constexpr int RD_PS = 40, RD_WS = 81;
+static_assert(RD_WX / 2 <= RD_PS && RD_PS + (RD_WX - 1) / 2 < RD_WS && RD_QX == 34, "window planes");
+__device__ __forceinline__ int rd_col(int i) { return (i & 1) * RD_PS + (i >> 1); }
 }
 
 union RbDespeckleLds {
     struct {
-        float r[RD_WY][RD_WX], g[RD_WY][RD_WX];
+        float r[RD_WY][RD_WS], g[RD_WY][RD_WS];
     } win;
     float4 t[RD_RY][RD_RX];
 };
@@ -607,8 +616,8 @@ __global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img gr
             return make_float2(raw(rx0 - 1 + i, ry0 - 1 + j), ld1(green, rx0 - 1 + i, ry0 - 1 + j));
         },
         [&](int k, float2 v) {
-            sm.win.r[k / RD_WX][k % RD_WX] = v.x;
-            sm.win.g[k / RD_WX][k % RD_WX] = v.y;
+            sm.win.r[k / RD_WX][rd_col(k % RD_WX)] = v.x;
+            sm.win.g[k / RD_WX][rd_col(k % RD_WX)] = v.y;
         });
     __syncthreads();
     float3 o[RD_QPT][2][2];
@@ -623,8 +632,8 @@ __global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img gr
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                R[j][i] = sm.win.r[2 * qy + j][2 * qx + i];
-                G[j][i] = sm.win.g[2 * qy + j][2 * qx + i];
+                R[j][i] = sm.win.r[2 * qy + j][rd_col(2 * qx + i)];
+                G[j][i] = sm.win.g[2 * qy + j][rd_col(2 * qx + i)];
             }
         red_blue_from<true>(R, G, b, red_is_c1_site, cvar, clip, m, o[n]);
 #pragma unroll
