@@ -33,6 +33,10 @@ struct DngError {
 
 [[noreturn]] void fail(const std::string& msg) { throw DngError{msg}; }
 
+// largest raw image decoded (samples of the full, uncropped image): 1 Gi
+// samples (2 GiB), far above any camera's mosaic
+constexpr size_t kMaxSamples = size_t(1) << 30;
+
 // ------------------------------------------------------------ TIFF container
 enum : uint16_t {
     kNewSubFileType = 254, kImageWidth = 256, kImageLength = 257, kBitsPerSample = 258, kCompression = 259,
@@ -168,6 +172,10 @@ void build_huff(const uint8_t counts[16], const uint8_t* vals, int nvals, Huff& 
     for (int len = 1; len <= 16; len++) {
         h.valptr[len] = k;
         h.mincode[len] = code;
+        // Annex C: the codes of one length must fit in `len` bits; checked
+        // before any lookup entry of this length is written
+        if (code + counts[len - 1] > (1 << len)) fail("lossless JPEG: bad Huffman table");
+        if (k + counts[len - 1] > nvals) fail("lossless JPEG: Huffman table with fewer symbols than codes");
         for (int i = 0; i < counts[len - 1]; i++, k++, code++) {
             if (len <= 9) {
                 const int shift = 9 - len;
@@ -175,7 +183,6 @@ void build_huff(const uint8_t counts[16], const uint8_t* vals, int nvals, Huff& 
             }
         }
         h.maxcode[len] = counts[len - 1] ? code - 1 : -1;
-        if (code > (1 << len)) fail("lossless JPEG: bad Huffman table");
         code <<= 1;
     }
     h.maxcode[17] = 0x7fffffff;
@@ -313,6 +320,8 @@ void lj92_decode(const uint8_t* d, size_t n, uint16_t* out, size_t count) {
         p += len;
     }
     if (P < 2 || P > 16 || X <= 0 || Y <= 0 || pred < 1 || pred > 7) fail("lossless JPEG: unsupported frame");
+    // the first predictor 2^(P - Pt - 1) needs a point transform below P
+    if (al >= P) fail("lossless JPEG: point transform not below the sample precision");
     // a restart interval is a whole number of rows (T.81 H.1.1; the reference's
     // decoder counts it in rows, src/gls_dng_lossless_jpeg.cpp:387,1173)
     if (restart && restart % X) fail("lossless JPEG: restart interval not a whole number of rows");
@@ -500,17 +509,26 @@ void decode_pixels(const Tiff& t, const RawIfd& r, const Decoded& dec, uint16_t*
             unpack_block(t, off, bytes, bits, buf, (size_t)bw * bh);
         }
     };
+    // sizes come from the file: bound them before anything is allocated from them
+    if ((size_t)W * (size_t)H > kMaxSamples) fail("DNG: image larger than " + std::to_string(kMaxSamples) + " samples");
     for (int y = 0; y < in.height; y++) std::fill(pixels + (size_t)y * pitch, pixels + (size_t)y * pitch + in.width, 0);
     if (in.tiled) {
-        const int tw = (int)ifd.get(kTileWidth, 0), th = (int)ifd.get(kTileLength, 0);
+        const double twd = ifd.get(kTileWidth, 0), thd = ifd.get(kTileLength, 0);
+        // a tile may overhang the image, but by less than one tile per axis
+        if (!(twd >= 1 && thd >= 1 && twd <= 2.0 * W + 16 && thd <= 2.0 * H + 16))
+            fail("DNG: tile size outside the image's bounds");
+        const int tw = (int)twd, th = (int)thd;
         const Tag* to = ifd.find(kTileOffsets);
         const Tag* tc = ifd.find(kTileByteCounts);
-        if (tw <= 0 || th <= 0 || !to || !tc || to->num.size() != tc->num.size()) fail("DNG: bad tile layout");
+        if (!to || !tc || to->num.size() != tc->num.size()) fail("DNG: bad tile layout");
         const int tx = (W + tw - 1) / tw, ty = (H + th - 1) / th;
         if ((int)to->num.size() < tx * ty) fail("DNG: missing tiles");
         const int ntiles = tx * ty;
         const int nthreads = std::max(1, std::min<int>(ntiles, std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+        // a worker's exception must not leave its thread (std::terminate):
+        // each keeps its message, and out-of-memory is rethrown as such
         std::vector<std::string> errs(nthreads);
+        std::vector<char> oom(nthreads, 0);
         std::vector<std::thread> pool;
         for (int k = 0; k < nthreads; k++) {
             pool.emplace_back([&, k]() {
@@ -523,10 +541,16 @@ void decode_pixels(const Tiff& t, const RawIfd& r, const Decoded& dec, uint16_t*
                     }
                 } catch (const DngError& e) {
                     errs[k] = e.msg;
+                } catch (const std::bad_alloc&) {
+                    oom[k] = 1;
+                } catch (const std::exception& e) {
+                    errs[k] = std::string("DNG: ") + e.what();
                 }
             });
         }
         for (auto& th_ : pool) th_.join();
+        for (char o : oom)
+            if (o) throw std::bad_alloc();
         for (const auto& e : errs)
             if (!e.empty()) fail(e);
     } else {
@@ -674,6 +698,8 @@ extern "C" clim_status clim_dng_decode(const uint8_t* data, int64_t size, clim_d
         return dng_fail(CLIM_EINVAL, e.msg);
     } catch (const std::bad_alloc&) {
         return dng_fail(CLIM_ENOMEM, "clim_dng_decode: out of memory");
+    } catch (const std::exception& e) {
+        return dng_fail(CLIM_EINVAL, std::string("clim_dng_decode: ") + e.what());
     }
 }
 

@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# CLIM_ORACLE_LIB: another build of the same sources (the sanitizer build of
+# `make -C oracle sanitize`, loaded by tests/test_oracle_sanitize.py)
+LIB_PATH = os.environ.get("CLIM_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 _lib = None
 
