@@ -113,7 +113,7 @@ def exported_symbols():
                         "clim_dng_last_error",
                         "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
                         "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_raw_to_rgb_batch",
-                        "clim_generate_bayer")]
+                        "clim_generate_bayer", "clim_compute_raw_noise_statistics_noalloc")]
 
 
 def _f(vals):
@@ -276,6 +276,15 @@ class Context:
         self._check(lib().clim_compute_raw_noise_statistics(self.h, scaled.ref(), bayer, out),
                     "computeRawNoiseStatistics")
         return np.array(out[:], np.float32)
+
+    def computeRawNoiseStatisticsNoAlloc(self, scaled, bayer, nlf_dev, status_dev=None):
+        """clim_compute_raw_noise_statistics_noalloc: asynchronous, no device
+        allocation; nlf_dev (a device image of at least 8 floats) receives
+        A[4], B[4], status_dev (nullable, a device image) the fit status."""
+        self._check(lib().clim_compute_raw_noise_statistics_noalloc(
+            self.h, scaled.ref(), bayer, C.c_void_p(nlf_dev.im.data),
+            C.c_void_p(status_dev.im.data) if status_dev is not None else None),
+            "computeRawNoiseStatisticsNoAlloc")
 
     def interpolateGreen(self, scaled, green, bayer, luma_variance):
         self._check(lib().clim_interpolate_green(self.h, scaled.ref(), green.ref(), bayer,

@@ -153,6 +153,7 @@ extern "C" clim_status clim_create(clim_ctx** out, int device) {
     if (hipSetDevice(device) != hipSuccess) return CLIM_ENODEV;
     clim_ctx* ctx = new clim_ctx();
     ctx->device = device;
+    clim_noalloc_init(device, &ctx->noalloc_scratch);
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&ctx->fit, sizeof(clim::FitState) * kFitJobs);
     if (e == hipSuccess) e = hipMemset(ctx->fit, 0, sizeof(clim::FitState) * kFitJobs);

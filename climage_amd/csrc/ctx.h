@@ -85,6 +85,9 @@ struct clim_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<ProfRec> prof_recs;
     size_t ev_next = 0;
+    // device address of the static scratch array of
+    // clim_compute_raw_noise_statistics_noalloc on this context's device
+    void* noalloc_scratch = nullptr;
 };
 
 // brackets the launches issued in its scope with two events when profiling
@@ -102,6 +105,9 @@ clim_status owned_alloc(clim_ctx* ctx, OwnedImg& o, int w, int h, int format);
 void owned_free(OwnedImg& o);
 clim::Bayer make_bayer(int bayer);
 int scene_colors(int bayer);
+// the static scratch array's device address on `device` (current), taken once
+// per device (noalloc.cpp); null when hipGetSymbolAddress failed
+void clim_noalloc_init(int device, void** ptr);
 
 #define CLIM_TRY_HIP(ctx, call)                                  \
     do {                                                         \

@@ -379,6 +379,25 @@ clim_status clim_generate_bayer(clim_ctx* ctx, clim_image* raw, clim_image* clea
                                 uint32_t scene_seed, uint32_t noise_seed, float noise_a,
                                 float noise_b, float black_level, float white_level);
 
+/* ------------------------------------------- allocation-free variant
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  clim_compute_raw_noise_statistics with every work buffer
+ * (sample lists, block partials, fit state) in one static device array of
+ * the library, CLIM_NOALLOC_SCRATCH_BYTES on each device it runs on, held
+ * until the process ends; the call allocates nothing and does not block.
+ * nlf_dev (device memory, 8 floats) receives A[4], B[4]; status_dev
+ * (nullable, device memory) receives 0 (fitted), 1 (no sample in range:
+ * nlf_dev receives zeros) or 2 (no inliers in the refit: the first-pass
+ * model).  Mosaics whose quad grid, each axis rounded up to 16, exceeds
+ * CLIM_NOALLOC_MAX_QUADS (e.g. larger than 4096 x 3072 pixels) give
+ * CLIM_EINVAL.  Every call on one device shares the array: calls that
+ * overlap on one GPU (two streams or two contexts) race. */
+#define CLIM_NOALLOC_MAX_QUADS 3145728
+#define CLIM_NOALLOC_SCRATCH_BYTES (CLIM_NOALLOC_MAX_QUADS * 32 + 1048576)
+clim_status clim_compute_raw_noise_statistics_noalloc(clim_ctx* ctx, const clim_image* scaled, int32_t bayer,
+                                                      float* nlf_dev, int32_t* status_dev);
+
 /* ------------------------------------------------------------ profiling
  * Per-stage device timing with HIP events on the context's stream (the
  * counterpart of the reference's PRINT_EXECUTION_TIME wall-clock block,
