@@ -95,7 +95,7 @@ def lib():
     return _lib
 
 
-PROFILE_STAGES = 13  # CLIM_PROFILE_STAGES
+PROFILE_STAGES = 14  # CLIM_PROFILE_STAGES
 
 
 def exported_symbols():
@@ -113,7 +113,31 @@ def exported_symbols():
                         "clim_dng_last_error",
                         "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
                         "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_raw_to_rgb_batch",
-                        "clim_generate_bayer", "clim_compute_raw_noise_statistics_noalloc")]
+                        "clim_generate_bayer", "clim_compute_raw_noise_statistics_noalloc",
+                        "clim_strength_set_weights", "clim_strength_get_weights", "clim_strength_set_mode",
+                        "clim_strength_gate_read", "clim_strength_map_read", "clim_strength_loss_grad",
+                        "clim_strength_train")]
+
+
+# learned denoise strength (include/climage.h, clim_strength_*)
+STRENGTH_PARAMS = 26612
+STRENGTH_OFF, STRENGTH_GATED, STRENGTH_FORCE_PLAIN, STRENGTH_FORCE_LEARNED = 0, 1, 2, 3
+STRENGTH_WEIGHTS = os.path.join(HERE, "weights", "strength.f32")
+
+
+class StrengthTrainConfig(C.Structure):
+    _fields_ = [("frames", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("crop", C.c_int32),
+                ("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("seed", C.c_uint32),
+                ("bayer", C.c_int32), ("noise_levels", C.c_int32)]
+
+
+def load_strength_weights(path=STRENGTH_WEIGHTS):
+    """The trained network parameters shipped with the package (float32,
+    the canonical layout of clim_strength_set_weights)."""
+    p = np.fromfile(path, dtype="<f4")
+    if p.size != STRENGTH_PARAMS:
+        raise ValueError(f"{path}: {p.size} parameters, expected {STRENGTH_PARAMS}")
+    return p
 
 
 def _f(vals):
@@ -417,6 +441,60 @@ class Context:
         return [(np.array(NM[i].rawNlf[:], np.float32),
                  np.array([NM[i].pyramidNlf[l][:] for l in range(params[i].levels)], np.float32))
                 for i in range(n)]
+
+    # ---- learned denoise strength ----------------------------------------
+    def strength_set_weights(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(lib().clim_strength_set_weights(self.h, p.ctypes.data_as(C.c_void_p), C.c_int64(p.size)),
+                    "strength_set_weights")
+
+    def strength_get_weights(self):
+        p = np.zeros(STRENGTH_PARAMS, np.float32)
+        self._check(lib().clim_strength_get_weights(self.h, p.ctypes.data_as(C.c_void_p), C.c_int64(p.size)),
+                    "strength_get_weights")
+        return p
+
+    def strength_set_mode(self, mode):
+        self._check(lib().clim_strength_set_mode(self.h, int(mode)), "strength_set_mode")
+
+    def strength_gate_read(self):
+        o = (C.c_int32 * 5)()
+        m = (C.c_float * 5)()
+        self._check(lib().clim_strength_gate_read(self.h, o, m), "strength_gate_read")
+        return np.array(o[:], np.int32), np.array(m[:], np.float32)
+
+    def strength_map_read(self):
+        gw, gh = C.c_int32(), C.c_int32()
+        self._check(lib().clim_strength_map_read(self.h, None, C.c_int64(0), C.byref(gw), C.byref(gh)),
+                    "strength_map_read")
+        m = np.zeros((gh.value, gw.value, 8), np.float32)
+        self._check(lib().clim_strength_map_read(self.h, m.ctypes.data_as(C.c_void_p), C.c_int64(m.size),
+                                                 C.byref(gw), C.byref(gh)), "strength_map_read")
+        return m
+
+    def strength_loss_grad(self, params, feat, curves, norm):
+        """(loss (2,), gradient (STRENGTH_PARAMS,), raw outputs (n, h, w, 32))."""
+        p = np.ascontiguousarray(params, np.float32)
+        f = np.ascontiguousarray(feat, np.float32)
+        n, h, w, _ = f.shape
+        cv = np.ascontiguousarray(curves, np.float32).reshape(n, h, w, 70)
+        nm = np.ascontiguousarray(norm, np.float32).reshape(n, 10)
+        out = np.zeros((n, h, w, 32), np.float32)
+        g = np.zeros(STRENGTH_PARAMS, np.float32)
+        loss = (C.c_double * 2)()
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(lib().clim_strength_loss_grad(self.h, ptr(p), ptr(f), ptr(cv), ptr(nm), n, h, w, ptr(out),
+                                                  ptr(g), loss), "strength_loss_grad")
+        return np.array(loss[:]), g, out
+
+    def strength_train(self, frames, width=4032, height=3024, crop=64, epochs=10, batch=32, lr=1e-3, seed=10000,
+                       bayer=RGGB, noise_levels=6):
+        """clim_strength_train; returns the (epochs, 2) sampled losses."""
+        cfg = StrengthTrainConfig(frames, width, height, crop, epochs, batch, lr, seed, bayer, noise_levels)
+        log = np.zeros(2 * epochs, np.float32)
+        self._check(lib().clim_strength_train(self.h, C.byref(cfg), log.ctypes.data_as(C.c_void_p),
+                                              C.c_int64(log.size)), "strength_train")
+        return log.reshape(epochs, 2)
 
     def generate_bayer(self, raw, bayer, scene_seed, noise_seed, a, b, black, white, clean=None):
         self._check(lib().clim_generate_bayer(self.h, raw.ref() if raw is not None else None,

@@ -188,6 +188,7 @@ extern "C" void clim_destroy(clim_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     free_workspace(ctx->ws);
+    sn_release(ctx);
     for (auto& o : ctx->op_guided_imgs) owned_free(o);
     owned_free(ctx->op_mean);
     owned_free(ctx->op_var);
@@ -681,7 +682,7 @@ extern "C" void clim_default_params(clim_demosaic_params* p, int32_t bayer, floa
 
 static const char* kStageNames[CLIM_PROFILE_STAGES] = {
     "scale_raw", "raw_stats", "raw_fit", "green", "red_blue_ycc", "despeckle",
-    "downsample", "level_stats", "level_fit", "denoise", "reassemble", "tone", "raw_denoise"};
+    "downsample", "level_stats", "level_fit", "denoise", "reassemble", "tone", "raw_denoise", "strength"};
 
 static constexpr size_t kProfEvents = 4096;
 

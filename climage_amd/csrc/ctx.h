@@ -6,6 +6,7 @@
 
 #include "climage.h"
 #include "kernels.h"
+#include "strength.h"
 
 // streams a clim_raw_to_rgb_batch call spreads its frames over (two fill
 // the device: three and four measured no faster on 12 MP frames)
@@ -44,7 +45,18 @@ constexpr int kFitK = 21;                       // largest accumulator count (ra
 
 enum Stage {
     ST_SCALE = 0, ST_RAW_STATS, ST_RAW_FIT, ST_GREEN, ST_RED_BLUE, ST_DESPECKLE,
-    ST_DOWNSAMPLE, ST_LEVEL_STATS, ST_LEVEL_FIT, ST_DENOISE, ST_REASSEMBLE, ST_TONE, ST_RAW_DENOISE
+    ST_DOWNSAMPLE, ST_LEVEL_STATS, ST_LEVEL_FIT, ST_DENOISE, ST_REASSEMBLE, ST_TONE, ST_RAW_DENOISE, ST_STRENGTH
+};
+
+// learned denoise strength (strength.h, strength.cpp) on one context
+struct SnTrainer;
+struct SnState {
+    int mode = 0;                       // clim_strength_mode
+    clim::SnWeights wt{};               // device weights loaded on this context (null until set)
+    const clim::SnWeights* use = nullptr;  // the weights inference reads (a batch lane: its owner's)
+    DevBuf feat, act0, act1, map, gate_sum, open;  // per-frame buffers (grid of level 2)
+    int gw = 0, gh = 0, levels = 0;
+    SnTrainer* trainer = nullptr;       // training / gradient buffers, created on first use
 };
 
 struct ProfRec {
@@ -85,6 +97,7 @@ struct clim_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<ProfRec> prof_recs;
     size_t ev_next = 0;
+    SnState sn;
     // device address of the static scratch array of
     // clim_compute_raw_noise_statistics_noalloc on this context's device
     void* noalloc_scratch = nullptr;
@@ -108,6 +121,11 @@ int scene_colors(int bayer);
 // the static scratch array's device address on `device` (current), taken once
 // per device (noalloc.cpp); null when hipGetSymbolAddress failed
 void clim_noalloc_init(int device, void** ptr);
+// learned strength (strength.cpp): the network on level 2 of a frame's
+// pyramid after the per-level fits, filling sr for the level filters (off
+// unless a mode and weights are set); the context's strength buffers freed
+clim_status strength_infer(clim_ctx* ctx, const clim::Img* lvl, int levels, int algorithm, clim::StrengthRef* sr);
+void sn_release(clim_ctx* ctx);
 
 #define CLIM_TRY_HIP(ctx, call)                                  \
     do {                                                         \

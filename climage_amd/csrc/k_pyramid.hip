@@ -286,7 +286,7 @@ bool launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipS
 template <bool SAME_EDGE>
 __device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int tx, int ty, float va0, float va1,
                                                float va2, float vb0, float vb1, float vb2, float chromaBoost,
-                                               float gradientBoost) {
+                                               float gradientBoost, float m) {
     const int X = tx + HR, Y = ty + HR;
     const float4 cp = t.p[Y][X];
     const float cY = cp.x, cCb = cp.y, cCr = cp.z;
@@ -313,6 +313,12 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int 
     float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
     e0 = e0 * e0;
     e1 = e1 * e1;
+    if (m > 0.0f) {
+        // learned strength: the range sigma times m, i.e. the squared edges times m^2
+        const float m2 = m * m;
+        e0 = e0 * m2;
+        e1 = e1 * m2;
+    }
     // 1 / sigma per channel; the squared distance by clim_range_len2
     const float r0 = clim_inv_sigma(va0 + vb0 * cY), r1 = clim_inv_sigma(va1 + vb1 * cY),
                 r2 = clim_inv_sigma(va2 + vb2 * cY);
@@ -352,7 +358,8 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int 
 // one 32x16 tile (x0, y0) of denoiseImage
 template <bool SAME_EDGE>
 __device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, const Img& in, const Img& out, int x0, int y0,
-                                               const ParamRef& nlf, float chromaBoost, float gradientBoost) {
+                                               const ParamRef& nlf, float chromaBoost, float gradientBoost,
+                                               const StrengthRef& sr, int level) {
     t.load(in, x0, y0, threadIdx.y * 32 + threadIdx.x);
     __syncthreads();
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
@@ -362,17 +369,18 @@ __device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, const Img&
         const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
         const int x = x0 + tx, y = y0 + ty;
         if (x >= out.w || y >= out.h) continue;
+        const float m = strength_at(sr, level, x, y);
         wrow_f4(out, y)[x] =
-            bilateral_px<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost);
+            bilateral_px<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost, m);
     }
 }
 
 template <bool SAME_EDGE>
 __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
-                                                   float gradientBoost) {
+                                                   float gradientBoost, StrengthRef sr, int level) {
     __shared__ LevelTile4<TX, TY> t;
     const TileId tl = xcd_tile();
-    bilateral_tile<SAME_EDGE>(t, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost);
+    bilateral_tile<SAME_EDGE>(t, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost, sr, level);
 }
 
 // denoiseImage of every pyramid level in one launch (the levels' denoising
@@ -386,20 +394,22 @@ __global__ void __launch_bounds__(256) k_bilateral_levels(BilateralLevels lv) {
     while (l + 1 < lv.n && b >= lv.base[l + 1]) l++;
     const int i = b - lv.base[l], tiles_x = (lv.out[l].w + TX - 1) / TX;
     bilateral_tile<SAME_EDGE>(t, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY, lv.nlf[l],
-                              lv.chromaBoost[l], lv.gradientBoost[l]);
+                              lv.chromaBoost[l], lv.gradientBoost[l], lv.sr, l);
 }
 
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
-                      hipStream_t st) {
+                      hipStream_t st, const StrengthRef* srp, int level) {
     dim3 blk(32, 8), grd((out.w + TX - 1) / TX, (out.h + TY - 1) / TY);
+    const StrengthRef sr = srp ? *srp : StrengthRef{nullptr, 0, 0, nullptr, 0.0f};
     // the edges as the kernel computes them (lumaBoost = 1 without gradient boost)
     float e0 = 1.0f * 1.41421356f, e1 = chromaBoost * 1.41421356f;
     e0 = e0 * e0;
     e1 = e1 * e1;
     if (!(gradientBoost > 1.0f) && e0 == e1)
-        hipLaunchKernelGGL(k_bilateral<true>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost);
+        hipLaunchKernelGGL(k_bilateral<true>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost, sr, level);
     else
-        hipLaunchKernelGGL(k_bilateral<false>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost);
+        hipLaunchKernelGGL(k_bilateral<false>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost, sr,
+                           level);
 }
 
 // the one-test form when launch_bilateral would take it for every level
@@ -428,7 +438,7 @@ struct GuidedTile {
     float h[9][TY + 4][TX + 1];
 };
 
-__global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef nlf) {
+__global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef nlf, StrengthRef sr, int level) {
     __shared__ GuidedTile sm;
     const TileId tl = xcd_tile();
     const int x0 = tl.x * TX, y0 = tl.y * TY;
@@ -465,7 +475,14 @@ __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef n
         if (x >= out.w || y >= out.h) continue;
         const int X = tx + HR, Y = ty + HR;
         const float ix = sm.t.x[Y][X], iy = sm.t.y[Y][X], iz = sm.t.z[Y][X];
-        const float eps0 = va0 + vb0 * ix, eps1 = va1 + vb1 * ix, eps2 = va2 + vb2 * ix;
+        float eps0 = va0 + vb0 * ix, eps1 = va1 + vb1 * ix, eps2 = va2 + vb2 * ix;
+        const float m = strength_at(sr, level, x, y);
+        if (m > 0.0f) {
+            // learned strength: epsilon times m
+            eps0 = eps0 * m;
+            eps1 = eps1 * m;
+            eps2 = eps2 * m;
+        }
         float acc[9];
 #pragma unroll
         for (int q = 0; q < 9; q++) {
@@ -508,9 +525,11 @@ __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef n
     }
 }
 
-void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st) {
+void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st, const StrengthRef* srp,
+                        int level) {
     dim3 blk(32, 8), grd((out.w + TX - 1) / TX, (out.h + TY - 1) / TY);
-    hipLaunchKernelGGL(k_guided_fast, grd, blk, 0, st, in, out, nlf);
+    const StrengthRef sr = srp ? *srp : StrengthRef{nullptr, 0, 0, nullptr, 0.0f};
+    hipLaunchKernelGGL(k_guided_fast, grd, blk, 0, st, in, out, nlf, sr, level);
 }
 
 // -------------------------------------------------------------- convertTosRGB

@@ -109,18 +109,52 @@ void launch_rb_despeckle(const MosaicArg& raw, const Img& green, const Img& out,
 // ------------------------------------------------------------- pyramid stages
 void launch_downsample(const Img& in, const Img& out, hipStream_t st);
 void launch_noise_stats(const Img& in, const Img& out, hipStream_t st);
+// Per-texel denoise strength of the level filters (the learned-strength
+// network's map, strength.h): the bilateral range threshold is multiplied by
+// m^2 (the range sigma by m), the guided epsilon by m.  Off (the plain
+// filters, untouched arithmetic) when map is null and constant <= 0.
+struct StrengthRef {
+    const float* map;   // [gh][gw][8] on the grid of level 2: multiplier of level l in channel l
+    int gw, gh;
+    const int* open;    // [5] device: level l reads the map only when open[l] != 0
+    float constant;     // > 0: this multiplier at every texel (the map is not read)
+};
+// The multiplier at texel (x, y) of `level`, or 0 when the strength is off
+// there.  The map is sampled bilinearly (clamped) at the texel's centre,
+// grid coordinate (x + 1/2) 2^(level - 2) - 1/2.
+__device__ __forceinline__ float strength_at(const StrengthRef& s, int level, int x, int y) {
+    if (s.constant > 0.0f) return s.constant;
+    if (!s.map || !s.open[level]) return 0.0f;
+    const float sc = level == 0 ? 0.25f : level == 1 ? 0.5f : level == 2 ? 1.0f : level == 3 ? 2.0f : 4.0f;
+    const float gx = ((float)x + 0.5f) * sc - 0.5f, gy = ((float)y + 0.5f) * sc - 0.5f;
+    const float fx = floorf(gx), fy = floorf(gy);
+    const float ax = gx - fx, ay = gy - fy;
+    const int ix = (int)fx, iy = (int)fy;
+    const int x0 = clampi(ix, 0, s.gw - 1), x1 = clampi(ix + 1, 0, s.gw - 1);
+    const int y0 = clampi(iy, 0, s.gh - 1), y1 = clampi(iy + 1, 0, s.gh - 1);
+    const float* m = s.map + level;
+    const float v00 = m[((size_t)y0 * s.gw + x0) * 8], v01 = m[((size_t)y0 * s.gw + x1) * 8];
+    const float v10 = m[((size_t)y1 * s.gw + x0) * 8], v11 = m[((size_t)y1 * s.gw + x1) * 8];
+    const float top = v00 + (v01 - v00) * ax, bot = v10 + (v11 - v10) * ax;
+    return top + (bot - top) * ay;
+}
+
+// level: the texels' pyramid level (for the strength map)
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
-                      hipStream_t st);
-// denoiseImage of n pyramid levels in one launch (base: set by the launcher)
+                      hipStream_t st, const StrengthRef* sr = nullptr, int level = 0);
+// denoiseImage of n pyramid levels in one launch (base: set by the launcher);
+// entry l is pyramid level l
 struct BilateralLevels {
     Img in[5], out[5];
     ParamRef nlf[5];
     float chromaBoost[5], gradientBoost[5];
     int base[5];
     int n;
+    StrengthRef sr;
 };
 void launch_bilateral_levels(BilateralLevels& lv, hipStream_t st);
-void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st);
+void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st,
+                        const StrengthRef* sr = nullptr, int level = 0);
 // convertTosRGB applied to reassembleImage's result in the same kernel
 struct ToneLaunch {
     Img out8, outf;  // outf.p may be null

@@ -131,6 +131,9 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         ProfScope ps(ctx, ST_LEVEL_FIT, nlev);
         CLIM_TRY(run_fit_rest(ctx, jobs, false));
     }
+    // the learned strength of the level filters (off: sr leaves them plain)
+    clim::StrengthRef sr{nullptr, 0, 0, nullptr, 0.0f};
+    CLIM_TRY(strength_infer(ctx, lvl, levels, algorithm, &sr));
 
     // synthesis, coarse to fine.  The bilateral denoiser reads only the noisy
     // levels, so every level is denoised in one launch ahead of the
@@ -139,6 +142,7 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         ProfScope ps(ctx, ST_DENOISE, nlev);
         clim::BilateralLevels bl{};
         bl.n = levels;
+        bl.sr = sr;
         for (int i = 0; i < levels; i++) {
             bl.in[i] = lvl[i];
             bl.out[i] = den[i];
@@ -163,9 +167,9 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
                 // GuidedPreciseDenoiser passes var_b as eps (cpp/pyramidal_denoise.cpp:65)
                 clim::launch_guided_filter(lvl[i], den[i], 5, dev_params(ctx->fit[1 + i].cal + 3), gv, st);
             } else if (bilateral) {
-                clim::launch_bilateral(lvl[i], den[i], cal, dp[i].chromaBoost, dp[i].gradientBoost, st);
+                clim::launch_bilateral(lvl[i], den[i], cal, dp[i].chromaBoost, dp[i].gradientBoost, st, &sr, i);
             } else {
-                clim::launch_guided_fast(lvl[i], den[i], cal, st);
+                clim::launch_guided_fast(lvl[i], den[i], cal, st, &sr, i);
             }
         }
         if (i < levels - 1) {
@@ -484,6 +488,11 @@ extern "C" clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const cli
         CLIM_TRY_HIP(ctx, hipEventRecord(ctx->lane_ev[0], ctx->stream));
         for (int k = 1; k < nl; k++)
             CLIM_TRY_HIP(ctx, hipStreamWaitEvent(ctx->lanes[k - 1]->stream, ctx->lane_ev[0], 0));
+    }
+    // the lanes run the learned strength as this context does, on its weights
+    for (int k = 1; k < nl; k++) {
+        ctx->lanes[k - 1]->sn.mode = ctx->sn.mode;
+        ctx->lanes[k - 1]->sn.use = ctx->sn.use;
     }
     clim_status st = CLIM_OK;
     for (int i = 0; i < n && st == CLIM_OK; i++) {

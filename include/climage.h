@@ -379,6 +379,67 @@ clim_status clim_generate_bayer(clim_ctx* ctx, clim_image* raw, clim_image* clea
                                 uint32_t scene_seed, uint32_t noise_seed, float noise_a,
                                 float noise_b, float black_level, float white_level);
 
+/* --------------------------------------------- learned denoise strength
+ * No reference counterpart: a method change at the per-level filter plug
+ * point (ImageDenoiser::denoise, cpp/pyramidal_denoise.hpp:24-34).  A
+ * 4-layer 3x3 convolutional network (8 -> 32 -> 32 -> 32 -> 20 channels,
+ * bfloat16 weights, FP32 accumulation on the matrix cores) runs once per
+ * frame on the grid of pyramid level 2, after the per-level noise fits, and
+ * predicts for every texel and level a multiplier of the bilateral range
+ * sigma (cl/demosaic.cl:735-765) or of the guided filter's epsilon
+ * (cl/demosaic.cl:767-851), and the level's error relative to the plain
+ * strength.  A level uses the multipliers only when its gate is open; shut,
+ * it runs the plain filter and its output is the plain pyramid's bit for
+ * bit.  It applies to the Bilateral and GuidedFast denoisers of pyramids with
+ * at least 3 levels (otherwise every level is plain).
+ *
+ * Parameters: CLIM_STRENGTH_PARAMS floats, per layer l the weights
+ * W[tap = 3 ky + kx][cin][cout] then the biases b[cout]; weights are rounded
+ * to bfloat16 when set. */
+#define CLIM_STRENGTH_PARAMS 26612
+typedef enum {
+    CLIM_STRENGTH_OFF = 0,            /* the plain pyramid (default) */
+    CLIM_STRENGTH_GATED = 1,          /* a level opens when its mean predicted log2 error ratio is < 0 */
+    CLIM_STRENGTH_FORCE_PLAIN = 2,    /* the network runs, every gate shut: the plain output */
+    CLIM_STRENGTH_FORCE_LEARNED = 3   /* every gate open */
+} clim_strength_mode;
+clim_status clim_strength_set_weights(clim_ctx* ctx, const float* params, int64_t n);
+clim_status clim_strength_get_weights(clim_ctx* ctx, float* params, int64_t n);
+/* applies to later clim_raw_to_rgb / clim_raw_to_rgb_batch /
+ * clim_pyramidal_denoise calls on the context; needs weights unless OFF */
+clim_status clim_strength_set_mode(clim_ctx* ctx, int32_t mode);
+/* Blocking.  The last frame's gate: open[l], and (nullable) the mean
+ * predicted log2 error ratio of level l over the grid. */
+clim_status clim_strength_gate_read(clim_ctx* ctx, int32_t open[5], float mean_pred[5]);
+/* Blocking.  The last frame's multiplier map: gw x gh texels (the grid of
+ * level 2) of 8 floats, level l's multiplier in float l.  With map null only
+ * gw and gh are written. */
+clim_status clim_strength_map_read(clim_ctx* ctx, float* map, int64_t capacity, int32_t* gw, int32_t* gh);
+/* Blocking.  The training loss and its gradient on one batch (host arrays):
+ * feat n x h x w x 8 network inputs (rounded to bfloat16), curves
+ * n x h x w x 70 error curves ([level][filter][k] for log2 strength (k-3)/2),
+ * norm n x 10 normalisers ([level][filter], > 0 to count).  out_raw
+ * (nullable): n x h x w x 32 raw outputs (channels >= 20 zero); grad
+ * (nullable): CLIM_STRENGTH_PARAMS; loss: the strength and prediction terms. */
+clim_status clim_strength_loss_grad(clim_ctx* ctx, const float* params, const float* feat, const float* curves,
+                                    const float* norm, int32_t n, int32_t h, int32_t w, float* out_raw, float* grad,
+                                    double loss[2]);
+typedef struct {
+    int32_t frames;         /* synthetic frames (seeds seed + i, noise seeds seed + 100000 + i) */
+    int32_t width, height;  /* frame size (multiples of 4, at least 256) */
+    int32_t crop;           /* crop size on the level-2 grid (64: 256 x 256 frame pixels) */
+    int32_t epochs, batch;  /* passes over the crops, crops per step */
+    float lr;               /* Adam step (a tenth over the last fifth of the epochs) */
+    uint32_t seed;
+    int32_t bayer;
+    int32_t noise_levels;   /* frame i has Poisson gain a_(i mod noise_levels) of 0.0005, 0.001, 0.002,
+                               0.004, 0.008, 0.016, 0.003, 0.006; read noise a / 200 */
+} clim_strength_train_config;
+/* Blocking.  Generates the crops, trains from seeded initial weights with
+ * the library's kernels, and sets the result as the context's weights.
+ * log (nullable, 2 x epochs floats): each epoch's sampled losses. */
+clim_status clim_strength_train(clim_ctx* ctx, const clim_strength_train_config* cfg, float* log, int64_t log_n);
+
 /* ------------------------------------------- allocation-free variant
  * No counterpart in the reference.  For callers that cannot allocate device
  * memory: code under HIP graph capture, tight serving loops, harnesses that
@@ -405,7 +466,7 @@ clim_status clim_compute_raw_noise_statistics_noalloc(clim_ctx* ctx, const clim_
  * launch of clim_raw_to_rgb is bracketed by two events; clim_profile_read
  * waits for the stream, adds the elapsed times per stage and clears the
  * record.  Stage names: clim_profile_stage_name(i), i < CLIM_PROFILE_STAGES. */
-#define CLIM_PROFILE_STAGES 13
+#define CLIM_PROFILE_STAGES 14
 typedef struct {
     double ms[CLIM_PROFILE_STAGES];         /* summed kernel time per stage */
     int64_t launches[CLIM_PROFILE_STAGES];  /* launches per stage */

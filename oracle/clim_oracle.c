@@ -711,12 +711,55 @@ void P(noise_statistics)(const real* in, int w, int h, real* out) {
     }
 }
 
+/* ----------------------------------------------------- learned strength map
+ * The learned-strength network's per-level multipliers (no reference
+ * counterpart; climage_amd/csrc/kernels.h strength_at): a map of gw x gh
+ * texels (the grid of pyramid level 2) of 8 floats, level l's multiplier in
+ * float l, sampled bilinearly (clamped) at the level texel's centre, grid
+ * coordinate (x + 1/2) 2^(l - 2) - 1/2, in FP32 in both builds.  The
+ * bilateral range threshold (squared) is multiplied by m^2, the guided
+ * epsilon by m.  Level l reads the map only when open[l] != 0. */
+typedef struct {
+    const float* map;
+    int gw, gh;
+    const int* open;
+} o_strength;
+
+#ifndef O_STRENGTH_AT
+#define O_STRENGTH_AT
+static float o_strength_at(const o_strength* s, int level, int x, int y) {
+    if (!s || !s->map || !s->open[level]) return 0.0f;
+    const float sc = level == 0 ? 0.25f : level == 1 ? 0.5f : level == 2 ? 1.0f : level == 3 ? 2.0f : 4.0f;
+    const float gx = ((float)x + 0.5f) * sc - 0.5f, gy = ((float)y + 0.5f) * sc - 0.5f;
+    const float fx = floorf(gx), fy = floorf(gy);
+    const float ax = gx - fx, ay = gy - fy;
+    const int ix = (int)fx, iy = (int)fy;
+    const int x0 = ix < 0 ? 0 : (ix > s->gw - 1 ? s->gw - 1 : ix);
+    const int x1 = ix + 1 < 0 ? 0 : (ix + 1 > s->gw - 1 ? s->gw - 1 : ix + 1);
+    const int y0 = iy < 0 ? 0 : (iy > s->gh - 1 ? s->gh - 1 : iy);
+    const int y1 = iy + 1 < 0 ? 0 : (iy + 1 > s->gh - 1 ? s->gh - 1 : iy + 1);
+    const float* m = s->map + level;
+    const float v00 = m[((size_t)y0 * s->gw + x0) * 8], v01 = m[((size_t)y0 * s->gw + x1) * 8];
+    const float v10 = m[((size_t)y1 * s->gw + x0) * 8], v11 = m[((size_t)y1 * s->gw + x1) * 8];
+    const float top = v00 + (v01 - v00) * ax, bot = v10 + (v11 - v10) * ax;
+    return top + (bot - top) * ay;
+}
+#endif
+
 /* --------------------------------------------------------------- denoiseImage
  * cl/demosaic.cl:735-765: 5x5 hard-threshold sigma filter.  The per-tap
  * division by sigma is a multiplication by 1/sigma (computed once per pixel),
  * and length(d) < edge is evaluated as dot(d, d) < edge * edge. */
+static void P(denoise_bilateral_s)(const real* in, int w, int h, const float var_a[3], const float var_b[3],
+                                   float chroma_boost, float gradient_boost, const o_strength* sm, int level,
+                                   real* out);
 void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], const float var_b[3],
                           float chroma_boost, float gradient_boost, real* out) {
+    P(denoise_bilateral_s)(in, w, h, var_a, var_b, chroma_boost, gradient_boost, NULL, 0, out);
+}
+static void P(denoise_bilateral_s)(const real* in, int w, int h, const float var_a[3], const float var_b[3],
+                                   float chroma_boost, float gradient_boost, const o_strength* sm, int level,
+                                   real* out) {
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < h; y++) {
         for (int x = 0; x < w; x++) {
@@ -749,6 +792,14 @@ void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], co
             real edge[3] = {lumaBoost * K(1.41421356), (real)chroma_boost * K(1.41421356),
                             (real)chroma_boost * K(1.41421356)};
             for (int c = 0; c < 3; c++) edge[c] = edge[c] * edge[c];
+            {
+                /* learned strength: the range sigma times m */
+                const float m = o_strength_at(sm, level, x, y);
+                if (m > 0.0f) {
+                    const real m2 = (real)m * (real)m;
+                    for (int c = 0; c < 3; c++) edge[c] = edge[c] * m2;
+                }
+            }
 #if ORACLE_DOUBLE
             const real isig[3] = {K(1.0) / sigma[0], K(1.0) / sigma[1], K(1.0) / sigma[2]};
 #else
@@ -788,8 +839,14 @@ void P(denoise_bilateral)(const real* in, int w, int h, const float var_a[3], co
 
 /* ------------------------------------------------------- denoiseImageGuided
  * cl/demosaic.cl:767-851: one-pass 5x5 self-guided colour guided filter. */
+static void P(denoise_guided_fast_s)(const real* in, int w, int h, const float var_a[3], const float var_b[3],
+                                     const o_strength* sm, int level, real* out);
 void P(denoise_guided_fast)(const real* in, int w, int h, const float var_a[3], const float var_b[3],
                             real* out) {
+    P(denoise_guided_fast_s)(in, w, h, var_a, var_b, NULL, 0, out);
+}
+static void P(denoise_guided_fast_s)(const real* in, int w, int h, const float var_a[3], const float var_b[3],
+                                     const o_strength* sm, int level, real* out) {
     const real norm = K(1.0) / K(25.0);
 #pragma omp parallel for schedule(static)
     for (int y = 0; y < h; y++) {
@@ -798,6 +855,12 @@ void P(denoise_guided_fast)(const real* in, int w, int h, const float var_a[3], 
             real ix = c0[0], iy = c0[1], iz = c0[2];
             real eps[3];
             for (int c = 0; c < 3; c++) eps[c] = (real)var_a[c] + (real)var_b[c] * ix;
+            {
+                /* learned strength: epsilon times m */
+                const float m = o_strength_at(sm, level, x, y);
+                if (m > 0.0f)
+                    for (int c = 0; c < 3; c++) eps[c] = eps[c] * (real)m;
+            }
             /* 9 box sums: mean of I and of its 6 products (separable order) */
             real acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             for (int j = -2; j <= 2; j++) {
@@ -1334,8 +1397,18 @@ typedef struct {
 /* nlf_inout: levels x 6 (input used as fallback when a fit has no sample).
  * image: level 0 (w x h RGBA), out: w x h RGBA.  Returns a bit mask of the
  * levels whose fit fell back. */
+static int P(pyramidal_denoise_s)(const real* image, int w, int h, int levels, int algorithm,
+                                  const o_denoise_params* dp, float* nlf_inout, real* out, real* level3_out,
+                                  const o_strength* sm);
 int P(pyramidal_denoise)(const real* image, int w, int h, int levels, int algorithm,
                          const o_denoise_params* dp, float* nlf_inout, real* out, real* level3_out) {
+    return P(pyramidal_denoise_s)(image, w, h, levels, algorithm, dp, nlf_inout, out, level3_out, NULL);
+}
+/* sm (nullable): the learned strength of the level filters (Bilateral and
+ * GuidedFast; GuidedPrecise stays plain) */
+static int P(pyramidal_denoise_s)(const real* image, int w, int h, int levels, int algorithm,
+                                  const o_denoise_params* dp, float* nlf_inout, real* out, real* level3_out,
+                                  const o_strength* sm) {
     real* pyr[5] = {0};
     real* den[5] = {0};
     int lw[6], lh[6];
@@ -1370,9 +1443,10 @@ int P(pyramidal_denoise)(const real* image, int w, int h, int levels, int algori
         if (algorithm == 2) {
             P(guided_filter)(lvl[i], lw[i], lh[i], 5, np + 3, den[i]);
         } else if (use_bilateral) {
-            P(denoise_bilateral)(lvl[i], lw[i], lh[i], np, np + 3, dp[i].chromaBoost, dp[i].gradientBoost, den[i]);
+            P(denoise_bilateral_s)(lvl[i], lw[i], lh[i], np, np + 3, dp[i].chromaBoost, dp[i].gradientBoost, sm, i,
+                                   den[i]);
         } else {
-            P(denoise_guided_fast)(lvl[i], lw[i], lh[i], np, np + 3, den[i]);
+            P(denoise_guided_fast_s)(lvl[i], lw[i], lh[i], np, np + 3, sm, i, den[i]);
         }
         if (i < levels - 1) {
             const float nl[2] = {np[0], np[3]};
@@ -1419,8 +1493,19 @@ void P(fast_demosaic)(const uint16_t* raw, int w, int h, const o_pipeline_params
     free(rgb);
 }
 
+/* The full path with the learned strength map (map, gw, gh, open as in
+ * o_strength; map null: the plain path). */
+int P(raw_to_rgb_strength)(const uint16_t* raw, int w, int h, const o_pipeline_params* pp, uint8_t* out8,
+                           real* outf, float* nlf_out, real* ycc_dump, real* den_dump, const float* map, int gw,
+                           int gh, const int* open);
 int P(raw_to_rgb)(const uint16_t* raw, int w, int h, const o_pipeline_params* pp, uint8_t* out8,
                   real* outf, float* nlf_out, real* ycc_dump, real* den_dump) {
+    return P(raw_to_rgb_strength)(raw, w, h, pp, out8, outf, nlf_out, ycc_dump, den_dump, NULL, 0, 0, NULL);
+}
+int P(raw_to_rgb_strength)(const uint16_t* raw, int w, int h, const o_pipeline_params* pp, uint8_t* out8,
+                           real* outf, float* nlf_out, real* ycc_dump, real* den_dump, const float* map, int sgw,
+                           int sgh, const int* open) {
+    const o_strength strength = {map, sgw, sgh, open};
     size_t n = (size_t)w * h;
     int fallbacks = 0;
     real* scaled = (real*)malloc(n * sizeof(real));
@@ -1472,7 +1557,8 @@ int P(raw_to_rgb)(const uint16_t* raw, int w, int h, const o_pipeline_params* pp
     const int ltm = pp->localToneMapping && pp->levels > 3;
     const int gw = w >> 3, gh = h >> 3;  /* w/2/2/2 with truncation at every level */
     real* guide = ltm ? (real*)malloc((size_t)gw * gh * 4 * sizeof(real)) : NULL;
-    fallbacks |= P(pyramidal_denoise)(rgbB, w, h, pp->levels, pp->algorithm, pp->dp, nlf, rgbA, guide);
+    fallbacks |= P(pyramidal_denoise_s)(rgbB, w, h, pp->levels, pp->algorithm, pp->dp, nlf, rgbA, guide,
+                                        map ? &strength : NULL);
     if (den_dump) memcpy(den_dump, rgbA, n * 4 * sizeof(real));
 
     if (P(inverse3x3)(c2y, (double)pp->exposure_multiplier, y2c)) fallbacks |= 1 << 9;

@@ -331,7 +331,9 @@ class Oracle:
         res = (rc, out, np.array(nlf[:], np.float32).reshape(-1, 6))
         return res + (g,) if level3 else res
 
-    def raw_to_rgb(self, raw, params, dumps=False):
+    def raw_to_rgb(self, raw, params, dumps=False, strength=None):
+        """strength (optional): (map (gh, gw, 8) float32, open (5,) ints), the
+        learned-strength multipliers of the level filters."""
         h, w = raw.shape
         raw = np.ascontiguousarray(raw, np.uint16)
         out8 = np.zeros((h, w, 4), np.uint8)
@@ -339,8 +341,16 @@ class Oracle:
         nlf = (C.c_float * 34)()
         ycc = self._new(h, w, 4) if dumps else None
         den = self._new(h, w, 4) if dumps else None
-        rc = self._fn("raw_to_rgb")(_ptr(raw), w, h, C.byref(params), _ptr(out8), _ptr(outf), nlf,
-                                    _ptr(ycc) if dumps else None, _ptr(den) if dumps else None)
+        if strength is None:
+            rc = self._fn("raw_to_rgb")(_ptr(raw), w, h, C.byref(params), _ptr(out8), _ptr(outf), nlf,
+                                        _ptr(ycc) if dumps else None, _ptr(den) if dumps else None)
+        else:
+            smap = np.ascontiguousarray(strength[0], np.float32)
+            sopen = np.ascontiguousarray(strength[1], np.int32)
+            assert smap.ndim == 3 and smap.shape[2] == 8 and sopen.shape == (5,)
+            rc = self._fn("raw_to_rgb_strength")(_ptr(raw), w, h, C.byref(params), _ptr(out8), _ptr(outf), nlf,
+                                                 _ptr(ycc) if dumps else None, _ptr(den) if dumps else None,
+                                                 _ptr(smap), smap.shape[1], smap.shape[0], _ptr(sopen))
         nl = np.array(nlf[:], np.float32)
         res = {"rc": rc, "rgba8": out8, "rgbf": outf, "rawNlf": nl[:4],
                "pyramidNlf": nl[4:4 + 6 * params.levels].reshape(params.levels, 6)}
