@@ -128,7 +128,7 @@ STRENGTH_WEIGHTS = os.path.join(HERE, "weights", "strength.f32")
 class StrengthTrainConfig(C.Structure):
     _fields_ = [("frames", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("crop", C.c_int32),
                 ("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("seed", C.c_uint32),
-                ("bayer", C.c_int32), ("noise_levels", C.c_int32)]
+                ("bayer", C.c_int32), ("noise_levels", C.c_int32), ("verbose", C.c_int32)]
 
 
 def load_strength_weights(path=STRENGTH_WEIGHTS):
@@ -488,9 +488,10 @@ class Context:
         return np.array(loss[:]), g, out
 
     def strength_train(self, frames, width=4032, height=3024, crop=64, epochs=10, batch=32, lr=1e-3, seed=10000,
-                       bayer=RGGB, noise_levels=6):
+                       bayer=RGGB, noise_levels=6, verbose=False):
         """clim_strength_train; returns the (epochs, 2) sampled losses."""
-        cfg = StrengthTrainConfig(frames, width, height, crop, epochs, batch, lr, seed, bayer, noise_levels)
+        cfg = StrengthTrainConfig(frames, width, height, crop, epochs, batch, lr, seed, bayer, noise_levels,
+                                  int(verbose))
         log = np.zeros(2 * epochs, np.float32)
         self._check(lib().clim_strength_train(self.h, C.byref(cfg), log.ctypes.data_as(C.c_void_p),
                                               C.c_int64(log.size)), "strength_train")

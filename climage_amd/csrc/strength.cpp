@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <random>
@@ -534,6 +535,10 @@ extern "C" clim_status clim_strength_train(clim_ctx* ctx, const clim_strength_tr
                                static_cast<float*>(scurves.p) + base * cs * cs * clim::kSnCurves,
                                static_cast<float*>(snorm.p) + base * 10, st);
         SN_TRY_HIP(hipGetLastError());
+        if (c.verbose && (f % 16 == 15 || f == c.frames - 1)) {
+            SN_TRY_HIP(hipStreamSynchronize(st));
+            std::fprintf(stderr, "strength_train: %d of %d frames generated\n", f + 1, c.frames);
+        }
     }
     ctx->sn.mode = saved_mode;
 
@@ -584,6 +589,7 @@ extern "C" clim_status clim_strength_train(clim_ctx* ctx, const clim_strength_tr
             log[2 * e] = (float)ls;
             log[2 * e + 1] = (float)lr_sum;
         }
+        if (c.verbose) std::fprintf(stderr, "strength_train: epoch %d: losses %.5f %.5f\n", e, ls, lr_sum);
     }
     // the trained weights become the context's inference weights
     std::vector<float> pt(clim::kSnParams);

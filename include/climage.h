@@ -434,6 +434,7 @@ typedef struct {
     int32_t bayer;
     int32_t noise_levels;   /* frame i has Poisson gain a_(i mod noise_levels) of 0.0005, 0.001, 0.002,
                                0.004, 0.008, 0.016, 0.003, 0.006; read noise a / 200 */
+    int32_t verbose;        /* progress lines on stderr (frames generated, epochs) */
 } clim_strength_train_config;
 /* Blocking.  Generates the crops, trains from seeded initial weights with
  * the library's kernels, and sets the result as the context's weights.

@@ -11,6 +11,8 @@
   mode opens exactly the levels whose mean predicted error ratio is below 0.
 * Batched frames (two lanes) equal single-frame calls.
 * Training runs end to end on a small set.
+* The committed trained weights gain PSNR against the clean truth over the
+  plain pyramid on held-out scenes.
 """
 import numpy as np
 import pytest
@@ -187,3 +189,39 @@ def test_training_runs_end_to_end():
         raw = _frame(ctx, 512, 384, ca.RGGB, 0.004, 10)
         o8, of = _run(ctx, raw, ca.default_params(ca.RGGB, 64.0, 4095.0))
         assert np.all(np.isfinite(of))
+
+
+def _psnr(a, b):
+    mse = float(np.mean((a[..., :3].astype(np.float64) - b[..., :3]) ** 2))
+    return 10.0 * np.log10(1.0 / mse)
+
+
+@pytest.mark.parametrize("a,algorithm", [(0.002, ca.BILATERAL), (0.016, ca.BILATERAL), (0.008, ca.GUIDED_FAST)])
+def test_trained_network_gains_over_the_plain_pyramid(a, algorithm):
+    """The committed weights (climage_amd/weights/strength.f32) on a held-out
+    scene at a size the network was not trained on (2016 x 1512): PSNR
+    against the clean truth (the clean scene through the pipeline's tone map)
+    with the gate deciding is at least the plain pyramid's, and the gain where
+    the gate opens is at least 0.3 dB."""
+    w, h = 2016, 1512
+    with ca.Context(0) as ctx:
+        ctx.strength_set_weights(ca.load_strength_weights())
+        raw = ctx.image(w, h, ca.FMT_R16U)
+        clean = ctx.image(w, h, ca.FMT_RGBA32F)
+        ctx.generate_bayer(raw, ca.RGGB, 4321, 5321, a, a / 200.0, 64.0, 4095.0, clean=clean)
+        p = ca.default_params(ca.RGGB, 64.0, 4095.0, levels=5, algorithm=algorithm)
+        o8 = ctx.image(w, h, ca.FMT_RGBA8)
+        truth = ctx.image(w, h, ca.FMT_RGBA32F)
+        ctx.convertTosRGB(clean, o8, np.eye(3, dtype=np.float32), p.rgbConversionParameters, out_f32=truth)
+        t = truth.download()
+        print(f"gain: a {a} algorithm {algorithm}", flush=True)
+        _, plain = _run(ctx, raw, p)
+        ctx.strength_set_mode(ca.STRENGTH_GATED)
+        _, gated = _run(ctx, raw, p)
+        opened, pred = ctx.strength_gate_read()
+    pp, pg = _psnr(plain, t), _psnr(gated, t)
+    print(f"  PSNR vs clean truth: plain {pp:.3f} dB, gated {pg:.3f} dB, gate {opened.tolist()}, "
+          f"predicted log2 error ratio {np.round(pred, 3).tolist()}")
+    assert pg >= pp
+    if opened.any():
+        assert pg >= pp + 0.3
