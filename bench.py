@@ -9,18 +9,28 @@ all six frames with one clim_raw_to_rgb_batch call (the frames alternate
 between two streams; --sequential: one clim_raw_to_rgb call per frame);
 inputs are resident in HBM before timing starts.
 
-Prints one JSON line.  `stages` comes from a few untimed steps with every
-stage bracketed by HIP events; `roofline` is measured for the dominant
-single-kernel stage with HIP events on the library's stream during the timed
-steps (only that stage records events there, so the headline rate is not
-perturbed; batched, those launches share the device with the other stream's
-frame, and `roofline.isolated` gives the same kernel timed alone in the
-profiled steps); `cpu_baseline` times the FP32 CPU restatement (the oracle) on
-the same frames.
+Prints one JSON line:
+* value: Mpix/s over the timed steps (at least about a second of work);
+* stages: a few untimed steps with every stage bracketed by HIP events;
+* roofline: the dominant single-kernel stage; frac from its launches timed
+  alone (the profiled steps, one frame at a time), the same kernel's
+  launches inside the batched timed steps under "batched";
+* path_roofline: the whole path's algorithmic bytes (each reference stage's
+  inputs read and outputs written once, FP32 texels) and its measured HBM
+  bytes (the committed counter profile) per pixel, times the pixel rate;
+* fp64: the first frame against the FP64 CPU restatement (max-abs and PSNR
+  of the RGB output before quantisation, relative errors of the fitted noise
+  models) and whether it equals the FP32 restatement bit for bit;
+* learned: the same frames with the learned denoise strength (the committed
+  weights, gated), its rate, the strength stage's cost, the gates and the
+  PSNR gain against the clean truth on one frame;
+* cpu_baseline: the FP32 CPU restatement built -O3 -march=native, all host
+  threads, on a sample of the same frames.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -33,6 +43,10 @@ import climage_amd as ca  # noqa: E402
 
 NOISE_A = [0.0005, 0.001, 0.002, 0.004, 0.008, 0.016]
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
+# flops of the strength network per texel of its grid (pyramid level 3): 2 x 9 x
+# (8 x 32 + 32 x 32 + 32 x 32 + 32 x 20) multiply-adds
+STRENGTH_FLOP_PER_TEXEL = 2 * 9 * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 20)
 
 
 def path_bytes_per_px(levels):
@@ -44,7 +58,7 @@ def path_bytes_per_px(levels):
     return 206 + 60 * S(levels - 2) + 80 * S(levels - 1)
 
 
-def stage_bytes(stage, texels, levels_px=None, preview=False):
+def stage_bytes(stage, texels, preview=False):
     """Algorithmic bytes of one stage: each input read once, each output written
     once, FP32 texels (luma 4 B, RGBA 16 B), raw 2 B, RGBA8 4 B."""
     per = {
@@ -77,19 +91,30 @@ SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green"
                  "tone": "k_reassemble<true, true, false>", "raw_denoise": "k_raw_rgba_filter<2, true>"}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, h, bayer, ltm=False, noise_level=0.0, seconds_min=10.0):
-    """The FP32 CPU restatement on the bench frames (same sizes and noise
-    levels), as many as fit in about `seconds_min` seconds (at least one)."""
+    """The FP32 CPU restatement built for this host (-O3 -march=native, OpenMP
+    row bands over every host thread) on the bench frames (same sizes and
+    noise levels), as many as fit in about `seconds_min` seconds (at least one)."""
     from oracle import oracle as O
-    import subprocess
-    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    o = O.Oracle(double=False)
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "fast"], check=True)
+    o = O.Oracle(double=False, lib_path=os.path.join(ROOT, "oracle", "_build", "liboracle_fast.so"))
     p = O.default_params(bayer, 64.0, 4095.0, levels=5)
     p.localToneMapping = int(ltm)
     p.noiseLevel = noise_level
     raws = [O.generate_bayer(w, h, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
             for i, a in enumerate(NOISE_A)]
-    # the six frames in turn until seconds_min of CPU time (at most 8 passes)
+    o.raw_to_rgb(raws[0], p)  # first touch of the buffers, thread start-up
     done, dt = 0, 0.0
     while dt < seconds_min and done < 8 * len(raws):
         raw = raws[done % len(raws)]
@@ -99,8 +124,32 @@ def cpu_baseline(w, h, bayer, ltm=False, noise_level=0.0, seconds_min=10.0):
         done += 1
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(done * w * h / dt / 1e6, 2), "unit": "Mpix/s", "cores": cores, "kind": "port",
+            "cpu": cpu_model(), "nproc": os.cpu_count(), "build": "gcc -O3 -march=native -fopenmp",
             "sample": f"{done} frames ({w}x{h} RGGB, the {len(NOISE_A)} bench noise levels in turn) through the "
-                      f"FP32 CPU restatement in {dt:.1f} s, OpenMP row bands"}
+                      f"FP32 CPU restatement in {dt:.1f} s, OpenMP row bands on {cores} threads"}
+
+
+def fp64_parity(raw_host, bayer, gpu_f32, gpu_nlf, levels=5):
+    """The GPU frame against the FP64 and FP32 CPU restatements."""
+    from oracle import oracle as O
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    p = O.default_params(bayer, 64.0, 4095.0, levels=levels)
+    r64 = O.Oracle(double=True).raw_to_rgb(raw_host, p)
+    r32 = O.Oracle(double=False).raw_to_rgb(raw_host, p)
+    d = gpu_f32[..., :3].astype(np.float64) - r64["rgbf"][..., :3]
+    mse = float(np.mean(d * d))
+    raw_nlf, pyr_nlf = gpu_nlf
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
+
+    return {"frame": "first bench frame (a = 0.0005)",
+            "rgb": {"max_abs": float(np.abs(d).max()), "psnr_db": round(10 * np.log10(1.0 / max(mse, 1e-30)), 2),
+                    "pixels_above_1e-3": int(np.sum(np.abs(d) > 1e-3))},
+            "raw_nlf_max_rel": rel(raw_nlf, r64["rawNlf"]),
+            "pyramid_nlf_max_rel": rel(pyr_nlf, r64["pyramidNlf"]),
+            "bit_exact_vs_fp32": bool(np.array_equal(gpu_f32.view(np.uint32), r32["rgbf"].view(np.uint32)))}
 
 
 def measured_traffic(kernel, w, h):
@@ -112,6 +161,15 @@ def measured_traffic(kernel, w, h):
         return None
     with open(path) as f:
         return json.load(f)["kernels"].get(kernel)
+
+
+def measured_frame_bytes(w, h):
+    """HBM bytes of one frame (every dispatch) from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "traffic_raw_to_rgb_12mp.json")
+    if (w, h) != (4032, 3024) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("frame_bytes")
 
 
 def copy_calibration(nbytes):
@@ -132,14 +190,23 @@ def copy_calibration(nbytes):
         hip.hipFree(dst)
 
 
+def psnr(a, b):
+    d = a[..., :3].astype(np.float64) - b[..., :3]
+    return 10.0 * np.log10(1.0 / max(float(np.mean(d * d)), 1e-30))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=240)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4032)
     ap.add_argument("--height", type=int, default=3024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true", help="skip the FP64 / FP32 CPU comparison of one frame")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-strength leg")
+    ap.add_argument("--learned", action="store_true",
+                    help="the learned denoise strength (committed weights, gated) is the headline path")
     ap.add_argument("--ltm", action="store_true",
                     help="local tone mapping on (its mask is computed in the level-0 reassembly + tone kernel)")
     ap.add_argument("--preview", action="store_true",
@@ -160,15 +227,15 @@ def main():
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(frames are independent: a burst shards whole frames across GPUs)", file=sys.stderr)
         sys.exit(2)
+    if args.burst and args.preview:
+        print("bench.py: --burst runs the full path", file=sys.stderr)
+        sys.exit(2)
 
     w, h, bayer = args.width, args.height, ca.RGGB
     lib = ca.lib()
     ctx = ca.Context(0)
     # frames: (width, height, CFA, noise a, frame id)
     if args.burst:
-        if args.preview:
-            print("bench.py: --burst runs the full path", file=sys.stderr)
-            sys.exit(2)
         sizes = [(4032, 3024), (4000, 3000), (3264, 2448), (1920, 1080)]
         draws = np.random.default_rng(3000).integers(0, len(NOISE_A), 16)
         frames = [(*sizes[k // 4], ca.RGGB if k % 2 == 0 else ca.BGGR, NOISE_A[draws[k]], k) for k in range(16)]
@@ -184,13 +251,17 @@ def main():
         fp.rgbConversionParameters.localToneMapping = int(args.ltm)
         fp.noiseLevel = args.noise_level
         fparams.append(fp)
-    params = fparams[0]
     px_per_step = sum(fw * fh for fw, fh, _, _, _ in frames)
     single_kernel = dict(SINGLE_KERNEL)
     if args.ltm:
         single_kernel["tone"] = "k_reassemble<true, true, true>"
     if args.preview:
         single_kernel["tone"] = "k_fast_preview<true>"
+    learned_on = not args.preview and not args.no_learned
+    if learned_on or args.learned:
+        ctx.strength_set_weights(ca.load_strength_weights())
+    if args.learned:
+        ctx.strength_set_mode(ca.STRENGTH_GATED)
     ctx.sync()
 
     def step(batched=not args.sequential):
@@ -202,10 +273,6 @@ def main():
                 ctx.fast_demosaic(r, o, fp)
             else:
                 ctx.raw_to_rgb(r, o, fp, writeback=False)
-
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
 
     import ctypes as C
 
@@ -222,36 +289,51 @@ def main():
         return {names[i]: {"ms_total": pr.ms[i], "launches": int(pr.launches[i]), "texels": int(pr.texels[i])}
                 for i in range(ca.PROFILE_STAGES) if pr.launches[i]}
 
-    # 1. stage breakdown: a few untimed steps with every stage bracketed by events
-    stages, dom = {}, None
-    prof_on = not args.no_profile
-    if prof_on:
-        psteps = min(args.steps, 3)
+    def profile_steps(n):
+        """n untimed steps, one frame at a time on the profiled stream, every
+        stage bracketed by events: per-stage totals and us per frame."""
         lib.clim_profile_select(ctx.h, C.c_uint32(0xFFFFFFFF))
         lib.clim_profile_enable(ctx.h, 1)
         read_profile()
-        for _ in range(psteps):
-            step(batched=False)  # every frame on the profiled stream
-        stages = read_profile()
-        for v in stages.values():
-            v["us_per_frame"] = round(v["ms_total"] * 1e3 / (psteps * len(raws)), 2)
-        dom = max((k for k in stages if k in single_kernel), key=lambda k: stages[k]["ms_total"])
-        # 2. the timed steps record events around the dominant kernel only
-        lib.clim_profile_select(ctx.h, C.c_uint32(1 << names.index(dom)))
-        read_profile()
+        for _ in range(n):
+            step(batched=False)
+        st = read_profile()
+        lib.clim_profile_enable(ctx.h, 0)
+        for v in st.values():
+            v["us_per_frame"] = round(v["ms_total"] * 1e3 / (n * len(raws)), 2)
+        return st
 
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    def timed(nsteps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            step()
+        ctx.sync()
+        return time.perf_counter() - t0
+
+    for _ in range(args.warmup):
         step()
     ctx.sync()
-    dt = time.perf_counter() - t0
+
+    # 1. stage breakdown (every stage evented, one frame at a time)
+    stages, dom = {}, None
+    prof_on = not args.no_profile
+    if prof_on:
+        stages = profile_steps(min(args.steps, 3))
+        dom = max((k for k in stages if k in single_kernel), key=lambda k: stages[k]["ms_total"])
+        # the timed steps record events around the dominant kernel only
+        lib.clim_profile_select(ctx.h, C.c_uint32(1 << names.index(dom)))
+        lib.clim_profile_enable(ctx.h, 1)
+        read_profile()
+
+    # 2. the timed steps
+    dt = timed(args.steps)
+    mpix = args.steps * px_per_step / dt / 1e6
+    batched = not args.sequential and not args.preview
 
     if args.copy_calibration:
         copy_calibration(256 << 20)
 
-    mpix = args.steps * px_per_step / dt / 1e6
-    batched = not args.sequential and not args.preview
     latency = None
     if args.burst:
         # per-frame latency: each frame launched on an idle device and waited for
@@ -268,22 +350,97 @@ def main():
     if prof_on:
         s = read_profile()[dom]
         lib.clim_profile_enable(ctx.h, 0)
-        avg_s = s["ms_total"] / s["launches"] / 1e3
-        bytes_per_launch = stage_bytes(dom, s["texels"] / s["launches"], preview=args.preview)
-        ach = bytes_per_launch / avg_s / 1e9
+        # primary: the kernel alone (the profiled steps, one frame at a time)
+        iso = stages[dom]
+        iso_s = iso["ms_total"] / iso["launches"] / 1e3
+        bytes_per_launch = stage_bytes(dom, iso["texels"] / iso["launches"], preview=args.preview)
+        ach = bytes_per_launch / iso_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None if args.burst else measured_traffic(single_kernel[dom], w, h),
-                    "kernel": single_kernel[dom], "stage": dom, "avg_launch_us": round(avg_s * 1e6, 2),
-                    "launches_timed": s["launches"], "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+                    "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": None if args.burst else measured_traffic(single_kernel[dom], w, h),
+                    "kernel": single_kernel[dom], "stage": dom, "avg_launch_us": round(iso_s * 1e6, 2),
+                    "launches_timed": iso["launches"], "algorithmic_bytes_per_launch": int(bytes_per_launch),
+                    "timing": "kernel alone (one frame at a time), HIP events on the library's stream"}
         if batched:
-            # the timed launches share the device with the second lane's frame;
-            # the same kernel alone (the profiled steps, one frame at a time):
-            iso = stages[dom]
-            iso_s = iso["ms_total"] / iso["launches"] / 1e3
-            iso_ach = stage_bytes(dom, iso["texels"] / iso["launches"], preview=args.preview) / iso_s / 1e9
-            roofline["lanes"] = 2
-            roofline["isolated"] = {"avg_launch_us": round(iso_s * 1e6, 2), "achieved": round(iso_ach, 1),
-                                    "frac": round(iso_ach / HBM_PEAK_GBS, 4)}
+            # the same kernel's launches in the timed batched steps, sharing the
+            # device with the other stream's frame
+            b_s = s["ms_total"] / s["launches"] / 1e3
+            b_ach = stage_bytes(dom, s["texels"] / s["launches"]) / b_s / 1e9
+            roofline["batched"] = {"avg_launch_us": round(b_s * 1e6, 2), "achieved": round(b_ach, 1),
+                                   "frac": round(b_ach / HBM_PEAK_GBS, 4), "lanes": 2,
+                                   "launches_timed": s["launches"]}
+
+    path = None
+    if not args.preview:
+        bpp = path_bytes_per_px(5)
+        path = {"algorithmic": {"bytes_per_px": bpp, "achieved": round(bpp * mpix * 1e6 / 1e9, 1),
+                                "frac": round(bpp * mpix * 1e6 / 1e9 / HBM_PEAK_GBS, 4)},
+                "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+        fb = None if (args.burst or args.learned) else measured_frame_bytes(w, h)
+        if fb:
+            mbpp = fb / (w * h)
+            path["measured"] = {"bytes_per_px": round(mbpp, 2), "achieved": round(mbpp * mpix * 1e6 / 1e9, 1),
+                                "frac": round(mbpp * mpix * 1e6 / 1e9 / HBM_PEAK_GBS, 4),
+                                "source": "profiles/traffic_raw_to_rgb_12mp.json (FETCH_SIZE + WRITE_SIZE)"}
+
+    # 3. the learned denoise strength on the same frames
+    learned = None
+    if learned_on and not args.learned:
+        ctx.strength_set_mode(ca.STRENGTH_GATED)
+        for _ in range(2):
+            step()
+        lst = profile_steps(1) if prof_on else {}
+        lsteps = max(args.steps // 2, 1)
+        ldt = timed(lsteps)
+        lmpix = lsteps * px_per_step / ldt / 1e6
+        opened, pred = ctx.strength_gate_read()
+        learned = {"value": round(lmpix, 1), "unit": "Mpix/s", "steps": lsteps,
+                   "ms_per_step": round(ldt / lsteps * 1e3, 3), "vs_plain": round(lmpix / mpix, 4),
+                   "weights": "climage_amd/weights/strength.f32 (gated)",
+                   "gate_last_frame": opened.tolist(), "predicted_log2_error_ratio": np.round(pred, 4).tolist()}
+        if "strength" in lst:
+            st_us = lst["strength"]["us_per_frame"]
+            texels = lst["strength"]["texels"] / lst["strength"]["launches"]
+            flops = STRENGTH_FLOP_PER_TEXEL * texels
+            learned["strength_stage"] = {"us_per_frame": st_us, "grid_texels": int(texels),
+                                         "flop_per_frame": int(flops),
+                                         "achieved_tflops": round(flops / (st_us * 1e-6) / 1e12, 2),
+                                         "mfma_peak_tflops": BF16_PEAK_TFLOPS,
+                                         "frac": round(flops / (st_us * 1e-6) / 1e12 / BF16_PEAK_TFLOPS, 4),
+                                         "note": "the fused network launch (features and four layers) and the gate, events around both"}
+            learned["denoise_us_per_frame"] = lst.get("denoise", {}).get("us_per_frame")
+        if not args.burst:
+            # PSNR against the clean truth (the clean scene through the pipeline's
+            # tone map) on the first frame, plain and learned
+            fw, fh, fb_, a, i = frames[2]
+            clean = ctx.image(fw, fh, ca.FMT_RGBA32F)
+            raw = ctx.image(fw, fh, ca.FMT_R16U)
+            ctx.generate_bayer(raw, fb_, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0, clean=clean)
+            truth = ctx.image(fw, fh, ca.FMT_RGBA32F)
+            of = ctx.image(fw, fh, ca.FMT_RGBA32F)
+            o8 = ctx.image(fw, fh, ca.FMT_RGBA8)
+            ctx.convertTosRGB(clean, o8, np.eye(3, dtype=np.float32), fparams[2].rgbConversionParameters,
+                              out_f32=truth)
+            t = truth.download()
+            res = {}
+            for name, mode in (("plain", ca.STRENGTH_OFF), ("learned", ca.STRENGTH_GATED)):
+                ctx.strength_set_mode(mode)
+                ctx.raw_to_rgb(raw, o8, fparams[2], out_f32=of, writeback=False)
+                res[name] = round(psnr(of.download(), t), 3)
+            learned["psnr_vs_clean_truth_db"] = {"frame": f"bench frame {i} (a = {a})", **res,
+                                                 "gain": round(res["learned"] - res["plain"], 3)}
+            for im in (clean, raw, truth, of, o8):
+                im.free()
+        ctx.strength_set_mode(ca.STRENGTH_OFF)
+
+    # 4. the first frame against the CPU restatements (outside any timing)
+    parity = None
+    if not (args.no_parity or args.preview or args.burst or args.learned or args.ltm or args.noise_level > 0.6):
+        of = ctx.image(w, h, ca.FMT_RGBA32F)
+        nlf = ctx.raw_to_rgb(raws[0], outs[0], fparams[0], out_f32=of)
+        parity = fp64_parity(raws[0].download(), bayer, of.download(), nlf)
+        of.free()
+
     cpu = None
     if not args.no_cpu_baseline and not args.preview:
         # the burst's CPU sample: its smallest frame size
@@ -296,9 +453,11 @@ def main():
                     "per-frame noise level drawn from the headline set, 5-level bilateral pyramid, one context")
     else:
         metric = ("Mpix/s RAW->RGB half-resolution preview (mosaic pixels) at 12 MP" if args.preview
+                  else "Mpix/s RAW->RGB (demosaic+learned-strength pyramid denoise) at 12 MP" if args.learned
                   else "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP")
         workload = (f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, "
                     + ("half-resolution preview (fastDemosaicImage)" if args.preview else "5-level bilateral pyramid")
+                    + (", learned strength (gated)" if args.learned else "")
                     + (", local tone mapping" if args.ltm else "")
                     + (f", noiseLevel {args.noise_level} (raw pre-denoise)" if args.noise_level > 0.6 else ""))
     res = {
@@ -314,11 +473,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded scenes generated on the device)",
-        "config": {"workload": workload, "frames_per_step": len(raws), "pixels_per_step": px_per_step},
+        "config": {"workload": workload, "frames_per_step": len(raws), "pixels_per_step": px_per_step,
+                   "timed_seconds": round(dt, 3), "batched": batched},
         "roofline": roofline,
-        "path_roofline": None if args.preview else {"bytes_per_px": path_bytes_per_px(5),
-                          "achieved": round(path_bytes_per_px(5) * mpix * 1e6 / 1e9, 1), "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": round(path_bytes_per_px(5) * mpix * 1e6 / 1e9 / HBM_PEAK_GBS, 4)},
+        "path_roofline": path,
+        "fp64": parity,
+        "learned": learned,
         "cpu_baseline": cpu,
         "stages": stages,
     }

@@ -467,7 +467,7 @@ class Context:
         gw, gh = C.c_int32(), C.c_int32()
         self._check(lib().clim_strength_map_read(self.h, None, C.c_int64(0), C.byref(gw), C.byref(gh)),
                     "strength_map_read")
-        m = np.zeros((gh.value, gw.value, 8), np.float32)
+        m = np.zeros((5, gh.value, gw.value), np.float32)
         self._check(lib().clim_strength_map_read(self.h, m.ctypes.data_as(C.c_void_p), C.c_int64(m.size),
                                                  C.byref(gw), C.byref(gh)), "strength_map_read")
         return m
@@ -487,7 +487,7 @@ class Context:
                                                   ptr(g), loss), "strength_loss_grad")
         return np.array(loss[:]), g, out
 
-    def strength_train(self, frames, width=4032, height=3024, crop=64, epochs=10, batch=32, lr=1e-3, seed=10000,
+    def strength_train(self, frames, width=4032, height=3024, crop=32, epochs=10, batch=32, lr=1e-3, seed=10000,
                        bayer=RGGB, noise_levels=6, verbose=False):
         """clim_strength_train; returns the (epochs, 2) sampled losses."""
         cfg = StrengthTrainConfig(frames, width, height, crop, epochs, batch, lr, seed, bayer, noise_levels,

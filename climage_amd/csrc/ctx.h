@@ -54,7 +54,7 @@ struct SnState {
     int mode = 0;                       // clim_strength_mode
     clim::SnWeights wt{};               // device weights loaded on this context (null until set)
     const clim::SnWeights* use = nullptr;  // the weights inference reads (a batch lane: its owner's)
-    DevBuf feat, act0, act1, map, gate_sum, open;  // per-frame buffers (grid of level 2)
+    DevBuf map, gate_sum, open;         // per-frame buffers (grid of level 3)
     int gw = 0, gh = 0, levels = 0;
     SnTrainer* trainer = nullptr;       // training / gradient buffers, created on first use
 };
@@ -121,7 +121,7 @@ int scene_colors(int bayer);
 // the static scratch array's device address on `device` (current), taken once
 // per device (noalloc.cpp); null when hipGetSymbolAddress failed
 void clim_noalloc_init(int device, void** ptr);
-// learned strength (strength.cpp): the network on level 2 of a frame's
+// learned strength (strength.cpp): the network on level 3 of a frame's
 // pyramid after the per-level fits, filling sr for the level filters (off
 // unless a mode and weights are set); the context's strength buffers freed
 clim_status strength_infer(clim_ctx* ctx, const clim::Img* lvl, int levels, int algorithm, clim::StrengthRef* sr);

@@ -357,10 +357,12 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int 
 
 // one 32x16 tile (x0, y0) of denoiseImage
 template <bool SAME_EDGE>
-__device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, const Img& in, const Img& out, int x0, int y0,
-                                               const ParamRef& nlf, float chromaBoost, float gradientBoost,
+__device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, float* mt, const Img& in, const Img& out, int x0,
+                                               int y0, const ParamRef& nlf, float chromaBoost, float gradientBoost,
                                                const StrengthRef& sr, int level) {
-    t.load(in, x0, y0, threadIdx.y * 32 + threadIdx.x);
+    const int tid = threadIdx.y * 32 + threadIdx.x;
+    t.load(in, x0, y0, tid);
+    const MapStage ms = stage_strength<TX, TY>(sr, level, x0, y0, mt, tid);
     __syncthreads();
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
     const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
@@ -369,7 +371,7 @@ __device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, const Img&
         const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
         const int x = x0 + tx, y = y0 + ty;
         if (x >= out.w || y >= out.h) continue;
-        const float m = strength_at(sr, level, x, y);
+        const float m = strength_at_staged(sr, ms, mt, level, x, y);
         wrow_f4(out, y)[x] =
             bilateral_px<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost, m);
     }
@@ -379,8 +381,9 @@ template <bool SAME_EDGE>
 __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
                                                    float gradientBoost, StrengthRef sr, int level) {
     __shared__ LevelTile4<TX, TY> t;
+    __shared__ float mt[kMapStageMax];
     const TileId tl = xcd_tile();
-    bilateral_tile<SAME_EDGE>(t, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost, sr, level);
+    bilateral_tile<SAME_EDGE>(t, mt, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost, sr, level);
 }
 
 // denoiseImage of every pyramid level in one launch (the levels' denoising
@@ -389,11 +392,12 @@ __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf
 template <bool SAME_EDGE>
 __global__ void __launch_bounds__(256) k_bilateral_levels(BilateralLevels lv) {
     __shared__ LevelTile4<TX, TY> t;
+    __shared__ float mt[kMapStageMax];
     const int b = xcd_block();
     int l = 0;
     while (l + 1 < lv.n && b >= lv.base[l + 1]) l++;
     const int i = b - lv.base[l], tiles_x = (lv.out[l].w + TX - 1) / TX;
-    bilateral_tile<SAME_EDGE>(t, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY, lv.nlf[l],
+    bilateral_tile<SAME_EDGE>(t, mt, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY, lv.nlf[l],
                               lv.chromaBoost[l], lv.gradientBoost[l], lv.sr, l);
 }
 
@@ -440,10 +444,12 @@ struct GuidedTile {
 
 __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef nlf, StrengthRef sr, int level) {
     __shared__ GuidedTile sm;
+    __shared__ float mt[kMapStageMax];
     const TileId tl = xcd_tile();
     const int x0 = tl.x * TX, y0 = tl.y * TY;
     const int tid = threadIdx.y * 32 + threadIdx.x;
     sm.t.load(in, x0, y0, tid);
+    const MapStage ms = stage_strength<TX, TY>(sr, level, x0, y0, mt, tid);
     __syncthreads();
     {
         constexpr int N = (TY + 4) * TX, IT = (N + 255) / 256;
@@ -476,7 +482,7 @@ __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef n
         const int X = tx + HR, Y = ty + HR;
         const float ix = sm.t.x[Y][X], iy = sm.t.y[Y][X], iz = sm.t.z[Y][X];
         float eps0 = va0 + vb0 * ix, eps1 = va1 + vb1 * ix, eps2 = va2 + vb2 * ix;
-        const float m = strength_at(sr, level, x, y);
+        const float m = strength_at_staged(sr, ms, mt, level, x, y);
         if (m > 0.0f) {
             // learned strength: epsilon times m
             eps0 = eps0 * m;

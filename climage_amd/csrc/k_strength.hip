@@ -16,6 +16,8 @@
 // The weight gradient (k_sn_wgrad) is a GEMM over pixels: D[co][(tap, ci)]
 // = sum_p dz[p][co] x[p + tap][ci], accumulated per block over a run of
 // tiles, written as per-block partials and summed in a fixed order.
+#include <algorithm>
+
 #include "strength.h"
 
 namespace clim {
@@ -47,7 +49,6 @@ __global__ void __launch_bounds__(256) k_sn_conv(SnConv a) {
     constexpr int KSTEPS = CIN == 32 ? 18 : 5;
     constexpr int KPAD = KSTEPS * 16;
     __shared__ __align__(16) uint16_t tile[LY * LX * PS];
-    __shared__ float rsum[EPI == SN_MAP ? kSnLevels : 1][EPI == SN_MAP ? 256 : 1];
 
     const int tiles_x = (a.w + kCTX - 1) / kCTX, tiles_y = (a.h + kCTY - 1) / kCTY;
     const int per_img = tiles_x * tiles_y;
@@ -122,27 +123,9 @@ __global__ void __launch_bounds__(256) k_sn_conv(SnConv a) {
                     o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
                     *reinterpret_cast<uint2*>(a.y + pix * 32 + c0) = o;
                 }
-            } else if (EPI == SN_RAW) {
+            } else {  // SN_RAW
                 if (ok) *reinterpret_cast<float4*>(a.y32 + pix * 32 + c0) = make_float4(v[0], v[1], v[2], v[3]);
-            } else {  // SN_MAP: channels 4l .. 4l + 3 of level l = 2 j + half
-                const int l = 2 * j + half;
-                if (l < kSnLevels) {
-                    const int f = (a.filter >> l) & 1;  // 1: the guided filter denoises level l
-                    const float sv = fminf(fmaxf(v[f], -kSnSMax), kSnSMax);
-                    const float m = exp2f(f ? 2.0f * sv : sv);
-                    if (ok) a.y32[pix * 8 + l] = m;
-                    rsum[l][(2 * wave + g) * 32 + col] = ok ? v[2 + f] : 0.0f;
-                }
             }
-        }
-    }
-    if (EPI == SN_MAP) {
-        __syncthreads();
-        if (tid < kSnLevels) {
-            float s = 0.0f;
-            for (int i = 0; i < 256; i++) s += rsum[tid][i];
-            const long long q = llrint((double)s * 65536.0);
-            atomicAdd(reinterpret_cast<unsigned long long*>(a.gate_sum + tid), (unsigned long long)q);
         }
     }
 }
@@ -156,8 +139,248 @@ void launch_sn_conv(const SnConv& a, int cin, SnEpilogue epi, hipStream_t st) {
     SN_CONV_CASE(32, SN_RELU)
     SN_CONV_CASE(32, SN_MASK)
     SN_CONV_CASE(32, SN_RAW)
-    SN_CONV_CASE(32, SN_MAP)
 #undef SN_CONV_CASE
+}
+
+// ---------------------------------------------------- fused inference
+// The features and all four layers of a 16 x 16 tile of the grid in one
+// block, the activations in LDS: layer l computes the region of its output
+// that the remaining layers need (22, 20, 18, 16 texels square), so the
+// activations never reach HBM (each round trip is 49 MB per 12 MP frame).
+// Region texels outside the image are zero, which is the zero padding every
+// layer's input has in the layer-by-layer form; each output texel sees the
+// same bfloat16 inputs and the same MFMA sequence, so the results are that
+// form's.  The features are computed from a 28 x 28 tile of the grid level staged in
+// LDS (clamped reads, the window sums in k_sn_features's order).
+// Two blocks share a CU (80 KB of LDS each); a block reads each layer's
+// weights (its A operand) from global memory into registers, the first
+// layer's while the features are computed.  The B operand is read from LDS
+// a chunk of MFMA steps ahead.
+namespace {
+constexpr int kFT = 16;              // output tile
+constexpr int kR0 = kFT + 8;         // feature region (halo 4)
+constexpr int kL2 = kR0 + 4;         // grid-level texels under the feature windows
+constexpr int kPS = 40;              // activation pixel stride (bf16)
+struct InferSmem {
+    uint16_t f[kR0 * kR0 * 8];                      // features
+    uint16_t a[(kFT + 6) * (kFT + 6) * kPS];        // layer 1 and layer 3 outputs
+    union {
+        float4 lv[kL2 * kL2];                       // grid-level texels (before layer 1)
+        uint16_t b[(kFT + 4) * (kFT + 4) * kPS];    // layer 2 output
+    };
+    float gate[4][2][kSnLevels];
+};
+}  // namespace
+
+__device__ __forceinline__ float sn_pred_f(const float* nlf, int c, float y) {
+    return fmaxf(nlf[c] + nlf[3 + c] * y, 1e-10f);
+}
+
+// the A operand of a layer (weights of output channel col, k = 16 s + 8 half
+// + [0, 8)) into registers
+template <int KSTEPS>
+__device__ __forceinline__ void sn_load_weights(bf16x8 (&af)[18], const uint16_t* w, int half, int col) {
+    const uint16_t* wrow = w + col * (KSTEPS * 16) + 8 * half;
+#pragma unroll
+    for (int s = 0; s < KSTEPS; s++) af[s] = as_bf16x8(*reinterpret_cast<const uint4*>(wrow + 16 * s));
+}
+
+// B operand of step s for output texel q of an S x S region over an
+// (S + 2)-wide input region (pixel stride PSI bf16)
+template <int CIN, int S, int PSI>
+__device__ __forceinline__ uint4 sn_bfrag(const uint16_t* in, int q, int s, int half) {
+    constexpr int SI = S + 2;
+    const int tap = CIN == 32 ? (s >> 1) : 2 * s + half;
+    const int ci0 = CIN == 32 ? 16 * (s & 1) + 8 * half : 0;
+    if (CIN == 8 && tap >= 9) return make_uint4(0u, 0u, 0u, 0u);  // K padding of the last step
+    const int qy = q / S, qx = q % S;
+    return *reinterpret_cast<const uint4*>(in + ((qy + tap / 3) * SI + (qx + tap % 3)) * PSI + ci0);
+}
+
+// Two 32-texel groups (q0, q1; the second only when two) through one layer:
+// the B fragments of the next chunk of steps are read from LDS while the
+// current chunk's MFMAs run.
+template <int CIN, int S>
+__device__ __forceinline__ void sn_pair(const uint16_t* in, const bf16x8 (&af)[18], int q0, int q1, bool two,
+                                        int half, f32x16& acc0, f32x16& acc1) {
+    constexpr int KSTEPS = CIN == 32 ? 18 : 5;
+    constexpr int PSI = CIN == 32 ? kPS : 8;
+    constexpr int CH = CIN == 32 ? 6 : 5, NCH = KSTEPS / CH;
+    uint4 bb[2][2][CH];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        acc0[i] = 0.0f;
+        acc1[i] = 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+        bb[0][0][u] = sn_bfrag<CIN, S, PSI>(in, q0, u, half);
+        bb[0][1][u] = sn_bfrag<CIN, S, PSI>(in, q1, u, half);
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+        if (c + 1 < NCH) {
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                bb[(c + 1) & 1][0][u] = sn_bfrag<CIN, S, PSI>(in, q0, (c + 1) * CH + u, half);
+                bb[(c + 1) & 1][1][u] = sn_bfrag<CIN, S, PSI>(in, q1, (c + 1) * CH + u, half);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[c * CH + u], as_bf16x8(bb[c & 1][0][u]), acc0, 0, 0, 0);
+            if (two)
+                acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[c * CH + u], as_bf16x8(bb[c & 1][1][u]), acc1, 0,
+                                                               0, 0);
+        }
+    }
+}
+
+// one hidden layer: out region S x S from in region (S + 2) x (S + 2); the
+// output texel (qx, qy) is at image position (ox + qx, oy + qy) and is zero
+// outside the image.  A wave takes 32-texel groups wave, wave + 4, ... two
+// at a time.
+template <int CIN, int S>
+__device__ __forceinline__ void sn_layer_lds(const uint16_t* in, uint16_t* out, const bf16x8 (&af)[18],
+                                             const float* bias, int ox, int oy, int gw, int gh, int wave, int half,
+                                             int col) {
+    constexpr int NG = (S * S + 31) / 32;
+    float bv[16];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) bv[4 * j + i] = bias[8 * j + 4 * half + i];
+    for (int g = wave; g < NG; g += 8) {
+        const int g2 = g + 4;
+        const int q0 = min(32 * g + col, S * S - 1), q1 = min(32 * g2 + col, S * S - 1);
+        f32x16 acc0, acc1;
+        sn_pair<CIN, S>(in, af, q0, q1, g2 < NG, half, acc0, acc1);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int gg = k ? g2 : g;
+            const int q = 32 * gg + col;
+            if (gg >= NG || q >= S * S) continue;
+            const int qy = q / S, qx = q % S;
+            const int x = ox + qx, y = oy + qy;
+            const bool inside = x >= 0 && x < gw && y >= 0 && y < gh;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float v[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float a = k ? acc1[4 * j + i] : acc0[4 * j + i];
+                    v[i] = inside ? fmaxf(a + bv[4 * j + i], 0.0f) : 0.0f;
+                }
+                uint2 o;
+                o.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+                o.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+                *reinterpret_cast<uint2*>(out + q * kPS + 8 * j + 4 * half) = o;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256, 2) k_sn_infer(Img lv, const float* nlf0, const float* nlfg, SnWeights wt,
+                                                     float* map, int filter, long long* gate_sum) {
+    __shared__ InferSmem sm;
+    const int gw = lv.w, gh = lv.h;
+    const int tiles_x = (gw + kFT - 1) / kFT;
+    const int x0 = (blockIdx.x % tiles_x) * kFT, y0 = (blockIdx.x / tiles_x) * kFT;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, half = lane >> 5, col = lane & 31;
+    bf16x8 wa[18];
+    sn_load_weights<5>(wa, wt.fwd[0], half, col);
+    // grid-level texels (clamped) under the feature windows: origin (x0 - 6, y0 - 6)
+    for (int i = tid; i < kL2 * kL2; i += 256) {
+        const int x = clampi(x0 - 6 + i % kL2, 0, gw - 1), y = clampi(y0 - 6 + i / kL2, 0, gh - 1);
+        sm.lv[i] = row_f4(lv, y)[x];
+    }
+    __syncthreads();
+    // features of the region (x0 - 4, y0 - 4) .. + 24, zero outside the image
+    for (int i = tid; i < kR0 * kR0; i += 256) {
+        const int rx = i % kR0, ry = i / kR0;
+        const int x = x0 - 4 + rx, y = y0 - 4 + ry;
+        uint4 o = make_uint4(0u, 0u, 0u, 0u);
+        if (x >= 0 && x < gw && y >= 0 && y < gh) {
+            float s[3] = {0, 0, 0}, qq[3] = {0, 0, 0};
+            for (int j = 0; j < 5; j++)
+                for (int k = 0; k < 5; k++) {
+                    const float4 v = sm.lv[(ry + j) * kL2 + rx + k];
+                    s[0] += v.x; s[1] += v.y; s[2] += v.z;
+                    qq[0] += v.x * v.x; qq[1] += v.y * v.y; qq[2] += v.z * v.z;
+                }
+            const float mY = s[0] / 25.0f;
+            float f[8];
+            f[0] = mY;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float m = s[c] / 25.0f, var = qq[c] / 25.0f - m * m;
+                f[1 + c] = log2f(fmaxf(var, 1e-12f) / sn_pred_f(nlfg, c, mY)) * 0.25f;
+            }
+            f[4] = log2f(sn_pred_f(nlfg, 0, mY)) * 0.125f + 1.0f;
+            f[5] = log2f(sn_pred_f(nlfg, 1, mY)) * 0.125f + 1.0f;
+            f[6] = log2f(sn_pred_f(nlf0, 0, mY)) * 0.125f + 1.0f;
+            f[7] = 1.0f;
+            o.x = (uint32_t)f2bf(f[0]) | ((uint32_t)f2bf(f[1]) << 16);
+            o.y = (uint32_t)f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16);
+            o.z = (uint32_t)f2bf(f[4]) | ((uint32_t)f2bf(f[5]) << 16);
+            o.w = (uint32_t)f2bf(f[6]) | ((uint32_t)f2bf(f[7]) << 16);
+        }
+        *reinterpret_cast<uint4*>(&sm.f[i * 8]) = o;
+    }
+    __syncthreads();
+    sn_layer_lds<8, kFT + 6>(sm.f, sm.a, wa, wt.bias[0], x0 - 3, y0 - 3, gw, gh, wave, half, col);
+    sn_load_weights<18>(wa, wt.fwd[1], half, col);
+    __syncthreads();
+    sn_layer_lds<32, kFT + 4>(sm.a, sm.b, wa, wt.bias[1], x0 - 2, y0 - 2, gw, gh, wave, half, col);
+    sn_load_weights<18>(wa, wt.fwd[2], half, col);
+    __syncthreads();
+    sn_layer_lds<32, kFT + 2>(sm.b, sm.a, wa, wt.bias[2], x0 - 1, y0 - 1, gw, gh, wave, half, col);
+    sn_load_weights<18>(wa, wt.fwd[3], half, col);
+    __syncthreads();
+    // last layer: 16 x 16 texels, two 32-texel groups per wave; the map and
+    // the predicted error ratios
+    {
+        constexpr int S = kFT;
+        const int g0 = 2 * wave;
+        f32x16 acc[2];
+        sn_pair<32, S>(sm.a, wa, 32 * g0 + col, 32 * (g0 + 1) + col, true, half, acc[0], acc[1]);
+#pragma unroll
+        for (int gg = 0; gg < 2; gg++) {
+            const int q = 32 * (g0 + gg) + col, qy = q / S, qx = q % S;
+            const int x = x0 + qx, y = y0 + qy;
+            const bool ok = x < gw && y < gh;
+            // lane (col, half) holds levels 2 j + half (channels 4 l .. 4 l + 3)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int l = 2 * j + half;
+                float r = 0.0f;
+                if (l < kSnLevels) {
+                    const int f = (filter >> l) & 1;
+                    const float sv = fminf(fmaxf(acc[gg][4 * j + f] + wt.bias[3][4 * l + f], -kSnSMax), kSnSMax);
+                    if (ok) map[((size_t)l * gh + y) * gw + x] = exp2f(f ? 2.0f * sv : sv);
+                    r = ok ? acc[gg][4 * j + 2 + f] + wt.bias[3][4 * l + 2 + f] : 0.0f;
+                }
+                // sum over the 32 texels of this half (fixed butterfly order)
+#pragma unroll
+                for (int off = 16; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+                if (col == 0 && l < kSnLevels) sm.gate[wave][gg][l] = r;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < kSnLevels) {
+        float s = 0.0f;
+        for (int w = 0; w < 4; w++)
+            for (int gg = 0; gg < 2; gg++) s += sm.gate[w][gg][tid];
+        atomicAdd(reinterpret_cast<unsigned long long*>(gate_sum + tid),
+                  (unsigned long long)llrint((double)s * 65536.0));
+    }
+}
+
+void launch_sn_infer(const Img& lvg, const float* nlf0, const float* nlfg, const SnWeights& wt, float* map,
+                     int filter, long long* gate_sum, hipStream_t st) {
+    const int tiles = ((lvg.w + kFT - 1) / kFT) * ((lvg.h + kFT - 1) / kFT);
+    hipLaunchKernelGGL(k_sn_infer, dim3(tiles), dim3(256), 0, st, lvg, nlf0, nlfg, wt, map, filter, gate_sum);
 }
 
 // ---------------------------------------------------------- weight gradient
@@ -420,16 +643,16 @@ int launch_sn_loss(const float* out32, const float* curves, const float* norm, i
 }
 
 // ----------------------------------------------------------------- features
-// Per texel of level 2: 5x5 (clamped) mean and variance of Y, Cb, Cr, and
-// the noise models' predicted variances at the mean luma:
-//   f0 mean Y, f1..f3 log2(var / predicted var) / 4 for Y, Cb, Cr (level 2's
-//   model), f4, f5 log2(predicted var Y, Cb at level 2) / 8 + 1, f6 the same
+// Per texel of the grid level: 5x5 (clamped) mean and variance of Y, Cb,
+// Cr, and the noise models' predicted variances at the mean luma:
+//   f0 mean Y, f1..f3 log2(var / predicted var) / 4 for Y, Cb, Cr (the grid
+//   level's model), f4, f5 log2(predicted var Y, Cb there) / 8 + 1, f6 the same
 //   for Y at level 0, f7 = 1.
 __device__ __forceinline__ float sn_pred(const float* nlf, int c, float y) {
     return fmaxf(nlf[c] + nlf[3 + c] * y, 1e-10f);
 }
 
-__global__ void __launch_bounds__(256) k_sn_features(Img lv, const float* nlf0, const float* nlf2, uint16_t* feat) {
+__global__ void __launch_bounds__(256) k_sn_features(Img lv, const float* nlf0, const float* nlfg, uint16_t* feat) {
     const int x = blockIdx.x * 32 + (threadIdx.x & 31), y = blockIdx.y * 8 + (threadIdx.x >> 5);
     if (x >= lv.w || y >= lv.h) return;
     float s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
@@ -447,10 +670,10 @@ __global__ void __launch_bounds__(256) k_sn_features(Img lv, const float* nlf0, 
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const float m = s[c] / 25.0f, var = q[c] / 25.0f - m * m;
-        f[1 + c] = log2f(fmaxf(var, 1e-12f) / sn_pred(nlf2, c, mY)) * 0.25f;
+        f[1 + c] = log2f(fmaxf(var, 1e-12f) / sn_pred(nlfg, c, mY)) * 0.25f;
     }
-    f[4] = log2f(sn_pred(nlf2, 0, mY)) * 0.125f + 1.0f;
-    f[5] = log2f(sn_pred(nlf2, 1, mY)) * 0.125f + 1.0f;
+    f[4] = log2f(sn_pred(nlfg, 0, mY)) * 0.125f + 1.0f;
+    f[5] = log2f(sn_pred(nlfg, 1, mY)) * 0.125f + 1.0f;
     f[6] = log2f(sn_pred(nlf0, 0, mY)) * 0.125f + 1.0f;
     f[7] = 1.0f;
     uint4 o;
@@ -461,9 +684,9 @@ __global__ void __launch_bounds__(256) k_sn_features(Img lv, const float* nlf0, 
     *reinterpret_cast<uint4*>(feat + ((size_t)y * lv.w + x) * 8) = o;
 }
 
-void launch_sn_features(const Img& lv2, const float* nlf0, const float* nlf2, uint16_t* feat, hipStream_t st) {
-    const dim3 grd((lv2.w + 31) / 32, (lv2.h + 7) / 8);
-    hipLaunchKernelGGL(k_sn_features, grd, dim3(256), 0, st, lv2, nlf0, nlf2, feat);
+void launch_sn_features(const Img& lvg, const float* nlf0, const float* nlfg, uint16_t* feat, hipStream_t st) {
+    const dim3 grd((lvg.w + 31) / 32, (lvg.h + 7) / 8);
+    hipLaunchKernelGGL(k_sn_features, grd, dim3(256), 0, st, lvg, nlf0, nlfg, feat);
 }
 
 // ------------------------------------------------------------ error curves
@@ -487,13 +710,13 @@ __global__ void __launch_bounds__(256) k_sn_curve(Img den, Img clean, Mat3 y2c, 
         return e;
     };
     float e = 0.0f;
-    if (level <= 2) {
-        const int sx = 1 << (2 - level);
+    if (level <= kSnGrid) {
+        const int sx = 1 << (kSnGrid - level);
         for (int j = 0; j < sx; j++)
             for (int i = 0; i < sx; i++) e += err(gx * sx + i, gy * sx + j);
         e /= (float)(sx * sx);
     } else {
-        const int sh = level - 2;
+        const int sh = level - kSnGrid;
         e = err(min(gx >> sh, den.w - 1), min(gy >> sh, den.h - 1));
     }
     curves[((size_t)gy * gw + gx) * kSnCurves + slot] = e;

@@ -114,27 +114,77 @@ void launch_noise_stats(const Img& in, const Img& out, hipStream_t st);
 // m^2 (the range sigma by m), the guided epsilon by m.  Off (the plain
 // filters, untouched arithmetic) when map is null and constant <= 0.
 struct StrengthRef {
-    const float* map;   // [gh][gw][8] on the grid of level 2: multiplier of level l in channel l
+    const float* map;   // [5][gh][gw] on the grid of level 3: plane l holds level l's multipliers
     int gw, gh;
     const int* open;    // [5] device: level l reads the map only when open[l] != 0
     float constant;     // > 0: this multiplier at every texel (the map is not read)
 };
+// grid texels per texel of `level`: 2^(level - 3)
+__device__ __forceinline__ float strength_scale(int level) {
+    return level == 0 ? 0.125f : level == 1 ? 0.25f : level == 2 ? 0.5f : level == 3 ? 1.0f : 2.0f;
+}
 // The multiplier at texel (x, y) of `level`, or 0 when the strength is off
 // there.  The map is sampled bilinearly (clamped) at the texel's centre,
-// grid coordinate (x + 1/2) 2^(level - 2) - 1/2.
+// grid coordinate (x + 1/2) 2^(level - 3) - 1/2.
 __device__ __forceinline__ float strength_at(const StrengthRef& s, int level, int x, int y) {
     if (s.constant > 0.0f) return s.constant;
     if (!s.map || !s.open[level]) return 0.0f;
-    const float sc = level == 0 ? 0.25f : level == 1 ? 0.5f : level == 2 ? 1.0f : level == 3 ? 2.0f : 4.0f;
+    const float sc = strength_scale(level);
     const float gx = ((float)x + 0.5f) * sc - 0.5f, gy = ((float)y + 0.5f) * sc - 0.5f;
     const float fx = floorf(gx), fy = floorf(gy);
     const float ax = gx - fx, ay = gy - fy;
     const int ix = (int)fx, iy = (int)fy;
     const int x0 = clampi(ix, 0, s.gw - 1), x1 = clampi(ix + 1, 0, s.gw - 1);
     const int y0 = clampi(iy, 0, s.gh - 1), y1 = clampi(iy + 1, 0, s.gh - 1);
-    const float* m = s.map + level;
-    const float v00 = m[((size_t)y0 * s.gw + x0) * 8], v01 = m[((size_t)y0 * s.gw + x1) * 8];
-    const float v10 = m[((size_t)y1 * s.gw + x0) * 8], v11 = m[((size_t)y1 * s.gw + x1) * 8];
+    const float* m = s.map + (size_t)level * s.gw * s.gh;
+    const float v00 = m[(size_t)y0 * s.gw + x0], v01 = m[(size_t)y0 * s.gw + x1];
+    const float v10 = m[(size_t)y1 * s.gw + x0], v11 = m[(size_t)y1 * s.gw + x1];
+    const float top = v00 + (v01 - v00) * ax, bot = v10 + (v11 - v10) * ax;
+    return top + (bot - top) * ay;
+}
+
+// The map texels under one tile (x0, y0, TX x TY texels) of a level no
+// coarser than the grid (level <= 3), read once per block into LDS (mt,
+// kMapStageMax floats, clamped like strength_at's reads); on is false when
+// the strength is off for the level, constant, or the level is coarser (then
+// strength_at reads the map directly).  Every thread of the block calls it
+// (256-thread blocks; a __syncthreads follows before strength_at_staged).
+constexpr int kMapStageMax = 34 * 18;
+struct MapStage {
+    int gx0, gy0, w;
+    bool on;
+};
+template <int TX, int TY>
+__device__ __forceinline__ MapStage stage_strength(const StrengthRef& s, int level, int x0, int y0, float* mt,
+                                                   int tid) {
+    MapStage r{0, 0, 0, false};
+    if (s.constant > 0.0f || !s.map || level > 3 || !s.open[level]) return r;
+    const float sc = strength_scale(level);
+    r.gx0 = (int)floorf(((float)x0 + 0.5f) * sc - 0.5f);
+    r.gy0 = (int)floorf(((float)y0 + 0.5f) * sc - 0.5f);
+    const int gx1 = (int)floorf(((float)(x0 + TX - 1) + 0.5f) * sc - 0.5f) + 1;
+    const int gy1 = (int)floorf(((float)(y0 + TY - 1) + 0.5f) * sc - 0.5f) + 1;
+    r.w = gx1 - r.gx0 + 1;
+    const int n = r.w * (gy1 - r.gy0 + 1);
+    for (int i = tid; i < n; i += 256) {
+        const int gx = clampi(r.gx0 + i % r.w, 0, s.gw - 1), gy = clampi(r.gy0 + i / r.w, 0, s.gh - 1);
+        mt[i] = s.map[((size_t)level * s.gh + gy) * s.gw + gx];
+    }
+    r.on = true;
+    return r;
+}
+// strength_at with the four texels from the staged tile (the same values and
+// arithmetic); falls back to strength_at when the tile is not staged
+__device__ __forceinline__ float strength_at_staged(const StrengthRef& s, const MapStage& m, const float* mt,
+                                                    int level, int x, int y) {
+    if (!m.on) return strength_at(s, level, x, y);
+    const float sc = strength_scale(level);
+    const float gx = ((float)x + 0.5f) * sc - 0.5f, gy = ((float)y + 0.5f) * sc - 0.5f;
+    const float fx = floorf(gx), fy = floorf(gy);
+    const float ax = gx - fx, ay = gy - fy;
+    const int i0 = (int)fx - m.gx0, j0 = (int)fy - m.gy0;
+    const float v00 = mt[j0 * m.w + i0], v01 = mt[j0 * m.w + i0 + 1];
+    const float v10 = mt[(j0 + 1) * m.w + i0], v11 = mt[(j0 + 1) * m.w + i0 + 1];
     const float top = v00 + (v01 - v00) * ax, bot = v10 + (v11 - v10) * ax;
     return top + (bot - top) * ay;
 }

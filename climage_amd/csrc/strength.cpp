@@ -5,7 +5,7 @@
 // Plug point: the per-level filter of PyramidalDenoise
 // (ImageDenoiser::denoise, cpp/pyramidal_denoise.hpp:24-34).  The network
 // runs once per frame after the per-level noise fits, on the grid of pyramid
-// level 2, and writes a map of per-level multipliers of the bilateral range
+// level 3, and writes a map of per-level multipliers of the bilateral range
 // sigma (cl/demosaic.cl:735-765) or of the guided filter's epsilon
 // (cl/demosaic.cl:767-851).  A level takes the map only when the gate is
 // open: the network's predicted error of the level against the plain
@@ -16,7 +16,7 @@
 // Training data: seeded synthetic frames with their clean scenes
 // (clim_generate_bayer) go through the plain pipeline; each level is then
 // denoised at seven constant strengths, and the error of each against the
-// clean scene's pyramid (camera RGB) on the grid of level 2 forms the error
+// clean scene's pyramid (camera RGB) on the grid of level 3 forms the error
 // curves the loss interpolates.  The plain pyramid is the judge: the loss of
 // each crop is normalised by its plain-strength error.
 #include <hip/hip_runtime.h>
@@ -142,7 +142,7 @@ void sn_release(clim_ctx* ctx) {
     delete s.trainer;
     s.trainer = nullptr;
     free_weights(s.wt);
-    DevBuf* b[] = {&s.feat, &s.act0, &s.act1, &s.map, &s.gate_sum, &s.open};
+    DevBuf* b[] = {&s.map, &s.gate_sum, &s.open};
     for (DevBuf* p : b) free_buf(*p);
     s.use = nullptr;
 }
@@ -232,7 +232,7 @@ static int sn_backward_train(const clim::SnWeights& wt, SnTrainer* t, int n, int
 
 // --------------------------------------------------------------- inference
 
-// Runs the network on level 2 of the frame's pyramid (lvl[0..levels)) with
+// Runs the network on level 3 of the frame's pyramid (lvl[0..levels)) with
 // the calibrated per-level noise models in the context's fit slots, and
 // fills `sr` for the level filters.  algorithm: which filter each level runs
 // (GuidedFast runs the bilateral on the last level, as the reference's
@@ -240,15 +240,12 @@ static int sn_backward_train(const clim::SnWeights& wt, SnTrainer* t, int n, int
 clim_status strength_infer(clim_ctx* ctx, const Img* lvl, int levels, int algorithm, clim::StrengthRef* sr) {
     SnState& s = ctx->sn;
     *sr = clim::StrengthRef{nullptr, 0, 0, nullptr, 0.0f};
-    if (s.mode == CLIM_STRENGTH_OFF || !s.use || levels < 3 ||
+    if (s.mode == CLIM_STRENGTH_OFF || !s.use || levels <= clim::kSnGrid ||
         (algorithm != CLIM_DENOISE_BILATERAL && algorithm != CLIM_DENOISE_GUIDED_FAST))
         return CLIM_OK;
-    const int gw = lvl[2].w, gh = lvl[2].h;
+    const int gw = lvl[clim::kSnGrid].w, gh = lvl[clim::kSnGrid].h;
     const size_t px = (size_t)gw * gh;
-    CLIM_TRY(devbuf(ctx, s.feat, px * 8 * sizeof(uint16_t)));
-    CLIM_TRY(devbuf(ctx, s.act0, px * 32 * sizeof(uint16_t)));
-    CLIM_TRY(devbuf(ctx, s.act1, px * 32 * sizeof(uint16_t)));
-    CLIM_TRY(devbuf(ctx, s.map, px * 8 * sizeof(float)));
+    CLIM_TRY(devbuf(ctx, s.map, px * clim::kSnLevels * sizeof(float)));
     if (!s.gate_sum.p) {
         CLIM_TRY(devbuf(ctx, s.gate_sum, clim::kSnLevels * sizeof(long long)));
         CLIM_TRY_HIP(ctx, hipMemset(s.gate_sum.p, 0, s.gate_sum.bytes));  // later frames: the gate kernel clears
@@ -259,30 +256,11 @@ clim_status strength_infer(clim_ctx* ctx, const Img* lvl, int levels, int algori
     s.levels = levels;
     hipStream_t st = ctx->stream;
     ProfScope ps(ctx, ST_STRENGTH, (long long)px);
-    clim::launch_sn_features(lvl[2], ctx->fit[1].cal, ctx->fit[3].cal, static_cast<uint16_t*>(s.feat.p), st);
-    const clim::SnWeights& wt = *s.use;
-    const uint16_t* in = static_cast<uint16_t*>(s.feat.p);
-    uint16_t* bufs[2] = {static_cast<uint16_t*>(s.act0.p), static_cast<uint16_t*>(s.act1.p)};
+    // GuidedFast runs the bilateral on its last level (the reference's pyramid does)
     const int guided_mask = algorithm == CLIM_DENOISE_GUIDED_FAST ? ((1 << (levels - 1)) - 1) : 0;
-    for (int l = 0; l < clim::kSnLayers; l++) {
-        clim::SnConv a{};
-        a.x = in;
-        a.wa = wt.fwd[l];
-        a.bias = wt.bias[l];
-        a.n = 1;
-        a.h = gh;
-        a.w = gw;
-        if (l < clim::kSnLayers - 1) {
-            a.y = bufs[l & 1];
-            clim::launch_sn_conv(a, clim::sn_cin(l), clim::SN_RELU, st);
-            in = bufs[l & 1];
-        } else {
-            a.y32 = static_cast<float*>(s.map.p);
-            a.filter = guided_mask;
-            a.gate_sum = static_cast<long long*>(s.gate_sum.p);
-            clim::launch_sn_conv(a, clim::sn_cin(l), clim::SN_MAP, st);
-        }
-    }
+    clim::launch_sn_infer(lvl[clim::kSnGrid], ctx->fit[1].cal, ctx->fit[1 + clim::kSnGrid].cal, *s.use,
+                          static_cast<float*>(s.map.p),
+                          guided_mask, static_cast<long long*>(s.gate_sum.p), st);
     clim::launch_sn_gate(static_cast<long long*>(s.gate_sum.p), levels, s.mode, static_cast<int*>(s.open.p), st);
     CLIM_TRY_HIP(ctx, hipGetLastError());
     *sr = clim::StrengthRef{static_cast<const float*>(s.map.p), gw, gh, static_cast<const int*>(s.open.p), 0.0f};
@@ -354,7 +332,7 @@ extern "C" clim_status clim_strength_map_read(clim_ctx* ctx, float* map, int64_t
     if (!s.map.p) return ctx_fail(ctx, CLIM_EINVAL, "strength_map_read: no frame ran the network");
     *gw = s.gw;
     *gh = s.gh;
-    const int64_t n = (int64_t)s.gw * s.gh * 8;
+    const int64_t n = (int64_t)s.gw * s.gh * clim::kSnLevels;
     if (!map) return CLIM_OK;
     if (capacity < n) return ctx_fail(ctx, CLIM_EINVAL, "strength_map_read: buffer too small");
     CLIM_TRY_HIP(ctx, hipMemcpyAsync(map, s.map.p, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -423,13 +401,13 @@ extern "C" clim_status clim_strength_train(clim_ctx* ctx, const clim_strength_tr
                                            int64_t log_n) {
     if (!ctx || !cfg) return ctx_fail(ctx, CLIM_EINVAL, "strength_train: null argument");
     const clim_strength_train_config c = *cfg;
-    if (c.frames <= 0 || c.width < 256 || c.height < 256 || (c.width & 3) || (c.height & 3) || c.crop <= 0 ||
+    if (c.frames <= 0 || c.width < 256 || c.height < 256 || (c.width & 7) || (c.height & 7) || c.crop <= 0 ||
         c.epochs <= 0 || c.batch <= 0 || !(c.lr > 0.0f) || c.bayer < 0 || c.bayer > 3 || c.noise_levels <= 0 ||
         c.noise_levels > 8)
         return ctx_fail(ctx, CLIM_EINVAL, "strength_train: bad configuration");
     const int levels = 5;
-    const int gw = c.width / 4, gh = c.height / 4, cs = c.crop;
-    if (gw < cs || gh < cs) return ctx_fail(ctx, CLIM_EINVAL, "strength_train: crop larger than the level-2 grid");
+    const int gw = c.width >> clim::kSnGrid, gh = c.height >> clim::kSnGrid, cs = c.crop;
+    if (gw < cs || gh < cs) return ctx_fail(ctx, CLIM_EINVAL, "strength_train: crop larger than the grid");
     const int cx = gw / cs, cy = gh / cs, per_frame = cx * cy;
     const long long ncrops = (long long)per_frame * c.frames;
     if (log && log_n < (int64_t)c.epochs * 2) return ctx_fail(ctx, CLIM_EINVAL, "strength_train: log too small");
@@ -527,7 +505,8 @@ extern "C" clim_status clim_strength_train(clim_ctx* ctx, const clim_strength_tr
                 }
             }
         }
-        clim::launch_sn_features(lv[2], ctx->fit[1].cal, ctx->fit[3].cal, static_cast<uint16_t*>(ffeat.p), st);
+        clim::launch_sn_features(lv[clim::kSnGrid], ctx->fit[1].cal, ctx->fit[1 + clim::kSnGrid].cal,
+                                 static_cast<uint16_t*>(ffeat.p), st);
         const size_t base = (size_t)f * per_frame;
         clim::launch_sn_gather(static_cast<uint16_t*>(ffeat.p), static_cast<float*>(fcurves.p), gw, cs,
                                static_cast<int2*>(origins.p), per_frame,

@@ -1,9 +1,9 @@
 // strength.h - the learned-strength network (internal to the library).
 //
 // A 4-layer 3x3 convolutional network (8 -> 32 -> 32 -> 32 -> 20 channels,
-// ReLU between layers, zero padding) evaluated on the grid of pyramid level 2
-// (a quarter of the frame's width and height).  Inputs: 5x5 statistics of
-// level 2 against the fitted noise models (k_sn_features).  Outputs per
+// ReLU between layers, zero padding) evaluated on the grid of pyramid level
+// kSnGrid = 3 (an eighth of the frame's width and height).  Inputs: 5x5
+// statistics of that level against the fitted noise models (k_sn_features).  Outputs per
 // level l (0..4): channel 4l + 0 the log2 bilateral range multiplier, 4l + 1
 // the log2 guided-epsilon multiplier (applied squared), 4l + 2 / 4l + 3 the
 // predicted log2 ratio of the level's error with the learned strength to its
@@ -25,6 +25,7 @@ constexpr int kSnFeat = 8;        // input channels
 constexpr int kSnHidden = 32;     // hidden channels
 constexpr int kSnOut = 20;        // output channels (4 per level)
 constexpr int kSnLevels = 5;
+constexpr int kSnGrid = 3;        // the pyramid level whose texels the network runs on
 constexpr int kSnCurveK = 7;      // strengths of a training error curve: log2 m = (k - 3) / 2
 constexpr int kSnCurves = kSnLevels * 2 * kSnCurveK;  // [level][filter][k] per grid texel
 constexpr float kSnSMax = 1.5f;   // |log2 m| bound (the curves' span)
@@ -52,8 +53,7 @@ struct SnWeights {
 enum SnEpilogue {
     SN_RELU = 0,   // bf16 out = relu(acc + bias)
     SN_MASK = 1,   // bf16 out = acc where mask > 0 (backward through the previous ReLU)
-    SN_RAW = 2,    // fp32 out [P][32] = acc + bias (training forward of the last layer)
-    SN_MAP = 3     // inference: strength map [P][8] and the gate's predicted-error sums
+    SN_RAW = 2     // fp32 out [P][32] = acc + bias (training forward of the last layer)
 };
 
 struct SnConv {
@@ -63,12 +63,19 @@ struct SnConv {
     int n, h, w;
     uint16_t* y;            // SN_RELU / SN_MASK: [N][H][W][32]
     const uint16_t* mask;   // SN_MASK: [N][H][W][32]
-    float* y32;             // SN_RAW: [N][H][W][32]; SN_MAP: the map [N][H][W][8]
-    int filter;             // SN_MAP: 0 bilateral, 1 guided
-    long long* gate_sum;    // SN_MAP: [5] fixed-point sums of the predicted log2 error ratio (2^-16 units)
+    float* y32;             // SN_RAW: [N][H][W][32]
 };
 // cin = 8 or 32
 void launch_sn_conv(const SnConv& a, int cin, SnEpilogue epi, hipStream_t st);
+
+// inference of one frame in one launch: features of the grid level (lvg) and all
+// four layers with the activations in LDS, writing the map [5][gh][gw]
+// (plane l: level l's multipliers) and adding the predicted log2 error
+// ratios' sums (2^-16 units) to gate_sum[5]; filter: bit l set when the
+// guided filter denoises level l (its epsilon multiplier, 4^s, is mapped),
+// else the bilateral's range multiplier 2^s
+void launch_sn_infer(const Img& lvg, const float* nlf0, const float* nlfg, const SnWeights& wt, float* map,
+                     int filter, long long* gate_sum, hipStream_t st);
 
 // weight gradient of one layer: grad (canonical layout of the layer, FP32)
 // = sum over pixels of x(shifted) * dz, bias gradient = sum dz; partials:
@@ -90,13 +97,13 @@ void launch_sn_adam(float* master, const float* grad, float* m, float* v, int n,
 int launch_sn_loss(const float* out32, const float* curves, const float* norm, int n, int hw, float scale,
                    uint16_t* dz, float* loss_partials, hipStream_t st);
 
-// features of the level-2 image lv2 (RGBA32F) with the calibrated noise
-// models of level 0 and level 2 (device: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr)
-void launch_sn_features(const Img& lv2, const float* nlf0, const float* nlf2, uint16_t* feat, hipStream_t st);
+// features of the grid level's image lvg (RGBA32F) with the calibrated noise
+// models of level 0 and of the grid level (device: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr)
+void launch_sn_features(const Img& lvg, const float* nlf0, const float* nlfg, uint16_t* feat, hipStream_t st);
 
 // error curve entry (level, filter, k) on the grid gw x gh: mean over the
-// level's texels in each grid texel (level <= 2) or the texel under it
-// (level > 2) of |y2c (den - clean)|^2
+// level's texels in each grid texel (level <= kSnGrid) or the texel under it
+// (coarser levels) of |y2c (den - clean)|^2
 void launch_sn_curve(const Img& den, const Img& clean, const float y2c[9], int level, int gw, int gh, float* curves,
                      int slot, hipStream_t st);
 

@@ -5,8 +5,8 @@
 Training runs inside the library (clim_strength_train): seeded synthetic
 4032x3024 RGGB frames and their clean scenes go through the plain pipeline;
 every pyramid level is denoised at seven constant strengths and compared with
-the clean scene's pyramid, giving the error curves of 64x64 crops of the
-level-2 grid (256x256 frame pixels; 165 crops a frame, 20130 for 122 frames);
+the clean scene's pyramid, giving the error curves of 32x32 crops of the
+level-3 grid (256x256 frame pixels; 165 crops a frame, 20130 for 122 frames);
 the network (BF16 MFMA kernels) is fitted to them with Adam.  The script then
 measures what the trained network gains: PSNR against the clean truth (the
 clean scene tone-mapped by the pipeline's own tone map) of the plain, gated
@@ -83,7 +83,7 @@ def main():
     with ca.Context(0) as ctx:
         t0 = time.perf_counter()
         print(f"training: {args.frames} frames {args.width}x{args.height}, {args.epochs} epochs", flush=True)
-        log = ctx.strength_train(frames=args.frames, width=args.width, height=args.height, crop=64,
+        log = ctx.strength_train(frames=args.frames, width=args.width, height=args.height, crop=32,
                                  epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, verbose=True)
         dt = time.perf_counter() - t0
         for e, (ls, lr) in enumerate(log):
@@ -96,8 +96,8 @@ def main():
         rows = evaluate(ctx, args.width, args.height, seeds)
         gain = {k: round(float(np.mean([r[k] - r["plain"] for r in rows])), 4) for k in ("gated", "learned")}
         print(f"mean PSNR gain over plain: gated {gain['gated']} dB, learned {gain['learned']} dB", flush=True)
-        meta = {"frames": args.frames, "size": [args.width, args.height], "crop": 64,
-                "crops": args.frames * (args.width // 4 // 64) * (args.height // 4 // 64), "epochs": args.epochs,
+        meta = {"frames": args.frames, "size": [args.width, args.height], "crop": 32, "grid_level": 3,
+                "crops": args.frames * (args.width // 8 // 32) * (args.height // 8 // 32), "epochs": args.epochs,
                 "batch": args.batch, "lr": args.lr, "seed": args.seed, "train_seconds": round(dt, 2),
                 "loss_log": log.round(6).tolist(), "eval": rows, "mean_gain_db": gain}
         with open(os.path.join(args.out, "strength.json"), "w") as f:

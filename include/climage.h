@@ -384,14 +384,14 @@ clim_status clim_generate_bayer(clim_ctx* ctx, clim_image* raw, clim_image* clea
  * point (ImageDenoiser::denoise, cpp/pyramidal_denoise.hpp:24-34).  A
  * 4-layer 3x3 convolutional network (8 -> 32 -> 32 -> 32 -> 20 channels,
  * bfloat16 weights, FP32 accumulation on the matrix cores) runs once per
- * frame on the grid of pyramid level 2, after the per-level noise fits, and
+ * frame on the grid of pyramid level 3, after the per-level noise fits, and
  * predicts for every texel and level a multiplier of the bilateral range
  * sigma (cl/demosaic.cl:735-765) or of the guided filter's epsilon
  * (cl/demosaic.cl:767-851), and the level's error relative to the plain
  * strength.  A level uses the multipliers only when its gate is open; shut,
  * it runs the plain filter and its output is the plain pyramid's bit for
  * bit.  It applies to the Bilateral and GuidedFast denoisers of pyramids with
- * at least 3 levels (otherwise every level is plain).
+ * at least 4 levels (otherwise every level is plain).
  *
  * Parameters: CLIM_STRENGTH_PARAMS floats, per layer l the weights
  * W[tap = 3 ky + kx][cin][cout] then the biases b[cout]; weights are rounded
@@ -411,8 +411,8 @@ clim_status clim_strength_set_mode(clim_ctx* ctx, int32_t mode);
 /* Blocking.  The last frame's gate: open[l], and (nullable) the mean
  * predicted log2 error ratio of level l over the grid. */
 clim_status clim_strength_gate_read(clim_ctx* ctx, int32_t open[5], float mean_pred[5]);
-/* Blocking.  The last frame's multiplier map: gw x gh texels (the grid of
- * level 2) of 8 floats, level l's multiplier in float l.  With map null only
+/* Blocking.  The last frame's multiplier map: 5 planes of gw x gh floats
+ * (the grid of level 3), plane l level l's multipliers.  With map null only
  * gw and gh are written. */
 clim_status clim_strength_map_read(clim_ctx* ctx, float* map, int64_t capacity, int32_t* gw, int32_t* gh);
 /* Blocking.  The training loss and its gradient on one batch (host arrays):
@@ -426,8 +426,8 @@ clim_status clim_strength_loss_grad(clim_ctx* ctx, const float* params, const fl
                                     double loss[2]);
 typedef struct {
     int32_t frames;         /* synthetic frames (seeds seed + i, noise seeds seed + 100000 + i) */
-    int32_t width, height;  /* frame size (multiples of 4, at least 256) */
-    int32_t crop;           /* crop size on the level-2 grid (64: 256 x 256 frame pixels) */
+    int32_t width, height;  /* frame size (multiples of 8, at least 256) */
+    int32_t crop;           /* crop size on the level-3 grid (32: 256 x 256 frame pixels) */
     int32_t epochs, batch;  /* passes over the crops, crops per step */
     float lr;               /* Adam step (a tenth over the last fifth of the epochs) */
     uint32_t seed;

@@ -713,10 +713,10 @@ void P(noise_statistics)(const real* in, int w, int h, real* out) {
 
 /* ----------------------------------------------------- learned strength map
  * The learned-strength network's per-level multipliers (no reference
- * counterpart; climage_amd/csrc/kernels.h strength_at): a map of gw x gh
- * texels (the grid of pyramid level 2) of 8 floats, level l's multiplier in
- * float l, sampled bilinearly (clamped) at the level texel's centre, grid
- * coordinate (x + 1/2) 2^(l - 2) - 1/2, in FP32 in both builds.  The
+ * counterpart; climage_amd/csrc/kernels.h strength_at): a map of 5 planes
+ * of gw x gh floats (the grid of pyramid level 3), plane l level l's
+ * multipliers, sampled bilinearly (clamped) at the level texel's centre, grid
+ * coordinate (x + 1/2) 2^(l - 3) - 1/2, in FP32 in both builds.  The
  * bilateral range threshold (squared) is multiplied by m^2, the guided
  * epsilon by m.  Level l reads the map only when open[l] != 0. */
 typedef struct {
@@ -729,7 +729,7 @@ typedef struct {
 #define O_STRENGTH_AT
 static float o_strength_at(const o_strength* s, int level, int x, int y) {
     if (!s || !s->map || !s->open[level]) return 0.0f;
-    const float sc = level == 0 ? 0.25f : level == 1 ? 0.5f : level == 2 ? 1.0f : level == 3 ? 2.0f : 4.0f;
+    const float sc = level == 0 ? 0.125f : level == 1 ? 0.25f : level == 2 ? 0.5f : level == 3 ? 1.0f : 2.0f;
     const float gx = ((float)x + 0.5f) * sc - 0.5f, gy = ((float)y + 0.5f) * sc - 0.5f;
     const float fx = floorf(gx), fy = floorf(gy);
     const float ax = gx - fx, ay = gy - fy;
@@ -738,9 +738,9 @@ static float o_strength_at(const o_strength* s, int level, int x, int y) {
     const int x1 = ix + 1 < 0 ? 0 : (ix + 1 > s->gw - 1 ? s->gw - 1 : ix + 1);
     const int y0 = iy < 0 ? 0 : (iy > s->gh - 1 ? s->gh - 1 : iy);
     const int y1 = iy + 1 < 0 ? 0 : (iy + 1 > s->gh - 1 ? s->gh - 1 : iy + 1);
-    const float* m = s->map + level;
-    const float v00 = m[((size_t)y0 * s->gw + x0) * 8], v01 = m[((size_t)y0 * s->gw + x1) * 8];
-    const float v10 = m[((size_t)y1 * s->gw + x0) * 8], v11 = m[((size_t)y1 * s->gw + x1) * 8];
+    const float* m = s->map + (size_t)level * s->gw * s->gh;
+    const float v00 = m[(size_t)y0 * s->gw + x0], v01 = m[(size_t)y0 * s->gw + x1];
+    const float v10 = m[(size_t)y1 * s->gw + x0], v11 = m[(size_t)y1 * s->gw + x1];
     const float top = v00 + (v01 - v00) * ax, bot = v10 + (v11 - v10) * ax;
     return top + (bot - top) * ay;
 }

@@ -107,11 +107,14 @@ def generate_bayer(w, h, bayer, scene_seed, noise_seed, a, b, black, white, clea
 
 
 class Oracle:
-    def __init__(self, double=False):
+    """lib_path: another build of the same sources (the speed-built CPU
+    baseline of `make -C oracle fast`)."""
+
+    def __init__(self, double=False, lib_path=None):
         self.double = double
         self.dt = np.float64 if double else np.float32
         self.pfx = "o64_" if double else "o32_"
-        self.L = lib()
+        self.L = C.CDLL(lib_path) if lib_path else lib()
 
     def _fn(self, name):
         return getattr(self.L, self.pfx + name)
@@ -332,8 +335,8 @@ class Oracle:
         return res + (g,) if level3 else res
 
     def raw_to_rgb(self, raw, params, dumps=False, strength=None):
-        """strength (optional): (map (gh, gw, 8) float32, open (5,) ints), the
-        learned-strength multipliers of the level filters."""
+        """strength (optional): (map (5, gh, gw) float32, open (5,) ints), the
+        learned-strength multipliers of the level filters (plane l: level l)."""
         h, w = raw.shape
         raw = np.ascontiguousarray(raw, np.uint16)
         out8 = np.zeros((h, w, 4), np.uint8)
@@ -347,10 +350,10 @@ class Oracle:
         else:
             smap = np.ascontiguousarray(strength[0], np.float32)
             sopen = np.ascontiguousarray(strength[1], np.int32)
-            assert smap.ndim == 3 and smap.shape[2] == 8 and sopen.shape == (5,)
+            assert smap.ndim == 3 and smap.shape[0] == 5 and sopen.shape == (5,)
             rc = self._fn("raw_to_rgb_strength")(_ptr(raw), w, h, C.byref(params), _ptr(out8), _ptr(outf), nlf,
                                                  _ptr(ycc) if dumps else None, _ptr(den) if dumps else None,
-                                                 _ptr(smap), smap.shape[1], smap.shape[0], _ptr(sopen))
+                                                 _ptr(smap), smap.shape[2], smap.shape[1], _ptr(sopen))
         nl = np.array(nlf[:], np.float32)
         res = {"rc": rc, "rgba8": out8, "rgbf": outf, "rawNlf": nl[:4],
                "pyramidNlf": nl[4:4 + 6 * params.levels].reshape(params.levels, 6)}

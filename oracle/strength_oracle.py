@@ -153,12 +153,14 @@ def loss_grad(params, feat, curves, norm, storage="bf16", target=None):
     return (l_s, l_r), np.concatenate(grads), out
 
 
-def features(lv2, nlf0, nlf2):
-    """Network inputs (H, W, 8) of a level-2 image (H, W, 4): 5x5 clamped mean
-    and variance of Y, Cb, Cr and the calibrated models' predicted variances
-    (nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr)."""
+def features(lvg, nlf0, nlfg):
+    """Network inputs (H, W, 8) of the grid level's image (pyramid level 3,
+    (H, W, 4)): 5x5 clamped mean and variance of Y, Cb, Cr and the calibrated
+    models' predicted variances (nlf: A_Y, A_Cb, A_Cr, B_Y, B_Cb, B_Cr) of
+    level 0 and of the grid level."""
     from scipy import ndimage
-    img = np.asarray(lv2, np.float64)
+    nlf2 = nlfg
+    img = np.asarray(lvg, np.float64)
     m = [ndimage.uniform_filter(img[..., c], 5, mode="nearest") for c in range(3)]
     q = [ndimage.uniform_filter(img[..., c] ** 2, 5, mode="nearest") for c in range(3)]
     mY = m[0]
