@@ -112,7 +112,7 @@ def exported_symbols():
                         "clim_fast_debayer", "clim_fast_demosaic", "clim_dng_decode", "clim_unpack_dng_metadata",
                         "clim_dng_last_error",
                         "clim_cam_ycbcr", "clim_inverse3x3", "clim_default_params",
-                        "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_raw_to_rgb_batch",
+                        "clim_pyramidal_denoise", "clim_pyramidal_denoise_into", "clim_raw_to_rgb", "clim_raw_to_rgb_ex", "clim_raw_to_rgb_batch",
                         "clim_generate_bayer", "clim_compute_raw_noise_statistics_noalloc",
                         "clim_strength_set_weights", "clim_strength_get_weights", "clim_strength_set_mode",
                         "clim_strength_gate_read", "clim_strength_map_read", "clim_strength_loss_grad",
@@ -409,13 +409,15 @@ class Context:
                     "pyramidalDenoiseInto")
         return np.array(buf[:6 * levels], np.float32).reshape(levels, 6)
 
-    def raw_to_rgb(self, raw, out8, params, out_f32=None, writeback=True):
-        """RawConverter::demosaicImage.  With writeback, blocks and returns the
-        measured noise model (rawNlf, pyramidNlf)."""
+    def raw_to_rgb(self, raw, out8, params, out_f32=None, writeback=True, rotate_180=False):
+        """RawConverter::demosaicImage (rotate_180: clim_raw_to_rgb_ex).  With
+        writeback, blocks and returns the measured noise model (rawNlf,
+        pyramidNlf)."""
         nm = NoiseModel() if writeback else None
-        self._check(lib().clim_raw_to_rgb(self.h, raw.ref(), out8.ref(),
-                                          out_f32.ref() if out_f32 is not None else None, C.byref(params),
-                                          C.byref(nm) if writeback else None), "raw_to_rgb")
+        self._check(lib().clim_raw_to_rgb_ex(self.h, raw.ref(), out8.ref(),
+                                             out_f32.ref() if out_f32 is not None else None, C.byref(params),
+                                             C.byref(nm) if writeback else None, int(bool(rotate_180))),
+                    "raw_to_rgb")
         if not writeback:
             return None
         return (np.array(nm.rawNlf[:], np.float32),

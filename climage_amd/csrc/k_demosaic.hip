@@ -427,16 +427,30 @@ void launch_red_blue(const MosaicArg& raw, const Img& green, const Img& out, int
     dim3 blk(64, 4), grd((out.w + 127) / 128, (out.h + 7) / 8);  // one quad per thread
     if (fuse && raw.u16)
         hipLaunchKernelGGL((k_red_blue<true, true>), grd, blk, 0, st, raw.src<true>(), green, out, b, red_is_c1, cv,
-                           0, clip, m);
+                           rotate_180, clip, m);
     else if (fuse)
         hipLaunchKernelGGL((k_red_blue<true, false>), grd, blk, 0, st, raw.src<false>(), green, out, b, red_is_c1, cv,
-                           0, clip, m);
+                           rotate_180, clip, m);
     else if (raw.u16)
         hipLaunchKernelGGL((k_red_blue<false, true>), grd, blk, 0, st, raw.src<true>(), green, out, b, red_is_c1, cv,
                            rotate_180, clip, m);
     else
         hipLaunchKernelGGL((k_red_blue<false, false>), grd, blk, 0, st, raw.src<false>(), green, out, b, red_is_c1,
                            cv, rotate_180, clip, m);
+}
+
+// Row 0 and column 0 to zero: the texels a rotated interpolateRedBlue never
+// writes (its target of source (x, y) is (w - x, h - y), kept when inside)
+__global__ void __launch_bounds__(256) k_zero_row0_col0(Img out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < out.w) wrow_f4(out, 0)[i] = z;
+    if (i < out.h) wrow_f4(out, i)[0] = z;
+}
+
+void launch_zero_row0_col0(const Img& out, hipStream_t st) {
+    const int n = out.w > out.h ? out.w : out.h;
+    hipLaunchKernelGGL(k_zero_row0_col0, dim3((n + 255) / 256), dim3(256), 0, st, out);
 }
 
 // ------------------------------------------- stand-alone pointwise operators

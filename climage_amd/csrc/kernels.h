@@ -98,6 +98,9 @@ void launch_green(const MosaicArg& raw, const Img& green, int bayer, const Param
 void launch_red_blue(const MosaicArg& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
                      const ParamRef& cv, int rotate_180, bool fuse, float clip, const float c2y[9],
                      hipStream_t st);
+// row 0 and column 0 of an RGBA32F image to zero (the texels a rotated
+// interpolateRedBlue leaves unwritten)
+void launch_zero_row0_col0(const Img& out, hipStream_t st);
 void launch_blend_highlights(const Img& in, const Img& out, float clip, hipStream_t st);
 void launch_transform(const Img& in, const Img& out, const float mat[9], hipStream_t st);
 void launch_despeckle(const Img& in, const Img& out, float var_a, float var_b, hipStream_t st);

@@ -271,8 +271,8 @@ extern "C" clim_status clim_pyramidal_denoise_into(clim_ctx* ctx, const clim_ima
     return CLIM_OK;
 }
 
-extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8,
-                                       clim_image* out_f32, const clim_demosaic_params* p, clim_noise_model* nlf_out) {
+static clim_status raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8, clim_image* out_f32,
+                              const clim_demosaic_params* p, clim_noise_model* nlf_out, int rotate_180) {
     if (!ctx || !p) return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: null argument");
     CLIM_TRY(check_params(ctx, p));
     CLIM_TRY(check_image(ctx, raw, CLIM_FMT_R16U, "raw"));
@@ -350,7 +350,17 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
     // despeckleImage with the caller's pyramidNlf[0] (cpp/raw_converter.cpp:144-145),
     // in the same kernel as interpolateRedBlue
     const float* np0 = p->noiseModel.pyramidNlf[0];
-    {
+    if (rotate_180) {
+        // cl/demosaic.cl:313-315: source texel (x, y) lands on (w - x, h - y)
+        // when that is inside the image, so row 0 and column 0 are never
+        // written; here they are zero (the reference leaves whatever its
+        // image held).  The despeckle then runs on the rotated image.
+        ProfScope ps(ctx, ST_RED_BLUE, npx);
+        clim::launch_zero_row0_col0(ws.rgbA.img, st);
+        clim::launch_red_blue(mosaic, ws.green.img, ws.rgbA.img, bayer, b, dev_params(rawfit), 1, true, 1.0f, c2y,
+                              st);
+        clim::launch_despeckle(ws.rgbA.img, ws.rgbB.img, np0[0], np0[3], st);
+    } else {
         ProfScope ps(ctx, ST_RED_BLUE, npx);
         clim::launch_rb_despeckle(mosaic, ws.green.img, ws.rgbB.img, bayer, b, dev_params(rawfit), 1.0f, c2y, np0[0],
                                   np0[3], st);
@@ -383,6 +393,17 @@ extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, cli
             std::memcpy(nlf_out->pyramidNlf[i], ctx->fit_host[1 + i].out, 6 * sizeof(float));
     }
     return CLIM_OK;
+}
+
+extern "C" clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8,
+                                       clim_image* out_f32, const clim_demosaic_params* p, clim_noise_model* nlf_out) {
+    return raw_to_rgb(ctx, raw, out_rgba8, out_f32, p, nlf_out, 0);
+}
+
+extern "C" clim_status clim_raw_to_rgb_ex(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8,
+                                          clim_image* out_f32, const clim_demosaic_params* p,
+                                          clim_noise_model* nlf_out, int32_t rotate_180) {
+    return raw_to_rgb(ctx, raw, out_rgba8, out_f32, p, nlf_out, rotate_180 ? 1 : 0);
 }
 
 extern "C" clim_status clim_fast_demosaic(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8,

@@ -351,6 +351,17 @@ clim_status clim_raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* ou
                             clim_image* out_f32, const clim_demosaic_params* params,
                             clim_noise_model* nlf_out);
 
+/* The same with RawConverter::demosaicImage's rotate_180 argument
+ * (cpp/raw_converter.hpp:54-55): non-zero rotates the image by 180 degrees
+ * where interpolateRedBlue writes it, cl/demosaic.cl:313-315, with the
+ * reference's target (width - x, height - y) - one texel off a true
+ * rotation; row 0 and column 0, which no source texel reaches, are zero
+ * before the despeckle.  The rest of the path runs on the rotated image.
+ * clim_raw_to_rgb is this call with rotate_180 = 0. */
+clim_status clim_raw_to_rgb_ex(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8,
+                               clim_image* out_f32, const clim_demosaic_params* params,
+                               clim_noise_model* nlf_out, int32_t rotate_180);
+
 /* n frames, each as clim_raw_to_rgb(raws[i], out_rgba8[i], out_f32 ? out_f32[i]
  * : null, &params[i]) - RawConverter::demosaicImage called once per frame of a
  * burst (cpp/raw_converter.cpp:73-186), the frames of any sizes and CFA

@@ -18,22 +18,30 @@ class RawConverter {
                           PyramidalDenoise<5>::DenoiseAlgorithm algorithm = PyramidalDenoise<5>::Bilateral)
         : _ctx(ctx), _levels(levels), _algorithm(algorithm) {}
 
+    // gmb_position is accepted and ignored, as in the reference: its only
+    // consumer, PyramidalDenoise::denoise, calibrates the noise model on
+    // every frame whatever the chart position (cpp/pyramidal_denoise.cpp:109,
+    // "calibrate_nlf = true; // gmb_position != nullptr").  rotate_180 turns
+    // the image by 180 degrees at interpolateRedBlue with the reference's
+    // (width - x, height - y) target (cl/demosaic.cl:313-315); row 0 and
+    // column 0, which no texel reaches, are zero (clim_raw_to_rgb_ex).
     gls::hip_image_2d<gls::rgba_pixel>* demosaicImage(const gls::image<gls::luma_pixel_16>& rawImage,
                                                       DemosaicParameters* demosaicParameters,
                                                       const gls::rectangle* /*gmb_position*/, bool rotate_180) {
-        if (rotate_180) throw gls::clim_error(CLIM_EINVAL, "demosaicImage: rotate_180 is not supported");
         allocateTextures(rawImage.width, rawImage.height);
         _raw->copyPixelsFrom(rawImage);
-        return demosaicImage(*_raw, demosaicParameters);
+        return demosaicImage(*_raw, demosaicParameters, rotate_180);
     }
 
     // device-resident input: no upload
     gls::hip_image_2d<gls::rgba_pixel>* demosaicImage(const gls::hip_image_2d<gls::luma_pixel_16>& raw,
-                                                      DemosaicParameters* demosaicParameters) {
+                                                      DemosaicParameters* demosaicParameters,
+                                                      bool rotate_180 = false) {
         allocateTextures(raw.width, raw.height);
         clim_demosaic_params p = to_clim(*demosaicParameters, _levels, _algorithm);
         clim_noise_model nm;
-        _ctx->check(clim_raw_to_rgb(_ctx->ctx(), raw.clim(), _srgb->clim(), nullptr, &p, &nm), "demosaicImage");
+        _ctx->check(clim_raw_to_rgb_ex(_ctx->ctx(), raw.clim(), _srgb->clim(), nullptr, &p, &nm, rotate_180 ? 1 : 0),
+                    "demosaicImage");
         for (int c = 0; c < 4; c++) demosaicParameters->noiseModel.rawNlf[c] = nm.rawNlf[c];
         for (int l = 0; l < _levels; l++)
             for (int k = 0; k < 6; k++) demosaicParameters->noiseModel.pyramidNlf[l][k] = nm.pyramidNlf[l][k];

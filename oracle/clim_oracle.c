@@ -1473,6 +1473,7 @@ typedef struct {
     int localToneMapping;  /* needs levels >= 4 */
     float ltm[4];          /* guidedFilterEps, shadows, highlights, detail */
     float noiseLevel;      /* > 0.6: the high-noise raw branch */
+    int rotate_180;        /* RawConverter::demosaicImage's rotate_180 */
 } o_pipeline_params;
 
 /* Full path.  nlf_out: rawNlf[4] then levels x 6.  Optional stage dumps
@@ -1544,7 +1545,10 @@ int P(raw_to_rgb_strength)(const uint16_t* raw, int w, int h, const o_pipeline_p
     }
 
     P(interpolate_green)(scaled, w, h, pp->bayer, rawNlf[1], green);
-    P(interpolate_red_blue)(scaled, green, w, h, pp->bayer, (rawNlf[0] + rawNlf[2]) / 2.0f, 0, rgbA);
+    /* cpp/raw_converter.cpp:127-128 passes rotate_180 to interpolateRedBlue,
+     * whose rotated writes never reach row 0 or column 0: they start at zero */
+    if (pp->rotate_180) memset(rgbA, 0, n * 4 * sizeof(real));
+    P(interpolate_red_blue)(scaled, green, w, h, pp->bayer, (rawNlf[0] + rawNlf[2]) / 2.0f, pp->rotate_180, rgbA);
     P(blend_highlights)(rgbA, w, h, 1.0f, rgbA);
     float c2y[9], y2c[9];
     P(cam_ycbcr)(pp->rgb_cam, c2y);

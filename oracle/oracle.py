@@ -46,7 +46,7 @@ class PipelineParams(C.Structure):
                 ("contrast", C.c_float), ("saturation", C.c_float), ("toneCurveSlope", C.c_float),
                 ("exposureBias", C.c_float), ("blacks", C.c_float), ("levels", C.c_int),
                 ("algorithm", C.c_int), ("localToneMapping", C.c_int), ("ltm", C.c_float * 4),
-                ("noiseLevel", C.c_float)]
+                ("noiseLevel", C.c_float), ("rotate_180", C.c_int)]
 
 
 # pyramidNlf row 0 of cpp/raw_pipeline.cpp:105-111 ("minimal noise model")
