@@ -42,6 +42,8 @@ sys.path.insert(0, ROOT)
 import climage_amd as ca  # noqa: E402
 
 NOISE_A = [0.0005, 0.001, 0.002, 0.004, 0.008, 0.016]
+ALGORITHMS = {"bilateral": ca.BILATERAL, "guided_fast": ca.GUIDED_FAST, "guided_precise": ca.GUIDED_PRECISE}
+ALGORITHM_NAMES = {ca.BILATERAL: "bilateral", ca.GUIDED_FAST: "GuidedFast", ca.GUIDED_PRECISE: "GuidedPrecise"}
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0  # dense MFMA
 # flops of the strength network per texel of its grid (pyramid level 3): 2 x 9 x
@@ -102,14 +104,14 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(w, h, bayer, ltm=False, noise_level=0.0, seconds_min=10.0):
+def cpu_baseline(w, h, bayer, ltm=False, noise_level=0.0, algorithm=0, seconds_min=10.0):
     """The FP32 CPU restatement built for this host (-O3 -march=native, OpenMP
     row bands over every host thread) on the bench frames (same sizes and
     noise levels), as many as fit in about `seconds_min` seconds (at least one)."""
     from oracle import oracle as O
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "fast"], check=True)
     o = O.Oracle(double=False, lib_path=os.path.join(ROOT, "oracle", "_build", "liboracle_fast.so"))
-    p = O.default_params(bayer, 64.0, 4095.0, levels=5)
+    p = O.default_params(bayer, 64.0, 4095.0, levels=5, algorithm=algorithm)
     p.localToneMapping = int(ltm)
     p.noiseLevel = noise_level
     raws = [O.generate_bayer(w, h, bayer, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
@@ -129,11 +131,11 @@ def cpu_baseline(w, h, bayer, ltm=False, noise_level=0.0, seconds_min=10.0):
                       f"FP32 CPU restatement in {dt:.1f} s, OpenMP row bands on {cores} threads"}
 
 
-def fp64_parity(raw_host, bayer, gpu_f32, gpu_nlf, levels=5):
+def fp64_parity(raw_host, bayer, gpu_f32, gpu_nlf, levels=5, algorithm=0):
     """The GPU frame against the FP64 and FP32 CPU restatements."""
     from oracle import oracle as O
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
-    p = O.default_params(bayer, 64.0, 4095.0, levels=levels)
+    p = O.default_params(bayer, 64.0, 4095.0, levels=levels, algorithm=algorithm)
     r64 = O.Oracle(double=True).raw_to_rgb(raw_host, p)
     r32 = O.Oracle(double=False).raw_to_rgb(raw_host, p)
     d = gpu_f32[..., :3].astype(np.float64) - r64["rgbf"][..., :3]
@@ -218,6 +220,8 @@ def main():
                          "RGGB and BGGR alternating, noise levels drawn from the headline set, one context")
     ap.add_argument("--sequential", action="store_true",
                     help="one clim_raw_to_rgb call per frame instead of clim_raw_to_rgb_batch over the step's frames")
+    ap.add_argument("--algorithm", choices=list(ALGORITHMS), default="bilateral",
+                    help="the pyramid's denoiser (PyramidalDenoise::DenoiseAlgorithm)")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device fill and device-to-device copy (reference kernels "
@@ -232,6 +236,7 @@ def main():
         sys.exit(2)
 
     w, h, bayer = args.width, args.height, ca.RGGB
+    algorithm = ALGORITHMS[args.algorithm]
     lib = ca.lib()
     ctx = ca.Context(0)
     # frames: (width, height, CFA, noise a, frame id)
@@ -247,7 +252,7 @@ def main():
         ctx.generate_bayer(r, fb, 1000 + i, 2000 + i, a, a / 200.0, 64.0, 4095.0)
         raws.append(r)
         outs.append(ctx.image(fw // 2, fh // 2, ca.FMT_RGBA8) if args.preview else ctx.image(fw, fh, ca.FMT_RGBA8))
-        fp = ca.default_params(fb, 64.0, 4095.0, levels=5, algorithm=ca.BILATERAL)
+        fp = ca.default_params(fb, 64.0, 4095.0, levels=5, algorithm=algorithm)
         fp.rgbConversionParameters.localToneMapping = int(args.ltm)
         fp.noiseLevel = args.noise_level
         fparams.append(fp)
@@ -257,6 +262,10 @@ def main():
         single_kernel["tone"] = "k_reassemble<true, true, true>"
     if args.preview:
         single_kernel["tone"] = "k_fast_preview<true>"
+    if algorithm == ca.GUIDED_FAST:
+        single_kernel["denoise"] = "k_guided_fast"
+    elif algorithm == ca.GUIDED_PRECISE:
+        del single_kernel["denoise"]  # seven kernels per level
     learned_on = not args.preview and not args.no_learned
     if learned_on or args.learned:
         ctx.strength_set_weights(ca.load_strength_weights())
@@ -376,7 +385,9 @@ def main():
         path = {"algorithmic": {"bytes_per_px": bpp, "achieved": round(bpp * mpix * 1e6 / 1e9, 1),
                                 "frac": round(bpp * mpix * 1e6 / 1e9 / HBM_PEAK_GBS, 4)},
                 "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-        fb = None if (args.burst or args.learned) else measured_frame_bytes(w, h)
+        default_workload = not (args.burst or args.learned or args.ltm or args.noise_level > 0.6
+                                or algorithm != ca.BILATERAL)
+        fb = measured_frame_bytes(w, h) if default_workload else None
         if fb:
             mbpp = fb / (w * h)
             path["measured"] = {"bytes_per_px": round(mbpp, 2), "achieved": round(mbpp * mpix * 1e6 / 1e9, 1),
@@ -438,14 +449,14 @@ def main():
     if not (args.no_parity or args.preview or args.burst or args.learned or args.ltm or args.noise_level > 0.6):
         of = ctx.image(w, h, ca.FMT_RGBA32F)
         nlf = ctx.raw_to_rgb(raws[0], outs[0], fparams[0], out_f32=of)
-        parity = fp64_parity(raws[0].download(), bayer, of.download(), nlf)
+        parity = fp64_parity(raws[0].download(), bayer, of.download(), nlf, algorithm=algorithm)
         of.free()
 
     cpu = None
     if not args.no_cpu_baseline and not args.preview:
         # the burst's CPU sample: its smallest frame size
         cw, ch = (1920, 1080) if args.burst else (w, h)
-        cpu = cpu_baseline(cw, ch, bayer, ltm=args.ltm, noise_level=args.noise_level)
+        cpu = cpu_baseline(cw, ch, bayer, ltm=args.ltm, noise_level=args.noise_level, algorithm=algorithm)
 
     if args.burst:
         metric = "Mpix/s RAW->RGB (demosaic+pyramid denoise), burst of 16 uneven frames"
@@ -456,7 +467,8 @@ def main():
                   else "Mpix/s RAW->RGB (demosaic+learned-strength pyramid denoise) at 12 MP" if args.learned
                   else "Mpix/s RAW->RGB (demosaic+pyramid denoise) at 12 MP")
         workload = (f"{w}x{h} RGGB uint16 12-bit, 6 noise levels per step, "
-                    + ("half-resolution preview (fastDemosaicImage)" if args.preview else "5-level bilateral pyramid")
+                    + ("half-resolution preview (fastDemosaicImage)" if args.preview
+                       else f"5-level {ALGORITHM_NAMES[algorithm]} pyramid")
                     + (", learned strength (gated)" if args.learned else "")
                     + (", local tone mapping" if args.ltm else "")
                     + (f", noiseLevel {args.noise_level} (raw pre-denoise)" if args.noise_level > 0.6 else ""))
