@@ -171,8 +171,7 @@ extern "C" clim_status clim_create(clim_ctx** out, int device) {
 }
 
 static void free_workspace(Workspace& ws) {
-    OwnedImg* all[] = {&ws.scaled, &ws.green, &ws.rgbA, &ws.rgbB, &ws.raw_mean, &ws.raw_var, &ws.g_meanI,
-                       &ws.g_s1, &ws.g_s2, &ws.g_ipr, &ws.g_ipg, &ws.g_ipb, &ws.g_tmp4, &ws.g_tmp2};
+    OwnedImg* all[] = {&ws.scaled, &ws.green, &ws.rgbA, &ws.rgbB, &ws.raw_mean, &ws.raw_var};
     for (OwnedImg* o : all) owned_free(*o);
     for (int i = 0; i < CLIM_MAX_LEVELS; i++) {
         owned_free(ws.pyr[i]);
@@ -584,9 +583,14 @@ extern "C" clim_status clim_guided_filter(clim_ctx* ctx, const clim_image* in, i
     CLIM_TRY(check_img(ctx, in, CLIM_FMT_RGBA32F, "in"));
     CLIM_TRY(check_img(ctx, out, CLIM_FMT_RGBA32F, "out"));
     CLIM_TRY(check_same(ctx, in, out, "GuidedFilter::filter"));
-    clim::GuidedScratch g{};
-    CLIM_TRY(guided_scratch(ctx, ctx->op_guided_imgs, in->width, in->height, &g));
-    clim::launch_guided_filter(view(in), view(out), filter_size, host_params(eps, 3), g, ctx->stream);
+    // the one-launch form reads neighbouring tiles' input: in place takes the passes
+    if (clim::guided_filter_fused(filter_size) && in->data != out->data) {
+        clim::launch_guided_filter_fused(view(in), view(out), filter_size, host_params(eps, 3), ctx->stream);
+    } else {
+        clim::GuidedScratch g{};
+        CLIM_TRY(guided_scratch(ctx, ctx->op_guided_imgs, in->width, in->height, &g));
+        clim::launch_guided_filter(view(in), view(out), filter_size, host_params(eps, 3), g, ctx->stream);
+    }
     CLIM_TRY_HIP(ctx, hipGetLastError());
     return CLIM_OK;
 }

@@ -37,7 +37,6 @@ struct Workspace {
     OwnedImg pyr[CLIM_MAX_LEVELS];    // pyr[i], i >= 1: downsampled level i
     OwnedImg den[CLIM_MAX_LEVELS];    // den[i], i >= 1: denoised level i (den[0] is rgbA)
     OwnedImg stats[CLIM_MAX_LEVELS];  // noiseStatistics of level i
-    OwnedImg g_meanI, g_s1, g_s2, g_ipr, g_ipg, g_ipb, g_tmp4, g_tmp2;  // GuidedPrecise scratch
 };
 
 constexpr int kFitJobs = 1 + CLIM_MAX_LEVELS;  // raw + levels

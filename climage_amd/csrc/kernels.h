@@ -253,6 +253,10 @@ struct GuidedScratch {
 };
 void launch_guided_filter(const Img& in, const Img& out, int fs, const ParamRef& eps, const GuidedScratch& g,
                           hipStream_t st);
+// the same filter in one launch with no scratch, for the filter sizes where
+// guided_filter_fused(fs) holds (3 and 5; the pyramid uses 5)
+bool guided_filter_fused(int fs);
+void launch_guided_filter_fused(const Img& in, const Img& out, int fs, const ParamRef& eps, hipStream_t st);
 
 // -------------------------------------------------------- synthetic scenes
 constexpr int kSceneBlobs = 200, kSceneRects = 50;

@@ -56,19 +56,12 @@ static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int
             CLIM_TRY(owned_alloc(ctx, ws.den[i], lw, lh, CLIM_FMT_RGBA32F));
         }
     }
-    if (algorithm == CLIM_DENOISE_GUIDED_PRECISE) {
-        OwnedImg* g[8] = {&ws.g_meanI, &ws.g_s1, &ws.g_ipr, &ws.g_ipg, &ws.g_ipb, &ws.g_tmp4, &ws.g_tmp2, &ws.g_s2};
-        for (int i = 0; i < 7; i++) CLIM_TRY(owned_alloc(ctx, *g[i], w, h, CLIM_FMT_RGBA32F));
-        CLIM_TRY(owned_alloc(ctx, *g[7], w, h, CLIM_FMT_RG32F));
-    }
     ws.w = w;
     ws.h = h;
     ws.levels = levels;
     ws.algorithm = algorithm;
     return CLIM_OK;
 }
-
-static Img sub_view(const Img& im, int w, int h) { return Img{im.p, w, h, im.pitch}; }
 
 // The pyramid of cpp/pyramidal_denoise.cpp:103-149 on device images.
 // level0: despeckled YCbCr; out: denoised level 0 (may not alias level0).
@@ -158,14 +151,8 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         if (algorithm != CLIM_DENOISE_BILATERAL) {
             ProfScope ps(ctx, ST_DENOISE, (long long)lvl[i].w * lvl[i].h);
             if (algorithm == CLIM_DENOISE_GUIDED_PRECISE) {
-                clim::GuidedScratch g{ws.g_meanI.img, ws.g_s1.img, ws.g_s2.img, ws.g_ipr.img,
-                                      ws.g_ipg.img,   ws.g_ipb.img, ws.g_tmp4.img, ws.g_tmp2.img};
-                clim::GuidedScratch gv{sub_view(g.mean_I, lvl[i].w, lvl[i].h), sub_view(g.s1, lvl[i].w, lvl[i].h),
-                                       sub_view(g.s2, lvl[i].w, lvl[i].h),     sub_view(g.ipr, lvl[i].w, lvl[i].h),
-                                       sub_view(g.ipg, lvl[i].w, lvl[i].h),    sub_view(g.ipb, lvl[i].w, lvl[i].h),
-                                       sub_view(g.tmp4, lvl[i].w, lvl[i].h),   sub_view(g.tmp2, lvl[i].w, lvl[i].h)};
-                // GuidedPreciseDenoiser passes var_b as eps (cpp/pyramidal_denoise.cpp:65)
-                clim::launch_guided_filter(lvl[i], den[i], 5, dev_params(ctx->fit[1 + i].cal + 3), gv, st);
+                // GuidedPreciseDenoiser: filter size 5, var_b as eps (cpp/pyramidal_denoise.cpp:65)
+                clim::launch_guided_filter_fused(lvl[i], den[i], 5, dev_params(ctx->fit[1 + i].cal + 3), st);
             } else if (bilateral) {
                 clim::launch_bilateral(lvl[i], den[i], cal, dp[i].chromaBoost, dp[i].gradientBoost, st, &sr, i);
             } else {
