@@ -24,6 +24,9 @@ Prints one JSON line:
 * learned: the same frames with the learned denoise strength (the committed
   weights, gated), its rate, the strength stage's cost, the gates and the
   PSNR gain against the clean truth on one frame;
+* burst: the uneven 16-frame burst recorded once as a HIP graph and
+  replayed: Mpix/s, the frames' completion times within a replay (p50/p99)
+  and the autotuner's choice for its cross-frame launches;
 * cpu_baseline: the FP32 CPU restatement built -O3 -march=native, all host
   threads, on a sample of the same frames.
 """
@@ -192,6 +195,56 @@ def copy_calibration(nbytes):
         hip.hipFree(dst)
 
 
+BURST_SIZES = [(4032, 3024), (4000, 3000), (3264, 2448), (1920, 1080)]
+
+
+def burst_leg(ctx, replays=20):
+    """The uneven burst (16 frames: 4 each of 4032x3024, 4000x3000, 3264x2448
+    and 1920x1080, RGGB and BGGR alternating, noise levels drawn from the
+    headline set) recorded once as a HIP graph (clim_burst_create: the
+    cross-frame persistent green and red/blue launches, autotuned) and
+    replayed: Mpix/s over the replays (host clock, inputs resident), and the
+    frames' completion times within a replay (HIP events in the graph)."""
+    draws = np.random.default_rng(3000).integers(0, len(NOISE_A), 16)
+    frames = [(*BURST_SIZES[k // 4], ca.RGGB if k % 2 == 0 else ca.BGGR, NOISE_A[draws[k]]) for k in range(16)]
+    raws, outs, params = [], [], []
+    for k, (fw, fh, fb, a) in enumerate(frames):
+        r = ctx.image(fw, fh, ca.FMT_R16U)
+        ctx.generate_bayer(r, fb, 1000 + k, 2000 + k, a, a / 200.0, 64.0, 4095.0)
+        raws.append(r)
+        outs.append(ctx.image(fw, fh, ca.FMT_RGBA8))
+        params.append(ca.default_params(fb, 64.0, 4095.0, levels=5, algorithm=ca.BILATERAL))
+    px = sum(fw * fh for fw, fh, _, _ in frames)
+    ctx.sync()
+    t0 = time.perf_counter()
+    burst = ca.Burst(ctx, raws, outs, params, frame_times=True)
+    create_s = time.perf_counter() - t0
+    try:
+        lat = []
+        for _ in range(3):  # untimed replays; their frame completion times
+            burst.launch()
+            lat.extend(burst.frame_times().tolist())
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(replays):
+            burst.launch()
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        tuning = burst.tuning()
+    finally:
+        burst.close()
+        for im in raws + outs:
+            im.free()
+    return {"value": round(replays * px / dt / 1e6, 1), "unit": "Mpix/s", "frames": len(frames), "pixels": px,
+            "replays": replays, "ms_per_burst": round(dt / replays * 1e3, 3),
+            "frame_completion_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                    "p99": round(float(np.percentile(lat, 99)), 3), "samples": len(lat),
+                                    "note": "each frame's last kernel, from the replay's start (events in the graph)"},
+            "create_s": round(create_s, 3), "autotune": tuning,
+            "form": "one hipGraphLaunch per burst: per-frame raw fits and pyramids on two lanes, the 16 frames' "
+                    "green and red/blue + despeckle stages as two cross-frame persistent launches"}
+
+
 def psnr(a, b):
     d = a[..., :3].astype(np.float64) - b[..., :3]
     return 10.0 * np.log10(1.0 / max(float(np.mean(d * d)), 1e-30))
@@ -207,6 +260,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true", help="skip the FP64 / FP32 CPU comparison of one frame")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-strength leg")
+    ap.add_argument("--no-burst", action="store_true", help="skip the recorded-burst leg")
     ap.add_argument("--learned", action="store_true",
                     help="the learned denoise strength (committed weights, gated) is the headline path")
     ap.add_argument("--ltm", action="store_true",
@@ -444,7 +498,12 @@ def main():
                 im.free()
         ctx.strength_set_mode(ca.STRENGTH_OFF)
 
-    # 4. the first frame against the CPU restatements (outside any timing)
+    # 4. the uneven burst recorded as one graph (its own frames and timing)
+    burst = None
+    if not (args.no_burst or args.preview or args.burst or args.learned):
+        burst = burst_leg(ctx)
+
+    # 5. the first frame against the CPU restatements (outside any timing)
     parity = None
     if not (args.no_parity or args.preview or args.burst or args.learned or args.ltm or args.noise_level > 0.6):
         of = ctx.image(w, h, ca.FMT_RGBA32F)
@@ -491,6 +550,7 @@ def main():
         "path_roofline": path,
         "fp64": parity,
         "learned": learned,
+        "burst": burst,
         "cpu_baseline": cpu,
         "stages": stages,
     }

@@ -91,6 +91,7 @@ def lib():
         L.clim_last_error.restype = C.c_char_p
         L.clim_status_string.restype = C.c_char_p
         L.clim_stream.restype = C.c_void_p
+        L.clim_burst_destroy.restype = None
         _lib = L
     return _lib
 
@@ -116,7 +117,8 @@ def exported_symbols():
                         "clim_generate_bayer", "clim_compute_raw_noise_statistics_noalloc",
                         "clim_strength_set_weights", "clim_strength_get_weights", "clim_strength_set_mode",
                         "clim_strength_gate_read", "clim_strength_map_read", "clim_strength_loss_grad",
-                        "clim_strength_train")]
+                        "clim_strength_train", "clim_burst_create", "clim_burst_launch", "clim_burst_frame_times",
+                        "clim_burst_tuning_read", "clim_burst_destroy")]
 
 
 # learned denoise strength (include/climage.h, clim_strength_*)
@@ -248,6 +250,66 @@ class DeviceImage:
     def free(self):
         if self.im.data:
             lib().clim_image_free(self.ctx.h, C.byref(self.im))
+
+
+BURST_FRAME_TIMES = 1  # CLIM_BURST_FRAME_TIMES
+
+
+class BurstTuning(C.Structure):
+    _fields_ = [("waves", C.c_int32 * 2), ("blocks_per_cu", C.c_int32 * 2), ("grab", C.c_int32 * 2),
+                ("us", C.c_float * 2), ("us_first", C.c_float * 2), ("candidates", C.c_int32 * 2),
+                ("tiles", C.c_int32 * 2), ("cross_frames", C.c_int32), ("cached", C.c_int32)]
+
+
+class Burst:
+    """A burst recorded once and replayed (clim_burst_*): frame i from raws[i]
+    into outs8[i] (and outs_f32[i]) with params[i].  The images must outlive
+    the burst."""
+
+    def __init__(self, ctx, raws, outs8, params, outs_f32=None, frame_times=False):
+        n = len(raws)
+        if len(outs8) != n or len(params) != n or (outs_f32 is not None and len(outs_f32) != n):
+            raise ValueError("Burst: lists of different lengths")
+        self.ctx, self.n = ctx, n
+        self._keep = (list(raws), list(outs8), list(outs_f32 or []))
+        R = (C.POINTER(Image) * n)(*[C.pointer(r.im) for r in raws])
+        O8 = (C.POINTER(Image) * n)(*[C.pointer(o.im) for o in outs8])
+        OF = None
+        if outs_f32 is not None:
+            OF = (C.POINTER(Image) * n)(*[C.pointer(o.im) if o is not None else None for o in outs_f32])
+        P = (DemosaicParameters * n)(*params)
+        self.h = C.c_void_p()
+        ctx._check(lib().clim_burst_create(ctx.h, n, R, O8, OF, P, C.c_uint32(BURST_FRAME_TIMES if frame_times else 0),
+                                           C.byref(self.h)), "burst_create")
+
+    def launch(self):
+        self.ctx._check(lib().clim_burst_launch(self.h), "burst_launch")
+
+    def frame_times(self):
+        """ms from the last launch's start to each frame's end (blocks)."""
+        ms = (C.c_float * self.n)()
+        self.ctx._check(lib().clim_burst_frame_times(self.h, ms), "burst_frame_times")
+        return np.array(ms[:], np.float32)
+
+    def tuning(self):
+        t = BurstTuning()
+        self.ctx._check(lib().clim_burst_tuning_read(self.h, C.byref(t)), "burst_tuning_read")
+        names = ("green", "red_blue_despeckle")
+        return {names[k]: {"waves_per_simd": t.waves[k], "blocks_per_cu": t.blocks_per_cu[k], "grab": t.grab[k],
+                           "us": round(t.us[k], 2), "us_first_candidate": round(t.us_first[k], 2),
+                           "candidates": t.candidates[k], "tiles": t.tiles[k]} for k in range(2)} | \
+            {"cross_frames": t.cross_frames, "cached": bool(t.cached)}
+
+    def close(self):
+        if self.h:
+            lib().clim_burst_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Context:

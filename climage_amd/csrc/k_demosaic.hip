@@ -6,6 +6,8 @@
 //
 // Reference: cl/demosaic.cl (cl/ = CLImage/app/src/main/cpp/OpenCL/); line
 // numbers are cited per kernel.  All loads clamp to the image.
+#include <type_traits>
+
 #include "fit_acc.h"
 #include "kernels.h"
 #include "tiles.h"
@@ -194,15 +196,23 @@ constexpr int G_RS = 48;
 static_assert(G_RS >= GR_X && G_RS >= GS_X, "tile rows fit");
 }
 
+struct GreenLds {
+    float r[GR_Y][G_RS];
+    float sgx[GS_Y][G_RS];
+    float sgy[GS_Y][G_RS];
+    float sgl[GS_Y][G_RS];
+};
+
+// one GT_X x GT_Y tile at (x0, y0) by the block's 256 threads (tid); returns
+// with the LDS still being read by other threads (the caller's next use of
+// sm needs a barrier first)
 template <bool U16>
-__global__ void __launch_bounds__(256) k_green(MosaicSrc<U16> raw, Img green, int gxo, int gyo, ParamRef lv) {
-    __shared__ float r[GR_Y][G_RS];
-    __shared__ float sgx[GS_Y][G_RS];
-    __shared__ float sgy[GS_Y][G_RS];
-    __shared__ float sgl[GS_Y][G_RS];
-    const TileId tl = xcd_tile();
-    const int x0 = tl.x * GT_X, y0 = tl.y * GT_Y;
-    const int tid = threadIdx.y * 32 + threadIdx.x;
+__device__ __forceinline__ void green_tile(const MosaicSrc<U16>& raw, const Img& green, int gxo, int gyo,
+                                           const ParamRef& lv, int x0, int y0, int tid, GreenLds& sm) {
+    auto& r = sm.r;
+    auto& sgx = sm.sgx;
+    auto& sgy = sm.sgy;
+    auto& sgl = sm.sgl;
     stage<GR_X * GR_Y, 256>(
         tid, [&](int i) { return raw(x0 - 3 + i % GR_X, y0 - 3 + i / GR_X); },
         [&](int i, float v) { r[i / GR_X][i % GR_X] = v; });
@@ -295,6 +305,13 @@ __global__ void __launch_bounds__(256) k_green(MosaicSrc<U16> raw, Img green, in
         if (xe < green.w) row[xe] = ev;
         if (xe + 1 < green.w) row[xe + 1] = od;
     }
+}
+
+template <bool U16>
+__global__ void __launch_bounds__(256) k_green(MosaicSrc<U16> raw, Img green, int gxo, int gyo, ParamRef lv) {
+    __shared__ GreenLds sm;
+    const TileId tl = xcd_tile();
+    green_tile<U16>(raw, green, gxo, gyo, lv, tl.x * GT_X, tl.y * GT_Y, threadIdx.y * 32 + threadIdx.x, sm);
 }
 
 void launch_green(const MosaicArg& raw, const Img& green, int bayer, const ParamRef& lv, hipStream_t st) {
@@ -612,15 +629,14 @@ union RbDespeckleLds {
     float4 t[RD_RY][RD_RX];
 };
 
+// one RD_X x RD_Y output tile at (x0, y0) by the block's 256 threads; as
+// green_tile, returns with the LDS possibly still being read
 template <bool U16>
-__global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img green, Img out, Bayer b,
-                                                      int red_is_c1_site, ParamRef cv, float clip, Mat3 m,
-                                                      float var_a, float var_b) {
-    __shared__ RbDespeckleLds sm;
-    const TileId tl = xcd_tile();
-    const int x0 = tl.x * RD_X, y0 = tl.y * RD_Y;
+__device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, const Img& green, const Img& out,
+                                                  const Bayer& b, int red_is_c1_site, const ParamRef& cv, float clip,
+                                                  const Mat3& m, float var_a, float var_b, int x0, int y0, int tid,
+                                                  RbDespeckleLds& sm) {
     const int rx0 = x0 - 2, ry0 = y0 - 2;
-    const int tid = threadIdx.x;
     const float cvar = chroma_variance(cv);
     // the region's mosaic (scaled) and green, texel (rx0 - 1 + i, ry0 - 1 + j)
     stage<RD_WX * RD_WY, 256>(
@@ -690,6 +706,103 @@ __global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img gr
         };
         wrow_f4(out, y)[x] = nonan ? despeckle_px<true>(fetch, var_a, var_b) : despeckle_px<false>(fetch, var_a, var_b);
     }
+}
+
+template <bool U16>
+__global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img green, Img out, Bayer b,
+                                                      int red_is_c1_site, ParamRef cv, float clip, Mat3 m,
+                                                      float var_a, float var_b) {
+    __shared__ RbDespeckleLds sm;
+    const TileId tl = xcd_tile();
+    rb_despeckle_tile<U16>(raw, green, out, b, red_is_c1_site, cv, clip, m, var_a, var_b, tl.x * RD_X, tl.y * RD_Y,
+                           threadIdx.x, sm);
+}
+
+// ----------------------------------------------------- burst (cross-frame)
+// The green and red/blue + despeckle stages of every frame of a burst in one
+// launch each.  The frames' tiles form one list (frame-major, row-major
+// within a frame) cut into 8 contiguous segments, one per XCD; a persistent
+// block on XCD x (launch index mod 8) takes `grab` consecutive tiles of
+// segment x at a time from that segment's counter until it runs dry, so
+// neighbouring tiles (whose halos overlap) share an L2 while the blocks of
+// an XCD balance among themselves.  No block waits for another: a block that
+// starts after its segment is done leaves at once.  The last block out
+// clears the counters, so the next launch (or graph replay) starts from zero.
+__device__ __forceinline__ int burst_frame_of(const BurstFrame* frames, int kind, int t) {
+    int f = 0;
+    while (t >= frames[f].tile_end[kind]) f++;
+    return f;
+}
+
+__device__ __forceinline__ void burst_leave(BurstQueue* q) {
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&q->done, 1u) == gridDim.x - 1) {
+            __threadfence();
+            for (int i = 0; i < 8; i++) atomicExch(&q->next[i], 0u);
+            atomicExch(&q->done, 0u);
+        }
+    }
+}
+
+template <int KIND, int WAVES>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_demosaic_burst(const BurstFrame* __restrict__ frames, int total, BurstQueue* q, int grab) {
+    __shared__ typename std::conditional<KIND == 0, GreenLds, RbDespeckleLds>::type sm;
+    __shared__ int s_t;
+    const int xcd = blockIdx.x & 7;
+    const int lo = (int)((long long)total * xcd / 8), hi = (int)((long long)total * (xcd + 1) / 8);
+    for (;;) {
+        if (threadIdx.x == 0) s_t = lo + (int)atomicAdd(&q->next[xcd], (unsigned int)grab);
+        __syncthreads();
+        const int t0 = __builtin_amdgcn_readfirstlane(s_t);  // uniform: the frame table is read by scalar loads
+        if (t0 >= hi) break;
+        const int t1 = min(t0 + grab, hi);
+        for (int t = t0; t < t1; t++) {
+            const int f = burst_frame_of(frames, KIND, t);
+            const BurstFrame& fr = frames[f];
+            const int lt = t - (f ? frames[f - 1].tile_end[KIND] : 0);
+            const int ty = lt / fr.tiles_x[KIND], tx = lt - ty * fr.tiles_x[KIND];
+            if (KIND == 0) {
+                green_tile<true>(fr.raw, fr.green, fr.gxo, fr.gyo, fr.lv, tx * GT_X, ty * GT_Y, threadIdx.x,
+                                 reinterpret_cast<GreenLds&>(sm));
+            } else {
+                Mat3 m;
+#pragma unroll
+                for (int i = 0; i < 9; i++) m.m[i] = fr.c2y[i];
+                rb_despeckle_tile<true>(fr.raw, fr.green, fr.ycc, fr.b, fr.red_is_c1, fr.cv, fr.clip, m, fr.var_a,
+                                        fr.var_b, tx * RD_X, ty * RD_Y, threadIdx.x,
+                                        reinterpret_cast<RbDespeckleLds&>(sm));
+            }
+            __syncthreads();  // the tile's LDS is free
+        }
+    }
+    burst_leave(q);
+}
+
+void burst_tiles(int w, int h, int kind, int* tiles_x, int* tiles) {
+    const int tx = kind == 0 ? GT_X : RD_X, ty = kind == 0 ? GT_Y : RD_Y;
+    *tiles_x = (w + tx - 1) / tx;
+    *tiles = *tiles_x * ((h + ty - 1) / ty);
+}
+
+template <int KIND>
+static void launch_burst_kind(int waves, const BurstFrame* frames, int total, BurstQueue* q, int blocks, int grab,
+                              hipStream_t st) {
+    if (waves >= 8)
+        hipLaunchKernelGGL((k_demosaic_burst<KIND, 8>), dim3(blocks), dim3(256), 0, st, frames, total, q, grab);
+    else if (waves >= 6)
+        hipLaunchKernelGGL((k_demosaic_burst<KIND, 6>), dim3(blocks), dim3(256), 0, st, frames, total, q, grab);
+    else
+        hipLaunchKernelGGL((k_demosaic_burst<KIND, 4>), dim3(blocks), dim3(256), 0, st, frames, total, q, grab);
+}
+
+void launch_demosaic_burst(int kind, int waves, const BurstFrame* frames, int total_tiles, BurstQueue* q, int blocks,
+                           int grab, hipStream_t st) {
+    if (kind == 0)
+        launch_burst_kind<0>(waves, frames, total_tiles, q, blocks, grab, st);
+    else
+        hipLaunchKernelGGL((k_demosaic_burst<1, 4>), dim3(blocks), dim3(256), 0, st, frames, total_tiles, q, grab);
 }
 
 void launch_rb_despeckle(const MosaicArg& raw, const Img& green, const Img& out, int bayer, const Bayer& b,

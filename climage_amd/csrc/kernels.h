@@ -109,6 +109,37 @@ void launch_rb_despeckle(const MosaicArg& raw, const Img& green, const Img& out,
                          const ParamRef& cv, float clip, const float c2y[9], float var_a, float var_b,
                          hipStream_t st);
 
+// ----------------------------------------------------- burst (cross-frame)
+// One frame of a burst in the cross-frame green and red/blue + despeckle
+// launches (a device table, frame-major).
+struct BurstFrame {
+    MosaicSrc<true> raw;   // the uint16 mosaic, scaled on the fly
+    Img green, ycc;        // the frame's green plane and despeckled YCbCr image
+    int gxo, gyo;          // raw_green site
+    ParamRef lv, cv;       // luma / chroma variance for interpolateGreen / RedBlue
+    Bayer b;
+    int red_is_c1;
+    float clip;
+    float c2y[9];          // camera -> YCbCr
+    float var_a, var_b;    // despeckleImage (pyramidNlf[0])
+    int tiles_x[2];        // tiles per row: [0] green, [1] red/blue + despeckle
+    int tile_end[2];       // running totals of the frames' tile counts
+};
+// tile counters of one cross-frame kernel (zero between launches)
+struct BurstQueue {
+    unsigned int next[8];
+    unsigned int done;
+    unsigned int pad[7];
+};
+// tiles of a w x h frame for kind 0 (green) / 1 (red/blue + despeckle)
+void burst_tiles(int w, int h, int kind, int* tiles_x, int* tiles);
+// every frame's tiles of one stage, `blocks` persistent blocks (a multiple of
+// 8), `grab` tiles per take; waves (green: 4, 6 or 8): the register budget
+// the kernel is built for, as waves per SIMD (red/blue + despeckle has one
+// build: its registers allow 4)
+void launch_demosaic_burst(int kind, int waves, const BurstFrame* frames, int total_tiles, BurstQueue* q, int blocks,
+                           int grab, hipStream_t st);
+
 // ------------------------------------------------------------- pyramid stages
 void launch_downsample(const Img& in, const Img& out, hipStream_t st);
 void launch_noise_stats(const Img& in, const Img& out, hipStream_t st);

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "pipeline.h"
 
 using clim::Img;
 using clim::ParamRef;
@@ -258,8 +259,8 @@ extern "C" clim_status clim_pyramidal_denoise_into(clim_ctx* ctx, const clim_ima
     return CLIM_OK;
 }
 
-static clim_status raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8, clim_image* out_f32,
-                              const clim_demosaic_params* p, clim_noise_model* nlf_out, int rotate_180) {
+clim_status frame_prepare(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8, clim_image* out_f32,
+                          const clim_demosaic_params* p, FrameCall* fc) {
     if (!ctx || !p) return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: null argument");
     CLIM_TRY(check_params(ctx, p));
     CLIM_TRY(check_image(ctx, raw, CLIM_FMT_R16U, "raw"));
@@ -270,108 +271,144 @@ static clim_status raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* 
         return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: image too small for the pyramid depth");
     if (out_rgba8->width != w || out_rgba8->height != h)
         return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: output size differs from the mosaic");
-    Img outf{nullptr, 0, 0, 0};
+    FrameCall f{};
+    f.p = p;
+    f.w = w;
+    f.h = h;
+    f.out8 = view(out_rgba8);
+    f.outf = Img{nullptr, 0, 0, 0};
     if (out_f32) {
         CLIM_TRY(check_image(ctx, out_f32, CLIM_FMT_RGBA32F, "out_f32"));
         if (out_f32->width != w || out_f32->height != h)
             return ctx_fail(ctx, CLIM_EINVAL, "raw_to_rgb: float output size differs from the mosaic");
-        outf = view(out_f32);
+        f.outf = view(out_f32);
     }
-    float c2y[9], y2c[9], cam[9];
-    std::memcpy(cam, p->rgb_cam, sizeof(cam));
-    clim_cam_ycbcr(cam, c2y);
-    if (inverse3x3_scaled(c2y, (double)p->exposure_multiplier, y2c))
+    std::memcpy(f.cam, p->rgb_cam, sizeof(f.cam));
+    clim_cam_ycbcr(f.cam, f.c2y);
+    if (inverse3x3_scaled(f.c2y, (double)p->exposure_multiplier, f.y2c))
         return ctx_fail(ctx, CLIM_ENUMERIC, "raw_to_rgb: camera->YCbCr matrix is singular");
     // cpp/raw_converter.cpp:82: the raw-domain pre-denoise of high-noise frames
-    const bool high_noise = p->noiseLevel > 0.6f;
-    CLIM_TRY(ensure_workspace(ctx, w, h, p->levels, p->denoiseAlgorithm, high_noise));
-    Workspace& ws = ctx->ws;
-    hipStream_t st = ctx->stream;
-    const int bayer = p->bayerPattern;
-    const clim::Bayer b = make_bayer(bayer);
-
+    f.high_noise = p->noiseLevel > 0.6f;
     // scaleRawData (black level normalised as in cpp/raw_converter.cpp:105-106)
     // is applied on the fly by the kernels that read the mosaic
-    const long long npx = (long long)w * h;
-    const float black = p->black_level / 65535.0f;
-    clim::MosaicArg mosaic{};
-    mosaic.im = view(raw);
-    mosaic.b = b;
-    mosaic.smul = make_float4(p->scale_mul[0], p->scale_mul[1], p->scale_mul[2], p->scale_mul[3]);
-    mosaic.black = black;
-    mosaic.u16 = true;
-    // computeRawNoiseStatistics (cpp/raw_converter.cpp:108-109): fit slot 0,
-    // rawNlf = B; the input rawNlf is the fallback when no quad is in range
+    f.mosaic = clim::MosaicArg{};
+    f.mosaic.im = view(raw);
+    f.mosaic.b = make_bayer(p->bayerPattern);
+    f.mosaic.smul = make_float4(p->scale_mul[0], p->scale_mul[1], p->scale_mul[2], p->scale_mul[3]);
+    f.mosaic.black = p->black_level / 65535.0f;
+    f.mosaic.u16 = true;
+    f.raw = f.mosaic;
+    CLIM_TRY(frame_workspace(ctx, f));
+    *fc = f;
+    return CLIM_OK;
+}
+
+clim_status frame_workspace(clim_ctx* ctx, const FrameCall& f) {
+    return ensure_workspace(ctx, f.w, f.h, f.p->levels, f.p->denoiseAlgorithm, f.high_noise);
+}
+
+clim_status frame_raw_fit(clim_ctx* ctx, FrameCall& f, clim::FitState* st_out) {
+    Workspace& ws = ctx->ws;
+    const clim_demosaic_params* p = f.p;
+    const long long npx = (long long)f.w * f.h;
+    f.mosaic = f.raw;
+    // computeRawNoiseStatistics (cpp/raw_converter.cpp:108-109): rawNlf = B;
+    // the input rawNlf is the fallback when no quad is in range
     clim::FitJobs rjobs{};
     {
         const float in8[8] = {0, 0, 0, 0, p->noiseModel.rawNlf[0], p->noiseModel.rawNlf[1], p->noiseModel.rawNlf[2],
                               p->noiseModel.rawNlf[3]};
         rjobs.njobs = 1;
-        rjobs.j[0] = make_fit_job(ctx, 0, view(raw), ws.raw_mean.img, ws.raw_var.img, in8, nullptr);
-        rjobs.j[0].b = b;
+        rjobs.j[0] = make_fit_job(ctx, 0, f.mosaic.im, ws.raw_mean.img, ws.raw_var.img, in8, nullptr);
+        rjobs.j[0].b = f.mosaic.b;
+        if (st_out) rjobs.j[0].st = st_out;
     }
+    f.rawfit = rjobs.j[0].st->out;  // device: A[4], B[4]
     {
         ProfScope ps(ctx, ST_RAW_STATS, npx / 4);
-        CLIM_TRY(run_stats_fit(ctx, rjobs, true, p->scale_mul, black));
+        CLIM_TRY(run_stats_fit(ctx, rjobs, true, p->scale_mul, f.mosaic.black));
     }
     {
         ProfScope ps(ctx, ST_RAW_FIT, npx / 4);
         CLIM_TRY(run_fit_rest(ctx, rjobs, true));
     }
-    const float* rawfit = ctx->fit[0].out;  // device: A[4], B[4]
-    if (high_noise) {
+    if (f.high_noise) {
         // bayerToRawRGBA -> despeckleRawRGBAImage -> denoiseRawRGBAImage(rawNlf)
         // -> rawRGBAToBayer (cpp/raw_converter.cpp:111-123) in one kernel; the
         // demosaic then reads the denoised FP32 mosaic
         ProfScope ps(ctx, ST_RAW_DENOISE, npx / 4);
-        clim::launch_raw_predenoise(mosaic, dev_params(rawfit + 4), ws.scaled.img, st);
-        mosaic.im = ws.scaled.img;
-        mosaic.u16 = false;
+        clim::launch_raw_predenoise(f.mosaic, dev_params(f.rawfit + 4), ws.scaled.img, ctx->stream);
+        f.mosaic.im = ws.scaled.img;
+        f.mosaic.u16 = false;
     }
+    CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
+
+clim_status frame_demosaic(clim_ctx* ctx, const FrameCall& f, const Img& green, const Img& ycc, int rotate_180) {
+    hipStream_t st = ctx->stream;
+    const int bayer = f.p->bayerPattern;
+    const long long npx = (long long)f.w * f.h;
     // interpolateGreen(rawNlf[1]) and interpolateRedBlue((rawNlf[0]+rawNlf[2])/2)
     // + blendHighlightsImage(clip = 1) + transformImage(cam_ycbcr(rgb_cam))
     {
         ProfScope ps(ctx, ST_GREEN, npx);
-        clim::launch_green(mosaic, ws.green.img, bayer, dev_params(rawfit + 5), st);
+        clim::launch_green(f.mosaic, green, bayer, dev_params(f.rawfit + 5), st);
     }
     // despeckleImage with the caller's pyramidNlf[0] (cpp/raw_converter.cpp:144-145),
     // in the same kernel as interpolateRedBlue
-    const float* np0 = p->noiseModel.pyramidNlf[0];
+    const float* np0 = f.p->noiseModel.pyramidNlf[0];
     if (rotate_180) {
         // cl/demosaic.cl:313-315: source texel (x, y) lands on (w - x, h - y)
         // when that is inside the image, so row 0 and column 0 are never
         // written; here they are zero (the reference leaves whatever its
         // image held).  The despeckle then runs on the rotated image.
         ProfScope ps(ctx, ST_RED_BLUE, npx);
-        clim::launch_zero_row0_col0(ws.rgbA.img, st);
-        clim::launch_red_blue(mosaic, ws.green.img, ws.rgbA.img, bayer, b, dev_params(rawfit), 1, true, 1.0f, c2y,
+        const Img rot = ctx->ws.rgbA.img;
+        clim::launch_zero_row0_col0(rot, st);
+        clim::launch_red_blue(f.mosaic, green, rot, bayer, f.mosaic.b, dev_params(f.rawfit), 1, true, 1.0f, f.c2y,
                               st);
-        clim::launch_despeckle(ws.rgbA.img, ws.rgbB.img, np0[0], np0[3], st);
+        clim::launch_despeckle(rot, ycc, np0[0], np0[3], st);
     } else {
         ProfScope ps(ctx, ST_RED_BLUE, npx);
-        clim::launch_rb_despeckle(mosaic, ws.green.img, ws.rgbB.img, bayer, b, dev_params(rawfit), 1.0f, c2y, np0[0],
+        clim::launch_rb_despeckle(f.mosaic, green, ycc, bayer, f.mosaic.b, dev_params(f.rawfit), 1.0f, f.c2y, np0[0],
                                   np0[3], st);
     }
     CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
 
+clim_status frame_pyramid(clim_ctx* ctx, const FrameCall& f, const Img& ycc) {
+    Workspace& ws = ctx->ws;
+    const clim_demosaic_params* p = f.p;
     // transformImage(inverse(cam_to_ycbcr) * exposure) + convertTosRGB, in the
     // kernel that reassembles pyramid level 0, with localToneMappingMask of the
     // denoised image (guide: denoisedImagePyramid[3], cpp/raw_converter.cpp:164-167)
     const clim_rgb_conversion_params& rc = p->rgbConversionParameters;
-    clim::ToneLaunch tone{view(out_rgba8), outf,          y2c,           cam,
-                          rc.contrast,      rc.saturation, rc.toneCurveSlope, rc.exposureBias,
-                          rc.blacks,        Img{nullptr, 0, 0, 0}, {0, 0, 0, 0}};
+    clim::ToneLaunch tone{f.out8,          f.outf,        f.y2c,              f.cam,
+                          rc.contrast,     rc.saturation, rc.toneCurveSlope, rc.exposureBias,
+                          rc.blacks,       Img{nullptr, 0, 0, 0}, {0, 0, 0, 0}};
     if (rc.localToneMapping) {
         tone.ltm_guide = ws.den[3].img;
         const clim_ltm_params& lp = p->ltmParameters;
         const float l4[4] = {lp.guidedFilterEps, lp.shadows, lp.highlights, lp.detail};
         std::memcpy(tone.ltm, l4, sizeof(l4));
     }
-    CLIM_TRY(pyramid(ctx, ws.rgbB.img, ws.rgbA.img, p->levels, p->denoiseAlgorithm, p->denoiseParameters,
+    CLIM_TRY(pyramid(ctx, ycc, ws.rgbA.img, p->levels, p->denoiseAlgorithm, p->denoiseParameters,
                      p->noiseModel.pyramidNlf, &tone));
     CLIM_TRY_HIP(ctx, hipGetLastError());
+    return CLIM_OK;
+}
 
+static clim_status raw_to_rgb(clim_ctx* ctx, const clim_image* raw, clim_image* out_rgba8, clim_image* out_f32,
+                              const clim_demosaic_params* p, clim_noise_model* nlf_out, int rotate_180) {
+    FrameCall f;
+    CLIM_TRY(frame_prepare(ctx, raw, out_rgba8, out_f32, p, &f));
+    CLIM_TRY(frame_raw_fit(ctx, f, nullptr));
+    CLIM_TRY(frame_demosaic(ctx, f, ctx->ws.green.img, ctx->ws.rgbB.img, rotate_180));
+    CLIM_TRY(frame_pyramid(ctx, f, ctx->ws.rgbB.img));
     if (nlf_out) {
+        hipStream_t st = ctx->stream;
         CLIM_TRY_HIP(ctx, hipMemcpyAsync(ctx->fit_host, ctx->fit, sizeof(clim::FitState) * (1 + p->levels),
                                          hipMemcpyDeviceToHost, st));
         CLIM_TRY_HIP(ctx, hipStreamSynchronize(st));

@@ -378,6 +378,58 @@ clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const clim_image* co
                                   clim_image* const* out_rgba8, clim_image* const* out_f32,
                                   const clim_demosaic_params* params, clim_noise_model* nlf_out);
 
+/* ------------------------------------------------- bursts (recorded once)
+ * n frames, each as clim_raw_to_rgb(raws[i], out_rgba8[i], out_f32 ? out_f32[i]
+ * : null, &params[i]) - RawConverter::demosaicImage once per frame of a burst
+ * (cpp/raw_converter.cpp:73-186), frames of any sizes and CFA patterns - set
+ * up once and then run any number of times with one call each.  No reference
+ * counterpart: for callers that convert bursts of the same geometry again
+ * and again (new mosaics in the same input images).
+ *
+ * clim_burst_create validates the frames, allocates the burst's own work
+ * buffers (lanes of its own, each frame's green plane and YCbCr image),
+ * runs the burst once, times the candidate shapes of its two cross-frame
+ * persistent launches (the green and the red/blue + despeckle stages of
+ * every frame, one launch each) on these frames and keeps the fastest (the
+ * choice is remembered in the process for bursts of the same frame sizes),
+ * and records the whole burst into one HIP graph.  Work queued on the
+ * context before the call is complete first.  The caller's images must
+ * outlive the burst; the context's learned-strength weights and mode are
+ * copied in at creation.
+ *
+ * clim_burst_launch replays the graph on the context's stream (one graph
+ * launch, no allocation, no host work per frame): it reads the current
+ * contents of raws and writes the outputs, equal bit for bit to the
+ * single-frame calls'.  Launches of one burst must not overlap (they share
+ * its work buffers); the context's stream orders them.
+ *
+ * With CLIM_BURST_FRAME_TIMES the graph also records, after a launch,
+ * when each frame's last kernel finished: clim_burst_frame_times waits for
+ * the context's stream and writes ms[i], frame i's completion time
+ * measured from the launch's start. */
+typedef struct clim_burst clim_burst;
+#define CLIM_BURST_FRAME_TIMES 1u
+
+typedef struct {
+    int32_t waves[2];          /* register budget of the chosen build, as waves per SIMD: [0] green, [1] red/blue */
+    int32_t blocks_per_cu[2];  /* persistent blocks per CU */
+    int32_t grab[2];           /* tiles a block takes from its queue at a time */
+    float us[2];               /* the chosen shape's time per launch (us), at creation */
+    float us_first[2];         /* the first candidate's (one block per CU, one tile per take) */
+    int32_t candidates[2];     /* shapes timed */
+    int32_t tiles[2];          /* tiles per launch (all frames) */
+    int32_t cross_frames;      /* frames in the cross-frame launches (the high-noise ones demosaic on their own) */
+    int32_t cached;            /* 1: the shapes came from an earlier burst of the same frame sizes */
+} clim_burst_tuning;
+
+clim_status clim_burst_create(clim_ctx* ctx, int32_t n, const clim_image* const* raws,
+                              clim_image* const* out_rgba8, clim_image* const* out_f32,
+                              const clim_demosaic_params* params, uint32_t flags, clim_burst** burst);
+clim_status clim_burst_launch(clim_burst* burst);
+clim_status clim_burst_frame_times(clim_burst* burst, float* ms);
+clim_status clim_burst_tuning_read(const clim_burst* burst, clim_burst_tuning* tuning);
+void clim_burst_destroy(clim_burst* burst);
+
 /* ------------------------------------------------- synthetic scenes
  * Seeded synthetic RAW frames (smooth background, 200 Gaussian blobs, 50
  * hard-edged rectangles; Poisson-Gaussian noise y = a*Poisson(x/a) + N(0,b);
