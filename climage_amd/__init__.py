@@ -18,14 +18,14 @@ LIB_PATH = os.path.join(HERE, "lib", "libclimage_hip.so")
 # clim_status
 OK, EINVAL, ENOMEM, EHIP, ENUMERIC, ENODEV = range(6)
 # clim_format
-FMT_R16U, FMT_R32F, FMT_RGBA32F, FMT_RGBA8, FMT_RG32F = 1, 2, 3, 4, 5
+FMT_R16U, FMT_R32F, FMT_RGBA32F, FMT_RGBA8, FMT_RG32F, FMT_RGBA16F = 1, 2, 3, 4, 5, 6
 # BayerPattern (cpp/demosaic.hpp:23-28)
 GRBG, GBRG, RGGB, BGGR = 0, 1, 2, 3
 # PyramidalDenoise::DenoiseAlgorithm (cpp/pyramidal_denoise.hpp:38)
 BILATERAL, GUIDED_FAST, GUIDED_PRECISE = 0, 1, 2
 
 _NP = {FMT_R16U: (np.uint16, 1), FMT_R32F: (np.float32, 1), FMT_RGBA32F: (np.float32, 4),
-       FMT_RGBA8: (np.uint8, 4), FMT_RG32F: (np.float32, 2)}
+       FMT_RGBA8: (np.uint8, 4), FMT_RG32F: (np.float32, 2), FMT_RGBA16F: (np.float16, 4)}
 
 
 class ClimError(RuntimeError):
@@ -118,7 +118,7 @@ def exported_symbols():
                         "clim_strength_set_weights", "clim_strength_get_weights", "clim_strength_set_mode",
                         "clim_strength_gate_read", "clim_strength_map_read", "clim_strength_loss_grad",
                         "clim_strength_train", "clim_burst_create", "clim_burst_launch", "clim_burst_frame_times",
-                        "clim_burst_tuning_read", "clim_burst_destroy")]
+                        "clim_burst_tuning_read", "clim_burst_destroy", "clim_set_precision")]
 
 
 # learned denoise strength (include/climage.h, clim_strength_*)
@@ -253,6 +253,7 @@ class DeviceImage:
 
 
 BURST_FRAME_TIMES = 1  # CLIM_BURST_FRAME_TIMES
+PRECISION_FP32, PRECISION_MIXED = 0, 1  # clim_precision
 
 
 class BurstTuning(C.Structure):
@@ -347,6 +348,11 @@ class Context:
 
     def sync(self):
         self._check(lib().clim_sync(self.h), "sync")
+
+    def set_precision(self, precision):
+        """CLIM_PRECISION_FP32 (default) or CLIM_PRECISION_MIXED (FP16 level
+        storage, packed-FP16 bilateral weights with FP32 sums)."""
+        self._check(lib().clim_set_precision(self.h, int(precision)), "set_precision")
 
     # ---- operators (cpp/demosaic_cl.hpp names) ---------------------------
     def scaleRawData(self, raw, scaled, bayer, scale_mul, black_level):

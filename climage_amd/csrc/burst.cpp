@@ -60,6 +60,7 @@ struct clim_burst {
     clim::BurstQueue* queues = nullptr;  // [2]
     int total[2] = {0, 0};
     int cus = 0;
+    bool half = false;  // FP16 YCbCr images (the mixed-precision form)
     Tuned tune{};
     bool cached = false;
     hipEvent_t fork = nullptr, mid = nullptr, lane_end[kLanes] = {}, lane_mid[kLanes] = {};
@@ -108,8 +109,9 @@ static clim_status record(clim_burst* b, bool with_times) {
     }
     if (!b->cross.empty()) {
         for (int kind = 0; kind < 2; kind++)
-            clim::launch_demosaic_burst(kind, b->tune.waves[kind], b->table, b->total[kind], b->queues + kind,
-                                        b->tune.blocks_per_cu[kind] * b->cus, b->tune.grab[kind], s0);
+            clim::launch_demosaic_burst(kind, b->tune.waves[kind], b->half, b->table, b->total[kind],
+                                        b->queues + kind, b->tune.blocks_per_cu[kind] * b->cus, b->tune.grab[kind],
+                                        s0);
         BURST_TRY_HIP(b, hipGetLastError());
     }
     BURST_TRY_HIP(b, hipEventRecord(b->mid, s0));
@@ -137,10 +139,12 @@ static clim_status time_shape(clim_burst* b, int kind, int waves, int bpc, int g
     BURST_TRY_HIP(b, hipEventCreate(&e1));
     constexpr int kReps = 2;
     hipError_t e = hipSuccess;
-    clim::launch_demosaic_burst(kind, waves, b->table, b->total[kind], b->queues + kind, bpc * b->cus, grab, s0);
+    clim::launch_demosaic_burst(kind, waves, b->half, b->table, b->total[kind], b->queues + kind, bpc * b->cus, grab,
+                                s0);
     if (e == hipSuccess) e = hipEventRecord(e0, s0);
     for (int r = 0; r < kReps && e == hipSuccess; r++)
-        clim::launch_demosaic_burst(kind, waves, b->table, b->total[kind], b->queues + kind, bpc * b->cus, grab, s0);
+        clim::launch_demosaic_burst(kind, waves, b->half, b->table, b->total[kind], b->queues + kind, bpc * b->cus,
+                                    grab, s0);
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(e1, s0);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -220,6 +224,7 @@ static clim_status burst_build(clim_burst* b, int32_t n, const clim_image* const
     b->green.resize(n);
     b->ycc.resize(n);
     b->timed = (flags & CLIM_BURST_FRAME_TIMES) != 0;
+    b->half = ctx->precision == CLIM_PRECISION_MIXED;
     BURST_TRY_HIP(b, hipSetDevice(ctx->device));
     BURST_TRY_HIP(b, hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     // lanes: contexts of the burst's own, with the context's learned strength
@@ -231,6 +236,7 @@ static clim_status burst_build(clim_burst* b, int32_t n, const clim_image* const
     for (int k = 0; k < kLanes; k++) {
         const clim_status s = clim_create(&b->lane[k], ctx->device);
         if (s != CLIM_OK) return burst_fail(b, s, "lane context");
+        b->lane[k]->precision = ctx->precision;
         if (!weights.empty()) {
             if (clim_strength_set_weights(b->lane[k], weights.data(), (int64_t)weights.size()) != CLIM_OK ||
                 clim_strength_set_mode(b->lane[k], ctx->sn.mode) != CLIM_OK)
@@ -244,7 +250,8 @@ static clim_status burst_build(clim_burst* b, int32_t n, const clim_image* const
         if (s != CLIM_OK) return burst_fail(b, s, "frame " + std::to_string(i) + ": " + c->last_error);
         const FrameCall& f = b->calls[i];
         CLIM_TRY(owned_alloc(ctx, b->green[i], f.w, f.h, CLIM_FMT_R32F));
-        CLIM_TRY(owned_alloc(ctx, b->ycc[i], f.w, f.h, CLIM_FMT_RGBA32F));
+        CLIM_TRY(owned_alloc(ctx, b->ycc[i], f.w, f.h,
+                             ctx->precision == CLIM_PRECISION_MIXED ? CLIM_FMT_RGBA16F : CLIM_FMT_RGBA32F));
         // the high-noise frames read their pre-denoised plane from the lane's
         // work buffers: their demosaic stays with the lane
         if (!f.high_noise) b->cross.push_back(i);
@@ -321,8 +328,8 @@ static clim_status burst_build(clim_burst* b, int32_t n, const clim_image* const
         // the frames' fits are in place: time the shapes of each stage (the
         // green planes the second stage reads are the chosen shape's)
         CLIM_TRY(autotune(b, 0));
-        clim::launch_demosaic_burst(0, b->tune.waves[0], b->table, b->total[0], b->queues, b->tune.blocks_per_cu[0] * b->cus,
-                                    b->tune.grab[0], b->lane[0]->stream);
+        clim::launch_demosaic_burst(0, b->tune.waves[0], b->half, b->table, b->total[0], b->queues,
+                                    b->tune.blocks_per_cu[0] * b->cus, b->tune.grab[0], b->lane[0]->stream);
         CLIM_TRY(autotune(b, 1));
         BURST_TRY_HIP(b, hipStreamSynchronize(b->lane[0]->stream));
         std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -24,6 +24,13 @@ clim_status ctx_hip(clim_ctx* ctx, hipError_t e, const char* what) {
                     std::string(what) + ": " + hipGetErrorString(e));
 }
 
+extern "C" clim_status clim_set_precision(clim_ctx* ctx, int32_t precision) {
+    if (!ctx || (precision != CLIM_PRECISION_FP32 && precision != CLIM_PRECISION_MIXED))
+        return ctx_fail(ctx, CLIM_EINVAL, "set_precision: bad precision");
+    ctx->precision = precision;
+    return CLIM_OK;
+}
+
 int32_t clim_format_texel_bytes(int32_t format) {
     switch (format) {
         case CLIM_FMT_R16U: return 2;
@@ -31,6 +38,7 @@ int32_t clim_format_texel_bytes(int32_t format) {
         case CLIM_FMT_RGBA32F: return 16;
         case CLIM_FMT_RGBA8: return 4;
         case CLIM_FMT_RG32F: return 8;
+        case CLIM_FMT_RGBA16F: return 8;
         default: return 0;
     }
 }
@@ -45,6 +53,7 @@ clim_status owned_alloc(clim_ctx* ctx, OwnedImg& o, int w, int h, int format) {
     if (same && w <= o.cap_w && h <= o.cap_h) {
         o.img.w = w;
         o.img.h = h;
+        o.img.half = format == CLIM_FMT_RGBA16F ? 1 : 0;
         return CLIM_OK;
     }
     const int cw = same ? std::max(w, o.cap_w) : w, ch = same ? std::max(h, o.cap_h) : h;
@@ -52,7 +61,7 @@ clim_status owned_alloc(clim_ctx* ctx, OwnedImg& o, int w, int h, int format) {
     const size_t pitch = pitch_for(cw, format);
     void* p = nullptr;
     CLIM_TRY_HIP(ctx, hipMalloc(&p, pitch * (size_t)ch));
-    o.img = Img{(char*)p, w, h, (int)pitch};
+    o.img = Img{(char*)p, w, h, (int)pitch, format == CLIM_FMT_RGBA16F ? 1 : 0};
     o.format = format;
     o.cap_w = cw;
     o.cap_h = ch;
@@ -116,7 +125,7 @@ static clim_status check_same(clim_ctx* ctx, const clim_image* a, const clim_ima
 }
 
 static Img view(const clim_image* im) {
-    return Img{(char*)im->data, im->width, im->height, im->pitch_bytes};
+    return Img{(char*)im->data, im->width, im->height, im->pitch_bytes, im->format == CLIM_FMT_RGBA16F ? 1 : 0};
 }
 
 static ParamRef host_params(const float* v, int n) {

@@ -97,6 +97,7 @@ struct clim_ctx {
     std::vector<ProfRec> prof_recs;
     size_t ev_next = 0;
     SnState sn;
+    int precision = 0;  // clim_precision
     // device address of the static scratch array of
     // clim_compute_raw_noise_statistics_noalloc on this context's device
     void* noalloc_scratch = nullptr;

@@ -12,15 +12,23 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "clim_math.h"
 
 namespace clim {
 
-// pitch-linear device image view
+// pitch-linear device image view.  RGBA images hold FP32 texels, or FP16
+// texels (8 bytes) when `half` is set: the mixed-precision pyramid's storage.
+// Single-channel images are FP32 (or uint16 mosaics).
 struct Img {
     char* p;
     int w, h, pitch;
+    int half = 0;
 };
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -41,8 +49,56 @@ __device__ __forceinline__ float4* wrow_f4(const Img& im, int y) {
 __device__ __forceinline__ float ld1(const Img& im, int x, int y) {
     return row_f(im, clampi(y, 0, im.h - 1))[clampi(x, 0, im.w - 1)];
 }
+// RGBA texel (x, y) as FP32, from FP16 storage when H (the kernels of the
+// pyramid take H as a template parameter, so that neither form branches per
+// access; ldt4 / st4 / ld4 below decide at run time, for the other kernels)
+template <bool H>
+__device__ __forceinline__ float4 ldt4_(const Img& im, int x, int y) {
+    if (H) {
+        const half4_t v = reinterpret_cast<const half4_t*>(im.p + (size_t)y * im.pitch)[x];
+        return make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+    }
+    return row_f4(im, y)[x];
+}
+// store of RGBA texel (x, y) (FP16 storage rounds to nearest even)
+template <bool H>
+__device__ __forceinline__ void st4_(const Img& im, int x, int y, float4 v) {
+    if (H) {
+        half4_t h;
+        h.x = (_Float16)v.x;
+        h.y = (_Float16)v.y;
+        h.z = (_Float16)v.z;
+        h.w = (_Float16)v.w;
+        reinterpret_cast<half4_t*>(im.p + (size_t)y * im.pitch)[x] = h;
+    } else {
+        wrow_f4(im, y)[x] = v;
+    }
+}
+template <bool H>
+__device__ __forceinline__ float4 ld4_(const Img& im, int x, int y) {
+    return ldt4_<H>(im, clampi(x, 0, im.w - 1), clampi(y, 0, im.h - 1));
+}
+__device__ __forceinline__ float4 ldt4(const Img& im, int x, int y) {
+    return im.half ? ldt4_<true>(im, x, y) : ldt4_<false>(im, x, y);
+}
+__device__ __forceinline__ void st4(const Img& im, int x, int y, float4 v) {
+    if (im.half)
+        st4_<true>(im, x, y, v);
+    else
+        st4_<false>(im, x, y, v);
+}
 __device__ __forceinline__ float4 ld4(const Img& im, int x, int y) {
-    return row_f4(im, clampi(y, 0, im.h - 1))[clampi(x, 0, im.w - 1)];
+    return ldt4(im, clampi(x, 0, im.w - 1), clampi(y, 0, im.h - 1));
+}
+
+// Runs f(std::integral_constant<bool, H>) with H = the image's storage:
+// launchers pick the kernel instance this way.
+template <typename F>
+inline void with_storage(const Img& im, F f) {
+    if (im.half)
+        f(std::true_type{});
+    else
+        f(std::false_type{});
 }
 
 // OpenCL built-ins

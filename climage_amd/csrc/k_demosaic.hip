@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(256) k_red_blue(MosaicSrc<U16> raw, Img green,
                 oy = out.h - y;
                 if (ox >= out.w || oy >= out.h) continue;
             }
-            wrow_f4(out, oy)[ox] = make_float4(q[dy][dx].x, q[dy][dx].y, q[dy][dx].z, 0.0f);
+            st4(out, ox, oy, make_float4(q[dy][dx].x, q[dy][dx].y, q[dy][dx].z, 0.0f));
         }
     }
 }
@@ -461,8 +461,8 @@ void launch_red_blue(const MosaicArg& raw, const Img& green, const Img& out, int
 __global__ void __launch_bounds__(256) k_zero_row0_col0(Img out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (i < out.w) wrow_f4(out, 0)[i] = z;
-    if (i < out.h) wrow_f4(out, i)[0] = z;
+    if (i < out.w) st4(out, i, 0, z);
+    if (i < out.h) st4(out, 0, i, z);
 }
 
 void launch_zero_row0_col0(const Img& out, hipStream_t st) {
@@ -588,7 +588,7 @@ __global__ void __launch_bounds__(256) k_despeckle(Img in, Img out, float var_a,
     const int x = tl.x * 64 + threadIdx.x;
     const int y = tl.y * 4 + threadIdx.y;
     if (x >= out.w || y >= out.h) return;
-    wrow_f4(out, y)[x] = despeckle_px([&](int dx, int dy) { return ld4(in, x + dx, y + dy); }, var_a, var_b);
+    st4(out, x, y, despeckle_px([&](int dx, int dy) { return ld4(in, x + dx, y + dy); }, var_a, var_b));
 }
 
 void launch_despeckle(const Img& in, const Img& out, float var_a, float var_b, hipStream_t st) {
@@ -631,7 +631,7 @@ union RbDespeckleLds {
 
 // one RD_X x RD_Y output tile at (x0, y0) by the block's 256 threads; as
 // green_tile, returns with the LDS possibly still being read
-template <bool U16>
+template <bool U16, bool H>
 __device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, const Img& green, const Img& out,
                                                   const Bayer& b, int red_is_c1_site, const ParamRef& cv, float clip,
                                                   const Mat3& m, float var_a, float var_b, int x0, int y0, int tid,
@@ -692,8 +692,8 @@ __device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, con
     if (interior && nonan) {
         for (int i = tid; i < RD_X * RD_Y; i += 256) {
             const int lx = i % RD_X, ly = i / RD_X;
-            wrow_f4(out, y0 + ly)[x0 + lx] =
-                despeckle_px<true>([&](int dx, int dy) { return sm.t[ly + 2 + dy][lx + 2 + dx]; }, var_a, var_b);
+            st4_<H>(out, x0 + lx, y0 + ly,
+                    despeckle_px<true>([&](int dx, int dy) { return sm.t[ly + 2 + dy][lx + 2 + dx]; }, var_a, var_b));
         }
         return;
     }
@@ -704,17 +704,17 @@ __device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, con
             const int gx = clampi(x + dx, 0, out.w - 1), gy = clampi(y + dy, 0, out.h - 1);
             return sm.t[gy - ry0][gx - rx0];
         };
-        wrow_f4(out, y)[x] = nonan ? despeckle_px<true>(fetch, var_a, var_b) : despeckle_px<false>(fetch, var_a, var_b);
+        st4_<H>(out, x, y, nonan ? despeckle_px<true>(fetch, var_a, var_b) : despeckle_px<false>(fetch, var_a, var_b));
     }
 }
 
-template <bool U16>
+template <bool U16, bool H>
 __global__ void __launch_bounds__(256) k_rb_despeckle(MosaicSrc<U16> raw, Img green, Img out, Bayer b,
                                                       int red_is_c1_site, ParamRef cv, float clip, Mat3 m,
                                                       float var_a, float var_b) {
     __shared__ RbDespeckleLds sm;
     const TileId tl = xcd_tile();
-    rb_despeckle_tile<U16>(raw, green, out, b, red_is_c1_site, cv, clip, m, var_a, var_b, tl.x * RD_X, tl.y * RD_Y,
+    rb_despeckle_tile<U16, H>(raw, green, out, b, red_is_c1_site, cv, clip, m, var_a, var_b, tl.x * RD_X, tl.y * RD_Y,
                            threadIdx.x, sm);
 }
 
@@ -745,7 +745,7 @@ __device__ __forceinline__ void burst_leave(BurstQueue* q) {
     }
 }
 
-template <int KIND, int WAVES>
+template <int KIND, int WAVES, bool H>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_demosaic_burst(const BurstFrame* __restrict__ frames, int total, BurstQueue* q, int grab) {
     __shared__ typename std::conditional<KIND == 0, GreenLds, RbDespeckleLds>::type sm;
@@ -770,7 +770,7 @@ k_demosaic_burst(const BurstFrame* __restrict__ frames, int total, BurstQueue* q
                 Mat3 m;
 #pragma unroll
                 for (int i = 0; i < 9; i++) m.m[i] = fr.c2y[i];
-                rb_despeckle_tile<true>(fr.raw, fr.green, fr.ycc, fr.b, fr.red_is_c1, fr.cv, fr.clip, m, fr.var_a,
+                rb_despeckle_tile<true, H>(fr.raw, fr.green, fr.ycc, fr.b, fr.red_is_c1, fr.cv, fr.clip, m, fr.var_a,
                                         fr.var_b, tx * RD_X, ty * RD_Y, threadIdx.x,
                                         reinterpret_cast<RbDespeckleLds&>(sm));
             }
@@ -786,23 +786,24 @@ void burst_tiles(int w, int h, int kind, int* tiles_x, int* tiles) {
     *tiles = *tiles_x * ((h + ty - 1) / ty);
 }
 
-template <int KIND>
-static void launch_burst_kind(int waves, const BurstFrame* frames, int total, BurstQueue* q, int blocks, int grab,
-                              hipStream_t st) {
-    if (waves >= 8)
-        hipLaunchKernelGGL((k_demosaic_burst<KIND, 8>), dim3(blocks), dim3(256), 0, st, frames, total, q, grab);
-    else if (waves >= 6)
-        hipLaunchKernelGGL((k_demosaic_burst<KIND, 6>), dim3(blocks), dim3(256), 0, st, frames, total, q, grab);
+void launch_demosaic_burst(int kind, int waves, bool half, const BurstFrame* frames, int total_tiles, BurstQueue* q,
+                           int blocks, int grab, hipStream_t st) {
+    // the green stage writes FP32 green planes whatever the level storage
+    if (kind == 0 && waves >= 8)
+        hipLaunchKernelGGL((k_demosaic_burst<0, 8, false>), dim3(blocks), dim3(256), 0, st, frames, total_tiles, q,
+                           grab);
+    else if (kind == 0 && waves >= 6)
+        hipLaunchKernelGGL((k_demosaic_burst<0, 6, false>), dim3(blocks), dim3(256), 0, st, frames, total_tiles, q,
+                           grab);
+    else if (kind == 0)
+        hipLaunchKernelGGL((k_demosaic_burst<0, 4, false>), dim3(blocks), dim3(256), 0, st, frames, total_tiles, q,
+                           grab);
+    else if (half)
+        hipLaunchKernelGGL((k_demosaic_burst<1, 4, true>), dim3(blocks), dim3(256), 0, st, frames, total_tiles, q,
+                           grab);
     else
-        hipLaunchKernelGGL((k_demosaic_burst<KIND, 4>), dim3(blocks), dim3(256), 0, st, frames, total, q, grab);
-}
-
-void launch_demosaic_burst(int kind, int waves, const BurstFrame* frames, int total_tiles, BurstQueue* q, int blocks,
-                           int grab, hipStream_t st) {
-    if (kind == 0)
-        launch_burst_kind<0>(waves, frames, total_tiles, q, blocks, grab, st);
-    else
-        hipLaunchKernelGGL((k_demosaic_burst<1, 4>), dim3(blocks), dim3(256), 0, st, frames, total_tiles, q, grab);
+        hipLaunchKernelGGL((k_demosaic_burst<1, 4, false>), dim3(blocks), dim3(256), 0, st, frames, total_tiles, q,
+                           grab);
 }
 
 void launch_rb_despeckle(const MosaicArg& raw, const Img& green, const Img& out, int bayer, const Bayer& b,
@@ -812,12 +813,14 @@ void launch_rb_despeckle(const MosaicArg& raw, const Img& green, const Img& out,
     Mat3 m;
     for (int i = 0; i < 9; i++) m.m[i] = c2y[i];
     dim3 blk(256), grd((out.w + RD_X - 1) / RD_X, (out.h + RD_Y - 1) / RD_Y);
-    if (raw.u16)
-        hipLaunchKernelGGL(k_rb_despeckle<true>, grd, blk, 0, st, raw.src<true>(), green, out, b, red_is_c1, cv,
-                           clip, m, var_a, var_b);
-    else
-        hipLaunchKernelGGL(k_rb_despeckle<false>, grd, blk, 0, st, raw.src<false>(), green, out, b, red_is_c1, cv,
-                           clip, m, var_a, var_b);
+    with_storage(out, [&](auto H) {
+        if (raw.u16)
+            hipLaunchKernelGGL((k_rb_despeckle<true, H()>), grd, blk, 0, st, raw.src<true>(), green, out, b,
+                               red_is_c1, cv, clip, m, var_a, var_b);
+        else
+            hipLaunchKernelGGL((k_rb_despeckle<false, H()>), grd, blk, 0, st, raw.src<false>(), green, out, b,
+                               red_is_c1, cv, clip, m, var_a, var_b);
+    });
 }
 
 }  // namespace clim

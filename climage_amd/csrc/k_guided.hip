@@ -191,7 +191,7 @@ template <int R> struct Dims {
 };
 }  // namespace gp
 
-template <int R>
+template <int R, bool H>
 __global__ void __launch_bounds__(gp::NT, 2) k_guided_precise(Img in, Img out, ParamRef eps) {
     using D = gp::Dims<R>;
     constexpr int TX = gp::TX, TY = gp::TY, NT = gp::NT, FS = D::FS;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(gp::NT, 2) k_guided_precise(Img in, Img out, P
         tid,
         [&](int i) {
             const int r = i / INW, c = i - r * INW;
-            return row_f4(in, clampi(y0 - 2 * R + r, 0, h - 1))[clampi(x0 - 2 * R + c, 0, w - 1)];
+            return ldt4_<H>(in, clampi(x0 - 2 * R + c, 0, w - 1), clampi(y0 - 2 * R + r, 0, h - 1));
         },
         [&](int i, float4 v) {
             IN[i] = v.x;
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(gp::NT, 2) k_guided_precise(Img in, Img out, P
 #pragma unroll
         for (int c = 0; c < 3; c++) q[c] = a[c] * p0 + a[4 + c] * p1 + a[8 + c] * p2;
         q[0] += a[3]; q[1] += a[7]; q[2] += a[11];
-        wrow_f4(out, y)[x] = make_float4(q[0], q[1], q[2], 0.0f);
+        st4_<H>(out, x, y, make_float4(q[0], q[1], q[2], 0.0f));
     }
 }
 
@@ -341,10 +341,12 @@ bool guided_filter_fused(int fs) { return fs == 3 || fs == 5; }
 
 void launch_guided_filter_fused(const Img& in, const Img& out, int fs, const ParamRef& eps, hipStream_t st) {
     dim3 grd((in.w + gp::TX - 1) / gp::TX, (in.h + gp::TY - 1) / gp::TY);
-    if (fs == 5)
-        hipLaunchKernelGGL(k_guided_precise<2>, grd, dim3(gp::NT), 0, st, in, out, eps);
-    else
-        hipLaunchKernelGGL(k_guided_precise<1>, grd, dim3(gp::NT), 0, st, in, out, eps);
+    with_storage(in, [&](auto H) {  // in and out share their storage
+        if (fs == 5)
+            hipLaunchKernelGGL((k_guided_precise<2, H()>), grd, dim3(gp::NT), 0, st, in, out, eps);
+        else
+            hipLaunchKernelGGL((k_guided_precise<1, H()>), grd, dim3(gp::NT), 0, st, in, out, eps);
+    });
 }
 
 template <int NC>

@@ -27,10 +27,12 @@ __device__ __forceinline__ float4 bilinear4_at(Fetch ld, int i0, int j0, float a
     r.w = w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w;
     return r;
 }
+template <bool H = false>
 __device__ __forceinline__ float4 bilinear4_at(const Img& im, int i0, int j0, float a, float b) {
-    return bilinear4_at([&](int i, int j) { return ld4(im, i, j); }, i0, j0, a, b);
+    return bilinear4_at([&](int i, int j) { return ld4_<H>(im, i, j); }, i0, j0, a, b);
 }
 
+template <bool H = false>
 __device__ __forceinline__ float4 bilinear4(const Img& im, float un, float vn) {
     const float u = un * (float)im.w - 0.5f;
     const float v = vn * (float)im.h - 0.5f;
@@ -38,7 +40,7 @@ __device__ __forceinline__ float4 bilinear4(const Img& im, float un, float vn) {
     float a = u - fu, b = v - fv;
     a = floorf(a * 256.0f + 0.5f) * 0.00390625f;
     b = floorf(b * 256.0f + 0.5f) * 0.00390625f;
-    return bilinear4_at(im, (int)fu, (int)fv, a, b);
+    return bilinear4_at<H>(im, (int)fu, (int)fv, a, b);
 }
 
 // Exact 2x ratios.  For an output of at most kDown2xMax texels per axis
@@ -53,7 +55,7 @@ constexpr int kDown2xMax = 6144, kUp2xMax = 16384;
 
 // ------------------------------------------------------------ downsampleImage
 // cl/demosaic.cl:853-865: four linear taps at +-1/2 output texel, averaged.
-template <bool EXACT2X>
+template <bool EXACT2X, bool H>
 __global__ void __launch_bounds__(256) k_downsample(Img in, Img out) {
     const TileId tl = xcd_tile();
     const int x = tl.x * 64 + threadIdx.x;
@@ -61,24 +63,24 @@ __global__ void __launch_bounds__(256) k_downsample(Img in, Img out) {
     if (x >= out.w || y >= out.h) return;
     float4 t0, t1, t2, t3;
     if (EXACT2X) {
-        t0 = bilinear4_at(in, 2 * x - 1, 2 * y - 1, 0.5f, 0.5f);
-        t1 = bilinear4_at(in, 2 * x + 1, 2 * y - 1, 0.5f, 0.5f);
-        t2 = bilinear4_at(in, 2 * x - 1, 2 * y + 1, 0.5f, 0.5f);
-        t3 = bilinear4_at(in, 2 * x + 1, 2 * y + 1, 0.5f, 0.5f);
+        t0 = bilinear4_at<H>(in, 2 * x - 1, 2 * y - 1, 0.5f, 0.5f);
+        t1 = bilinear4_at<H>(in, 2 * x + 1, 2 * y - 1, 0.5f, 0.5f);
+        t2 = bilinear4_at<H>(in, 2 * x - 1, 2 * y + 1, 0.5f, 0.5f);
+        t3 = bilinear4_at<H>(in, 2 * x + 1, 2 * y + 1, 0.5f, 0.5f);
     } else {
         const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
         const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
         const float sx = 0.5f * nx, sy = 0.5f * ny;
-        t0 = bilinear4(in, px - sx, py - sy);
-        t1 = bilinear4(in, px + sx, py - sy);
-        t2 = bilinear4(in, px - sx, py + sy);
-        t3 = bilinear4(in, px + sx, py + sy);
+        t0 = bilinear4<H>(in, px - sx, py - sy);
+        t1 = bilinear4<H>(in, px + sx, py - sy);
+        t2 = bilinear4<H>(in, px - sx, py + sy);
+        t3 = bilinear4<H>(in, px + sx, py + sy);
     }
     float4 acc = t0;
     acc.x += t1.x; acc.y += t1.y; acc.z += t1.z; acc.w += t1.w;
     acc.x += t2.x; acc.y += t2.y; acc.z += t2.z; acc.w += t2.w;
     acc.x += t3.x; acc.y += t3.y; acc.z += t3.z; acc.w += t3.w;
-    wrow_f4(out, y)[x] = make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.25f * acc.w);
+    st4_<H>(out, x, y, make_float4(0.25f * acc.x, 0.25f * acc.y, 0.25f * acc.z, 0.25f * acc.w));
 }
 
 bool downsample_exact2x(const Img& in, const Img& out) {
@@ -87,10 +89,13 @@ bool downsample_exact2x(const Img& in, const Img& out) {
 
 void launch_downsample(const Img& in, const Img& out, hipStream_t st) {
     dim3 blk(64, 4), grd((out.w + 63) / 64, (out.h + 3) / 4);
-    if (downsample_exact2x(in, out))
-        hipLaunchKernelGGL(k_downsample<true>, grd, blk, 0, st, in, out);
-    else
-        hipLaunchKernelGGL(k_downsample<false>, grd, blk, 0, st, in, out);
+    // a level and the next share their storage
+    with_storage(in, [&](auto H) {
+        if (downsample_exact2x(in, out))
+            hipLaunchKernelGGL((k_downsample<true, H()>), grd, blk, 0, st, in, out);
+        else
+            hipLaunchKernelGGL((k_downsample<false, H()>), grd, blk, 0, st, in, out);
+    });
 }
 
 // 5x5 stencils over an RGBA level: a 32x16 output tile plus a 2-texel halo in
@@ -162,6 +167,7 @@ __device__ __forceinline__ void down_taps(const LevelTile4<TX, TY>& t, float2 (*
 
 // thread tid writes channels 2p, 2p + 1 (p = tid / 128) of texel tid % 128
 // (16x8, row-major)
+template <bool H>
 __device__ __forceinline__ void down_write(const float2 (*tp)[kTapH][kTapW], const Img& out, int x0, int y0,
                                            int tid) {
     const int p = tid >> 7, o = tid & 127, lx = o & 15, ly = o >> 4;
@@ -172,7 +178,14 @@ __device__ __forceinline__ void down_write(const float2 (*tp)[kTapH][kTapW], con
     ax += t1.x; ay += t1.y;
     ax += t2.x; ay += t2.y;
     ax += t3.x; ay += t3.y;
-    reinterpret_cast<float2*>(wrow_f4(out, Y) + X)[p] = make_float2(0.25f * ax, 0.25f * ay);
+    if (H) {
+        half2_t hv;
+        hv.x = (_Float16)(0.25f * ax);
+        hv.y = (_Float16)(0.25f * ay);
+        reinterpret_cast<half2_t*>(out.p + (size_t)Y * out.pitch)[2 * X + p] = hv;
+    } else {
+        reinterpret_cast<float2*>(wrow_f4(out, Y) + X)[p] = make_float2(0.25f * ax, 0.25f * ay);
+    }
 }
 
 // noiseStatistics of pyramid levels (one job per level) with the first
@@ -182,7 +195,7 @@ __device__ __forceinline__ void down_write(const float2 (*tp)[kTapH][kTapW], con
 // statistics in range are listed for the refit.
 // DOWN (one job): each tile also writes the next level's texels it covers
 // (downsampleImage), so that level is made without reading this one again.
-template <bool DOWN>
+template <bool DOWN, bool H>
 __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down) {
     using L = Layout<3, true, false>;
     using Tile = LevelTile4<TX, TY>;
@@ -207,16 +220,16 @@ __global__ void __launch_bounds__(256) k_noise_stats_fit(FitJobs jobs, Img down)
     acc.zero();
     // the next tile's loads are issued before the current tile's compute
     typename Tile::Regs pf;
-    if (t0 < t1) Tile::fetch(pf, in, (t0 % tiles_x) * TX, (t0 / tiles_x) * TY, tid);
+    if (t0 < t1) Tile::template fetch<H>(pf, in, (t0 % tiles_x) * TX, (t0 / tiles_x) * TY, tid);
     for (int t = t0; t < t1; t++) {
         const int x0 = (t % tiles_x) * TX, y0 = (t / tiles_x) * TY;
         sm.t.put(pf, tid);
         __syncthreads();
-        if (t + 1 < t1) Tile::fetch(pf, in, ((t + 1) % tiles_x) * TX, ((t + 1) / tiles_x) * TY, tid);
+        if (t + 1 < t1) Tile::template fetch<H>(pf, in, ((t + 1) % tiles_x) * TX, ((t + 1) / tiles_x) * TY, tid);
         sm.r.compute(sm.t, tid);
         if constexpr (DOWN) down_taps(sm.t, sm.taps, tid);
         __syncthreads();
-        if constexpr (DOWN) down_write(sm.taps, down, x0, y0, tid);
+        if constexpr (DOWN) down_write<H>(sm.taps, down, x0, y0, tid);
         // this thread's outputs: rows 2 (tid / 32) and 2 (tid / 32) + 1 of column tid % 32
         float4 s4[2];
         bool ok[2];
@@ -269,10 +282,13 @@ bool launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipS
     }
     const int grid = fit_share_blocks(sub, tiles, kLevelStatsGrid);
     for (int j = j0; j < j1; j++) jobs.j[j] = sub.j[j - j0];
-    if (down)
-        hipLaunchKernelGGL(k_noise_stats_fit<true>, dim3(grid), dim3(256), 0, st, sub, *down);
-    else
-        hipLaunchKernelGGL(k_noise_stats_fit<false>, dim3(grid), dim3(256), 0, st, sub, Img{});
+    // the jobs' levels share their storage (and the next level's)
+    with_storage(sub.j[0].src, [&](auto H) {
+        if (down)
+            hipLaunchKernelGGL((k_noise_stats_fit<true, H()>), dim3(grid), dim3(256), 0, st, sub, *down);
+        else
+            hipLaunchKernelGGL((k_noise_stats_fit<false, H()>), dim3(grid), dim3(256), 0, st, sub, Img{});
+    });
     return true;
 }
 
@@ -356,12 +372,12 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int 
 }
 
 // one 32x16 tile (x0, y0) of denoiseImage
-template <bool SAME_EDGE>
+template <bool SAME_EDGE, bool H>
 __device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, float* mt, const Img& in, const Img& out, int x0,
                                                int y0, const ParamRef& nlf, float chromaBoost, float gradientBoost,
                                                const StrengthRef& sr, int level) {
     const int tid = threadIdx.y * 32 + threadIdx.x;
-    t.load(in, x0, y0, tid);
+    t.template load<H>(in, x0, y0, tid);
     const MapStage ms = stage_strength<TX, TY>(sr, level, x0, y0, mt, tid);
     __syncthreads();
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
@@ -372,18 +388,18 @@ __device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, float* mt,
         const int x = x0 + tx, y = y0 + ty;
         if (x >= out.w || y >= out.h) continue;
         const float m = strength_at_staged(sr, ms, mt, level, x, y);
-        wrow_f4(out, y)[x] =
-            bilateral_px<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost, m);
+        st4_<H>(out, x, y,
+                bilateral_px<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost, m));
     }
 }
 
-template <bool SAME_EDGE>
+template <bool SAME_EDGE, bool H>
 __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf, float chromaBoost,
                                                    float gradientBoost, StrengthRef sr, int level) {
     __shared__ LevelTile4<TX, TY> t;
     __shared__ float mt[kMapStageMax];
     const TileId tl = xcd_tile();
-    bilateral_tile<SAME_EDGE>(t, mt, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost, sr, level);
+    bilateral_tile<SAME_EDGE, H>(t, mt, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost, sr, level);
 }
 
 // denoiseImage of every pyramid level in one launch (the levels' denoising
@@ -397,8 +413,8 @@ __global__ void __launch_bounds__(256) k_bilateral_levels(BilateralLevels lv) {
     int l = 0;
     while (l + 1 < lv.n && b >= lv.base[l + 1]) l++;
     const int i = b - lv.base[l], tiles_x = (lv.out[l].w + TX - 1) / TX;
-    bilateral_tile<SAME_EDGE>(t, mt, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY, lv.nlf[l],
-                              lv.chromaBoost[l], lv.gradientBoost[l], lv.sr, l);
+    bilateral_tile<SAME_EDGE, false>(t, mt, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY,
+                                     lv.nlf[l], lv.chromaBoost[l], lv.gradientBoost[l], lv.sr, l);
 }
 
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
@@ -409,12 +425,18 @@ void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float 
     float e0 = 1.0f * 1.41421356f, e1 = chromaBoost * 1.41421356f;
     e0 = e0 * e0;
     e1 = e1 * e1;
-    if (!(gradientBoost > 1.0f) && e0 == e1)
-        hipLaunchKernelGGL(k_bilateral<true>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost, sr, level);
-    else
-        hipLaunchKernelGGL(k_bilateral<false>, grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost, sr,
-                           level);
+    with_storage(in, [&](auto H) {
+        if (!(gradientBoost > 1.0f) && e0 == e1)
+            hipLaunchKernelGGL((k_bilateral<true, H()>), grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost, sr,
+                               level);
+        else
+            hipLaunchKernelGGL((k_bilateral<false, H()>), grd, blk, 0, st, in, out, nlf, chromaBoost, gradientBoost,
+                               sr, level);
+    });
 }
+
+template <bool SAME_EDGE>
+__global__ void __launch_bounds__(256) k_bilateral_levels_mixed(BilateralLevels lv);
 
 // the one-test form when launch_bilateral would take it for every level
 void launch_bilateral_levels(BilateralLevels& lv, hipStream_t st) {
@@ -428,10 +450,203 @@ void launch_bilateral_levels(BilateralLevels& lv, hipStream_t st) {
         lv.base[l] = base;
         base += ((lv.out[l].w + TX - 1) / TX) * ((lv.out[l].h + TY - 1) / TY);
     }
-    if (same)
+    // FP16 level images: the mixed-precision form (a pyramid's levels share
+    // their storage; the FP32 form reads FP32 levels only)
+    const bool mixed = lv.in[0].half != 0;
+    if (same && mixed)
+        hipLaunchKernelGGL(k_bilateral_levels_mixed<true>, dim3(base), dim3(32, 8), 0, st, lv);
+    else if (mixed)
+        hipLaunchKernelGGL(k_bilateral_levels_mixed<false>, dim3(base), dim3(32, 8), 0, st, lv);
+    else if (same)
         hipLaunchKernelGGL(k_bilateral_levels<true>, dim3(base), dim3(32, 8), 0, st, lv);
     else
         hipLaunchKernelGGL(k_bilateral_levels<false>, dim3(base), dim3(32, 8), 0, st, lv);
+}
+
+// --------------------------------------- denoiseImage, mixed precision form
+// The same filter on FP16 level images with the range test and the weights in
+// packed FP16 and the weighted sums in FP32.  Two vertically adjacent taps go
+// through each step together: q[c][r][x] holds channel c of texels (x, r) and
+// (x, r + 1) as one FP16 pair, so a 5x5 window is two pair rows and a row of
+// single taps (second half zero) of five taps each.  Per tap pair:
+//   u_c = (s_c - c_c) r_c            v_pk_add_f16, v_pk_mul_f16 per channel
+//                                    (the difference first: FP16 s_c r_c -
+//                                    c_c r_c would carry the rounding of
+//                                    c_c r_c, up to 2^-11 c_c / sigma_c)
+//   l - e = u_Y^2 + u_Cb^2 + u_Cr^2 - e   three v_pk_fma_f16 from -e
+//   w = clamp((e - l) 2^12 2^12, 0, 1)    two v_pk_mul_f16, the second clamping
+//   f_c += s_c.lo w.lo + s_c.hi w.hi      v_dot2_f32_f16: FP32 sums (and count)
+// w is exactly 1 when l - e < 0 and 0 otherwise (and for NaN): a negative
+// FP16 value is at most -2^-24, and 2^24 times its magnitude is at least 1
+// or overflows to +inf, both of which clamp to 1.  (With separate luma and
+// chroma edges, l is formed first and each edge subtracted.)  r_c = 1 /
+// sigma_c is computed in FP32 per centre pixel and rounded to FP16 (finite
+// values above the FP16 range saturate; inf and NaN pass, giving no weights
+// as the FP32 form does); the edges e = (boost sqrt 2)^2 (m^2 with the
+// learned strength) are rounded to FP16; the sums are divided through the
+// hardware reciprocal, the result being stored as FP16.
+struct PairTile {
+    static constexpr int SX = TX + 4, SY = TY + 4;
+    // qv[r][x] = the (Y, Cb, Cr) pairs (texel (x, r), texel (x, r + 1)), one
+    // 16-byte LDS read per tap pair; the last row's second halves are zero
+    uint4 qv[SY][SX];
+    __device__ __forceinline__ static uint32_t pack(float a, float b) {
+        half2_t h;
+        h.x = (_Float16)a;
+        h.y = (_Float16)b;
+        return __builtin_bit_cast(uint32_t, h);
+    }
+    // Thread t < 36 x 7 takes column t % SX and rows 3 (t / SX) .. + 2: it
+    // loads the four texels of those rows and the next (zero past the tile)
+    // and writes the three pairs, so each texel is loaded once or twice.
+    static constexpr int kRun = 3, kRuns = (SY + kRun - 1) / kRun;
+    static_assert(SX * kRuns <= 256, "one run per thread");
+    __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
+        if (tid >= SX * kRuns) return;
+        const int x = tid % SX, r0 = (tid / SX) * kRun;
+        float4 v[kRun + 1];
+#pragma unroll
+        for (int k = 0; k <= kRun; k++)
+            v[k] = r0 + k < SY ? ld4_<true>(in, x0 - 2 + x, y0 - 2 + r0 + k) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < kRun; k++)
+            if (r0 + k < SY)
+                qv[r0 + k][x] = make_uint4(pack(v[k].x, v[k + 1].x), pack(v[k].y, v[k + 1].y),
+                                           pack(v[k].z, v[k + 1].z), 0u);
+    }
+    // texel (x, r) of channel c
+    __device__ __forceinline__ _Float16 at(int c, int r, int x) const {
+        const uint4 v = qv[r][x];
+        const uint32_t w = c == 0 ? v.x : c == 1 ? v.y : v.z;
+        return __builtin_bit_cast(half2_t, w).x;
+    }
+};
+
+// (w.x, 0)
+__device__ __forceinline__ half2_t only_first(half2_t w) {
+    return __builtin_bit_cast(half2_t, __builtin_bit_cast(uint32_t, w) & 0xffffu);
+}
+
+__device__ __forceinline__ _Float16 to_half_sat(float r) {
+    // finite values beyond the FP16 range saturate; inf and NaN pass
+    return (_Float16)((r <= 65504.0f || !(r <= 3.0e38f)) ? r : 65504.0f);
+}
+
+template <bool SAME_EDGE>
+__device__ __forceinline__ float4 bilateral_px_mixed(const PairTile& t, int tx, int ty, float va0, float va1,
+                                                     float va2, float vb0, float vb1, float vb2, float chromaBoost,
+                                                     float gradientBoost, float m) {
+    const int X = tx + HR, Y = ty + HR;
+    const float cY = (float)t.at(0, Y, X);
+    float lumaBoost = 1.0f;
+    if (!SAME_EDGE && gradientBoost > 1.0f) {
+        const float sig0 = sqrtf(va0 + vb0 * cY);
+        const float kx[3][3] = {{1, 0, -1}, {2, 0, -2}, {1, 0, -1}};
+        const float ky[3][3] = {{1, 2, 1}, {0, 0, 0}, {-1, -2, -1}};
+        float vx = 0.0f, vy = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const float s = (float)t.at(0, Y - 1 + j, X - 1 + i);
+                vx += kx[j][i] * s;
+                vy += ky[j][i] * s;
+            }
+        vx = vx * 0.471404521f;
+        vy = vy * 0.471404521f;
+        const float is0 = 1.0f / sig0;
+        lumaBoost = gradientBoost - (gradientBoost - 1.0f) * smoothstep(0.5f, 2.0f, len2(vx * is0, vy * is0));
+    }
+    float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
+    e0 = e0 * e0;
+    e1 = e1 * e1;
+    if (m > 0.0f) {
+        const float m2 = m * m;
+        e0 = e0 * m2;
+        e1 = e1 * m2;
+    }
+    const float r0 = clim_inv_sigma(va0 + vb0 * cY), r1 = clim_inv_sigma(va1 + vb1 * cY),
+                r2 = clim_inv_sigma(va2 + vb2 * cY);
+    const half2_t rY = {to_half_sat(r0), to_half_sat(r0)}, rCb = {to_half_sat(r1), to_half_sat(r1)},
+                  rCr = {to_half_sat(r2), to_half_sat(r2)};
+    const half2_t cY2 = {t.at(0, Y, X), t.at(0, Y, X)}, cCb2 = {t.at(1, Y, X), t.at(1, Y, X)},
+                  cCr2 = {t.at(2, Y, X), t.at(2, Y, X)};
+    const _Float16 e0h = to_half_sat(e0), e1h = to_half_sat(e1);
+    const half2_t nE0 = {-e0h, -e0h}, E0 = {e0h, e0h}, E1 = {e1h, e1h};
+    const half2_t big = {(_Float16)4096.0f, (_Float16)4096.0f}, nbig = {(_Float16)-4096.0f, (_Float16)-4096.0f},
+                  zero = {(_Float16)0.0f, (_Float16)0.0f}, one = {(_Float16)1.0f, (_Float16)1.0f};
+    float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, c0 = 0.0f, c1 = 0.0f;
+#pragma unroll
+    for (int jp = 0; jp < 3; jp++) {
+        // the fifth row has no partner: its pairs' second weights are cleared
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const uint4 v = t.qv[ty + 2 * jp][tx + i];
+            const half2_t sY = __builtin_bit_cast(half2_t, v.x), sCb = __builtin_bit_cast(half2_t, v.y),
+                          sCr = __builtin_bit_cast(half2_t, v.z);
+            const half2_t uY = (sY - cY2) * rY;
+            const half2_t uCb = (sCb - cCb2) * rCb;
+            const half2_t uCr = (sCr - cCr2) * rCr;
+            if (SAME_EDGE) {
+                // l - e accumulated from -e; its sign gives the weight
+                const half2_t le = __builtin_elementwise_fma(
+                    uCr, uCr, __builtin_elementwise_fma(uCb, uCb, __builtin_elementwise_fma(uY, uY, nE0)));
+                half2_t w0 = __builtin_elementwise_min(__builtin_elementwise_max(le * nbig * big, zero), one);
+                if (jp == 2) w0 = only_first(w0);
+                f0 = __builtin_amdgcn_fdot2(sY, w0, f0, false);
+                c0 = __builtin_amdgcn_fdot2(w0, one, c0, false);
+                f1 = __builtin_amdgcn_fdot2(sCb, w0, f1, false);
+                f2 = __builtin_amdgcn_fdot2(sCr, w0, f2, false);
+            } else {
+                const half2_t l =
+                    __builtin_elementwise_fma(uCr, uCr, __builtin_elementwise_fma(uCb, uCb, uY * uY));
+                half2_t w0 = __builtin_elementwise_min(__builtin_elementwise_max((E0 - l) * big * big, zero), one);
+                half2_t w1 = __builtin_elementwise_min(__builtin_elementwise_max((E1 - l) * big * big, zero), one);
+                if (jp == 2) {
+                    w0 = only_first(w0);
+                    w1 = only_first(w1);
+                }
+                f0 = __builtin_amdgcn_fdot2(sY, w0, f0, false);
+                c0 = __builtin_amdgcn_fdot2(w0, one, c0, false);
+                f1 = __builtin_amdgcn_fdot2(sCb, w1, f1, false);
+                f2 = __builtin_amdgcn_fdot2(sCr, w1, f2, false);
+                c1 = __builtin_amdgcn_fdot2(w1, one, c1, false);
+            }
+        }
+    }
+    // the quotients by the weight counts (1..25) through the reciprocal: the
+    // result is stored as FP16
+    const float k0 = __builtin_amdgcn_rcpf(c0), k1 = SAME_EDGE ? k0 : __builtin_amdgcn_rcpf(c1);
+    return make_float4(f0 * k0, f1 * k1, f2 * k1, 0.0f);
+}
+
+template <bool SAME_EDGE>
+__global__ void __launch_bounds__(256) k_bilateral_levels_mixed(BilateralLevels lv) {
+    __shared__ PairTile t;
+    __shared__ float mt[kMapStageMax];
+    const int b = xcd_block();
+    int l = 0;
+    while (l + 1 < lv.n && b >= lv.base[l + 1]) l++;
+    const int i = b - lv.base[l], tiles_x = (lv.out[l].w + TX - 1) / TX;
+    const int x0 = (i % tiles_x) * TX, y0 = (i / tiles_x) * TY;
+    const int tid = threadIdx.y * 32 + threadIdx.x;
+    const Img& out = lv.out[l];
+    const MapStage ms = stage_strength<TX, TY>(lv.sr, l, x0, y0, mt, tid);
+    t.load(lv.in[l], x0, y0, tid);
+    __syncthreads();
+    const ParamRef& nlf = lv.nlf[l];
+    const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
+    const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
+#pragma unroll 1
+    for (int rr = 0; rr < 2; rr++) {
+        const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= out.w || y >= out.h) continue;
+        const float m = strength_at_staged(lv.sr, ms, mt, l, x, y);
+        st4_<true>(out, x, y,
+                   bilateral_px_mixed<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, lv.chromaBoost[l],
+                                                 lv.gradientBoost[l], m));
+    }
 }
 
 // --------------------------------------------------------- denoiseImageGuided
@@ -442,13 +657,14 @@ struct GuidedTile {
     float h[9][TY + 4][TX + 1];
 };
 
+template <bool H>
 __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef nlf, StrengthRef sr, int level) {
     __shared__ GuidedTile sm;
     __shared__ float mt[kMapStageMax];
     const TileId tl = xcd_tile();
     const int x0 = tl.x * TX, y0 = tl.y * TY;
     const int tid = threadIdx.y * 32 + threadIdx.x;
-    sm.t.load(in, x0, y0, tid);
+    sm.t.load<H>(in, x0, y0, tid);
     const MapStage ms = stage_strength<TX, TY>(sr, level, x0, y0, mt, tid);
     __syncthreads();
     {
@@ -527,7 +743,7 @@ __global__ void __launch_bounds__(256) k_guided_fast(Img in, Img out, ParamRef n
             const float b = mI[c] - ar[c] * m0 - ag[c] * m1 - ab[c] * m2;
             q[c] = ar[c] * ix + ag[c] * iy + ab[c] * iz + b;
         }
-        wrow_f4(out, y)[x] = make_float4(q[0], q[1], q[2], 0.0f);
+        st4_<H>(out, x, y, make_float4(q[0], q[1], q[2], 0.0f));
     }
 }
 
@@ -535,7 +751,9 @@ void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipS
                         int level) {
     dim3 blk(32, 8), grd((out.w + TX - 1) / TX, (out.h + TY - 1) / TY);
     const StrengthRef sr = srp ? *srp : StrengthRef{nullptr, 0, 0, nullptr, 0.0f};
-    hipLaunchKernelGGL(k_guided_fast, grd, blk, 0, st, in, out, nlf, sr, level);
+    with_storage(in, [&](auto H) {
+        hipLaunchKernelGGL((k_guided_fast<H()>), grd, blk, 0, st, in, out, nlf, sr, level);
+    });
 }
 
 // -------------------------------------------------------------- convertTosRGB
@@ -547,7 +765,7 @@ __global__ void __launch_bounds__(256) k_tone(Img in, Img mask, Img out8, Img ou
     const int y = blockIdx.y * 4 + threadIdx.y;
     if (x >= out8.w || y >= out8.h) return;
     float v[3];
-    tone_pre(row_f4(in, y)[x], a, v);
+    tone_pre(ldt4(in, x, y), a, v);
     tone_store_v<LTMB>(out8, outf, x, y, v, a, LTMB ? row_f(mask, y)[x] : 1.0f);
 }
 
@@ -592,7 +810,7 @@ struct LtmGuideLds {
 struct LtmGuideMem {
     Img g;
     __device__ __forceinline__ float at(int i, int j) const {
-        return reinterpret_cast<const float*>(row_f4(g, clampi(j, 0, g.h - 1)) + clampi(i, 0, g.w - 1))[0];
+        return ld4(g, i, j).x;
     }
 };
 
@@ -705,7 +923,7 @@ __global__ void __launch_bounds__(256) k_ltm_mask(Img in, Img out, LtmArgs a) {
     const LtmFoot f = ltm_stage(sh, &ax, tl.x * 64, min(tl.x * 64 + 63, out.w - 1), tl.y * 4,
                                 min(tl.y * 4 + 3, out.h - 1), nx, ny, a.guide);
     if (x >= out.w || y >= out.h) return;
-    const float4 p = row_f4(in, y)[x];
+    const float4 p = ldt4(in, x, y);
     const float rgb[3] = {a.T[0] * p.x + a.T[1] * p.y + a.T[2] * p.z, a.T[3] * p.x + a.T[4] * p.y + a.T[5] * p.z,
                           a.T[6] * p.x + a.T[7] * p.y + a.T[8] * p.z};
     wrow_f(out, y)[x] = ltm_mask_at(f, sh, ax, threadIdx.x, threadIdx.y, p.x, rgb, a);
@@ -746,7 +964,7 @@ void launch_tone(const Img& in, const Img& mask, const Img& out8, const Img& out
 // convertTosRGB (out8 / outf) instead of writing it.  LTM (with TONE): the
 // localToneMapping mask of the result (guide: pyramid level 3) is computed
 // in the same thread and applied by the tone map (cpp/raw_converter.cpp:164-174).
-template <bool EXACT2X, bool TONE, bool LTM>
+template <bool EXACT2X, bool TONE, bool LTM, bool H>
 __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img out, float sharpening0,
                                                     ParamRef nlf, Img out8, Img outf, ToneArgs ta, LtmArgs la) {
     const TileId tl = xcd_tile();
@@ -772,15 +990,15 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
         const int tid = threadIdx.y * 64 + threadIdx.x;
         for (int k = tid; k < 2 * RH * RW; k += 256) {
             const int m = k / (RH * RW), r = (k / RW) % RH, c = k % RW;
-            hs[m][r][c] = ld4(m ? d1 : n1, hx0 + c, hy0 + r);
+            hs[m][r][c] = ld4_<H>(m ? d1 : n1, hx0 + c, hy0 + r);
         }
         __syncthreads();
     }
-    auto half_n1 = [&](int i, int j) { return STAGE ? hs[0][j - hy0][i - hx0] : ld4(n1, i, j); };
-    auto half_d1 = [&](int i, int j) { return STAGE ? hs[STAGE ? 1 : 0][j - hy0][i - hx0] : ld4(d1, i, j); };
+    auto half_n1 = [&](int i, int j) { return STAGE ? hs[0][j - hy0][i - hx0] : ld4_<H>(n1, i, j); };
+    auto half_d1 = [&](int i, int j) { return STAGE ? hs[STAGE ? 1 : 0][j - hy0][i - hx0] : ld4_<H>(d1, i, j); };
     if (x >= out.w || y >= out.h) return;
     const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
-    const float4 p0 = row_f4(d0, y)[x];
+    const float4 p0 = ldt4_<H>(d0, x, y);
     float4 a1, b1;
     // exact 2x: the taps of the samples at px, py (i0, j0; weights a, b); those
     // at px +- nx, py +- ny (the sharpening gradient) sit at texels x/2 and
@@ -791,8 +1009,8 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
         a1 = bilinear4_at(half_n1, i0, j0, a, b);
         b1 = bilinear4_at(half_d1, i0, j0, a, b);
     } else {
-        a1 = bilinear4(n1, px, py);
-        b1 = bilinear4(d1, px, py);
+        a1 = bilinear4<H>(n1, px, py);
+        b1 = bilinear4<H>(d1, px, py);
     }
     float4 d;
     d.x = p0.x - (a1.x - b1.x);
@@ -808,8 +1026,8 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
             dx = (bilinear4_at(half_d1, xh, j0, ax, b).x - bilinear4_at(half_d1, xh - 1, j0, ax, b).x) / 2.0f;
             dy = (bilinear4_at(half_d1, i0, yh, a, by).x - bilinear4_at(half_d1, i0, yh - 1, a, by).x) / 2.0f;
         } else {
-            dx = (bilinear4(d1, px + nx, py).x - bilinear4(d1, px - nx, py).x) / 2.0f;
-            dy = (bilinear4(d1, px, py + ny).x - bilinear4(d1, px, py - ny).x) / 2.0f;
+            dx = (bilinear4<H>(d1, px + nx, py).x - bilinear4<H>(d1, px - nx, py).x) / 2.0f;
+            dy = (bilinear4<H>(d1, px, py + ny).x - bilinear4<H>(d1, px, py - ny).x) / 2.0f;
         }
         const float threshold = 0.25f * sqrtf(nlf.get(0) + nlf.get(3) * b1.x);
         const float detail = smoothstep(0.25f * threshold, threshold, len2(dx, dy)) *
@@ -825,7 +1043,7 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
     } else if (TONE) {
         tone_store(out8, outf, x, y, d, ta);
     } else {
-        wrow_f4(out, y)[x] = d;
+        st4_<H>(out, x, y, d);
     }
 }
 
@@ -845,21 +1063,25 @@ void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& o
         of = tone->outf;
         if (ltm) la = make_ltm_args(tone->ltm_guide, tone->ltm, nullptr);
     }
-#define CLIM_REASSEMBLE(E, T, L) \
-    hipLaunchKernelGGL((k_reassemble<E, T, L>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta, la)
-    if (exact && ltm)
-        CLIM_REASSEMBLE(true, true, true);
-    else if (exact && tone)
-        CLIM_REASSEMBLE(true, true, false);
-    else if (exact)
-        CLIM_REASSEMBLE(true, false, false);
-    else if (ltm)
-        CLIM_REASSEMBLE(false, true, true);
-    else if (tone)
-        CLIM_REASSEMBLE(false, true, false);
-    else
-        CLIM_REASSEMBLE(false, false, false);
+    // the three level images (and out) share their storage
+    with_storage(d0, [&](auto H) {
+#define CLIM_REASSEMBLE(E, T, L)                                                                                  \
+    hipLaunchKernelGGL((k_reassemble<E, T, L, H()>), grd, blk, 0, st, d0, n1, d1, out, sharpening, nlf, o8, of, ta, \
+                       la)
+        if (exact && ltm)
+            CLIM_REASSEMBLE(true, true, true);
+        else if (exact && tone)
+            CLIM_REASSEMBLE(true, true, false);
+        else if (exact)
+            CLIM_REASSEMBLE(true, false, false);
+        else if (ltm)
+            CLIM_REASSEMBLE(false, true, true);
+        else if (tone)
+            CLIM_REASSEMBLE(false, true, false);
+        else
+            CLIM_REASSEMBLE(false, false, false);
 #undef CLIM_REASSEMBLE
+    });
 }
 
 }  // namespace clim

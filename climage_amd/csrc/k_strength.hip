@@ -280,6 +280,7 @@ __device__ __forceinline__ void sn_layer_lds(const uint16_t* in, uint16_t* out, 
     }
 }
 
+template <bool H>
 __global__ void __launch_bounds__(256, 2) k_sn_infer(Img lv, const float* nlf0, const float* nlfg, SnWeights wt,
                                                      float* map, int filter, long long* gate_sum) {
     __shared__ InferSmem sm;
@@ -292,7 +293,7 @@ __global__ void __launch_bounds__(256, 2) k_sn_infer(Img lv, const float* nlf0, 
     // grid-level texels (clamped) under the feature windows: origin (x0 - 6, y0 - 6)
     for (int i = tid; i < kL2 * kL2; i += 256) {
         const int x = clampi(x0 - 6 + i % kL2, 0, gw - 1), y = clampi(y0 - 6 + i / kL2, 0, gh - 1);
-        sm.lv[i] = row_f4(lv, y)[x];
+        sm.lv[i] = ldt4_<H>(lv, x, y);
     }
     __syncthreads();
     // features of the region (x0 - 4, y0 - 4) .. + 24, zero outside the image
@@ -380,7 +381,10 @@ __global__ void __launch_bounds__(256, 2) k_sn_infer(Img lv, const float* nlf0, 
 void launch_sn_infer(const Img& lvg, const float* nlf0, const float* nlfg, const SnWeights& wt, float* map,
                      int filter, long long* gate_sum, hipStream_t st) {
     const int tiles = ((lvg.w + kFT - 1) / kFT) * ((lvg.h + kFT - 1) / kFT);
-    hipLaunchKernelGGL(k_sn_infer, dim3(tiles), dim3(256), 0, st, lvg, nlf0, nlfg, wt, map, filter, gate_sum);
+    with_storage(lvg, [&](auto H) {
+        hipLaunchKernelGGL((k_sn_infer<H()>), dim3(tiles), dim3(256), 0, st, lvg, nlf0, nlfg, wt, map, filter,
+                           gate_sum);
+    });
 }
 
 // ---------------------------------------------------------- weight gradient
@@ -652,14 +656,14 @@ __device__ __forceinline__ float sn_pred(const float* nlf, int c, float y) {
     return fmaxf(nlf[c] + nlf[3 + c] * y, 1e-10f);
 }
 
+template <bool H>
 __global__ void __launch_bounds__(256) k_sn_features(Img lv, const float* nlf0, const float* nlfg, uint16_t* feat) {
     const int x = blockIdx.x * 32 + (threadIdx.x & 31), y = blockIdx.y * 8 + (threadIdx.x >> 5);
     if (x >= lv.w || y >= lv.h) return;
     float s[3] = {0, 0, 0}, q[3] = {0, 0, 0};
     for (int j = -2; j <= 2; j++) {
-        const float4* row = row_f4(lv, clampi(y + j, 0, lv.h - 1));
         for (int i = -2; i <= 2; i++) {
-            const float4 v = row[clampi(x + i, 0, lv.w - 1)];
+            const float4 v = ld4_<H>(lv, x + i, y + j);
             s[0] += v.x; s[1] += v.y; s[2] += v.z;
             q[0] += v.x * v.x; q[1] += v.y * v.y; q[2] += v.z * v.z;
         }
@@ -686,7 +690,9 @@ __global__ void __launch_bounds__(256) k_sn_features(Img lv, const float* nlf0, 
 
 void launch_sn_features(const Img& lvg, const float* nlf0, const float* nlfg, uint16_t* feat, hipStream_t st) {
     const dim3 grd((lvg.w + 31) / 32, (lvg.h + 7) / 8);
-    hipLaunchKernelGGL(k_sn_features, grd, dim3(256), 0, st, lvg, nlf0, nlfg, feat);
+    with_storage(lvg, [&](auto H) {
+        hipLaunchKernelGGL((k_sn_features<H()>), grd, dim3(256), 0, st, lvg, nlf0, nlfg, feat);
+    });
 }
 
 // ------------------------------------------------------------ error curves

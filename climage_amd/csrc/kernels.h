@@ -137,8 +137,9 @@ void burst_tiles(int w, int h, int kind, int* tiles_x, int* tiles);
 // 8), `grab` tiles per take; waves (green: 4, 6 or 8): the register budget
 // the kernel is built for, as waves per SIMD (red/blue + despeckle has one
 // build: its registers allow 4)
-void launch_demosaic_burst(int kind, int waves, const BurstFrame* frames, int total_tiles, BurstQueue* q, int blocks,
-                           int grab, hipStream_t st);
+// half: the YCbCr images are FP16 (the mixed-precision form)
+void launch_demosaic_burst(int kind, int waves, bool half, const BurstFrame* frames, int total_tiles, BurstQueue* q,
+                           int blocks, int grab, hipStream_t st);
 
 // ------------------------------------------------------------- pyramid stages
 void launch_downsample(const Img& in, const Img& out, hipStream_t st);

@@ -40,21 +40,24 @@ static ParamRef dev_params(const float* p) {
     return r;
 }
 
+// half_levels: the level images in FP16 (the mixed-precision pipeline; the
+// stand-alone pyramid entry points take the caller's FP32 level 0)
 static clim_status ensure_workspace(clim_ctx* ctx, int w, int h, int levels, int algorithm,
-                                   bool scaled_plane = false) {
+                                   bool scaled_plane = false, bool half_levels = false) {
     Workspace& ws = ctx->ws;
+    const int lf = half_levels ? CLIM_FMT_RGBA16F : CLIM_FMT_RGBA32F;
     if (scaled_plane) CLIM_TRY(owned_alloc(ctx, ws.scaled, w, h, CLIM_FMT_R32F));
     CLIM_TRY(owned_alloc(ctx, ws.green, w, h, CLIM_FMT_R32F));
-    CLIM_TRY(owned_alloc(ctx, ws.rgbA, w, h, CLIM_FMT_RGBA32F));
-    CLIM_TRY(owned_alloc(ctx, ws.rgbB, w, h, CLIM_FMT_RGBA32F));
+    CLIM_TRY(owned_alloc(ctx, ws.rgbA, w, h, lf));
+    CLIM_TRY(owned_alloc(ctx, ws.rgbB, w, h, lf));
     CLIM_TRY(fit_list_alloc(ctx, ws.raw_mean, true, w, h));
     CLIM_TRY(fit_list_alloc(ctx, ws.raw_var, true, w, h));
     for (int i = 0, scale = 1; i < levels; i++, scale *= 2) {
         const int lw = w / scale, lh = h / scale;
         CLIM_TRY(fit_list_alloc(ctx, ws.stats[i], false, lw, lh));
         if (i > 0) {
-            CLIM_TRY(owned_alloc(ctx, ws.pyr[i], lw, lh, CLIM_FMT_RGBA32F));
-            CLIM_TRY(owned_alloc(ctx, ws.den[i], lw, lh, CLIM_FMT_RGBA32F));
+            CLIM_TRY(owned_alloc(ctx, ws.pyr[i], lw, lh, lf));
+            CLIM_TRY(owned_alloc(ctx, ws.den[i], lw, lh, lf));
         }
     }
     ws.w = w;
@@ -189,7 +192,9 @@ static clim_status check_image(clim_ctx* ctx, const clim_image* im, int format, 
     return CLIM_OK;
 }
 
-static Img view(const clim_image* im) { return Img{(char*)im->data, im->width, im->height, im->pitch_bytes}; }
+static Img view(const clim_image* im) {
+    return Img{(char*)im->data, im->width, im->height, im->pitch_bytes, im->format == CLIM_FMT_RGBA16F ? 1 : 0};
+}
 
 extern "C" clim_status clim_pyramidal_denoise(clim_ctx* ctx, const clim_image* image, int32_t levels,
                                               int32_t algorithm, const clim_denoise_params* params, clim_image* out,
@@ -304,7 +309,8 @@ clim_status frame_prepare(clim_ctx* ctx, const clim_image* raw, clim_image* out_
 }
 
 clim_status frame_workspace(clim_ctx* ctx, const FrameCall& f) {
-    return ensure_workspace(ctx, f.w, f.h, f.p->levels, f.p->denoiseAlgorithm, f.high_noise);
+    return ensure_workspace(ctx, f.w, f.h, f.p->levels, f.p->denoiseAlgorithm, f.high_noise,
+                            ctx->precision == CLIM_PRECISION_MIXED);
 }
 
 clim_status frame_raw_fit(clim_ctx* ctx, FrameCall& f, clim::FitState* st_out) {
@@ -534,10 +540,12 @@ extern "C" clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const cli
         for (int k = 1; k < nl; k++)
             CLIM_TRY_HIP(ctx, hipStreamWaitEvent(ctx->lanes[k - 1]->stream, ctx->lane_ev[0], 0));
     }
-    // the lanes run the learned strength as this context does, on its weights
+    // the lanes run the learned strength as this context does, on its
+    // weights, and at its precision
     for (int k = 1; k < nl; k++) {
         ctx->lanes[k - 1]->sn.mode = ctx->sn.mode;
         ctx->lanes[k - 1]->sn.use = ctx->sn.use;
+        ctx->lanes[k - 1]->precision = ctx->precision;
     }
     clim_status st = CLIM_OK;
     for (int i = 0; i < n && st == CLIM_OK; i++) {

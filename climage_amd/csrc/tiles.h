@@ -21,8 +21,10 @@ struct LevelTile {
 
     // the tile's texels in registers (fetch, then put)
     using Regs = Staged<SX * SY, 256, float4>;
+    // H: the level is stored as FP16
+    template <bool H = false>
     __device__ __forceinline__ static void fetch(Regs& r, const Img& in, int x0, int y0, int tid) {
-        r.fetch(tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); });
+        r.fetch(tid, [&](int i) { return ld4_<H>(in, x0 - 2 + i % SX, y0 - 2 + i / SX); });
     }
     __device__ __forceinline__ void put(const Regs& r, int tid) {
         r.put(tid, [&](int i, float4 p) {
@@ -31,9 +33,10 @@ struct LevelTile {
             z[i / SX][i % SX] = p.z;
         });
     }
+    template <bool H = false>
     __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
         Regs r;
-        fetch(r, in, x0, y0, tid);
+        fetch<H>(r, in, x0, y0, tid);
         put(r, tid);
     }
 };
@@ -44,15 +47,17 @@ template <int TX, int TY>
 struct LevelTile4 {
     static constexpr int SX = TX + 4, SY = TY + 4;
     float4 p[SY][SX];
+    template <bool H = false>
     __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
         stage<SX * SY, 256>(
-            tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
+            tid, [&](int i) { return ld4_<H>(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
             [&](int i, float4 v) { p[i / SX][i % SX] = v; });
     }
     // the tile's texels in registers (fetch, then put)
     using Regs = Staged<SX * SY, 256, float4>;
+    template <bool H = false>
     __device__ __forceinline__ static void fetch(Regs& r, const Img& in, int x0, int y0, int tid) {
-        r.fetch(tid, [&](int i) { return ld4(in, x0 - 2 + i % SX, y0 - 2 + i / SX); });
+        r.fetch(tid, [&](int i) { return ld4_<H>(in, x0 - 2 + i % SX, y0 - 2 + i / SX); });
     }
     __device__ __forceinline__ void put(const Regs& r, int tid) {
         r.put(tid, [&](int i, float4 v) { p[i / SX][i % SX] = v; });

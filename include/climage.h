@@ -48,7 +48,8 @@ typedef enum {
     CLIM_FMT_R32F = 2,      /* float luma */
     CLIM_FMT_RGBA32F = 3,   /* float4 */
     CLIM_FMT_RGBA8 = 4,     /* UNORM8 x4 */
-    CLIM_FMT_RG32F = 5      /* float2 (guided-filter scratch) */
+    CLIM_FMT_RG32F = 5,     /* float2 (guided-filter scratch) */
+    CLIM_FMT_RGBA16F = 6    /* half4: the mixed-precision pyramid's level storage */
 } clim_format;
 
 /* device image, pitch-linear; pitch_bytes >= width * texel size */
@@ -133,6 +134,21 @@ const char* clim_status_string(clim_status s);
 clim_status clim_sync(clim_ctx* ctx);
 /* the context's stream, as a hipStream_t cast to void* */
 void* clim_stream(clim_ctx* ctx);
+
+/* ---------------------------------------------------------- precision
+ * CLIM_PRECISION_FP32 (default): every stage in FP32 (the FP64 fits and 3x3
+ * inversions aside), bit for bit the FP32 restatement of the reference.
+ * CLIM_PRECISION_MIXED: the level images of the pyramid (the despeckled
+ * YCbCr level 0, the downsampled and the denoised levels) are stored as FP16
+ * (CLIM_FMT_RGBA16F), as the reference's ARM build stores every intermediate
+ * image (src/gls_image.hpp:32,189-210), and the bilateral denoiser's range
+ * test and weights run in packed FP16 with the weighted sums accumulated in
+ * FP32; the noise-model regressions (FP64 over FP32 statistics), the colour
+ * matrices (FP64) and the other stages' arithmetic stay as in the FP32 form.
+ * Applies to the pipeline calls (clim_raw_to_rgb*, bursts created later) on
+ * this context. */
+typedef enum { CLIM_PRECISION_FP32 = 0, CLIM_PRECISION_MIXED = 1 } clim_precision;
+clim_status clim_set_precision(clim_ctx* ctx, int32_t precision);
 
 /* -------------------------------------------------- images and copies
  * cl_image_2d constructor / copyPixelsFrom / copyPixelsTo

@@ -1399,16 +1399,37 @@ typedef struct {
  * levels whose fit fell back. */
 static int P(pyramidal_denoise_s)(const real* image, int w, int h, int levels, int algorithm,
                                   const o_denoise_params* dp, float* nlf_inout, real* out, real* level3_out,
-                                  const o_strength* sm);
+                                  const o_strength* sm, int half);
 int P(pyramidal_denoise)(const real* image, int w, int h, int levels, int algorithm,
                          const o_denoise_params* dp, float* nlf_inout, real* out, real* level3_out) {
-    return P(pyramidal_denoise_s)(image, w, h, levels, algorithm, dp, nlf_inout, out, level3_out, NULL);
+    return P(pyramidal_denoise_s)(image, w, h, levels, algorithm, dp, nlf_inout, out, level3_out, NULL, 0);
+}
+
+/* FP16 storage (the mixed-precision form's level images, and the reference's
+ * ARM build, src/gls_image.hpp:32,189-210): each value rounded to the
+ * nearest IEEE half, ties to even, with subnormals, overflow to infinity. */
+static real P(half_round)(real x) {
+    if (!(x == x) || x == 0) return x;
+    const real ax = x < 0 ? -x : x;
+    if (ax >= K(65520.0)) return x < 0 ? -(real)INFINITY : (real)INFINITY;
+    int e;
+    frexp((double)ax, &e);                 /* ax in [2^(e-1), 2^e) */
+    if (e < -13) e = -13;                  /* below 2^-14: the subnormal step 2^-24 */
+    const double ulp = ldexp(1.0, e - 11);
+    const double r = nearbyint((double)ax / ulp) * ulp;
+    return (real)(x < 0 ? -r : r);
+}
+void P(half_round_img)(real* img, size_t values) {
+    for (size_t i = 0; i < values; i++) img[i] = P(half_round)(img[i]);
 }
 /* sm (nullable): the learned strength of the level filters (Bilateral and
  * GuidedFast; GuidedPrecise stays plain) */
+/* half: the level images are stored as FP16 (the downsampled levels, the
+ * denoised levels and the reassembled levels above 0; level 0's reassembly
+ * goes straight to the tone map) */
 static int P(pyramidal_denoise_s)(const real* image, int w, int h, int levels, int algorithm,
                                   const o_denoise_params* dp, float* nlf_inout, real* out, real* level3_out,
-                                  const o_strength* sm) {
+                                  const o_strength* sm, int half) {
     real* pyr[5] = {0};
     real* den[5] = {0};
     int lw[6], lh[6];
@@ -1424,7 +1445,10 @@ static int P(pyramidal_denoise_s)(const real* image, int w, int h, int levels, i
 
     float cal[5][6];
     for (int i = 0; i < levels; i++) {
-        if (i < levels - 1) P(downsample)(lvl[i], lw[i], lh[i], pyr[i + 1], lw[i + 1], lh[i + 1]);
+        if (i < levels - 1) {
+            P(downsample)(lvl[i], lw[i], lh[i], pyr[i + 1], lw[i + 1], lh[i + 1]);
+            if (half) P(half_round_img)(pyr[i + 1], (size_t)lw[i + 1] * lh[i + 1] * 4);
+        }
         real* st = (real*)malloc((size_t)lw[i] * lh[i] * 4 * sizeof(real));
         P(noise_statistics)(lvl[i], lw[i], lh[i], st);
         float fit[6];
@@ -1448,10 +1472,12 @@ static int P(pyramidal_denoise_s)(const real* image, int w, int h, int levels, i
         } else {
             P(denoise_guided_fast_s)(lvl[i], lw[i], lh[i], np, np + 3, sm, i, den[i]);
         }
+        if (half) P(half_round_img)(den[i], (size_t)lw[i] * lh[i] * 4);
         if (i < levels - 1) {
             const float nl[2] = {np[0], np[3]};
             P(reassemble)(den[i], lvl[i + 1], den[i + 1], lw[i], lh[i], lw[i + 1], lh[i + 1],
                           dp[i].sharpening, nl, den[i]);
+            if (half && i > 0) P(half_round_img)(den[i], (size_t)lw[i] * lh[i] * 4);
         }
     }
     /* denoisedImagePyramid[3], the local tone mapping guide */
@@ -1474,6 +1500,7 @@ typedef struct {
     float ltm[4];          /* guidedFilterEps, shadows, highlights, detail */
     float noiseLevel;      /* > 0.6: the high-noise raw branch */
     int rotate_180;        /* RawConverter::demosaicImage's rotate_180 */
+    int fp16_storage;      /* the level images stored as FP16 (the mixed-precision form's storage) */
 } o_pipeline_params;
 
 /* Full path.  nlf_out: rawNlf[4] then levels x 6.  Optional stage dumps
@@ -1553,7 +1580,11 @@ int P(raw_to_rgb_strength)(const uint16_t* raw, int w, int h, const o_pipeline_p
     float c2y[9], y2c[9];
     P(cam_ycbcr)(pp->rgb_cam, c2y);
     P(transform)(rgbA, w, h, c2y, rgbA);
+    /* the rotated YCbCr image is stored before its despeckle (the unrotated
+     * one goes to the despeckle without a store) */
+    if (pp->fp16_storage && pp->rotate_180) P(half_round_img)(rgbA, n * 4);
     P(despeckle)(rgbA, w, h, pp->pyramidNlf[0], pp->pyramidNlf[0] + 3, rgbB);
+    if (pp->fp16_storage) P(half_round_img)(rgbB, n * 4);
     if (ycc_dump) memcpy(ycc_dump, rgbB, n * 4 * sizeof(real));
 
     float nlf[5 * 6];
@@ -1562,7 +1593,7 @@ int P(raw_to_rgb_strength)(const uint16_t* raw, int w, int h, const o_pipeline_p
     const int gw = w >> 3, gh = h >> 3;  /* w/2/2/2 with truncation at every level */
     real* guide = ltm ? (real*)malloc((size_t)gw * gh * 4 * sizeof(real)) : NULL;
     fallbacks |= P(pyramidal_denoise_s)(rgbB, w, h, pp->levels, pp->algorithm, pp->dp, nlf, rgbA, guide,
-                                        map ? &strength : NULL);
+                                        map ? &strength : NULL, pp->fp16_storage);
     if (den_dump) memcpy(den_dump, rgbA, n * 4 * sizeof(real));
 
     if (P(inverse3x3)(c2y, (double)pp->exposure_multiplier, y2c)) fallbacks |= 1 << 9;

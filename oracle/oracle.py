@@ -46,7 +46,7 @@ class PipelineParams(C.Structure):
                 ("contrast", C.c_float), ("saturation", C.c_float), ("toneCurveSlope", C.c_float),
                 ("exposureBias", C.c_float), ("blacks", C.c_float), ("levels", C.c_int),
                 ("algorithm", C.c_int), ("localToneMapping", C.c_int), ("ltm", C.c_float * 4),
-                ("noiseLevel", C.c_float), ("rotate_180", C.c_int)]
+                ("noiseLevel", C.c_float), ("rotate_180", C.c_int), ("fp16_storage", C.c_int)]
 
 
 # pyramidNlf row 0 of cpp/raw_pipeline.cpp:105-111 ("minimal noise model")
@@ -319,6 +319,12 @@ class Oracle:
         self._fn("inverse3x3").argtypes = [C.c_void_p, C.c_double, C.c_void_p]
         rc = self._fn("inverse3x3")(_f(np.asarray(m).ravel()), C.c_double(scale), out)
         return rc, np.array(out[:], np.float32).reshape(3, 3)
+
+    def half_round(self, arr):
+        """Values rounded to FP16 storage (the mixed-precision form's level images)."""
+        a = np.array(arr, dtype=self.dt, copy=True)
+        self._fn("half_round_img")(_ptr(a), C.c_size_t(a.size))
+        return a
 
     def pyramidal_denoise(self, img, levels, algorithm, params, nlf_in, level3=False):
         """Returns (fallback mask, denoised level 0, fitted NLF) and, with
