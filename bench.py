@@ -24,6 +24,10 @@ Prints one JSON line:
 * learned: the same frames with the learned denoise strength (the committed
   weights, gated), its rate, the strength stage's cost, the gates and the
   PSNR gain against the clean truth on one frame;
+* mixed: the mixed-precision form (FP16 level storage, packed-FP16
+  bilateral weights with FP32 sums) on the same frames, its error against
+  the FP64 and FP16-storage restatements, and config 2's 48 MP 14-bit frame
+  at both precisions;
 * burst: the uneven 16-frame burst recorded once as a HIP graph and
   replayed: Mpix/s, the frames' completion times within a replay (p50/p99)
   and the autotuner's choice for its cross-frame launches;
@@ -245,6 +249,84 @@ def burst_leg(ctx, replays=20):
                     "green and red/blue + despeckle stages as two cross-frame persistent launches"}
 
 
+# the path's algorithmic bytes per pixel with every float texel stored as FP16
+# (the reference's ARM build), five levels
+PATH_BYTES_FP16 = 199.1
+
+
+def mixed_leg(ctx, raws, outs, fparams, steps, mpix_fp32):
+    """The mixed-precision form (CLIM_PRECISION_MIXED) on the bench frames:
+    Mpix/s batched, its first frame against the FP64 and the FP16-storage
+    FP32 restatements, and config 2's geometry (8064x6048 RGGB, 14-bit, one
+    frame) at both precisions."""
+    from oracle import oracle as O
+    ctx.set_precision(ca.PRECISION_MIXED)
+    try:
+        for _ in range(2):
+            ctx.raw_to_rgb_batch(raws, outs, fparams)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.raw_to_rgb_batch(raws, outs, fparams)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        px = sum(r.width * r.height for r in raws)
+        mpix = steps * px / dt / 1e6
+        # the first frame's error (outside the timing)
+        w, h = raws[0].width, raws[0].height
+        of = ctx.image(w, h, ca.FMT_RGBA32F)
+        raw_nlf, pyr_nlf = ctx.raw_to_rgb(raws[0], outs[0], fparams[0], out_f32=of)
+        got = of.download()
+        of.free()
+        host = raws[0].download()
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+        p = O.default_params(ca.RGGB, 64.0, 4095.0, levels=5)
+        r64 = O.Oracle(double=True).raw_to_rgb(host, p)
+        p.fp16_storage = 1
+        r16 = O.Oracle(double=False).raw_to_rgb(host, p)
+
+        def err(ref):
+            d = got[..., :3].astype(np.float64) - ref["rgbf"][..., :3]
+            mse = float(np.mean(d * d))
+            return {"max_abs": round(float(np.abs(d).max()), 5), "psnr_db": round(10 * np.log10(1.0 / max(mse, 1e-30)), 2),
+                    "raw_nlf_max_rel": float(np.max(np.abs(raw_nlf - ref["rawNlf"]) / np.abs(ref["rawNlf"]))),
+                    "pyramid_nlf_max_rel": float(np.max(np.abs(pyr_nlf - ref["pyramidNlf"]) /
+                                                        np.abs(ref["pyramidNlf"])))}
+        res = {"value": round(mpix, 1), "unit": "Mpix/s", "steps": steps, "vs_fp32": round(mpix / mpix_fp32, 4),
+               "storage": "FP16 level images (despeckled level 0, downsampled and denoised levels)",
+               "arithmetic": "bilateral range test and weights in packed FP16, sums in FP32 (v_dot2_f32_f16); "
+                             "FP64 fits and colour matrices",
+               "error_first_frame": {"vs_fp64": err(r64), "vs_fp16_storage_fp32": err(r16)},
+               "path_roofline_fp16_accounting": {"bytes_per_px": PATH_BYTES_FP16,
+                                                 "achieved": round(PATH_BYTES_FP16 * mpix * 1e6 / 1e9, 1),
+                                                 "frac": round(PATH_BYTES_FP16 * mpix * 1e6 / 1e9 / HBM_PEAK_GBS, 4)}}
+    finally:
+        ctx.set_precision(ca.PRECISION_FP32)
+    # config 2: one 48 MP 14-bit frame per call, both precisions
+    w2, h2 = 8064, 6048
+    r2 = ctx.image(w2, h2, ca.FMT_R16U)
+    ctx.generate_bayer(r2, ca.RGGB, 1000, 2000, 0.002, 1e-5, 64.0, 16383.0)
+    o2 = ctx.image(w2, h2, ca.FMT_RGBA8)
+    p2 = ca.default_params(ca.RGGB, 64.0, 16383.0, levels=5)
+    c2 = {"workload": "8064x6048 RGGB uint16 14-bit (white 16383), one frame per call"}
+    for name, prec in (("fp32", ca.PRECISION_FP32), ("mixed", ca.PRECISION_MIXED)):
+        ctx.set_precision(prec)
+        for _ in range(2):
+            ctx.raw_to_rgb(r2, o2, p2, writeback=False)
+        ctx.sync()
+        n2 = max(steps // 2, 4)
+        t0 = time.perf_counter()
+        for _ in range(n2):
+            ctx.raw_to_rgb(r2, o2, p2, writeback=False)
+        ctx.sync()
+        c2[name] = round(n2 * w2 * h2 / (time.perf_counter() - t0) / 1e6, 1)
+    ctx.set_precision(ca.PRECISION_FP32)
+    for im in (r2, o2):
+        im.free()
+    res["config2_48mp"] = c2
+    return res
+
+
 def psnr(a, b):
     d = a[..., :3].astype(np.float64) - b[..., :3]
     return 10.0 * np.log10(1.0 / max(float(np.mean(d * d)), 1e-30))
@@ -261,6 +343,7 @@ def main():
     ap.add_argument("--no-parity", action="store_true", help="skip the FP64 / FP32 CPU comparison of one frame")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-strength leg")
     ap.add_argument("--no-burst", action="store_true", help="skip the recorded-burst leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--learned", action="store_true",
                     help="the learned denoise strength (committed weights, gated) is the headline path")
     ap.add_argument("--ltm", action="store_true",
@@ -503,7 +586,13 @@ def main():
     if not (args.no_burst or args.preview or args.burst or args.learned):
         burst = burst_leg(ctx)
 
-    # 5. the first frame against the CPU restatements (outside any timing)
+    # 5. the mixed-precision form on the same frames, and config 2's geometry
+    mixed = None
+    if not (args.no_mixed or args.preview or args.burst or args.learned or args.ltm or args.noise_level > 0.6
+            or algorithm != ca.BILATERAL):
+        mixed = mixed_leg(ctx, raws, outs, fparams, max(args.steps // 4, 10), mpix)
+
+    # 6. the first frame against the CPU restatements (outside any timing)
     parity = None
     if not (args.no_parity or args.preview or args.burst or args.learned or args.ltm or args.noise_level > 0.6):
         of = ctx.image(w, h, ca.FMT_RGBA32F)
@@ -551,6 +640,7 @@ def main():
         "fp64": parity,
         "learned": learned,
         "burst": burst,
+        "mixed": mixed,
         "cpu_baseline": cpu,
         "stages": stages,
     }

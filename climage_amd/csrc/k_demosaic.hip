@@ -187,20 +187,21 @@ namespace {
 constexpr int GT_X = 32, GT_Y = 16;
 constexpr int GR_X = GT_X + 6, GR_Y = GT_Y + 6;   // raw tile
 constexpr int GS_X = GT_X + 4, GS_Y = GT_Y + 4;   // sobel tile
-// Row length of the LDS tiles: a wave reads four tile rows, each at every
-// second column (one non-green pixel per pair) with the column parity
-// alternating by row.  48-float rows put rows two apart 96 words (half the
-// 64 banks) apart and neighbouring rows on banks of the other parity, so the
-// 64 reads of a tap meet 64 distinct banks.
-constexpr int G_RS = 48;
-static_assert(G_RS >= GR_X && G_RS >= GS_X, "tile rows fit");
+// The LDS tiles are unpadded (rows of 38 and 36 floats).  LDS serves 32
+// lanes at a time from 32 four-byte banks.  The staging and Sobel passes
+// walk the tiles in order, so 32 consecutive lanes meet 32 consecutive
+// words.  In the interpolation pass a half-wave reads two tile rows, each
+// at every second column (one non-green pixel per pair) with the column
+// parity alternating by row: the two rows' words are 38 +- 1 or 36 +- 1
+// apart, an odd number of banks, so one row meets the even banks and the
+// other the odd ones.
 }
 
 struct GreenLds {
-    float r[GR_Y][G_RS];
-    float sgx[GS_Y][G_RS];
-    float sgy[GS_Y][G_RS];
-    float sgl[GS_Y][G_RS];
+    float r[GR_Y][GR_X];
+    float sgx[GS_Y][GS_X];
+    float sgy[GS_Y][GS_X];
+    float sgl[GS_Y][GS_X];
 };
 
 // one GT_X x GT_Y tile at (x0, y0) by the block's 256 threads (tid); returns
@@ -221,8 +222,13 @@ __device__ __forceinline__ void green_tile(const MosaicSrc<U16>& raw, const Img&
     // 9-tap sums in their order with the zero taps left out and the +-1, +-2
     // taps as exact adds: every partial sum is the same value (up to the sign
     // of a zero, which the abs() removes).
+    // items (sx, sy): the first 32 columns with one row per half-wave (32
+    // consecutive words per access), then the last GS_X - 32 columns
+    static_assert(GS_X > 32 && 32 * GS_Y + (GS_X - 32) * GS_Y <= 4 * 256, "four rounds");
     for (int i = tid; i < GS_X * GS_Y; i += 256) {
-        const int sy = i / GS_X, sx = i % GS_X;
+        constexpr int kMain = 32 * GS_Y, kRest = GS_X - 32;
+        const int sy = i < kMain ? i >> 5 : (i - kMain) / kRest;
+        const int sx = i < kMain ? i & 31 : 32 + (i - kMain) % kRest;
         const float s00 = r[sy][sx], s01 = r[sy][sx + 1], s02 = r[sy][sx + 2];
         const float s10 = r[sy + 1][sx], s12 = r[sy + 1][sx + 2];
         const float s20 = r[sy + 2][sx], s21 = r[sy + 2][sx + 1], s22 = r[sy + 2][sx + 2];
@@ -614,10 +620,12 @@ constexpr int RD_QPT = (RD_QX * RD_QY + 255) / 256;  // quads per thread
 // A window row in LDS: its even and its odd columns as two planes of
 // RD_PS floats, RD_WS floats per row.  Quad (qx, qy) reads column 2 qx + i
 // of row 2 qy + j, i.e. word (2 qy + j) RD_WS + (i & 1) RD_PS + qx + i / 2;
-// two rows are 162 words, 34 mod 64, so the bank is q mod 64 plus a
-// constant (q = 34 qy + qx) and a wave's 64 consecutive quads meet 64
-// distinct banks.
-constexpr int RD_PS = 40, RD_WS = 81;
+// two rows are 194 words, 2 mod 32, so the bank is q mod 32 plus a
+// constant (q = 34 qy + qx) and the 32 consecutive quads LDS serves at a
+// time meet 32 distinct banks.  The staging pass writes consecutive
+// columns, alternately to the two planes, 48 words (16 banks) apart: the
+// even lanes meet one half of the banks and the odd lanes the other.
+constexpr int RD_PS = 48, RD_WS = 97;
 static_assert(RD_WX / 2 <= RD_PS && RD_PS + (RD_WX - 1) / 2 < RD_WS && RD_QX == 34, "window planes");
 __device__ __forceinline__ int rd_col(int i) { return (i & 1) * RD_PS + (i >> 1); }
 }
@@ -627,6 +635,7 @@ union RbDespeckleLds {
         float r[RD_WY][RD_WS], g[RD_WY][RD_WS];
     } win;
     float4 t[RD_RY][RD_RX];
+    __device__ __forceinline__ float4& at(int y, int x) { return t[y][x]; }
 };
 
 // one RD_X x RD_Y output tile at (x0, y0) by the block's 256 threads; as
@@ -684,7 +693,7 @@ __device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, con
         for (int dy = 0; dy < 2; dy++)
 #pragma unroll
             for (int dx = 0; dx < 2; dx++)
-                sm.t[2 * qy + dy][2 * qx + dx] = make_float4(o[n][dy][dx].x, o[n][dy][dx].y, o[n][dy][dx].z, 0.0f);
+                sm.at(2 * qy + dy, 2 * qx + dx) = make_float4(o[n][dy][dx].x, o[n][dy][dx].y, o[n][dy][dx].z, 0.0f);
     }
     __syncthreads();
     // every 3x3 neighbourhood of an interior tile lies inside the image: no clamps
@@ -693,7 +702,7 @@ __device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, con
         for (int i = tid; i < RD_X * RD_Y; i += 256) {
             const int lx = i % RD_X, ly = i / RD_X;
             st4_<H>(out, x0 + lx, y0 + ly,
-                    despeckle_px<true>([&](int dx, int dy) { return sm.t[ly + 2 + dy][lx + 2 + dx]; }, var_a, var_b));
+                    despeckle_px<true>([&](int dx, int dy) { return sm.at(ly + 2 + dy, lx + 2 + dx); }, var_a, var_b));
         }
         return;
     }
@@ -702,7 +711,7 @@ __device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, con
         if (x >= out.w || y >= out.h) continue;
         auto fetch = [&](int dx, int dy) {
             const int gx = clampi(x + dx, 0, out.w - 1), gy = clampi(y + dy, 0, out.h - 1);
-            return sm.t[gy - ry0][gx - rx0];
+            return sm.at(gy - ry0, gx - rx0);
         };
         st4_<H>(out, x, y, nonan ? despeckle_px<true>(fetch, var_a, var_b) : despeckle_px<false>(fetch, var_a, var_b));
     }
