@@ -67,22 +67,24 @@ def path_bytes_per_px(levels):
     return 206 + 60 * S(levels - 2) + 80 * S(levels - 1)
 
 
-def stage_bytes(stage, texels, preview=False):
+def stage_bytes(stage, texels, preview=False, half=False):
     """Algorithmic bytes of one stage: each input read once, each output written
-    once, FP32 texels (luma 4 B, RGBA 16 B), raw 2 B, RGBA8 4 B."""
+    once, FP32 texels (luma 4 B, RGBA 16 B; the level images' RGBA texels 8 B
+    with half, the mixed-precision form's FP16 storage), raw 2 B, RGBA8 4 B."""
+    L = 8 if half else 16
     per = {
         "scale_raw": 2 + 4,             # u16 in, f32 out (per pixel; stand-alone operator only)
         "raw_stats": 8 + 32,            # 4 u16 texels in, mean+var RGBA out (per quad)
         "raw_fit": 32,                  # the refit pass over mean+var (per quad)
         "green": 2 + 4,                 # u16 mosaic in (scaled on the fly), f32 green out
-        "red_blue_ycc": 2 + 4 + 16,     # + highlights, YCbCr and despeckle in the same kernel
+        "red_blue_ycc": 2 + 4 + L,      # + highlights, YCbCr and despeckle in the same kernel
         "despeckle": 16 + 16,
-        "downsample": 4 * 16 + 16,      # per output texel: 4 input texels + 1 output
-        "level_stats": 16 + 16,
+        "downsample": 4 * L + L,        # per output texel: 4 input texels + 1 output
+        "level_stats": L + 16,          # the level in, the statistics (FP32 RGBA) out
         "level_fit": 16,                # the refit pass over the statistics image
-        "denoise": 16 + 16,
-        "reassemble": 16 + 16 + 16 / 4 + 16 / 4,
-        "tone": 16 + 16 / 4 + 16 / 4 + 4,  # level-0 reassembly fused with the tone map: RGBA8 out
+        "denoise": L + L,
+        "reassemble": L + L + L / 4 + L / 4,
+        "tone": L + L / 4 + L / 4 + 4,  # level-0 reassembly fused with the tone map: RGBA8 out
         "raw_denoise": 8 + 16,          # 4 u16 texels in, 4 FP32 texels out (per quad)
     }
     if preview:
@@ -94,10 +96,15 @@ def stage_bytes(stage, texels, preview=False):
 # level), so that their event timing is that kernel's launch duration; the
 # names are the kernel instances this workload runs (as rocprofv3 lists them)
 SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green": "k_green",
-                 "red_blue_ycc": "k_rb_despeckle<true>", "despeckle": "k_despeckle",
-                 "downsample": "k_downsample<true>",
-                 "denoise": "k_bilateral_levels<true>", "reassemble": "k_reassemble<true, false, false>",
-                 "tone": "k_reassemble<true, true, false>", "raw_denoise": "k_raw_rgba_filter<2, true>"}
+                 "red_blue_ycc": "k_rb_despeckle<true, false>", "despeckle": "k_despeckle",
+                 "downsample": "k_downsample<true, false>",
+                 "denoise": "k_bilateral_levels<true>", "reassemble": "k_reassemble<true, false, false, false>",
+                 "tone": "k_reassemble<true, true, false, false>", "raw_denoise": "k_raw_rgba_filter<2, true>"}
+# the same stages' kernels in the mixed-precision form
+SINGLE_KERNEL_MIXED = dict(SINGLE_KERNEL, **{
+    "red_blue_ycc": "k_rb_despeckle<true, true>", "downsample": "k_downsample<true, true>",
+    "denoise": "k_bilateral_levels_mixed<true>", "reassemble": "k_reassemble<true, false, false, true>",
+    "tone": "k_reassemble<true, true, false, true>"})
 
 
 def cpu_model():
@@ -169,7 +176,7 @@ def measured_traffic(kernel, w, h):
     if (w, h) != (4032, 3024) or not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)["kernels"].get(kernel)
+        return json.load(f)["per_dispatch"].get(kernel)
 
 
 def measured_frame_bytes(w, h):
@@ -355,6 +362,12 @@ def main():
     ap.add_argument("--burst", action="store_true",
                     help="the uneven burst: 16 frames, 4 each of 4032x3024, 4000x3000, 3264x2448 and 1920x1080, "
                          "RGGB and BGGR alternating, noise levels drawn from the headline set, one context")
+    ap.add_argument("--graph", action="store_true",
+                    help="with --burst: the burst recorded once as a HIP graph (clim_burst_create) and replayed, "
+                         "one graph launch per step")
+    ap.add_argument("--mixed", action="store_true",
+                    help="the mixed-precision form (FP16 level storage, packed-FP16 bilateral weights) as the "
+                         "timed path")
     ap.add_argument("--sequential", action="store_true",
                     help="one clim_raw_to_rgb call per frame instead of clim_raw_to_rgb_batch over the step's frames")
     ap.add_argument("--algorithm", choices=list(ALGORITHMS), default="bilateral",
@@ -371,11 +384,16 @@ def main():
     if args.burst and args.preview:
         print("bench.py: --burst runs the full path", file=sys.stderr)
         sys.exit(2)
+    if args.graph and not args.burst:
+        print("bench.py: --graph records the --burst frames", file=sys.stderr)
+        sys.exit(2)
 
     w, h, bayer = args.width, args.height, ca.RGGB
     algorithm = ALGORITHMS[args.algorithm]
     lib = ca.lib()
     ctx = ca.Context(0)
+    if args.mixed:
+        ctx.set_precision(ca.PRECISION_MIXED)
     # frames: (width, height, CFA, noise a, frame id)
     if args.burst:
         sizes = [(4032, 3024), (4000, 3000), (3264, 2448), (1920, 1080)]
@@ -394,13 +412,13 @@ def main():
         fp.noiseLevel = args.noise_level
         fparams.append(fp)
     px_per_step = sum(fw * fh for fw, fh, _, _, _ in frames)
-    single_kernel = dict(SINGLE_KERNEL)
+    single_kernel = dict(SINGLE_KERNEL_MIXED if args.mixed else SINGLE_KERNEL)
     if args.ltm:
-        single_kernel["tone"] = "k_reassemble<true, true, true>"
+        single_kernel["tone"] = f"k_reassemble<true, true, true, {'true' if args.mixed else 'false'}>"
     if args.preview:
         single_kernel["tone"] = "k_fast_preview<true>"
     if algorithm == ca.GUIDED_FAST:
-        single_kernel["denoise"] = "k_guided_fast"
+        single_kernel["denoise"] = f"k_guided_fast<{'true' if args.mixed else 'false'}>"
     elif algorithm == ca.GUIDED_PRECISE:
         del single_kernel["denoise"]  # seven kernels per level
     learned_on = not args.preview and not args.no_learned
@@ -409,8 +427,12 @@ def main():
     if args.learned:
         ctx.strength_set_mode(ca.STRENGTH_GATED)
     ctx.sync()
+    graph = ca.Burst(ctx, raws, outs, fparams, frame_times=True) if args.graph else None
 
     def step(batched=not args.sequential):
+        if batched and graph is not None:
+            graph.launch()
+            return
         if batched and not args.preview:
             ctx.raw_to_rgb_batch(raws, outs, fparams)
             return
@@ -481,7 +503,16 @@ def main():
         copy_calibration(256 << 20)
 
     latency = None
-    if args.burst:
+    if graph is not None:
+        # each frame's completion within a replay (events in the graph)
+        lat = []
+        for _ in range(5):
+            graph.launch()
+            lat.extend(graph.frame_times().tolist())
+        latency = {"p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                   "frames": len(lat), "note": "frame completion from the replay's start",
+                   "autotune": graph.tuning()}
+    elif args.burst:
         # per-frame latency: each frame launched on an idle device and waited for
         lat = []
         for _ in range(3):
@@ -494,35 +525,37 @@ def main():
                    "frames": len(lat)}
     roofline = None
     if prof_on:
-        s = read_profile()[dom]
+        # (a recorded burst's replays carry no stage events)
+        s = read_profile().get(dom)
         lib.clim_profile_enable(ctx.h, 0)
         # primary: the kernel alone (the profiled steps, one frame at a time)
         iso = stages[dom]
         iso_s = iso["ms_total"] / iso["launches"] / 1e3
-        bytes_per_launch = stage_bytes(dom, iso["texels"] / iso["launches"], preview=args.preview)
+        bytes_per_launch = stage_bytes(dom, iso["texels"] / iso["launches"], preview=args.preview, half=args.mixed)
         ach = bytes_per_launch / iso_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": None if args.burst else measured_traffic(single_kernel[dom], w, h),
+                    "traffic": None if (args.burst or args.mixed) else measured_traffic(single_kernel[dom], w, h),
                     "kernel": single_kernel[dom], "stage": dom, "avg_launch_us": round(iso_s * 1e6, 2),
                     "launches_timed": iso["launches"], "algorithmic_bytes_per_launch": int(bytes_per_launch),
                     "timing": "kernel alone (one frame at a time), HIP events on the library's stream"}
-        if batched:
+        if batched and s:
             # the same kernel's launches in the timed batched steps, sharing the
             # device with the other stream's frame
             b_s = s["ms_total"] / s["launches"] / 1e3
-            b_ach = stage_bytes(dom, s["texels"] / s["launches"]) / b_s / 1e9
+            b_ach = stage_bytes(dom, s["texels"] / s["launches"], half=args.mixed) / b_s / 1e9
             roofline["batched"] = {"avg_launch_us": round(b_s * 1e6, 2), "achieved": round(b_ach, 1),
                                    "frac": round(b_ach / HBM_PEAK_GBS, 4), "lanes": 2,
                                    "launches_timed": s["launches"]}
 
     path = None
     if not args.preview:
-        bpp = path_bytes_per_px(5)
+        # the FP16-storage accounting for the mixed form (every float texel halved)
+        bpp = PATH_BYTES_FP16 if args.mixed else path_bytes_per_px(5)
         path = {"algorithmic": {"bytes_per_px": bpp, "achieved": round(bpp * mpix * 1e6 / 1e9, 1),
                                 "frac": round(bpp * mpix * 1e6 / 1e9 / HBM_PEAK_GBS, 4)},
                 "peak": HBM_PEAK_GBS, "unit": "GB/s"}
-        default_workload = not (args.burst or args.learned or args.ltm or args.noise_level > 0.6
+        default_workload = not (args.burst or args.learned or args.ltm or args.noise_level > 0.6 or args.mixed
                                 or algorithm != ca.BILATERAL)
         fb = measured_frame_bytes(w, h) if default_workload else None
         if fb:
@@ -533,7 +566,8 @@ def main():
 
     # 3. the learned denoise strength on the same frames
     learned = None
-    if learned_on and not args.learned:
+    # (a recorded burst keeps the strength mode it was recorded with)
+    if learned_on and not args.learned and graph is None:
         ctx.strength_set_mode(ca.STRENGTH_GATED)
         for _ in range(2):
             step()
@@ -583,18 +617,19 @@ def main():
 
     # 4. the uneven burst recorded as one graph (its own frames and timing)
     burst = None
-    if not (args.no_burst or args.preview or args.burst or args.learned):
+    if not (args.no_burst or args.preview or args.burst or args.learned or args.mixed):
         burst = burst_leg(ctx)
 
     # 5. the mixed-precision form on the same frames, and config 2's geometry
     mixed = None
-    if not (args.no_mixed or args.preview or args.burst or args.learned or args.ltm or args.noise_level > 0.6
-            or algorithm != ca.BILATERAL):
+    if not (args.no_mixed or args.mixed or args.preview or args.burst or args.learned or args.ltm
+            or args.noise_level > 0.6 or algorithm != ca.BILATERAL):
         mixed = mixed_leg(ctx, raws, outs, fparams, max(args.steps // 4, 10), mpix)
 
     # 6. the first frame against the CPU restatements (outside any timing)
     parity = None
-    if not (args.no_parity or args.preview or args.burst or args.learned or args.ltm or args.noise_level > 0.6):
+    if not (args.no_parity or args.preview or args.burst or args.learned or args.mixed or args.ltm
+            or args.noise_level > 0.6):
         of = ctx.image(w, h, ca.FMT_RGBA32F)
         nlf = ctx.raw_to_rgb(raws[0], outs[0], fparams[0], out_f32=of)
         parity = fp64_parity(raws[0].download(), bayer, of.download(), nlf, algorithm=algorithm)
@@ -609,7 +644,8 @@ def main():
     if args.burst:
         metric = "Mpix/s RAW->RGB (demosaic+pyramid denoise), burst of 16 uneven frames"
         workload = ("burst: 4 each of 4032x3024, 4000x3000, 3264x2448, 1920x1080 RGGB/BGGR uint16 12-bit, "
-                    "per-frame noise level drawn from the headline set, 5-level bilateral pyramid, one context")
+                    "per-frame noise level drawn from the headline set, 5-level bilateral pyramid, one context"
+                    + (", recorded as one HIP graph, one graph launch per step" if graph is not None else ""))
     else:
         metric = ("Mpix/s RAW->RGB half-resolution preview (mosaic pixels) at 12 MP" if args.preview
                   else "Mpix/s RAW->RGB (demosaic+learned-strength pyramid denoise) at 12 MP" if args.learned
@@ -618,6 +654,7 @@ def main():
                     + ("half-resolution preview (fastDemosaicImage)" if args.preview
                        else f"5-level {ALGORITHM_NAMES[algorithm]} pyramid")
                     + (", learned strength (gated)" if args.learned else "")
+                    + (", mixed precision (FP16 level storage, packed-FP16 bilateral weights)" if args.mixed else "")
                     + (", local tone mapping" if args.ltm else "")
                     + (f", noiseLevel {args.noise_level} (raw pre-denoise)" if args.noise_level > 0.6 else ""))
     res = {
@@ -631,7 +668,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f16 storage + f32" if args.mixed else "f32",
         "data": "synthetic (seeded scenes generated on the device)",
         "config": {"workload": workload, "frames_per_step": len(raws), "pixels_per_step": px_per_step,
                    "timed_seconds": round(dt, 3), "batched": batched},
@@ -647,6 +684,8 @@ def main():
     if latency:
         res["latency"] = latency
     print(json.dumps(res))
+    if graph is not None:
+        graph.close()
     ctx.close()
 
 
