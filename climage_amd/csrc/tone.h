@@ -56,15 +56,21 @@ template <bool LTMB>
 __device__ __forceinline__ uint32_t tone_rest(float v[3], const ToneArgs& a, float rgb[3], float boost) {
 #pragma unroll
     for (int c = 0; c < 3; c++) v[c] *= a.ebmul;
+    // smoothstep's division by the constant edge spans as a reciprocal
+    // multiplication with one fused correction: the same values
+    // (clim_smoothstep_rc)
     if (a.saturation != 1.0f) {
         const float luma = 0.2126f * v[0] + 0.7152f * v[1] + 0.0722f * v[2];
+        constexpr float span = 2.0f - 0.75f;
 #pragma unroll
         for (int c = 0; c < 3; c++)
-            v[c] = mixf(luma, v[c], mixf(a.saturation, 1.0f, smoothstep(0.75f, 2.0f, v[c])));
+            v[c] = mixf(luma, v[c], mixf(a.saturation, 1.0f, clim_smoothstep_rc(0.75f, span, 1.0f / span, v[c])));
     }
     if (a.contrast != 1.0f) {
+        constexpr float span = 2.0f - 0.9f;
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = mixf(0.2f, v[c], mixf(a.contrast, 1.0f, smoothstep(0.9f, 2.0f, v[c])));
+        for (int c = 0; c < 3; c++)
+            v[c] = mixf(0.2f, v[c], mixf(a.contrast, 1.0f, clim_smoothstep_rc(0.9f, span, 1.0f / span, v[c])));
     }
 #pragma unroll
     for (int r = 0; r < 3; r++) rgb[r] = a.cam[r * 3 + 0] * v[0] + a.cam[r * 3 + 1] * v[1] + a.cam[r * 3 + 2] * v[2];

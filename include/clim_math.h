@@ -46,6 +46,20 @@ CLIM_HD float clim_smoothstepf(float e0, float e1, float x) {
     return t * t * (3.0f - 2.0f * t);
 }
 CLIM_HD float clim_mixf(float a, float b, float t) { return a + (b - a) * t; }
+/* smoothstep(e0, e0 + span, x) with the division by the constant span done
+ * as a multiplication by rspan = 1 / span and one fused correction
+ * (clim_div25's form).  Equal to clim_smoothstepf for every float x where
+ * that holds for the pair (e0, span): the tone map's (0.9, 2.0) and
+ * (0.75, 2.0) are checked over every x between the edges (outside them both
+ * forms clamp to the same end) and the special values.  An infinite x makes
+ * the correction NaN; the product, of the right sign, is kept then. */
+CLIM_HD float clim_smoothstep_rc(float e0, float span, float rspan, float x) {
+    const float d = x - e0;
+    const float q = d * rspan;
+    const float c = fmaf(fmaf(-q, span, d), rspan, q);
+    const float t = clim_clampf(c == c ? c : q, 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
 CLIM_HD float clim_stepf(float edge, float x) { return x < edge ? 0.0f : 1.0f; }
 
 /* ------------------------------------------------------------ atan2(y,x)/pi
@@ -150,11 +164,30 @@ CLIM_HD float clim_powf(float x, float y) {
     return (neg && odd) ? -r : r;
 }
 
-/* --------------------------------------------------------------- tanh(x) */
+/* --------------------------------------------------------------- tanh(x)
+ * (1 - e) / (1 + e) with e = exp(-2|x|), the exponential inlined without
+ * branches: below -20, e < 2^-28 and (1 - e) / (1 + e) rounds to 1 for any
+ * such e, so the argument is clamped there (a NaN compares false and stays
+ * NaN), which keeps 2^k a normal number (k >= -29).  The same arithmetic as
+ * clim_expf on the unclamped range: the same bits as
+ * (1 - clim_expf(-2|x|)) / (1 + clim_expf(-2|x|)) for every x (NaN for NaN;
+ * checked over all 2^32 patterns). */
 CLIM_HD float clim_tanhf(float x) {
-    float ax = fabsf(x);
-    float e = clim_expf(-2.0f * ax);
-    float t = (1.0f - e) / (1.0f + e);
+    float y = -2.0f * fabsf(x);
+    y = y < -20.0f ? -20.0f : y;
+    const float k = rintf(y * 1.442695022e+00f);
+    const float r = (y - k * 6.93359375e-01f) - k * -2.12194440e-04f;
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    const float ks = fmaxf(k, -29.0f); /* a NaN k (NaN x) gives a NaN p anyway */
+    const float e = p * clim_u2f((uint32_t)((int)ks + 127) << 23);
+    const float t = (1.0f - e) / (1.0f + e);
     return (clim_f2u(x) >> 31) ? -t : t;
 }
 

@@ -99,6 +99,24 @@ long om_div25nz_mismatches(uint32_t lo, uint32_t hi) {
     }
     return bad;
 }
+/* mismatches between clim_smoothstep_rc(e0, e1 - e0, 1 / (e1 - e0), x) and
+ * clim_smoothstepf(e0, e1, x) over the bit patterns [lo, hi) (NaN results
+ * equal when both are NaN; a zero's sign is not compared) */
+long om_smoothstep_rc_mismatches(float e0, float e1, uint32_t lo, uint32_t hi) {
+    const float span = e1 - e0, rspan = 1.0f / span;
+    long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = lo; i < (int64_t)hi; i++) {
+        const float x = clim_u2f((uint32_t)i);
+        const float a = clim_smoothstep_rc(e0, span, rspan, x), b = clim_smoothstepf(e0, e1, x);
+        if (a != a || b != b) {
+            if (!(a != a && b != b)) bad++;
+        } else if (a != b) {
+            bad++;
+        }
+    }
+    return bad;
+}
 /* clim_div25_nz at one value (its -0 case) */
 float om_div25_nz(float x) { return clim_div25_nz(x); }
 

@@ -207,6 +207,43 @@ def test_fit_q_solve_and_moment_residuals(L):
         assert abs(Fraction(rss) - exact) <= Fraction(1, 10 ** 12) * scale + Fraction(1, 10 ** 30), trial
 
 
+def test_tanh_is_the_exp_quotient(L):
+    """clim_tanhf (the exponential inlined, argument clamped at -20) equals
+    (1 - e) / (1 + e) with e = clim_expf(-2|x|), sign restored, bit for bit:
+    on samples across the clamp and the special values (the build checked
+    all 2^32 patterns once)."""
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.uniform(-12, 12, 200000), rng.uniform(-200, 200, 50000),
+                        10.0 ** rng.uniform(-40, 38, 50000) * rng.choice([-1, 1], 50000),
+                        [0.0, -0.0, 10.0, -10.0, 9.999, 10.001, 30.0, -30.0, 1e30, np.inf, -np.inf]]).astype(np.float32)
+    th = _vec(L, "om_tanh", x)
+    e = _vec(L, "om_exp", (np.float32(-2.0) * np.abs(x)).astype(np.float32))
+    t = ((np.float32(1) - e) / (np.float32(1) + e)).astype(np.float32)
+    t = np.where(np.signbit(x), -t, t).astype(np.float32)
+    assert np.array_equal(th.view(np.uint32), t.view(np.uint32))
+    assert np.isnan(_vec(L, "om_tanh", np.array([np.nan], np.float32))[0])
+    assert _vec(L, "om_tanh", np.array([np.inf, -np.inf], np.float32)).tolist() == [1.0, -1.0]
+
+
+@pytest.mark.parametrize("e0,e1", [(0.9, 2.0), (0.75, 2.0)])
+def test_smoothstep_rc_equals_division(L, e0, e1):
+    """The tone map's smoothsteps with the division by the edge span as a
+    reciprocal multiplication and one fused correction equal the literal
+    form: over every float between the edges (where the quotient is in
+    [0, 1]; outside, both forms clamp to the same end since the corrected
+    quotient is within an ulp of the true one), a sample of all other
+    patterns and the special values (infinities, NaN, zeros)."""
+    L.om_smoothstep_rc_mismatches.restype = C.c_long
+    L.om_smoothstep_rc_mismatches.argtypes = [C.c_float, C.c_float, C.c_uint32, C.c_uint32]
+    f0, f1 = np.float32(e0), np.float32(e1)
+    lo, hi = int(f0.view(np.uint32)), int(f1.view(np.uint32)) + 1
+    assert L.om_smoothstep_rc_mismatches(f0, f1, lo, hi) == 0
+    for start in range(0, 2 ** 32, 2 ** 26):  # every 64th block of 2^20 patterns
+        assert L.om_smoothstep_rc_mismatches(f0, f1, start, start + 2 ** 20) == 0, hex(start)
+    for v in (0x7F800000, 0xFF800000, 0x7FC00000, 0x00000000, 0x80000000, 0x7F7FFFFF, 0xFF7FFFFF):
+        assert L.om_smoothstep_rc_mismatches(f0, f1, v, v + 1) == 0, hex(v)
+
+
 def test_div65535_equals_division_for_every_uint16(L):
     """The kernels' scaleRawData quotient (reciprocal + one fused correction)
     is bit-identical to the IEEE division for all 65536 inputs."""
