@@ -620,13 +620,16 @@ constexpr int RD_QPT = (RD_QX * RD_QY + 255) / 256;  // quads per thread
 // A window row in LDS: its even and its odd columns as two planes of
 // RD_PS floats, RD_WS floats per row.  Quad (qx, qy) reads column 2 qx + i
 // of row 2 qy + j, i.e. word (2 qy + j) RD_WS + (i & 1) RD_PS + qx + i / 2;
-// two rows are 194 words, 2 mod 32, so the bank is q mod 32 plus a
+// two rows are 162 words, 2 mod 32, so the bank is q mod 32 plus a
 // constant (q = 34 qy + qx) and the 32 consecutive quads LDS serves at a
 // time meet 32 distinct banks.  The staging pass writes consecutive
-// columns, alternately to the two planes, 48 words (16 banks) apart: the
-// even lanes meet one half of the banks and the odd lanes the other.
-constexpr int RD_PS = 48, RD_WS = 97;
-static_assert(RD_WX / 2 <= RD_PS && RD_PS + (RD_WX - 1) / 2 < RD_WS && RD_QX == 34, "window planes");
+// columns, alternately to the two planes, 40 words apart (a 2-way overlap
+// of the even and odd lanes' banks on the writes, once per texel).
+// The YCbCr region is three planes of floats (alpha is 0 and not kept):
+// 22.8 KB, so seven blocks fit a CU's LDS (the window takes 19.4 KB).
+constexpr int RD_PS = 40, RD_WS = 81;
+static_assert(RD_WX / 2 <= RD_PS && RD_PS + (RD_WX - 1) / 2 < RD_WS && RD_QX == 34 && (2 * RD_WS) % 32 == 2,
+              "window planes");
 __device__ __forceinline__ int rd_col(int i) { return (i & 1) * RD_PS + (i >> 1); }
 }
 
@@ -634,8 +637,15 @@ union RbDespeckleLds {
     struct {
         float r[RD_WY][RD_WS], g[RD_WY][RD_WS];
     } win;
-    float4 t[RD_RY][RD_RX];
-    __device__ __forceinline__ float4& at(int y, int x) { return t[y][x]; }
+    float t[3][RD_RY][RD_RX];
+    __device__ __forceinline__ float4 at(int y, int x) const {
+        return make_float4(t[0][y][x], t[1][y][x], t[2][y][x], 0.0f);
+    }
+    __device__ __forceinline__ void set(int y, int x, float3 v) {
+        t[0][y][x] = v.x;
+        t[1][y][x] = v.y;
+        t[2][y][x] = v.z;
+    }
 };
 
 // one RD_X x RD_Y output tile at (x0, y0) by the block's 256 threads; as
@@ -693,7 +703,7 @@ __device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, con
         for (int dy = 0; dy < 2; dy++)
 #pragma unroll
             for (int dx = 0; dx < 2; dx++)
-                sm.at(2 * qy + dy, 2 * qx + dx) = make_float4(o[n][dy][dx].x, o[n][dy][dx].y, o[n][dy][dx].z, 0.0f);
+                sm.set(2 * qy + dy, 2 * qx + dx, o[n][dy][dx]);
     }
     __syncthreads();
     // every 3x3 neighbourhood of an interior tile lies inside the image: no clamps

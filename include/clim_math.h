@@ -168,23 +168,22 @@ CLIM_HD float clim_powf(float x, float y) {
  * (1 - e) / (1 + e) with e = exp(-2|x|), the exponential inlined without
  * branches: below -20, e < 2^-28 and (1 - e) / (1 + e) rounds to 1 for any
  * such e, so the argument is clamped there (a NaN compares false and stays
- * NaN), which keeps 2^k a normal number (k >= -29).  The same arithmetic as
- * clim_expf on the unclamped range: the same bits as
- * (1 - clim_expf(-2|x|)) / (1 + clim_expf(-2|x|)) for every x (NaN for NaN;
- * checked over all 2^32 patterns). */
+ * NaN), which keeps 2^k a normal number (k >= -29).  The exponential is
+ * clim_expf's method with the Cody-Waite steps and the Horner steps fused
+ * (fmaf: one rounding each, the same on every IEEE machine). */
 CLIM_HD float clim_tanhf(float x) {
     float y = -2.0f * fabsf(x);
     y = y < -20.0f ? -20.0f : y;
     const float k = rintf(y * 1.442695022e+00f);
-    const float r = (y - k * 6.93359375e-01f) - k * -2.12194440e-04f;
+    const float r = fmaf(-k, -2.12194440e-04f, fmaf(-k, 6.93359375e-01f, y));
     float p = 1.0f / 5040.0f;
-    p = p * r + 1.0f / 720.0f;
-    p = p * r + 1.0f / 120.0f;
-    p = p * r + 1.0f / 24.0f;
-    p = p * r + 1.0f / 6.0f;
-    p = p * r + 0.5f;
-    p = p * r + 1.0f;
-    p = p * r + 1.0f;
+    p = fmaf(p, r, 1.0f / 720.0f);
+    p = fmaf(p, r, 1.0f / 120.0f);
+    p = fmaf(p, r, 1.0f / 24.0f);
+    p = fmaf(p, r, 1.0f / 6.0f);
+    p = fmaf(p, r, 0.5f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
     const float ks = fmaxf(k, -29.0f); /* a NaN k (NaN x) gives a NaN p anyway */
     const float e = p * clim_u2f((uint32_t)((int)ks + 127) << 23);
     const float t = (1.0f - e) / (1.0f + e);
