@@ -252,6 +252,22 @@ struct MosaicSrc {
         const uint32_t v16 = reinterpret_cast<const uint16_t*>(im.p + (size_t)yc * im.pitch)[xc];
         return scale_texel(v16, b.site(xc, yc), smul, black);
     }
+    // texels (xg .. xg + 3, y), each the value operator() gives: one 8-byte
+    // (16-byte) load when the four lie in the image and xg is a multiple of 4
+    __device__ __forceinline__ float4 row4(int xg, int y) const {
+        const int yc = clampi(y, 0, im.h - 1);
+        if ((xg & 3) == 0 && xg >= 0 && xg + 3 < im.w) {
+            const char* row = im.p + (size_t)yc * im.pitch;
+            if (U16) {
+                const uint2 w = *reinterpret_cast<const uint2*>(row + 2 * (size_t)xg);
+                const int s0 = b.site(xg, yc), s1 = b.site(xg + 1, yc);  // sites alternate along the row
+                return make_float4(scale_texel(w.x & 0xffffu, s0, smul, black), scale_texel(w.x >> 16, s1, smul, black),
+                                   scale_texel(w.y & 0xffffu, s0, smul, black), scale_texel(w.y >> 16, s1, smul, black));
+            }
+            return *reinterpret_cast<const float4*>(row + 4 * (size_t)xg);
+        }
+        return make_float4((*this)(xg, y), (*this)(xg + 1, y), (*this)(xg + 2, y), (*this)(xg + 3, y));
+    }
 };
 
 struct RawQuadLoader {

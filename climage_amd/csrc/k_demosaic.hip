@@ -214,9 +214,19 @@ __device__ __forceinline__ void green_tile(const MosaicSrc<U16>& raw, const Img&
     auto& sgx = sm.sgx;
     auto& sgy = sm.sgy;
     auto& sgl = sm.sgl;
-    stage<GR_X * GR_Y, 256>(
-        tid, [&](int i) { return raw(x0 - 3 + i % GR_X, y0 - 3 + i / GR_X); },
-        [&](int i, float v) { r[i / GR_X][i % GR_X] = v; });
+    // the raw tile in groups of four texels, columns x0 - 4 + 4 k (x0 is a
+    // multiple of 4): one wide load per group; the tile keeps x0 - 3 .. x0 + 34
+    constexpr int GG = (GR_X + 1 + 3) / 4;
+    static_assert(GT_X % 4 == 0 && GR_X == GT_X + 6, "groups of four");
+    stage<GG * GR_Y, 256>(
+        tid, [&](int i) { return raw.row4(x0 - 4 + 4 * (i % GG), y0 - 3 + i / GG); },
+        [&](int i, float4 v) {
+            const int j = i / GG, c = 4 * (i % GG) - 1;
+            if (c >= 0) r[j][c] = v.x;
+            r[j][c + 1] = v.y;
+            r[j][c + 2] = v.z;
+            if (c + 3 < GR_X) r[j][c + 3] = v.w;
+        });
     __syncthreads();
     // sobel(), cl/demosaic.cl:96-112, value / sqrt(4.5) then abs().  The
     // 9-tap sums in their order with the zero taps left out and the +-1, +-2
@@ -281,7 +291,9 @@ __device__ __forceinline__ void green_tile(const MosaicSrc<U16>& raw, const Img&
                 wsum += wt;  // the x and y weight sums are equal
             }
         }
-        const float gradx = sx / wsum, grady = sy / wsum;
+        // the two quotients by one reciprocal (the FP32 restatement's form)
+        const float iw = 1.0f / wsum;
+        const float gradx = sx * iw, grady = sy * iw;
 
         const float g_lf_x = (g_left + g_right) / 2.0f, g_lf_y = (g_up + g_down) / 2.0f;
         const float g_hf_x = (2.0f * c_xy - (c_left + c_right)) / 4.0f;
@@ -290,7 +302,11 @@ __device__ __forceinline__ void green_tile(const MosaicSrc<U16>& raw, const Img&
             clampf(mmin(c_xy, mmin(g_ave, c2_ave)) / mmax(c_xy, mmax(g_ave, c2_ave)), 0.0f, 1.0f);
         const float glen = len2(gradx, grady);
         const float lgt = smoothstep(lumaStdDev, 4.0f * lumaStdDev, glen);
-        const float hf_gain = lgt * mmin(0.5f * whiteness + smoothstep(0.0f, 0.3f, glen / 1.41421356f), 1.0f);
+        // glen / sqrt(2) as a multiplication (the FP32 restatement's form);
+        // smoothstep(0, 0.3, .) with the span division as a reciprocal and one
+        // fused correction, the same values (clim_smoothstep_rc)
+        const float hf_gain =
+            lgt * mmin(0.5f * whiteness + clim_smoothstep_rc(0.0f, 0.3f, 1.0f / 0.3f, glen * 0.707106781f), 1.0f);
         const float g_est_x = g_lf_x + hf_gain * g_hf_x;
         const float g_est_y = g_lf_y + hf_gain * g_hf_y;
         float direction = 2.0f * clim_atan2pif(grady, gradx);

@@ -63,33 +63,32 @@ CLIM_HD float clim_smoothstep_rc(float e0, float span, float rspan, float x) {
 CLIM_HD float clim_stepf(float edge, float x) { return x < edge ? 0.0f : 1.0f; }
 
 /* ------------------------------------------------------------ atan2(y,x)/pi
- * Octant reduction to t = min/max in [0,1]; for t > tan(pi/8) the identity
+ * Octant reduction to t = min/max in [0,1]; past tan(pi/8) the identity
  * atan(t) = pi/4 + atan((t-1)/(t+1)) brings the argument into
  * [-tan(pi/8), tan(pi/8)], where atan(u) = u*P(u^2) with P a degree-5
  * least-squares fit at Chebyshev nodes (max abs error 2.7e-10 before float
- * rounding).  Result in units of pi (OpenCL atan2pi). */
+ * rounding).  The reduced argument is formed with one division, as
+ * min/max or (min - max)/(min + max), the octant chosen by
+ * min > tan(pi/8) max; the polynomial in fused steps.  Result in units of
+ * pi (OpenCL atan2pi). */
 CLIM_HD float clim_atan2pif(float y, float x) {
-    float ax = fabsf(x), ay = fabsf(y);
-    float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
-    float t = mx > 0.0f ? mn / mx : 0.0f;
-    if (x != x || y != y) return x + y;          /* NaN in, NaN out */
-    float base = 0.0f;
-    if (t > 4.142135680e-01f) {                  /* tan(pi/8) */
-        t = (t - 1.0f) / (t + 1.0f);
-        base = 0.25f;
-    }
-    float s = t * t;
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const int far = mn > 4.142135680e-01f * mx; /* tan(pi/8) */
+    const float num = far ? mn - mx : mn, den = far ? mn + mx : mx;
+    const float t = den > 0.0f ? num / den : 0.0f;
+    const float s = t * t;
     float p = -6.077074260e-02f;
-    p = p * s + 1.059265658e-01f;
-    p = p * s + -1.424319446e-01f;
-    p = p * s + 1.999843121e-01f;
-    p = p * s + -3.333331347e-01f;
-    p = p * s + 1.0f;
-    float r = base + (t * p) * 3.183098733e-01f; /* 1/pi */
+    p = fmaf(p, s, 1.059265658e-01f);
+    p = fmaf(p, s, -1.424319446e-01f);
+    p = fmaf(p, s, 1.999843121e-01f);
+    p = fmaf(p, s, -3.333331347e-01f);
+    p = fmaf(p, s, 1.0f);
+    float r = fmaf(t * p, 3.183098733e-01f, far ? 0.25f : 0.0f); /* 1/pi */
     if (ay > ax) r = 0.5f - r;
     if (clim_f2u(x) >> 31) r = 1.0f - r;
     if (clim_f2u(y) >> 31) r = -r;
-    return r;
+    return (x != x || y != y) ? x + y : r; /* NaN in, NaN out */
 }
 
 /* ----------------------------------------------------------------- exp(x)

@@ -417,8 +417,15 @@ static inline void bilateral_sobel(const real* img, int w, int h, int x, int y, 
             wy += wt;
         }
     }
+#if ORACLE_DOUBLE
     *ox = sx / wx;
     *oy = sy / wy;
+#else
+    /* the two quotients by one reciprocal (the weight sums are equal) */
+    const real iw = K(1.0) / wx;
+    *ox = sx * iw;
+    *oy = sy * iw;
+#endif
 }
 
 void P(interpolate_green)(const real* raw, int w, int h, int bayer, float luma_variance, real* green) {
@@ -461,8 +468,12 @@ void P(interpolate_green)(const real* raw, int w, int h, int bayer, float luma_v
                                     K(0.0), K(1.0));
             real glen = len2(gradx, grady);
             real lgt = smoothstepr(lumaStdDev, K(4.0) * lumaStdDev, glen);
-            real hf_gain = lgt * mmin(K(0.5) * whiteness + smoothstepr(K(0.0), K(0.3), glen / K(1.41421356)),
-                                      K(1.0));
+#if ORACLE_DOUBLE
+            real glen_s2 = glen / K(1.41421356);
+#else
+            real glen_s2 = glen * K(0.707106781); /* / sqrt(2) as a multiplication */
+#endif
+            real hf_gain = lgt * mmin(K(0.5) * whiteness + smoothstepr(K(0.0), K(0.3), glen_s2), K(1.0));
             real g_est_x = g_lf_x + hf_gain * g_hf_x;
             real g_est_y = g_lf_y + hf_gain * g_hf_y;
 

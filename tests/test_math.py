@@ -225,7 +225,7 @@ def test_tanh_across_the_clamp(L):
     assert _vec(L, "om_tanh", np.array([np.inf, -np.inf], np.float32)).tolist() == [1.0, -1.0]
 
 
-@pytest.mark.parametrize("e0,e1", [(0.9, 2.0), (0.75, 2.0)])
+@pytest.mark.parametrize("e0,e1", [(0.9, 2.0), (0.75, 2.0), (0.0, 0.3)])
 def test_smoothstep_rc_equals_division(L, e0, e1):
     """The tone map's smoothsteps with the division by the edge span as a
     reciprocal multiplication and one fused correction equal the literal
