@@ -49,6 +49,13 @@ __device__ __forceinline__ float4* wrow_f4(const Img& im, int y) {
 __device__ __forceinline__ float ld1(const Img& im, int x, int y) {
     return row_f(im, clampi(y, 0, im.h - 1))[clampi(x, 0, im.w - 1)];
 }
+// ld1 of (xg .. xg + 3, y): one 16-byte load when the four lie in the image
+// and xg is a multiple of 4 (rows are 16-byte aligned)
+__device__ __forceinline__ float4 ld1x4(const Img& im, int xg, int y) {
+    if ((xg & 3) == 0 && xg >= 0 && xg + 3 < im.w)
+        return *reinterpret_cast<const float4*>(row_f(im, clampi(y, 0, im.h - 1)) + xg);
+    return make_float4(ld1(im, xg, y), ld1(im, xg + 1, y), ld1(im, xg + 2, y), ld1(im, xg + 3, y));
+}
 // RGBA texel (x, y) as FP32, from FP16 storage when H (the kernels of the
 // pyramid take H as a template parameter, so that neither form branches per
 // access; ldt4 / st4 / ld4 below decide at run time, for the other kernels)

@@ -673,16 +673,29 @@ __device__ __forceinline__ void rb_despeckle_tile(const MosaicSrc<U16>& raw, con
                                                   RbDespeckleLds& sm) {
     const int rx0 = x0 - 2, ry0 = y0 - 2;
     const float cvar = chroma_variance(cv);
-    // the region's mosaic (scaled) and green, texel (rx0 - 1 + i, ry0 - 1 + j)
-    stage<RD_WX * RD_WY, 256>(
+    // the region's mosaic (scaled) and green, texel (rx0 - 1 + i, ry0 - 1 + j),
+    // in groups of four texels at columns rx0 - 2 + 4 k (x0 is a multiple of
+    // 4): one wide load of each image per group
+    struct Group {
+        float4 r, g;
+    };
+    constexpr int WG = (RD_WX + 1 + 3) / 4;
+    static_assert(RD_X % 4 == 0, "groups of four");
+    stage<WG * RD_WY, 256>(
         tid,
         [&](int k) {
-            const int i = k % RD_WX, j = k / RD_WX;
-            return make_float2(raw(rx0 - 1 + i, ry0 - 1 + j), ld1(green, rx0 - 1 + i, ry0 - 1 + j));
+            const int xg = rx0 - 2 + 4 * (k % WG), y = ry0 - 1 + k / WG;
+            return Group{raw.row4(xg, y), ld1x4(green, xg, y)};
         },
-        [&](int k, float2 v) {
-            sm.win.r[k / RD_WX][rd_col(k % RD_WX)] = v.x;
-            sm.win.g[k / RD_WX][rd_col(k % RD_WX)] = v.y;
+        [&](int k, const Group& v) {
+            const int j = k / WG, c = 4 * (k % WG) - 1;
+            const float rv[4] = {v.r.x, v.r.y, v.r.z, v.r.w}, gv[4] = {v.g.x, v.g.y, v.g.z, v.g.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (c + i < 0 || c + i >= RD_WX) continue;
+                sm.win.r[j][rd_col(c + i)] = rv[i];
+                sm.win.g[j][rd_col(c + i)] = gv[i];
+            }
         });
     __syncthreads();
     float3 o[RD_QPT][2][2];
