@@ -299,11 +299,21 @@ bool launch_noise_stats_fit(FitJobs& jobs, int j0, int j1, const Img* down, hipS
 // oracle evaluates the same expressions).
 // SAME_EDGE: gradientBoost <= 1 and the luma and chroma edges are equal, so
 // every tap's luma and chroma weights are equal (one test, one weight sum).
+//
+// One output texel's state: 1 / sigma per channel, the scaled centre, the
+// squared edges, the weighted sums and the weight counts (the weights are 0
+// or 1).
+struct BilCentre {
+    float r0, r1, r2, cr0, cr1, cr2, e0, e1;
+    float f0, f1, f2;
+    int c0, c1;
+};
+
+// the centre at tile texel (X, Y) (halo included)
 template <bool SAME_EDGE>
-__device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int tx, int ty, float va0, float va1,
-                                               float va2, float vb0, float vb1, float vb2, float chromaBoost,
-                                               float gradientBoost, float m) {
-    const int X = tx + HR, Y = ty + HR;
+__device__ __forceinline__ BilCentre bilateral_centre(const LevelTile4<TX, TY>& t, int X, int Y, float va0, float va1,
+                                                      float va2, float vb0, float vb1, float vb2, float chromaBoost,
+                                                      float gradientBoost, float m) {
     const float4 cp = t.p[Y][X];
     const float cY = cp.x, cCb = cp.y, cCr = cp.z;
     float lumaBoost = 1.0f;
@@ -326,6 +336,7 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int 
         const float is0 = 1.0f / sig0;
         lumaBoost = gradientBoost - (gradientBoost - 1.0f) * smoothstep(0.5f, 2.0f, len2(vx * is0, vy * is0));
     }
+    BilCentre c;
     float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
     e0 = e0 * e0;
     e1 = e1 * e1;
@@ -335,62 +346,139 @@ __device__ __forceinline__ float4 bilateral_px(const LevelTile4<TX, TY>& t, int 
         e0 = e0 * m2;
         e1 = e1 * m2;
     }
-    // 1 / sigma per channel; the squared distance by clim_range_len2
-    const float r0 = clim_inv_sigma(va0 + vb0 * cY), r1 = clim_inv_sigma(va1 + vb1 * cY),
-                r2 = clim_inv_sigma(va2 + vb2 * cY);
-    const float cr0 = cY * r0, cr1 = cCb * r1, cr2 = cCr * r2;
-    float f0 = 0, f1 = 0, f2 = 0;
-    int c0 = 0, c1 = 0;  // weight sums (the weights are 0 or 1)
+    c.e0 = e0;
+    c.e1 = e1;
+    // 1 / sigma per channel (clim_inv_sigma); the squared distance by
+    // clim_range_len2.  The Newton steps run on every lane and the special
+    // cases (a variance that is not a positive finite float) once per wave,
+    // only when some lane has one: the same values as clim_inv_sigma.
+    const float s0 = va0 + vb0 * cY, s1 = va1 + vb1 * cY, s2 = va2 + vb2 * cY;
+    c.r0 = clim_rsqrt_newton(s0);
+    c.r1 = clim_rsqrt_newton(s1);
+    c.r2 = clim_rsqrt_newton(s2);
+    constexpr int kPosFinite = 128 | 256;  // +subnormal, +normal
+    if (__builtin_expect(!(__builtin_amdgcn_classf(s0, kPosFinite) && __builtin_amdgcn_classf(s1, kPosFinite) &&
+                           __builtin_amdgcn_classf(s2, kPosFinite)),
+                         0)) {
+        c.r0 = clim_inv_sigma(s0);
+        c.r1 = clim_inv_sigma(s1);
+        c.r2 = clim_inv_sigma(s2);
+    }
+    c.cr0 = cY * c.r0;
+    c.cr1 = cCb * c.r1;
+    c.cr2 = cCr * c.r2;
+    c.f0 = c.f1 = c.f2 = 0.0f;
+    c.c0 = c.c1 = 0;
+    return c;
+}
+
+// one tap (a, b, cc) of the centre's 5x5 window, in window order
+template <bool SAME_EDGE>
+__device__ __forceinline__ void bilateral_tap(BilCentre& c, float a, float b, float cc) {
+    const float l = clim_range_len2(a, b, cc, c.r0, c.r1, c.r2, c.cr0, c.cr1, c.cr2);
+    // f += w * s with w in {0, 1}: the product is exact, so the fused form
+    // rounds once on the same value (the same bits)
+    const bool in0 = l < c.e0;
+    const float w0 = in0 ? 1.0f : 0.0f;
+    c.f0 = fmaf(w0, a, c.f0);
+    c.c0 += in0;
+    if (SAME_EDGE) {
+        c.f1 = fmaf(w0, b, c.f1);
+        c.f2 = fmaf(w0, cc, c.f2);
+    } else {
+        const bool in1 = l < c.e1;
+        const float w1 = in1 ? 1.0f : 0.0f;
+        c.f1 = fmaf(w1, b, c.f1);
+        c.f2 = fmaf(w1, cc, c.f2);
+        c.c1 += in1;
+    }
+}
+
+// the weighted means: sum / count by clim_div_count (the IEEE quotient's
+// bits; rcp[n] = 1.0f / n); the Cr count equals the Cb one (same threshold,
+// same weights)
+template <bool SAME_EDGE>
+__device__ __forceinline__ float4 bilateral_mean(const BilCentre& c, const float* rcp) {
+    const int k1 = SAME_EDGE ? c.c0 : c.c1;
+    const float n0 = (float)c.c0, n1 = (float)k1, q0 = rcp[c.c0], q1 = SAME_EDGE ? q0 : rcp[k1];
+    // the fused form on every lane; clim_div_count's other cases once per
+    // wave, only when some lane's product is not a normal float
+    float4 r = make_float4(clim_div_count_normal(c.f0, n0, q0), clim_div_count_normal(c.f1, n1, q1),
+                           clim_div_count_normal(c.f2, n1, q1), 0.0f);
+    constexpr int kNormal = 8 | 256;  // -normal, +normal
+    if (__builtin_expect(!(__builtin_amdgcn_classf(c.f0 * q0, kNormal) && __builtin_amdgcn_classf(c.f1 * q1, kNormal) &&
+                           __builtin_amdgcn_classf(c.f2 * q1, kNormal)),
+                         0)) {
+        r.x = clim_div_count(c.f0, n0, q0);
+        r.y = clim_div_count(c.f1, n1, q1);
+        r.z = clim_div_count(c.f2, n1, q1);
+    }
+    return r;
+}
+
+// Two vertically adjacent outputs (tile rows ty, ty + 1) of one thread: their
+// 5x5 windows share four rows, so the six rows of taps are read from LDS once
+// and each tap goes to the one or two windows it belongs to, every window in
+// its own row-major order (the sums are those of the one-texel form).
+template <bool SAME_EDGE>
+__device__ __forceinline__ void bilateral_pair(const LevelTile4<TX, TY>& t, int tx, int ty, BilCentre& A,
+                                               BilCentre& B) {
 #pragma unroll
-    for (int j = 0; j < 5; j++) {
+    for (int i = 0; i < 5; i++) {
+        const float4 v = t.p[ty][tx + i];
+        bilateral_tap<SAME_EDGE>(A, v.x, v.y, v.z);
+    }
+#pragma unroll 1
+    for (int j = 1; j < 5; j++) {
 #pragma unroll
         for (int i = 0; i < 5; i++) {
             const float4 v = t.p[ty + j][tx + i];
-            const float a = v.x, b = v.y, c = v.z;
-            const float l = clim_range_len2(a, b, c, r0, r1, r2, cr0, cr1, cr2);
-            // f += w * s with w in {0, 1}: the product is exact, so the fused
-            // form rounds once on the same value (the same bits)
-            const bool in0 = l < e0;
-            const float w0 = in0 ? 1.0f : 0.0f;
-            f0 = fmaf(w0, a, f0);
-            c0 += in0;
-            if (SAME_EDGE) {
-                f1 = fmaf(w0, b, f1);
-                f2 = fmaf(w0, c, f2);
-            } else {
-                const bool in1 = l < e1;
-                const float w1 = in1 ? 1.0f : 0.0f;
-                f1 = fmaf(w1, b, f1);
-                f2 = fmaf(w1, c, f2);
-                c1 += in1;
-            }
+            bilateral_tap<SAME_EDGE>(A, v.x, v.y, v.z);
+            bilateral_tap<SAME_EDGE>(B, v.x, v.y, v.z);
         }
     }
-    const float n0 = (float)c0, n1 = SAME_EDGE ? n0 : (float)c1;
-    // the Cr weight sum equals the Cb one (same threshold, same weights)
-    return make_float4(f0 / n0, f1 / n1, f2 / n1, 0.0f);
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const float4 v = t.p[ty + 5][tx + i];
+        bilateral_tap<SAME_EDGE>(B, v.x, v.y, v.z);
+    }
+}
+
+// The outputs of one 32x16 tile (x0, y0) of denoiseImage whose texels are in
+// LDS (t, and the strength map texels in mt); rcp[n] = 1.0f / n.
+template <bool SAME_EDGE, bool H>
+__device__ __forceinline__ void bilateral_compute(const LevelTile4<TX, TY>& t, const MapStage& ms, const float* mt,
+                                                  const float* rcp, const Img& out, int x0, int y0,
+                                                  const ParamRef& nlf, float chromaBoost, float gradientBoost,
+                                                  const StrengthRef& sr, int level) {
+    const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
+    const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
+    // thread (tx, ty) writes rows 2 ty and 2 ty + 1 of column tx
+    const int tx = threadIdx.x, ty = 2 * threadIdx.y;
+    const int x = x0 + tx, y = y0 + ty;
+    if (x >= out.w || y >= out.h) return;
+    const float ma = strength_at_staged(sr, ms, mt, level, x, y);
+    const float mb = strength_at_staged(sr, ms, mt, level, x, y + 1);
+    BilCentre A = bilateral_centre<SAME_EDGE>(t, tx + HR, ty + HR, va0, va1, va2, vb0, vb1, vb2, chromaBoost,
+                                              gradientBoost, ma);
+    BilCentre B = bilateral_centre<SAME_EDGE>(t, tx + HR, ty + 1 + HR, va0, va1, va2, vb0, vb1, vb2, chromaBoost,
+                                              gradientBoost, mb);
+    bilateral_pair<SAME_EDGE>(t, tx, ty, A, B);
+    st4_<H>(out, x, y, bilateral_mean<SAME_EDGE>(A, rcp));
+    if (y + 1 < out.h) st4_<H>(out, x, y + 1, bilateral_mean<SAME_EDGE>(B, rcp));
 }
 
 // one 32x16 tile (x0, y0) of denoiseImage
 template <bool SAME_EDGE, bool H>
-__device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, float* mt, const Img& in, const Img& out, int x0,
-                                               int y0, const ParamRef& nlf, float chromaBoost, float gradientBoost,
-                                               const StrengthRef& sr, int level) {
+__device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, float* mt, float* rcp, const Img& in,
+                                               const Img& out, int x0, int y0, const ParamRef& nlf, float chromaBoost,
+                                               float gradientBoost, const StrengthRef& sr, int level) {
     const int tid = threadIdx.y * 32 + threadIdx.x;
     t.template load<H>(in, x0, y0, tid);
     const MapStage ms = stage_strength<TX, TY>(sr, level, x0, y0, mt, tid);
+    if (tid < 26) rcp[tid] = 1.0f / (float)tid;
     __syncthreads();
-    const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
-    const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
-#pragma unroll 1
-    for (int rr = 0; rr < 2; rr++) {
-        const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
-        const int x = x0 + tx, y = y0 + ty;
-        if (x >= out.w || y >= out.h) continue;
-        const float m = strength_at_staged(sr, ms, mt, level, x, y);
-        st4_<H>(out, x, y,
-                bilateral_px<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, chromaBoost, gradientBoost, m));
-    }
+    bilateral_compute<SAME_EDGE, H>(t, ms, mt, rcp, out, x0, y0, nlf, chromaBoost, gradientBoost, sr, level);
 }
 
 template <bool SAME_EDGE, bool H>
@@ -398,8 +486,10 @@ __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf
                                                    float gradientBoost, StrengthRef sr, int level) {
     __shared__ LevelTile4<TX, TY> t;
     __shared__ float mt[kMapStageMax];
+    __shared__ float rcp[26];
     const TileId tl = xcd_tile();
-    bilateral_tile<SAME_EDGE, H>(t, mt, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost, sr, level);
+    bilateral_tile<SAME_EDGE, H>(t, mt, rcp, in, out, tl.x * TX, tl.y * TY, nlf, chromaBoost, gradientBoost, sr,
+                                 level);
 }
 
 // denoiseImage of every pyramid level in one launch (the levels' denoising
@@ -409,11 +499,12 @@ template <bool SAME_EDGE>
 __global__ void __launch_bounds__(256) k_bilateral_levels(BilateralLevels lv) {
     __shared__ LevelTile4<TX, TY> t;
     __shared__ float mt[kMapStageMax];
+    __shared__ float rcp[26];
     const int b = xcd_block();
     int l = 0;
     while (l + 1 < lv.n && b >= lv.base[l + 1]) l++;
     const int i = b - lv.base[l], tiles_x = (lv.out[l].w + TX - 1) / TX;
-    bilateral_tile<SAME_EDGE, false>(t, mt, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY,
+    bilateral_tile<SAME_EDGE, false>(t, mt, rcp, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY,
                                      lv.nlf[l], lv.chromaBoost[l], lv.gradientBoost[l], lv.sr, l);
 }
 

@@ -282,10 +282,16 @@ CLIM_HD float clim_inv_var(float s) { return s < 0.0f ? clim_u2f(0x7fc00000u) : 
  * binade pair decides the others while no intermediate leaves the normal
  * range), within 4 ulp for every normal s; subnormal s get a coarser
  * estimate.  NaN for s < 0 and NaN, +inf for s = 0, 0 for s = +inf. */
-CLIM_HD float clim_rsqrt(float s) {
+/* the bit-level estimate and the three Newton steps alone: clim_rsqrt for
+ * every positive finite s (normal or subnormal) */
+CLIM_HD float clim_rsqrt_newton(float s) {
     float y = clim_u2f(0x5f375a86u - (clim_f2u(s) >> 1));
     const float h = 0.5f * s;
     for (int i = 0; i < 3; i++) y = y * fmaf(-h, y * y, 1.5f);
+    return y;
+}
+CLIM_HD float clim_rsqrt(float s) {
+    const float y = clim_rsqrt_newton(s);
     if (!(s > 0.0f)) return s == 0.0f ? clim_u2f(0x7f800000u) : clim_u2f(0x7fc00000u);
     return s == clim_u2f(0x7f800000u) ? 0.0f : y;
 }
@@ -348,6 +354,30 @@ CLIM_HD float clim_div25_nz(float x) {
     const float r = 1.0f / 25.0f;
     const float q = x * r;
     return fmaf(fmaf(-q, 25.0f, x), r, q);
+}
+
+/* ------------------------------------------- x / n for a weight count n
+ * The bilateral's weighted mean divides a sum by its count of unit weights,
+ * n in 0..25, with rn = 1.0f / n (rn = +inf for n = 0).  Where the product
+ * q = x * rn is a normal float, one fused correction gives the correctly
+ * rounded quotient; where q is 0, inf or NaN, q itself is x / n (n = 0
+ * included: 0 * inf = NaN = 0 / 0); where q is subnormal the IEEE division
+ * is taken.  Equal to x / (float)n bit for bit for every float x and every
+ * n in 0..25 (checked over all 2^32 bit patterns per n; the test suite
+ * rechecks one binade and the subnormal-quotient range, which decide it).
+ * Without the subnormal branch the form differs for n = 6, 10, 12, 14, ...
+ * when x < 2^-125.  clim_div_count_normal is the fused form alone (equal to
+ * clim_div_count wherever x * rn is a normal float). */
+CLIM_HD float clim_div_count_normal(float x, float n, float rn) {
+    const float q = x * rn;
+    return fmaf(fmaf(-q, n, x), rn, q);
+}
+CLIM_HD float clim_div_count(float x, float n, float rn) {
+    const float q = x * rn;
+    const float c = clim_div_count_normal(x, n, rn);
+    const float a = fabsf(q);
+    if (a < 1.17549435e-38f && a > 0.0f) return x / n;
+    return a < INFINITY && a > 0.0f ? c : q;
 }
 
 /* ------------------------------------------ 128-bit fixed-point accumulator

@@ -117,6 +117,42 @@ long om_smoothstep_rc_mismatches(float e0, float e1, uint32_t lo, uint32_t hi) {
     }
     return bad;
 }
+/* mismatches between clim_div_count(x, n, 1 / n) and x / n over the bit
+ * patterns [lo, hi) (NaN results equal when both are NaN) */
+long om_div_count_mismatches(int n, uint32_t lo, uint32_t hi) {
+    const float fn = (float)n, rn = 1.0f / fn;
+    long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = lo; i < (int64_t)hi; i++) {
+        const float x = clim_u2f((uint32_t)i);
+        const float a = clim_div_count(x, fn, rn), b = x / fn;
+        if (a != a || b != b) {
+            if (!(a != a && b != b)) bad++;
+        } else if (clim_f2u(a) != clim_f2u(b)) {
+            bad++;
+        }
+    }
+    return bad;
+}
+/* the same count for clim_div_count without its subnormal branch (a planted
+ * mistake the test must catch) */
+long om_div_count_nobranch_mismatches(int n, uint32_t lo, uint32_t hi) {
+    const float fn = (float)n, rn = 1.0f / fn;
+    long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = lo; i < (int64_t)hi; i++) {
+        const float x = clim_u2f((uint32_t)i);
+        const float q = x * rn;
+        const float c = fmaf(fmaf(-q, fn, x), rn, q);
+        const float a = fabsf(q) < INFINITY && q != 0.0f ? c : q, b = x / fn;
+        if (a != a || b != b) {
+            if (!(a != a && b != b)) bad++;
+        } else if (clim_f2u(a) != clim_f2u(b)) {
+            bad++;
+        }
+    }
+    return bad;
+}
 /* clim_div25_nz at one value (its -0 case) */
 float om_div25_nz(float x) { return clim_div25_nz(x); }
 

@@ -277,6 +277,30 @@ def test_div25_equals_division(L):
     assert z == 0.0 and np.signbit(np.float32(z)) == False
 
 
+def test_div_count_equals_division(L):
+    """clim_div_count (the bilateral's sum / count, n = 0..25) against x / n.
+    As for clim_div25, division by a constant commutes with scaling by 2^k
+    unless the quotient is subnormal, so one ordinary binade, every float
+    below 2^-121 (quotients near and below the subnormal range) and the
+    specials decide it, both signs; the build also checked all 2^32 patterns
+    for every n.  A planted mistake (the form without its subnormal branch)
+    is caught by the same ranges."""
+    L.om_div_count_mismatches.restype = C.c_long
+    L.om_div_count_mismatches.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
+    ranges = [(0x3F800000, 0x40000000), (0x00000000, 0x03000000), (0x7F800000, 0x80000000)]
+    for n in range(26):
+        for lo, hi in ranges:
+            for sign in (0, 0x80000000):
+                bad = L.om_div_count_mismatches(n, lo | sign, (hi - 1) | sign)
+                assert bad == 0, (n, hex(lo), sign, bad)
+
+    # the planted mistake: without the subnormal branch the low range fails
+    L.om_div_count_nobranch_mismatches.restype = C.c_long
+    L.om_div_count_nobranch_mismatches.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
+    assert L.om_div_count_nobranch_mismatches(6, 0x00000000, 0x02FFFFFF) > 0
+    assert L.om_div_count_nobranch_mismatches(6, 0x3F800000, 0x3FFFFFFF) == 0
+
+
 def test_window_sums_are_never_negative_zero():
     """The premise of clim_div25_nz in the window statistics: under
     round-to-nearest, +0 + (-0) = +0 and x + (-x) = +0, so a float sum
