@@ -307,6 +307,7 @@ struct BilCentre {
     float r0, r1, r2, cr0, cr1, cr2, e0, e1;
     float f0, f1, f2;
     int c0, c1;
+    float nH, E, n;  // SAME_EDGE: the step of bilateral_tap, its weight count
 };
 
 // the centre at tile texel (X, Y) (halo included)
@@ -348,6 +349,13 @@ __device__ __forceinline__ BilCentre bilateral_centre(const LevelTile4<TX, TY>& 
     }
     c.e0 = e0;
     c.e1 = e1;
+    // SAME_EDGE's weight as one fused step (bilateral_tap): H = 2^(50 - e),
+    // e the exponent of e0, so that E = e0 H is in [2^50, 2^51) exactly
+    // (e0 is a positive normal float: 2 m^2 with m in [2^-1.5, 2^1.5] for the
+    // learned map, 2 without it)
+    c.nH = -clim_u2f((304u - (clim_f2u(e0) >> 23)) << 23);
+    c.E = e0 * -c.nH;
+    c.n = 0.0f;
     // 1 / sigma per channel (clim_inv_sigma); the squared distance by
     // clim_range_len2.  The Newton steps run on every lane and the special
     // cases (a variance that is not a positive finite float) once per wave,
@@ -378,20 +386,25 @@ __device__ __forceinline__ void bilateral_tap(BilCentre& c, float a, float b, fl
     const float l = clim_range_len2(a, b, cc, c.r0, c.r1, c.r2, c.cr0, c.cr1, c.cr2);
     // f += w * s with w in {0, 1}: the product is exact, so the fused form
     // rounds once on the same value (the same bits)
-    const bool in0 = l < c.e0;
-    const float w0 = in0 ? 1.0f : 0.0f;
+    if (SAME_EDGE) {
+        // w = (l < e0) as clamp(fma(l, -H, e0 H), 0, 1), one instruction (the
+        // clamp is the FMA's output modifier): for l < e0 the exact value
+        // H (e0 - l) >= H ulp(e0) / 2 = 2^26 rounds to >= 1; for l >= e0 it is
+        // <= 0; a NaN clamps to 0 (clim_step_below_scaled)
+        const float w = __builtin_amdgcn_fmed3f(fmaf(l, c.nH, c.E), 0.0f, 1.0f);
+        c.f0 = fmaf(w, a, c.f0);
+        c.f1 = fmaf(w, b, c.f1);
+        c.f2 = fmaf(w, cc, c.f2);
+        c.n += w;
+        return;
+    }
+    const bool in0 = l < c.e0, in1 = l < c.e1;
+    const float w0 = in0 ? 1.0f : 0.0f, w1 = in1 ? 1.0f : 0.0f;
     c.f0 = fmaf(w0, a, c.f0);
     c.c0 += in0;
-    if (SAME_EDGE) {
-        c.f1 = fmaf(w0, b, c.f1);
-        c.f2 = fmaf(w0, cc, c.f2);
-    } else {
-        const bool in1 = l < c.e1;
-        const float w1 = in1 ? 1.0f : 0.0f;
-        c.f1 = fmaf(w1, b, c.f1);
-        c.f2 = fmaf(w1, cc, c.f2);
-        c.c1 += in1;
-    }
+    c.f1 = fmaf(w1, b, c.f1);
+    c.f2 = fmaf(w1, cc, c.f2);
+    c.c1 += in1;
 }
 
 // the weighted means: sum / count by clim_div_count (the IEEE quotient's
@@ -399,8 +412,9 @@ __device__ __forceinline__ void bilateral_tap(BilCentre& c, float a, float b, fl
 // same weights)
 template <bool SAME_EDGE>
 __device__ __forceinline__ float4 bilateral_mean(const BilCentre& c, const float* rcp) {
-    const int k1 = SAME_EDGE ? c.c0 : c.c1;
-    const float n0 = (float)c.c0, n1 = (float)k1, q0 = rcp[c.c0], q1 = SAME_EDGE ? q0 : rcp[k1];
+    const int k0 = SAME_EDGE ? (int)c.n : c.c0, k1 = SAME_EDGE ? k0 : c.c1;
+    const float n0 = SAME_EDGE ? c.n : (float)k0, n1 = SAME_EDGE ? n0 : (float)k1;
+    const float q0 = rcp[k0], q1 = SAME_EDGE ? q0 : rcp[k1];
     // the fused form on every lane; clim_div_count's other cases once per
     // wave, only when some lane's product is not a normal float
     float4 r = make_float4(clim_div_count_normal(c.f0, n0, q0), clim_div_count_normal(c.f1, n1, q1),

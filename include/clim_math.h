@@ -380,6 +380,20 @@ CLIM_HD float clim_div_count(float x, float n, float rn) {
     return a < INFINITY && a > 0.0f ? c : q;
 }
 
+/* ----------------------------------------- the unit step (l < e0) scaled
+ * The bilateral's weight (l < e0 ? 1 : 0) as one fused step whose output
+ * clamp to [0, 1] the GPU applies for free: clamp(fma(l, -H, e0 H), 0, 1)
+ * with H = 2^(50 - e), e the exponent of e0 (a positive normal float of
+ * exponent >= -77), so e0 H is exact.  For l < e0 the exact value
+ * H (e0 - l) >= H ulp(e0) / 2 = 2^26 rounds to >= 1; for l >= e0 it is
+ * <= 0; a NaN l clamps to 0 (fmaxf(NaN, 0) = 0 here, the GPU's clamp with
+ * DX10 clamping likewise).  The kernels evaluate this form; the oracle keeps
+ * the comparison, and the test suite checks the two agree. */
+CLIM_HD float clim_step_below_scaled(float l, float e0) {
+    const float H = clim_u2f((304u - (clim_f2u(e0) >> 23)) << 23);
+    return clim_clampf(fmaf(l, -H, e0 * H), 0.0f, 1.0f);
+}
+
 /* ------------------------------------------ 128-bit fixed-point accumulator
  * Non-negative doubles below 2^37 are truncated to multiples of 2^-90 and
  * summed as unsigned 128-bit integers.  Integer addition is associative, so

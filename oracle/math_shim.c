@@ -153,6 +153,19 @@ long om_div_count_nobranch_mismatches(int n, uint32_t lo, uint32_t hi) {
     }
     return bad;
 }
+/* mismatches between clim_step_below_scaled(l, e0) and (l < e0 ? 1 : 0)
+ * over the bit patterns [lo, hi) of l (both signs' NaNs and infinities
+ * included when the range holds them) */
+long om_step_scaled_mismatches(float e0, uint32_t lo, uint32_t hi) {
+    long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+    for (int64_t i = lo; i < (int64_t)hi; i++) {
+        const float l = clim_u2f((uint32_t)i);
+        const float w = clim_step_below_scaled(l, e0);
+        if (clim_f2u(w) != clim_f2u(l < e0 ? 1.0f : 0.0f)) bad++;
+    }
+    return bad;
+}
 /* clim_div25_nz at one value (its -0 case) */
 float om_div25_nz(float x) { return clim_div25_nz(x); }
 

@@ -301,6 +301,30 @@ def test_div_count_equals_division(L):
     assert L.om_div_count_nobranch_mismatches(6, 0x3F800000, 0x3FFFFFFF) == 0
 
 
+def test_scaled_step_equals_the_comparison(L):
+    """The bilateral's weight as clamp(fma(l, -H, e0 H), 0, 1) (one GPU
+    instruction) against l < e0: every l within 2^22 ulps of e0 on either
+    side, every l of the positive binades from 2^-30 up to 2^8 and the
+    specials, for the squared edges the kernels meet (2 without a learned
+    map, 2 m^2 over the map's range m in [2^-1.5, 2^1.5], powers of two and
+    their neighbours), and the smallest and largest edges it admits."""
+    L.om_step_scaled_mismatches.restype = C.c_long
+    L.om_step_scaled_mismatches.argtypes = [C.c_float, C.c_uint32, C.c_uint32]
+    f = np.float32
+    edges = [f(2.0), f(2.0) * f(2.0 ** -1.5) ** 2, f(2.0) * f(2.0 ** 1.5) ** 2, f(0.25), f(16.0),
+             np.nextafter(f(2.0), f(0)), np.nextafter(f(2.0), f(4)), f(2.0 ** -77), f(3.0e38)]
+    m = np.float32(np.random.default_rng(4).uniform(2.0 ** -1.5, 2.0 ** 1.5, 6))
+    edges += [f(f(1.41421356) * f(1.41421356)) * mm * mm for mm in m]
+    for e0 in edges:
+        b = int(np.array(e0, np.float32).view(np.uint32))
+        assert L.om_step_scaled_mismatches(e0, max(b - (1 << 22), 0), min(b + (1 << 22), 0x7F800000)) == 0, e0
+        assert L.om_step_scaled_mismatches(e0, 0x7F800000, 0x80000000) == 0, e0  # inf, NaNs
+        assert L.om_step_scaled_mismatches(e0, 0xFF800000, 0xFFFFFFFF) == 0, e0
+        assert L.om_step_scaled_mismatches(e0, 0x80000000, 0x80400000) == 0, e0  # -0 and negatives
+    for e0 in edges[:7]:
+        assert L.om_step_scaled_mismatches(e0, 0x30800000, 0x43800000) == 0, e0
+
+
 def test_window_sums_are_never_negative_zero():
     """The premise of clim_div25_nz in the window statistics: under
     round-to-nearest, +0 + (-0) = +0 and x + (-x) = +0, so a float sum
