@@ -8,10 +8,10 @@
 #include "kernels.h"
 #include "strength.h"
 
-// streams a clim_raw_to_rgb_batch call spreads its frames over (two fill
-// the device: three and four measured no faster on 12 MP frames)
+// streams a clim_raw_to_rgb_batch call spreads its frames over (on batches
+// of 72 12 MP frames three measured 1.4 % faster than two, four 3 % slower)
 #ifndef CLIM_BATCH_LANES
-#define CLIM_BATCH_LANES 2
+#define CLIM_BATCH_LANES 3
 #endif
 
 struct DevBuf {

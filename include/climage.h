@@ -382,9 +382,9 @@ clim_status clim_raw_to_rgb_ex(clim_ctx* ctx, const clim_image* raw, clim_image*
  * : null, &params[i]) - RawConverter::demosaicImage called once per frame of a
  * burst (cpp/raw_converter.cpp:73-186), the frames of any sizes and CFA
  * patterns.  No reference counterpart as one call: for callers with several
- * frames at hand.  The frames alternate between the context's stream and a
- * second stream with its own work buffers (created on first use), so that
- * one frame's small launches overlap the other's; the results are those of
+ * frames at hand.  The frames are spread over the context's stream and two
+ * further streams with work buffers of their own (created on first use), so
+ * that one frame's small launches overlap the others'; the results are those of
  * the single-frame calls.  Work queued on the context before the call is
  * complete before any frame starts, and work queued after it sees every
  * output.  out_f32 may be null (or hold nulls).  nlf_out (nullable, n
