@@ -254,8 +254,11 @@ def burst_leg(ctx, replays=20):
                                     "p99": round(float(np.percentile(lat, 99)), 3), "samples": len(lat),
                                     "note": "each frame's last kernel, from the replay's start (events in the graph)"},
             "create_s": round(create_s, 3), "autotune": tuning,
-            "form": "one hipGraphLaunch per burst: per-frame raw fits and pyramids on two lanes, the 16 frames' "
-                    "green and red/blue + despeckle stages as two cross-frame persistent launches"}
+            "form": "one hipGraphLaunch per burst: per-frame raw fits and pyramids on three lanes, the 16 frames' "
+                    + ("green and red/blue + despeckle stages as two cross-frame persistent launches"
+                       if tuning["form"] == "cross-frame launches" else
+                       "green and red/blue + despeckle stages as per-frame kernels on the lanes (the faster "
+                       "recorded form at creation)")}
 
 
 # the path's algorithmic bytes per pixel with every float texel stored as FP16

@@ -253,21 +253,24 @@ class DeviceImage:
 
 
 BURST_FRAME_TIMES = 1  # CLIM_BURST_FRAME_TIMES
+BURST_CROSS_FRAME, BURST_PER_FRAME = 2, 4  # CLIM_BURST_CROSS_FRAME, CLIM_BURST_PER_FRAME
 PRECISION_FP32, PRECISION_MIXED = 0, 1  # clim_precision
 
 
 class BurstTuning(C.Structure):
     _fields_ = [("waves", C.c_int32 * 2), ("blocks_per_cu", C.c_int32 * 2), ("grab", C.c_int32 * 2),
                 ("us", C.c_float * 2), ("us_first", C.c_float * 2), ("candidates", C.c_int32 * 2),
-                ("tiles", C.c_int32 * 2), ("cross_frames", C.c_int32), ("cached", C.c_int32)]
+                ("tiles", C.c_int32 * 2), ("cross_frames", C.c_int32), ("cached", C.c_int32),
+                ("form", C.c_int32), ("form_ms", C.c_float * 2)]
 
 
 class Burst:
     """A burst recorded once and replayed (clim_burst_*): frame i from raws[i]
     into outs8[i] (and outs_f32[i]) with params[i].  The images must outlive
-    the burst."""
+    the burst.  form: None (both recorded forms timed, the faster kept),
+    "cross_frame" or "per_frame" (that form only)."""
 
-    def __init__(self, ctx, raws, outs8, params, outs_f32=None, frame_times=False):
+    def __init__(self, ctx, raws, outs8, params, outs_f32=None, frame_times=False, form=None):
         n = len(raws)
         if len(outs8) != n or len(params) != n or (outs_f32 is not None and len(outs_f32) != n):
             raise ValueError("Burst: lists of different lengths")
@@ -280,8 +283,10 @@ class Burst:
             OF = (C.POINTER(Image) * n)(*[C.pointer(o.im) if o is not None else None for o in outs_f32])
         P = (DemosaicParameters * n)(*params)
         self.h = C.c_void_p()
-        ctx._check(lib().clim_burst_create(ctx.h, n, R, O8, OF, P, C.c_uint32(BURST_FRAME_TIMES if frame_times else 0),
-                                           C.byref(self.h)), "burst_create")
+        flags = (BURST_FRAME_TIMES if frame_times else 0) | \
+            {None: 0, "cross_frame": BURST_CROSS_FRAME, "per_frame": BURST_PER_FRAME}[form]
+        ctx._check(lib().clim_burst_create(ctx.h, n, R, O8, OF, P, C.c_uint32(flags), C.byref(self.h)),
+                   "burst_create")
 
     def launch(self):
         self.ctx._check(lib().clim_burst_launch(self.h), "burst_launch")
@@ -299,7 +304,9 @@ class Burst:
         return {names[k]: {"waves_per_simd": t.waves[k], "blocks_per_cu": t.blocks_per_cu[k], "grab": t.grab[k],
                            "us": round(t.us[k], 2), "us_first_candidate": round(t.us_first[k], 2),
                            "candidates": t.candidates[k], "tiles": t.tiles[k]} for k in range(2)} | \
-            {"cross_frames": t.cross_frames, "cached": bool(t.cached)}
+            {"cross_frames": t.cross_frames, "cached": bool(t.cached),
+             "form": "cross-frame launches" if t.form == 0 else "per-frame kernels",
+             "replay_ms": {"cross_frame": round(t.form_ms[0], 3), "per_frame": round(t.form_ms[1], 3)}}
 
     def close(self):
         if self.h:

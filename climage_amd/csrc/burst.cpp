@@ -17,8 +17,11 @@
 //    no host work per frame;
 //  - the persistent launches' shape (register budget as waves per SIMD,
 //    blocks per CU, tiles per take) is chosen at creation by timing every
-//    candidate on the burst's own frames; the choice is kept per process for
-//    bursts of the same frame sizes.
+//    candidate on the burst's own frames; then the burst is recorded twice,
+//    with the cross-frame launches and with every frame's demosaic on its
+//    lane (the per-frame kernels), both graphs are timed, and the faster is
+//    kept; the choices are kept per process for bursts of the same frame
+//    sizes.
 // The burst owns every buffer its graph touches apart from the caller's
 // images, which must outlive it; the learned-strength weights and mode of
 // the context are copied in at creation.
@@ -35,12 +38,15 @@
 
 namespace {
 
-constexpr int kLanes = 2;  // frames alternate between two lanes
+constexpr int kLanes = 3;  // frame i runs on lane i mod 3
 
 struct Tuned {
     int waves[2], blocks_per_cu[2], grab[2];
     float us[2], us_first[2];
     int candidates[2];
+    int form;          // 0: cross-frame stencil launches, 1: per-frame kernels
+    float form_ms[2];  // a replay of each form at creation
+    int form_known;    // the form was chosen by timing (a cached choice can be reused)
 };  // a clim_burst_tuning without the per-burst fields
 std::mutex g_tune_mu;
 std::map<std::string, Tuned> g_tune;  // frame-size signature -> choice
@@ -85,8 +91,10 @@ static clim_status burst_fail(clim_burst* b, clim_status s, const std::string& m
 static int lane_of(int i) { return i % kLanes; }
 
 // the graph's work: raw fits (lanes), the cross-frame stencil launches (lane
-// 0), the pyramids (lanes); lane 0's stream is the origin
-static clim_status record(clim_burst* b, bool with_times) {
+// 0), the pyramids (lanes); lane 0's stream is the origin.  perframe: every
+// frame's demosaic runs on its lane after its fit (no cross-frame launches,
+// no join between the stages)
+static clim_status record(clim_burst* b, bool with_times, bool perframe) {
     hipStream_t s0 = b->lane[0]->stream;
     auto lane_error = [&](clim_ctx* c, clim_status st, int i) {
         return burst_fail(b, st, "frame " + std::to_string(i) + ": " + c->last_error);
@@ -99,9 +107,19 @@ static clim_status record(clim_burst* b, bool with_times) {
         FrameCall& f = b->calls[i];
         clim_status st = frame_workspace(c, f);
         if (st == CLIM_OK) st = frame_raw_fit(c, f, b->fits + i);
-        if (st == CLIM_OK && std::find(b->cross.begin(), b->cross.end(), i) == b->cross.end())
+        if (st == CLIM_OK && (perframe || std::find(b->cross.begin(), b->cross.end(), i) == b->cross.end()))
             st = frame_demosaic(c, f, b->green[i].img, b->ycc[i].img, 0);
+        if (st == CLIM_OK && perframe) st = frame_pyramid(c, f, b->ycc[i].img);
         if (st != CLIM_OK) return lane_error(c, st, i);
+        if (perframe && with_times)
+            BURST_TRY_HIP(b, hipEventRecordWithFlags(b->t_frame[i], c->stream, hipEventRecordExternal));
+    }
+    if (perframe) {
+        for (int k = 1; k < kLanes; k++) {
+            BURST_TRY_HIP(b, hipEventRecord(b->lane_end[k], b->lane[k]->stream));
+            BURST_TRY_HIP(b, hipStreamWaitEvent(s0, b->lane_end[k], 0));
+        }
+        return CLIM_OK;
     }
     for (int k = 1; k < kLanes; k++) {
         BURST_TRY_HIP(b, hipEventRecord(b->lane_mid[k], b->lane[k]->stream));
@@ -322,7 +340,7 @@ static clim_status burst_build(clim_burst* b, int32_t n, const clim_image* const
         }
     }
     BURST_TRY_HIP(b, hipStreamSynchronize(ctx->stream));  // the caller's queued work (e.g. the inputs) first
-    CLIM_TRY(record(b, false));
+    CLIM_TRY(record(b, false, false));
     BURST_TRY_HIP(b, hipStreamSynchronize(b->lane[0]->stream));
     if (!b->cached && !b->cross.empty()) {
         // the frames' fits are in place: time the shapes of each stage (the
@@ -332,18 +350,81 @@ static clim_status burst_build(clim_burst* b, int32_t n, const clim_image* const
                                     b->tune.blocks_per_cu[0] * b->cus, b->tune.grab[0], b->lane[0]->stream);
         CLIM_TRY(autotune(b, 1));
         BURST_TRY_HIP(b, hipStreamSynchronize(b->lane[0]->stream));
+        b->tune.form_known = 0;
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune[sig] = b->tune;
     }
-    // capture
-    BURST_TRY_HIP(b, hipStreamBeginCapture(b->lane[0]->stream, hipStreamCaptureModeThreadLocal));
-    const clim_status rs = record(b, b->timed);
-    hipGraph_t g = nullptr;
-    const hipError_t ce = hipStreamEndCapture(b->lane[0]->stream, &g);
-    b->graph = g;
-    if (rs != CLIM_OK) return rs;
-    if (ce != hipSuccess) return ctx_hip(ctx, ce, "burst: hipStreamEndCapture");
-    BURST_TRY_HIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    // the per-frame form once eagerly (its lane buffers sized), then both
+    // forms captured; unless the choice is known, both are timed and the
+    // faster is kept
+    CLIM_TRY(record(b, false, true));
+    BURST_TRY_HIP(b, hipStreamSynchronize(b->lane[0]->stream));
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    auto release = [&]() {
+        for (int k = 0; k < 2; k++) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            exec[k] = nullptr;
+            graph[k] = nullptr;
+        }
+    };
+    const int forced = (flags & CLIM_BURST_CROSS_FRAME) ? 0 : (flags & CLIM_BURST_PER_FRAME) ? 1 : -1;
+    const bool choose = forced < 0 && !(b->cached && b->tune.form_known);
+    if (forced >= 0) {
+        b->tune.form = b->cross.empty() ? 1 : forced;
+        b->tune.form_ms[0] = b->tune.form_ms[1] = 0.0f;
+    }
+    for (int form = 0; form < 2; form++) {
+        if (!choose && form != b->tune.form) continue;
+        if (form == 0 && b->cross.empty()) continue;
+        BURST_TRY_HIP(b, hipStreamBeginCapture(b->lane[0]->stream, hipStreamCaptureModeThreadLocal));
+        const clim_status rs = record(b, b->timed, form == 1);
+        const hipError_t ce = hipStreamEndCapture(b->lane[0]->stream, &graph[form]);
+        if (rs != CLIM_OK || ce != hipSuccess) {
+            release();
+            if (rs != CLIM_OK) return rs;
+            return ctx_hip(ctx, ce, "burst: hipStreamEndCapture");
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec[form], graph[form], nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            release();
+            return ctx_hip(ctx, ie, "burst: hipGraphInstantiate");
+        }
+    }
+    if (choose) {
+        // one replay of each to settle, then three timed
+        b->tune.form_ms[0] = b->tune.form_ms[1] = 0.0f;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipError_t e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        for (int form = 0; form < 2 && e == hipSuccess; form++) {
+            if (!exec[form]) continue;
+            e = hipGraphLaunch(exec[form], ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+            for (int r = 0; r < 3 && e == hipSuccess; r++) e = hipGraphLaunch(exec[form], ctx->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            b->tune.form_ms[form] = ms / 3.0f;
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (e != hipSuccess) {
+            release();
+            return ctx_hip(ctx, e, "burst: timing the recorded forms");
+        }
+        b->tune.form = !exec[0] || (exec[1] && b->tune.form_ms[1] < b->tune.form_ms[0]) ? 1 : 0;
+        b->tune.form_known = 1;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[sig] = b->tune;
+    }
+    b->graph = graph[b->tune.form];
+    b->exec = exec[b->tune.form];
+    graph[b->tune.form] = nullptr;
+    exec[b->tune.form] = nullptr;
+    release();
     return CLIM_OK;
 }
 
@@ -352,7 +433,10 @@ extern "C" clim_status clim_burst_create(clim_ctx* ctx, int32_t n, const clim_im
                                          const clim_demosaic_params* params, uint32_t flags, clim_burst** out) {
     if (!ctx || !out || n <= 0 || !raws || !out_rgba8 || !params)
         return ctx_fail(ctx, CLIM_EINVAL, "burst: bad argument");
-    if (flags & ~(uint32_t)CLIM_BURST_FRAME_TIMES) return ctx_fail(ctx, CLIM_EINVAL, "burst: unknown flag");
+    if (flags & ~(uint32_t)(CLIM_BURST_FRAME_TIMES | CLIM_BURST_CROSS_FRAME | CLIM_BURST_PER_FRAME))
+        return ctx_fail(ctx, CLIM_EINVAL, "burst: unknown flag");
+    if ((flags & CLIM_BURST_CROSS_FRAME) && (flags & CLIM_BURST_PER_FRAME))
+        return ctx_fail(ctx, CLIM_EINVAL, "burst: CLIM_BURST_CROSS_FRAME and CLIM_BURST_PER_FRAME together");
     *out = nullptr;
     clim_burst* b = new clim_burst();
     b->ctx = ctx;
@@ -395,5 +479,8 @@ extern "C" clim_status clim_burst_tuning_read(const clim_burst* b, clim_burst_tu
     }
     t->cross_frames = (int32_t)b->cross.size();
     t->cached = b->cached ? 1 : 0;
+    t->form = b->tune.form;
+    t->form_ms[0] = b->tune.form_ms[0];
+    t->form_ms[1] = b->tune.form_ms[1];
     return CLIM_OK;
 }

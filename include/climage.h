@@ -406,9 +406,12 @@ clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const clim_image* co
  * buffers (lanes of its own, each frame's green plane and YCbCr image),
  * runs the burst once, times the candidate shapes of its two cross-frame
  * persistent launches (the green and the red/blue + despeckle stages of
- * every frame, one launch each) on these frames and keeps the fastest (the
- * choice is remembered in the process for bursts of the same frame sizes),
- * and records the whole burst into one HIP graph.  Work queued on the
+ * every frame, one launch each) on these frames and keeps the fastest, then
+ * records the whole burst into a HIP graph twice - with the cross-frame
+ * launches, and with every frame's stages as the per-frame kernels on the
+ * burst's lanes - times a replay of each and keeps the faster graph (the
+ * choices are remembered in the process for bursts of the same frame
+ * sizes).  Work queued on the
  * context before the call is complete first.  The caller's images must
  * outlive the burst; the context's learned-strength weights and mode are
  * copied in at creation.
@@ -425,6 +428,10 @@ clim_status clim_raw_to_rgb_batch(clim_ctx* ctx, int32_t n, const clim_image* co
  * measured from the launch's start. */
 typedef struct clim_burst clim_burst;
 #define CLIM_BURST_FRAME_TIMES 1u
+/* record only one form instead of timing both: the cross-frame launches
+ * (ignored when no frame can take them) or the per-frame kernels */
+#define CLIM_BURST_CROSS_FRAME 2u
+#define CLIM_BURST_PER_FRAME 4u
 
 typedef struct {
     int32_t waves[2];          /* register budget of the chosen build, as waves per SIMD: [0] green, [1] red/blue */
@@ -434,8 +441,10 @@ typedef struct {
     float us_first[2];         /* the first candidate's (one block per CU, one tile per take) */
     int32_t candidates[2];     /* shapes timed */
     int32_t tiles[2];          /* tiles per launch (all frames) */
-    int32_t cross_frames;      /* frames in the cross-frame launches (the high-noise ones demosaic on their own) */
-    int32_t cached;            /* 1: the shapes came from an earlier burst of the same frame sizes */
+    int32_t cross_frames;      /* frames the cross-frame launches can take (the high-noise ones demosaic on their own) */
+    int32_t cached;            /* 1: the choices came from an earlier burst of the same frame sizes */
+    int32_t form;              /* the recorded graph kept: 0 with the cross-frame launches, 1 with per-frame kernels */
+    float form_ms[2];          /* a replay of each form at creation (ms; 0 when not timed) */
 } clim_burst_tuning;
 
 clim_status clim_burst_create(clim_ctx* ctx, int32_t n, const clim_image* const* raws,

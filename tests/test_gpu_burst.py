@@ -60,7 +60,10 @@ SMALL = [(256, 192, ca.RGGB, 0.002, 0.0, ca.BILATERAL), (322, 240, ca.BGGR, 0.00
          (480, 362, ca.GRBG, 0.002, 0.0, ca.BILATERAL)]
 
 
-def test_burst_replays_equal_single_frames(oracle32):
+@pytest.mark.parametrize("form", [None, "cross_frame", "per_frame"])
+def test_burst_replays_equal_single_frames(oracle32, form):
+    """Every recorded form (the timed choice, and each form forced) replays
+    the single-frame calls bit for bit."""
     ctx = ca.Context(0)
     try:
         frames = SMALL
@@ -68,10 +71,14 @@ def test_burst_replays_equal_single_frames(oracle32):
         outs8 = [ctx.image(w, h, ca.FMT_RGBA8) for w, h, *_ in frames]
         outsf = [ctx.image(w, h, ca.FMT_RGBA32F) for w, h, *_ in frames]
         params = [frame_params(b, nl, alg) for _w, _h, b, _a, nl, alg in frames]
-        burst = ca.Burst(ctx, raws, outs8, params, outs_f32=outsf, frame_times=True)
+        burst = ca.Burst(ctx, raws, outs8, params, outs_f32=outsf, frame_times=True, form=form)
         t = burst.tuning()
         print("tuning", t, flush=True)
         assert t["cross_frames"] == len(frames) - 1  # the high-noise frame demosaics on its lane
+        if form is None:
+            assert t["cached"] or (t["replay_ms"]["cross_frame"] > 0 and t["replay_ms"]["per_frame"] > 0)
+        else:
+            assert t["form"] == ("cross-frame launches" if form == "cross_frame" else "per-frame kernels")
         for k in ("green", "red_blue_despeckle"):
             assert t[k]["candidates"] >= 1 and t[k]["us"] > 0 and t[k]["grab"] in (1, 2, 4)
         for rnd, seed in enumerate((10, 500, 900)):
@@ -102,9 +109,11 @@ def test_burst_replays_equal_single_frames(oracle32):
             bitexact(outsf[i].download(), oracle32.raw_to_rgb(host[i], op)["rgbf"], f"frame {i} vs oracle")
         burst.close()
         # a second burst of the same frame sizes takes the recorded shapes
-        with ca.Burst(ctx, raws, outs8, params) as b2:
+        with ca.Burst(ctx, raws, outs8, params, form=form) as b2:
             t2 = b2.tuning()
             assert t2["cached"] and t2["green"]["grab"] == t["green"]["grab"]
+            if form is None:
+                assert t2["form"] == t["form"]
             b2.launch()
             ctx.sync()
             ref = single_frames(frames, host)
