@@ -190,7 +190,7 @@ int main(int argc, char** argv) {
         }
 
         // 4. two frames in one RawConverter::demosaicImages call (the second
-        //    on the context's second stream): each equals the fused result
+        //    on another of the context's streams): each equals the fused result
         DemosaicParameters p4 = synthetic_params(bayer, black, white), p5 = p4;
         for (DemosaicParameters* pp : {&p4, &p5}) {
             pp->rgbConversionParameters.localToneMapping = ltm;

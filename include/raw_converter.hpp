@@ -51,7 +51,7 @@ class RawConverter {
     // Several device-resident frames in one call (clim_raw_to_rgb_batch):
     // frame i from raws[i] into outs[i] (the caller's images, of the mosaic's
     // size); demosaicParameters[i]->noiseModel receives frame i's measured
-    // noise model.  The frames alternate between two streams of the context.
+    // noise model.  The frames are spread over three streams of the context.
     void demosaicImages(const std::vector<const gls::hip_image_2d<gls::luma_pixel_16>*>& raws,
                         const std::vector<gls::hip_image_2d<gls::rgba_pixel>*>& outs,
                         const std::vector<DemosaicParameters*>& demosaicParameters) {

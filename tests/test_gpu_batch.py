@@ -1,5 +1,5 @@
 """clim_raw_to_rgb_batch: frames of mixed sizes and CFA patterns through the
-context's two lanes give the single-frame results (the FP32 oracle's, bit
+context's lanes (frame i on stream i mod 3) give the single-frame results (the FP32 oracle's, bit
 for bit, and the same noise models), outputs are ready for work queued on
 the context afterwards, and a bad frame is reported with its index."""
 import numpy as np

@@ -123,9 +123,9 @@ def test_config1_rggb_12mp_guided_precise(ctx, oracle32, oracle64):
     assert e["rawNlf_rel"] < 1e-3 and e["pyramidNlf_rel"] < 2e-3
 
 
-def test_config1_six_frames_two_lane_batch(ctx, oracle32, oracle64):
+def test_config1_six_frames_batch(ctx, oracle32, oracle64):
     """The benchmark's call: six 12 MP RGGB frames at the six noise levels in
-    one clim_raw_to_rgb_batch call (frames alternate between two streams)."""
+    one clim_raw_to_rgb_batch call (frames spread over the context's three streams)."""
     w, h = 4032, 3024
     raws, ins, outs8, outsf, params = [], [], [], [], []
     for k, a in enumerate(NOISE_A):

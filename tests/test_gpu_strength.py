@@ -9,7 +9,7 @@
   for bit (the map sampling and the scaled thresholds are restated exactly).
 * A gate forced shut gives the plain pipeline's output bit for bit; the gated
   mode opens exactly the levels whose mean predicted error ratio is below 0.
-* Batched frames (two lanes) equal single-frame calls.
+* Batched frames (spread over the context's lanes) equal single-frame calls.
 * Training runs end to end on a small set.
 * The committed trained weights gain PSNR against the clean truth over the
   plain pyramid on held-out scenes.
