@@ -549,11 +549,11 @@ def main():
                     "timing": "kernel alone (one frame at a time), HIP events on the library's stream"}
         if batched and s:
             # the same kernel's launches in the timed batched steps, sharing the
-            # device with the other stream's frame
+            # device with the other lanes' frames
             b_s = s["ms_total"] / s["launches"] / 1e3
             b_ach = stage_bytes(dom, s["texels"] / s["launches"], half=args.mixed) / b_s / 1e9
             roofline["batched"] = {"avg_launch_us": round(b_s * 1e6, 2), "achieved": round(b_ach, 1),
-                                   "frac": round(b_ach / HBM_PEAK_GBS, 4), "lanes": 2,
+                                   "frac": round(b_ach / HBM_PEAK_GBS, 4), "lanes": 3,
                                    "launches_timed": s["launches"]}
 
     path = None
