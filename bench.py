@@ -85,6 +85,9 @@ def stage_bytes(stage, texels, preview=False, half=False):
         "level_stats": L + 16,          # the level in, the statistics (FP32 RGBA) out
         "level_fit": 16,                # the refit pass over the statistics image
         "denoise": L + L,
+        # per texel: the level in (the noisy level, which the FP32 bilateral
+        # path denoises in the same launch, or the denoised one), the next
+        # level's noisy and reassembled texels at a quarter each, the result out
         "reassemble": L + L + L / 4 + L / 4,
         "tone": L + L / 4 + L / 4 + 4,  # level-0 reassembly fused with the tone map: RGBA8 out
         "raw_denoise": 8 + 16,          # 4 u16 texels in, 4 FP32 texels out (per quad)
@@ -100,7 +103,7 @@ def stage_bytes(stage, texels, preview=False, half=False):
 SINGLE_KERNEL = {"scale_raw": "k_scale_raw", "raw_stats": "k_raw_stats", "green": "k_green",
                  "red_blue_ycc": "k_rb_despeckle<true, false>", "despeckle": "k_despeckle",
                  "downsample": "k_downsample<true, false>",
-                 "denoise": "k_bilateral_levels<true>", "reassemble": "k_reassemble<true, false, false, false>",
+                 "denoise": "k_bilateral_levels<true>", "reassemble": "k_bilateral_reassemble<true>",
                  "tone": "k_reassemble<true, true, false, false>", "raw_denoise": "k_raw_rgba_filter<2, true>"}
 # the same stages' kernels in the mixed-precision form
 SINGLE_KERNEL_MIXED = dict(SINGLE_KERNEL, **{
@@ -425,6 +428,10 @@ def main():
         single_kernel["tone"] = f"k_reassemble<true, true, true, {'true' if args.mixed else 'false'}>"
     if args.preview:
         single_kernel["tone"] = "k_fast_preview<true>"
+    if algorithm != ca.BILATERAL and not args.mixed:
+        # levels 1-3 reassembled on their own (the bilateral path denoises
+        # them in the same launch)
+        single_kernel["reassemble"] = "k_reassemble<true, false, false, false>"
     if algorithm == ca.GUIDED_FAST:
         single_kernel["denoise"] = f"k_guided_fast<{'true' if args.mixed else 'false'}>"
     elif algorithm == ca.GUIDED_PRECISE:

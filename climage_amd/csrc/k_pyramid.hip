@@ -458,19 +458,20 @@ __device__ __forceinline__ void bilateral_pair(const LevelTile4<TX, TY>& t, int 
     }
 }
 
-// The outputs of one 32x16 tile (x0, y0) of denoiseImage whose texels are in
-// LDS (t, and the strength map texels in mt); rcp[n] = 1.0f / n.
-template <bool SAME_EDGE, bool H>
+// The outputs of one 32x16 tile (x0, y0) of denoiseImage (a w x h level)
+// whose texels are in LDS (t, and the strength map texels in mt);
+// rcp[n] = 1.0f / n.  Each result goes to emit(x, y, value).
+template <bool SAME_EDGE, typename Emit>
 __device__ __forceinline__ void bilateral_compute(const LevelTile4<TX, TY>& t, const MapStage& ms, const float* mt,
-                                                  const float* rcp, const Img& out, int x0, int y0,
+                                                  const float* rcp, int w, int h, int x0, int y0,
                                                   const ParamRef& nlf, float chromaBoost, float gradientBoost,
-                                                  const StrengthRef& sr, int level) {
+                                                  const StrengthRef& sr, int level, Emit emit) {
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
     const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
-    // thread (tx, ty) writes rows 2 ty and 2 ty + 1 of column tx
+    // thread (tx, ty) finishes rows 2 ty and 2 ty + 1 of column tx
     const int tx = threadIdx.x, ty = 2 * threadIdx.y;
     const int x = x0 + tx, y = y0 + ty;
-    if (x >= out.w || y >= out.h) return;
+    if (x >= w || y >= h) return;
     const float ma = strength_at_staged(sr, ms, mt, level, x, y);
     const float mb = strength_at_staged(sr, ms, mt, level, x, y + 1);
     BilCentre A = bilateral_centre<SAME_EDGE>(t, tx + HR, ty + HR, va0, va1, va2, vb0, vb1, vb2, chromaBoost,
@@ -478,8 +479,21 @@ __device__ __forceinline__ void bilateral_compute(const LevelTile4<TX, TY>& t, c
     BilCentre B = bilateral_centre<SAME_EDGE>(t, tx + HR, ty + 1 + HR, va0, va1, va2, vb0, vb1, vb2, chromaBoost,
                                               gradientBoost, mb);
     bilateral_pair<SAME_EDGE>(t, tx, ty, A, B);
-    st4_<H>(out, x, y, bilateral_mean<SAME_EDGE>(A, rcp));
-    if (y + 1 < out.h) st4_<H>(out, x, y + 1, bilateral_mean<SAME_EDGE>(B, rcp));
+    const float4 ra = bilateral_mean<SAME_EDGE>(A, rcp), rb = bilateral_mean<SAME_EDGE>(B, rcp);
+    emit(x, y, ra);
+    if (y + 1 < h) emit(x, y + 1, rb);
+}
+
+// stages one 32x16 tile (x0, y0) of denoiseImage: the level's texels with
+// their halo, the strength map texels, the count reciprocals (the caller's
+// barrier follows)
+template <bool H>
+__device__ __forceinline__ MapStage bilateral_stage(LevelTile4<TX, TY>& t, float* mt, float* rcp, const Img& in,
+                                                    int x0, int y0, const StrengthRef& sr, int level, int tid) {
+    t.template load<H>(in, x0, y0, tid);
+    const MapStage ms = stage_strength<TX, TY>(sr, level, x0, y0, mt, tid);
+    if (tid < 26) rcp[tid] = 1.0f / (float)tid;
+    return ms;
 }
 
 // one 32x16 tile (x0, y0) of denoiseImage
@@ -488,11 +502,10 @@ __device__ __forceinline__ void bilateral_tile(LevelTile4<TX, TY>& t, float* mt,
                                                const Img& out, int x0, int y0, const ParamRef& nlf, float chromaBoost,
                                                float gradientBoost, const StrengthRef& sr, int level) {
     const int tid = threadIdx.y * 32 + threadIdx.x;
-    t.template load<H>(in, x0, y0, tid);
-    const MapStage ms = stage_strength<TX, TY>(sr, level, x0, y0, mt, tid);
-    if (tid < 26) rcp[tid] = 1.0f / (float)tid;
+    const MapStage ms = bilateral_stage<H>(t, mt, rcp, in, x0, y0, sr, level, tid);
     __syncthreads();
-    bilateral_compute<SAME_EDGE, H>(t, ms, mt, rcp, out, x0, y0, nlf, chromaBoost, gradientBoost, sr, level);
+    bilateral_compute<SAME_EDGE>(t, ms, mt, rcp, out.w, out.h, x0, y0, nlf, chromaBoost, gradientBoost, sr, level,
+                                 [&](int x, int y, float4 v) { st4_<H>(out, x, y, v); });
 }
 
 template <bool SAME_EDGE, bool H>
@@ -506,8 +519,8 @@ __global__ void __launch_bounds__(256) k_bilateral(Img in, Img out, ParamRef nlf
                                  level);
 }
 
-// denoiseImage of every pyramid level in one launch (the levels' denoising
-// reads only the noisy levels): a 1-D grid of all levels' tiles, level 0
+// denoiseImage of several pyramid levels in one launch (the levels' denoising
+// reads only the noisy levels): a 1-D grid of the levels' tiles, the finest
 // first, in XCD-contiguous runs.
 template <bool SAME_EDGE>
 __global__ void __launch_bounds__(256) k_bilateral_levels(BilateralLevels lv) {
@@ -519,7 +532,7 @@ __global__ void __launch_bounds__(256) k_bilateral_levels(BilateralLevels lv) {
     while (l + 1 < lv.n && b >= lv.base[l + 1]) l++;
     const int i = b - lv.base[l], tiles_x = (lv.out[l].w + TX - 1) / TX;
     bilateral_tile<SAME_EDGE, false>(t, mt, rcp, lv.in[l], lv.out[l], (i % tiles_x) * TX, (i / tiles_x) * TY,
-                                     lv.nlf[l], lv.chromaBoost[l], lv.gradientBoost[l], lv.sr, l);
+                                     lv.nlf[l], lv.chromaBoost[l], lv.gradientBoost[l], lv.sr, lv.level[l]);
 }
 
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
@@ -736,7 +749,7 @@ __global__ void __launch_bounds__(256) k_bilateral_levels_mixed(BilateralLevels 
     const int x0 = (i % tiles_x) * TX, y0 = (i / tiles_x) * TY;
     const int tid = threadIdx.y * 32 + threadIdx.x;
     const Img& out = lv.out[l];
-    const MapStage ms = stage_strength<TX, TY>(lv.sr, l, x0, y0, mt, tid);
+    const MapStage ms = stage_strength<TX, TY>(lv.sr, lv.level[l], x0, y0, mt, tid);
     t.load(lv.in[l], x0, y0, tid);
     __syncthreads();
     const ParamRef& nlf = lv.nlf[l];
@@ -747,7 +760,7 @@ __global__ void __launch_bounds__(256) k_bilateral_levels_mixed(BilateralLevels 
         const int tx = threadIdx.x, ty = threadIdx.y + 8 * rr;
         const int x = x0 + tx, y = y0 + ty;
         if (x >= out.w || y >= out.h) continue;
-        const float m = strength_at_staged(lv.sr, ms, mt, l, x, y);
+        const float m = strength_at_staged(lv.sr, ms, mt, lv.level[l], x, y);
         st4_<true>(out, x, y,
                    bilateral_px_mixed<SAME_EDGE>(t, tx, ty, va0, va1, va2, vb0, vb1, vb2, lv.chromaBoost[l],
                                                  lv.gradientBoost[l], m));
@@ -1063,7 +1076,57 @@ void launch_tone(const Img& in, const Img& mask, const Img& out8, const Img& out
 }
 
 // ------------------------------------------------------------ reassembleImage
-// cl/demosaic.cl:867-896.  nlf: (A_Y, B_Y) at nlf.get(0), nlf.get(3).
+// reassembleImage (cl/demosaic.cl:867-896) of texel (x, y) whose denoised
+// value is p0 (nx, ny: 1 / the level's size): p0 - (noisy - reassembled) of
+// the next level sampled at the texel, the luma blended towards the
+// reassembled next level by the detail sharpening.  EXACT2X: half_n1 /
+// half_d1 (i, j) give the next level's noisy and reassembled texels (clamped
+// to the image); otherwise n1 and d1 are sampled.
+template <bool EXACT2X, bool H, typename N1, typename D1>
+__device__ __forceinline__ float4 reassemble_px(float4 p0, int x, int y, float nx, float ny, float sharpening0,
+                                                const ParamRef& nlf, const Img& n1, const Img& d1, N1 half_n1,
+                                                D1 half_d1) {
+    const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
+    float4 a1, b1;
+    // exact 2x: the taps of the samples at px, py (i0, j0; weights a, b); those
+    // at px +- nx, py +- ny (the sharpening gradient) sit at texels x/2 and
+    // x/2 - 1 (y/2, y/2 - 1), rounded down, with weight 1 - a (1 - b)
+    const int i0 = (x & 1) ? (x - 1) >> 1 : (x >> 1) - 1, j0 = (y & 1) ? (y - 1) >> 1 : (y >> 1) - 1;
+    const float a = (x & 1) ? 0.25f : 0.75f, b = (y & 1) ? 0.25f : 0.75f;
+    if (EXACT2X) {
+        a1 = bilinear4_at(half_n1, i0, j0, a, b);
+        b1 = bilinear4_at(half_d1, i0, j0, a, b);
+    } else {
+        a1 = bilinear4<H>(n1, px, py);
+        b1 = bilinear4<H>(d1, px, py);
+    }
+    float4 d;
+    d.x = p0.x - (a1.x - b1.x);
+    d.y = p0.y - (a1.y - b1.y);
+    d.z = p0.z - (a1.z - b1.z);
+    d.w = p0.w - (a1.w - b1.w);
+    float sharpening = sharpening0;
+    if (sharpening > 1.0f) {
+        float dx, dy;
+        if (EXACT2X) {
+            const float ax = 1.0f - a, by = 1.0f - b;
+            const int xh = x >> 1, yh = y >> 1;
+            dx = (bilinear4_at(half_d1, xh, j0, ax, b).x - bilinear4_at(half_d1, xh - 1, j0, ax, b).x) / 2.0f;
+            dy = (bilinear4_at(half_d1, i0, yh, a, by).x - bilinear4_at(half_d1, i0, yh - 1, a, by).x) / 2.0f;
+        } else {
+            dx = (bilinear4<H>(d1, px + nx, py).x - bilinear4<H>(d1, px - nx, py).x) / 2.0f;
+            dy = (bilinear4<H>(d1, px, py + ny).x - bilinear4<H>(d1, px, py - ny).x) / 2.0f;
+        }
+        const float threshold = 0.25f * sqrtf(nlf.get(0) + nlf.get(3) * b1.x);
+        const float detail = smoothstep(0.25f * threshold, threshold, len2(dx, dy)) *
+                             (1.0f - smoothstep(0.95f, 1.0f, d.x)) * (0.6f + 0.4f * smoothstep(0.0f, 0.1f, d.x));
+        sharpening = 1.0f + (sharpening - 1.0f) * detail;
+    }
+    d.x = mixf(b1.x, d.x, sharpening);
+    return d;
+}
+
+// reassembleImage, cl/demosaic.cl:867-896.  nlf: (A_Y, B_Y) at nlf.get(0), nlf.get(3).
 // out may alias d0 (each texel is read, then written, by one thread).
 // TONE: the level-0 pass of the pipeline, which hands its result straight to
 // convertTosRGB (out8 / outf) instead of writing it.  LTM (with TONE): the
@@ -1102,44 +1165,8 @@ __global__ void __launch_bounds__(256) k_reassemble(Img d0, Img n1, Img d1, Img 
     auto half_n1 = [&](int i, int j) { return STAGE ? hs[0][j - hy0][i - hx0] : ld4_<H>(n1, i, j); };
     auto half_d1 = [&](int i, int j) { return STAGE ? hs[STAGE ? 1 : 0][j - hy0][i - hx0] : ld4_<H>(d1, i, j); };
     if (x >= out.w || y >= out.h) return;
-    const float px = ((float)x + 0.5f) * nx, py = ((float)y + 0.5f) * ny;
-    const float4 p0 = ldt4_<H>(d0, x, y);
-    float4 a1, b1;
-    // exact 2x: the taps of the samples at px, py (i0, j0; weights a, b); those
-    // at px +- nx, py +- ny (the sharpening gradient) sit at texels x/2 and
-    // x/2 - 1 (y/2, y/2 - 1), rounded down, with weight 1 - a (1 - b)
-    const int i0 = (x & 1) ? (x - 1) >> 1 : (x >> 1) - 1, j0 = (y & 1) ? (y - 1) >> 1 : (y >> 1) - 1;
-    const float a = (x & 1) ? 0.25f : 0.75f, b = (y & 1) ? 0.25f : 0.75f;
-    if (EXACT2X) {
-        a1 = bilinear4_at(half_n1, i0, j0, a, b);
-        b1 = bilinear4_at(half_d1, i0, j0, a, b);
-    } else {
-        a1 = bilinear4<H>(n1, px, py);
-        b1 = bilinear4<H>(d1, px, py);
-    }
-    float4 d;
-    d.x = p0.x - (a1.x - b1.x);
-    d.y = p0.y - (a1.y - b1.y);
-    d.z = p0.z - (a1.z - b1.z);
-    d.w = p0.w - (a1.w - b1.w);
-    float sharpening = sharpening0;
-    if (sharpening > 1.0f) {
-        float dx, dy;
-        if (EXACT2X) {
-            const float ax = 1.0f - a, by = 1.0f - b;
-            const int xh = x >> 1, yh = y >> 1;
-            dx = (bilinear4_at(half_d1, xh, j0, ax, b).x - bilinear4_at(half_d1, xh - 1, j0, ax, b).x) / 2.0f;
-            dy = (bilinear4_at(half_d1, i0, yh, a, by).x - bilinear4_at(half_d1, i0, yh - 1, a, by).x) / 2.0f;
-        } else {
-            dx = (bilinear4<H>(d1, px + nx, py).x - bilinear4<H>(d1, px - nx, py).x) / 2.0f;
-            dy = (bilinear4<H>(d1, px, py + ny).x - bilinear4<H>(d1, px, py - ny).x) / 2.0f;
-        }
-        const float threshold = 0.25f * sqrtf(nlf.get(0) + nlf.get(3) * b1.x);
-        const float detail = smoothstep(0.25f * threshold, threshold, len2(dx, dy)) *
-                             (1.0f - smoothstep(0.95f, 1.0f, d.x)) * (0.6f + 0.4f * smoothstep(0.0f, 0.1f, d.x));
-        sharpening = 1.0f + (sharpening - 1.0f) * detail;
-    }
-    d.x = mixf(b1.x, d.x, sharpening);
+    const float4 d = reassemble_px<EXACT2X, H>(ldt4_<H>(d0, x, y), x, y, nx, ny, sharpening0, nlf, n1, d1, half_n1,
+                                               half_d1);
     if (TONE && LTM) {
         float v[3];
         tone_pre(d, ta, v);  // the camera RGB of the mask is the tone map's input
@@ -1187,6 +1214,69 @@ void launch_reassemble(const Img& d0, const Img& n1, const Img& d1, const Img& o
             CLIM_REASSEMBLE(false, false, false);
 #undef CLIM_REASSEMBLE
     });
+}
+
+// ------------------------------------------ denoiseImage + reassembleImage
+// One FP32 pyramid level denoised and reassembled in one launch: the 32x16
+// tile of k_bilateral with the next level's noisy and reassembled texels
+// under it (columns x0/2 - 1 .. x0/2 + 16, rows y0/2 - 1 .. y0/2 + 8,
+// clamped to the image) staged before the one barrier; each denoised texel is
+// reassembled as k_reassemble<true, false, false, false> does and written.
+// The values are those of the two kernels in turn (the denoised FP32 texel
+// goes from the filter to the reassembly in registers instead of through
+// HBM), and out may not alias the level.
+namespace {
+constexpr int BR_X = TX / 2 + 2, BR_Y = TY / 2 + 2;
+}
+
+template <bool SAME_EDGE>
+__global__ void __launch_bounds__(256) k_bilateral_reassemble(Img in, Img n1, Img d1, Img out, ParamRef nlf,
+                                                              float chromaBoost, float gradientBoost,
+                                                              float sharpening, StrengthRef sr, int level) {
+    __shared__ LevelTile4<TX, TY> t;
+    __shared__ float mt[kMapStageMax];
+    __shared__ float rcp[26];
+    __shared__ float4 hs[2][BR_Y][BR_X];
+    const TileId tl = xcd_tile();
+    const int x0 = tl.x * TX, y0 = tl.y * TY;
+    const int tid = threadIdx.y * 32 + threadIdx.x;
+    const int hx0 = x0 / 2 - 1, hy0 = y0 / 2 - 1;
+    const MapStage ms = bilateral_stage<false>(t, mt, rcp, in, x0, y0, sr, level, tid);
+    for (int k = tid; k < 2 * BR_Y * BR_X; k += 256) {
+        const int m = k / (BR_Y * BR_X), r = (k / BR_X) % BR_Y, c = k % BR_X;
+        hs[m][r][c] = ld4_<false>(m ? d1 : n1, hx0 + c, hy0 + r);
+    }
+    __syncthreads();
+    const float nx = 1.0f / (float)out.w, ny = 1.0f / (float)out.h;
+    auto half_n1 = [&](int i, int j) { return hs[0][j - hy0][i - hx0]; };
+    auto half_d1 = [&](int i, int j) { return hs[1][j - hy0][i - hx0]; };
+    bilateral_compute<SAME_EDGE>(t, ms, mt, rcp, out.w, out.h, x0, y0, nlf, chromaBoost, gradientBoost, sr, level,
+                                 [&](int x, int y, float4 v) {
+                                     st4_<false>(out, x, y,
+                                                 reassemble_px<true, false>(v, x, y, nx, ny, sharpening, nlf, n1, d1,
+                                                                            half_n1, half_d1));
+                                 });
+}
+
+bool bilateral_reassemble_fits(const Img& in, const Img& n1, const Img& d1) {
+    return !in.half && !n1.half && !d1.half && in.w == 2 * n1.w && in.h == 2 * n1.h && n1.w == d1.w &&
+           n1.h == d1.h && in.w <= kUp2xMax && in.h <= kUp2xMax;
+}
+
+void launch_bilateral_reassemble(const Img& in, const Img& n1, const Img& d1, const Img& out, const ParamRef& nlf,
+                                 float chromaBoost, float gradientBoost, float sharpening, const StrengthRef& sr,
+                                 int level, hipStream_t st) {
+    dim3 blk(32, 8), grd((in.w + TX - 1) / TX, (in.h + TY - 1) / TY);
+    // the edges as the kernel computes them (lumaBoost = 1 without gradient boost)
+    float e0 = 1.0f * 1.41421356f, e1 = chromaBoost * 1.41421356f;
+    e0 = e0 * e0;
+    e1 = e1 * e1;
+    if (!(gradientBoost > 1.0f) && e0 == e1)
+        hipLaunchKernelGGL(k_bilateral_reassemble<true>, grd, blk, 0, st, in, n1, d1, out, nlf, chromaBoost,
+                           gradientBoost, sharpening, sr, level);
+    else
+        hipLaunchKernelGGL(k_bilateral_reassemble<false>, grd, blk, 0, st, in, n1, d1, out, nlf, chromaBoost,
+                           gradientBoost, sharpening, sr, level);
 }
 
 }  // namespace clim

@@ -228,16 +228,24 @@ __device__ __forceinline__ float strength_at_staged(const StrengthRef& s, const 
 void launch_bilateral(const Img& in, const Img& out, const ParamRef& nlf, float chromaBoost, float gradientBoost,
                       hipStream_t st, const StrengthRef* sr = nullptr, int level = 0);
 // denoiseImage of n pyramid levels in one launch (base: set by the launcher);
-// entry l is pyramid level l
+// entry l is pyramid level level[l] (the finest first)
 struct BilateralLevels {
     Img in[5], out[5];
     ParamRef nlf[5];
     float chromaBoost[5], gradientBoost[5];
-    int base[5];
+    int base[5], level[5];
     int n;
     StrengthRef sr;
 };
 void launch_bilateral_levels(BilateralLevels& lv, hipStream_t st);
+// denoiseImage of one FP32 pyramid level (in) and reassembleImage of the
+// result with the next level (n1 noisy, d1 reassembled) into out, in one
+// launch: the denoised level stays on chip.  Only for an exact 2x next level
+// (bilateral_reassemble_fits).
+bool bilateral_reassemble_fits(const Img& in, const Img& n1, const Img& d1);
+void launch_bilateral_reassemble(const Img& in, const Img& n1, const Img& d1, const Img& out, const ParamRef& nlf,
+                                 float chromaBoost, float gradientBoost, float sharpening, const StrengthRef& sr,
+                                 int level, hipStream_t st);
 void launch_guided_fast(const Img& in, const Img& out, const ParamRef& nlf, hipStream_t st,
                         const StrengthRef* sr = nullptr, int level = 0);
 // convertTosRGB applied to reassembleImage's result in the same kernel

@@ -133,24 +133,39 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
     CLIM_TRY(strength_infer(ctx, lvl, levels, algorithm, &sr));
 
     // synthesis, coarse to fine.  The bilateral denoiser reads only the noisy
-    // levels, so every level is denoised in one launch ahead of the
-    // reassembly chain.
+    // levels: level 0 and the coarsest are denoised in one launch ahead of
+    // the reassembly chain, and each FP32 level between them is denoised in
+    // the launch that reassembles it (its denoised texels stay on chip).
+    // FP16 levels, and a level whose next level is not half its size, take
+    // the one launch instead.
+    bool fused[CLIM_MAX_LEVELS] = {false};
     if (algorithm == CLIM_DENOISE_BILATERAL) {
-        ProfScope ps(ctx, ST_DENOISE, nlev);
         clim::BilateralLevels bl{};
-        bl.n = levels;
         bl.sr = sr;
+        long long ntex = 0;
         for (int i = 0; i < levels; i++) {
-            bl.in[i] = lvl[i];
-            bl.out[i] = den[i];
-            bl.nlf[i] = dev_params(ctx->fit[1 + i].cal);
-            bl.chromaBoost[i] = dp[i].chromaBoost;
-            bl.gradientBoost[i] = dp[i].gradientBoost;
+            fused[i] = i > 0 && i < levels - 1 && clim::bilateral_reassemble_fits(lvl[i], lvl[i + 1], den[i + 1]);
+            if (fused[i]) continue;
+            const int e = bl.n++;
+            bl.in[e] = lvl[i];
+            bl.out[e] = den[i];
+            bl.level[e] = i;
+            bl.nlf[e] = dev_params(ctx->fit[1 + i].cal);
+            bl.chromaBoost[e] = dp[i].chromaBoost;
+            bl.gradientBoost[e] = dp[i].gradientBoost;
+            ntex += (long long)lvl[i].w * lvl[i].h;
         }
+        ProfScope ps(ctx, ST_DENOISE, ntex);
         clim::launch_bilateral_levels(bl, st);
     }
     for (int i = levels - 1; i >= 0; i--) {
         const ParamRef cal = dev_params(ctx->fit[1 + i].cal);
+        if (fused[i]) {
+            ProfScope pr(ctx, ST_REASSEMBLE, (long long)lvl[i].w * lvl[i].h);
+            clim::launch_bilateral_reassemble(lvl[i], lvl[i + 1], den[i + 1], den[i], cal, dp[i].chromaBoost,
+                                              dp[i].gradientBoost, dp[i].sharpening, sr, i, st);
+            continue;
+        }
         const bool bilateral = algorithm == CLIM_DENOISE_GUIDED_FAST && i == levels - 1;
         if (algorithm != CLIM_DENOISE_BILATERAL) {
             ProfScope ps(ctx, ST_DENOISE, (long long)lvl[i].w * lvl[i].h);
