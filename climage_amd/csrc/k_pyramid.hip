@@ -310,8 +310,9 @@ struct BilCentre {
     float nH, E, n;  // SAME_EDGE: the step of bilateral_tap, its weight count
 };
 
-// the centre at tile texel (X, Y) (halo included)
-template <bool SAME_EDGE>
+// the centre at tile texel (X, Y) (halo included); MAP: the strength m may
+// be non-zero (without it m is 0 and the edges are the same for every texel)
+template <bool SAME_EDGE, bool MAP>
 __device__ __forceinline__ BilCentre bilateral_centre(const LevelTile4<TX, TY>& t, int X, int Y, float va0, float va1,
                                                       float va2, float vb0, float vb1, float vb2, float chromaBoost,
                                                       float gradientBoost, float m) {
@@ -341,7 +342,7 @@ __device__ __forceinline__ BilCentre bilateral_centre(const LevelTile4<TX, TY>& 
     float e0 = lumaBoost * 1.41421356f, e1 = chromaBoost * 1.41421356f;
     e0 = e0 * e0;
     e1 = e1 * e1;
-    if (m > 0.0f) {
+    if (MAP && m > 0.0f) {
         // learned strength: the range sigma times m, i.e. the squared edges times m^2
         const float m2 = m * m;
         e0 = e0 * m2;
@@ -461,27 +462,41 @@ __device__ __forceinline__ void bilateral_pair(const LevelTile4<TX, TY>& t, int 
 // The outputs of one 32x16 tile (x0, y0) of denoiseImage (a w x h level)
 // whose texels are in LDS (t, and the strength map texels in mt);
 // rcp[n] = 1.0f / n.  Each result goes to emit(x, y, value).
-template <bool SAME_EDGE, typename Emit>
-__device__ __forceinline__ void bilateral_compute(const LevelTile4<TX, TY>& t, const MapStage& ms, const float* mt,
-                                                  const float* rcp, int w, int h, int x0, int y0,
-                                                  const ParamRef& nlf, float chromaBoost, float gradientBoost,
-                                                  const StrengthRef& sr, int level, Emit emit) {
+template <bool SAME_EDGE, bool MAP, typename Emit>
+__device__ __forceinline__ void bilateral_compute_(const LevelTile4<TX, TY>& t, const MapStage& ms, const float* mt,
+                                                   const float* rcp, int w, int h, int x0, int y0,
+                                                   const ParamRef& nlf, float chromaBoost, float gradientBoost,
+                                                   const StrengthRef& sr, int level, Emit emit) {
     const float va0 = nlf.get(0), va1 = nlf.get(1), va2 = nlf.get(2);
     const float vb0 = nlf.get(3), vb1 = nlf.get(4), vb2 = nlf.get(5);
     // thread (tx, ty) finishes rows 2 ty and 2 ty + 1 of column tx
     const int tx = threadIdx.x, ty = 2 * threadIdx.y;
     const int x = x0 + tx, y = y0 + ty;
     if (x >= w || y >= h) return;
-    const float ma = strength_at_staged(sr, ms, mt, level, x, y);
-    const float mb = strength_at_staged(sr, ms, mt, level, x, y + 1);
-    BilCentre A = bilateral_centre<SAME_EDGE>(t, tx + HR, ty + HR, va0, va1, va2, vb0, vb1, vb2, chromaBoost,
-                                              gradientBoost, ma);
-    BilCentre B = bilateral_centre<SAME_EDGE>(t, tx + HR, ty + 1 + HR, va0, va1, va2, vb0, vb1, vb2, chromaBoost,
-                                              gradientBoost, mb);
+    const float ma = MAP ? strength_at_staged(sr, ms, mt, level, x, y) : 0.0f;
+    const float mb = MAP ? strength_at_staged(sr, ms, mt, level, x, y + 1) : 0.0f;
+    BilCentre A = bilateral_centre<SAME_EDGE, MAP>(t, tx + HR, ty + HR, va0, va1, va2, vb0, vb1, vb2, chromaBoost,
+                                                   gradientBoost, ma);
+    BilCentre B = bilateral_centre<SAME_EDGE, MAP>(t, tx + HR, ty + 1 + HR, va0, va1, va2, vb0, vb1, vb2,
+                                                   chromaBoost, gradientBoost, mb);
     bilateral_pair<SAME_EDGE>(t, tx, ty, A, B);
     const float4 ra = bilateral_mean<SAME_EDGE>(A, rcp), rb = bilateral_mean<SAME_EDGE>(B, rcp);
     emit(x, y, ra);
     if (y + 1 < h) emit(x, y + 1, rb);
+}
+template <bool SAME_EDGE, typename Emit>
+__device__ __forceinline__ void bilateral_compute(const LevelTile4<TX, TY>& t, const MapStage& ms, const float* mt,
+                                                  const float* rcp, int w, int h, int x0, int y0,
+                                                  const ParamRef& nlf, float chromaBoost, float gradientBoost,
+                                                  const StrengthRef& sr, int level, Emit emit) {
+    // without a strength map for the level (or a constant strength) every
+    // texel's m is 0 and the edge setup folds to constants
+    if (!(sr.constant > 0.0f) && (!sr.map || !sr.open[level]))
+        bilateral_compute_<SAME_EDGE, false>(t, ms, mt, rcp, w, h, x0, y0, nlf, chromaBoost, gradientBoost, sr, level,
+                                             emit);
+    else
+        bilateral_compute_<SAME_EDGE, true>(t, ms, mt, rcp, w, h, x0, y0, nlf, chromaBoost, gradientBoost, sr, level,
+                                            emit);
 }
 
 // stages one 32x16 tile (x0, y0) of denoiseImage: the level's texels with

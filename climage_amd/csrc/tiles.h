@@ -49,9 +49,16 @@ struct LevelTile4 {
     float4 p[SY][SX];
     template <bool H = false>
     __device__ __forceinline__ void load(const Img& in, int x0, int y0, int tid) {
-        stage<SX * SY, 256>(
-            tid, [&](int i) { return ld4_<H>(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
-            [&](int i, float4 v) { p[i / SX][i % SX] = v; });
+        // a tile whose halo lies inside the image reads without clamps (the
+        // same texels)
+        if (x0 >= 2 && y0 >= 2 && x0 + SX - 2 <= in.w && y0 + SY - 2 <= in.h)
+            stage<SX * SY, 256>(
+                tid, [&](int i) { return ldt4_<H>(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
+                [&](int i, float4 v) { p[i / SX][i % SX] = v; });
+        else
+            stage<SX * SY, 256>(
+                tid, [&](int i) { return ld4_<H>(in, x0 - 2 + i % SX, y0 - 2 + i / SX); },
+                [&](int i, float4 v) { p[i / SX][i % SX] = v; });
     }
     // the tile's texels in registers (fetch, then put)
     using Regs = Staged<SX * SY, 256, float4>;
