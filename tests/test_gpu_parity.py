@@ -286,6 +286,26 @@ def test_raw_to_rgb(ctx, oracle32, bayer, algorithm):
     bitexact(o8, r["rgba8"], "raw_to_rgb rgba8")
 
 
+@pytest.mark.parametrize("cb,gb,sharp", [(1.5, 1.0, 1.0), (1.0, 2.0, 1.6), (1.3, 1.5, 1.3)])
+def test_raw_to_rgb_denoise_parameters(ctx, oracle32, cb, gb, sharp):
+    """The bilateral pyramid with per-level chroma boost, gradient boost and
+    sharpening: levels 1-3 are denoised in the launch that reassembles them,
+    in the general-edge form when the boosts differ, and sharpen there."""
+    raw, _ = scene(ca.RGGB, seed=7)
+    p = ca.default_params(ca.RGGB, 64.0, 4095.0, levels=5)
+    op = O.default_params(ca.RGGB, 64.0, 4095.0, levels=5)
+    for lv in range(5):
+        for d in (p.denoiseParameters[lv], op.dp[lv]):
+            d.chromaBoost = cb
+            d.gradientBoost = gb
+            d.sharpening = sharp
+    o8, of, raw_nlf, pyr_nlf = run_pipeline(ctx, raw, p)
+    r = oracle32.raw_to_rgb(raw, op)
+    bitexact(pyr_nlf, r["pyramidNlf"], "pyramidNlf")
+    bitexact(of, r["rgbf"], f"raw_to_rgb float cb={cb} gb={gb} sharpening={sharp}")
+    bitexact(o8, r["rgba8"], "raw_to_rgb rgba8")
+
+
 def test_config0_1024x768(ctx, oracle32, oracle64):
     """1024x768 RGGB, 12-bit, black 64, a = 0.002, b = 1e-5, 4-level bilateral."""
     raw = O.generate_bayer(1024, 768, ca.RGGB, 1000, 2000, 0.002, 1e-5, 64.0, 4095.0)
