@@ -1273,9 +1273,9 @@ __global__ void __launch_bounds__(256) k_bilateral_reassemble(Img in, Img n1, Im
                                  });
 }
 
-bool bilateral_reassemble_fits(const Img& in, const Img& n1, const Img& d1) {
-    return !in.half && !n1.half && !d1.half && in.w == 2 * n1.w && in.h == 2 * n1.h && n1.w == d1.w &&
-           n1.h == d1.h && in.w <= kUp2xMax && in.h <= kUp2xMax;
+bool bilateral_reassemble_fits(const Img& in, const Img& n1, const Img& d1, const Img& out) {
+    return !in.half && !n1.half && !d1.half && !out.half && out.w == in.w && out.h == in.h && in.w == 2 * n1.w &&
+           in.h == 2 * n1.h && n1.w == d1.w && n1.h == d1.h && in.w <= kUp2xMax && in.h <= kUp2xMax;
 }
 
 void launch_bilateral_reassemble(const Img& in, const Img& n1, const Img& d1, const Img& out, const ParamRef& nlf,

@@ -242,7 +242,7 @@ void launch_bilateral_levels(BilateralLevels& lv, hipStream_t st);
 // result with the next level (n1 noisy, d1 reassembled) into out, in one
 // launch: the denoised level stays on chip.  Only for an exact 2x next level
 // (bilateral_reassemble_fits).
-bool bilateral_reassemble_fits(const Img& in, const Img& n1, const Img& d1);
+bool bilateral_reassemble_fits(const Img& in, const Img& n1, const Img& d1, const Img& out);
 void launch_bilateral_reassemble(const Img& in, const Img& n1, const Img& d1, const Img& out, const ParamRef& nlf,
                                  float chromaBoost, float gradientBoost, float sharpening, const StrengthRef& sr,
                                  int level, hipStream_t st);

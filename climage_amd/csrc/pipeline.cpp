@@ -144,7 +144,8 @@ static clim_status pyramid(clim_ctx* ctx, const Img& level0, const Img& out, int
         bl.sr = sr;
         long long ntex = 0;
         for (int i = 0; i < levels; i++) {
-            fused[i] = i > 0 && i < levels - 1 && clim::bilateral_reassemble_fits(lvl[i], lvl[i + 1], den[i + 1]);
+            fused[i] = i > 0 && i < levels - 1 &&
+                       clim::bilateral_reassemble_fits(lvl[i], lvl[i + 1], den[i + 1], den[i]);
             if (fused[i]) continue;
             const int e = bl.n++;
             bl.in[e] = lvl[i];
